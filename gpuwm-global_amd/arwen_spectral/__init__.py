@@ -1,0 +1,24 @@
+"""MI355X-native spherical-harmonic transform engine.
+
+Drop-in for the reference's ``arwen_global.spectral`` transform layer:
+``SphericalHarmonicTransform`` (analysis, synthesis, gradients),
+``VorticityDivergenceOperator`` (winds <-> vorticity/divergence) and
+``GaussianGrid``, computed by the HIP kernels of ``libarwen_sht.so``
+(Gauss nodes and Legendre tables in FP64 on the device, parity-folded
+grouped Legendre GEMMs on MFMA in FP32 or split BF16, fused radix-2/3/4
+longitude FFTs).  ``TransformBatch`` runs many transforms as one grouped
+launch per direction, optionally as a hipGraph.
+"""
+from __future__ import annotations
+
+from .backend import Backend, get_backend
+from .batch import TransformBatch
+from .constants import EARTH_RADIUS_M
+from .grid import GaussianGrid
+from .transform import DEFAULT_BAND, SphericalHarmonicTransform
+from .vector import VorticityDivergenceOperator
+
+__all__ = [
+    "Backend", "get_backend", "TransformBatch", "EARTH_RADIUS_M", "GaussianGrid",
+    "DEFAULT_BAND", "SphericalHarmonicTransform", "VorticityDivergenceOperator",
+]

@@ -1,0 +1,113 @@
+"""Grouped transform batches: one persistent Legendre GEMM launch per direction.
+
+A ``TransformBatch`` takes an ordered list of jobs -- synthesis-type jobs
+(``inverse``, ``zonal``, ``meridional``, ``gradient``, ``wind``) and
+analysis-type jobs (``forward``, ``vordiv``) on any mix of transforms (one
+precision) -- and runs them as
+
+    pack -> one grouped synthesis GEMM -> inverse FFTs
+    forward FFTs -> one grouped analysis GEMM -> unpack
+
+so all fields, levels, ensemble members and truncations of the batch share
+one launch per direction.  An analysis job may read a synthesis job's output
+of the same batch (a round trip).  ``capture()`` records the launches in a
+hipGraph; ``launch()`` replays it.  The reference has no counterpart: it
+transforms fields in chunks and ensemble members one at a time
+(reference da/ensemble.py:437-455).
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _native
+from .transform import SphericalHarmonicTransform
+
+OPS = {
+    "inverse": (_native.OP_SYNTHESIS, 1, 1),
+    "zonal": (_native.OP_ZONAL_DERIVATIVE, 1, 1),
+    "meridional": (_native.OP_MERIDIONAL_DERIVATIVE, 1, 1),
+    "gradient": (_native.OP_GRADIENT, 1, 2),
+    "wind": (_native.OP_UV_FROM_VORDIV, 2, 2),
+    "forward": (_native.OP_ANALYSIS, 1, 1),
+    "vordiv": (_native.OP_VORDIV_FROM_UV, 2, 2),
+}
+_SPECTRAL_IN = {"inverse", "zonal", "meridional", "gradient", "wind"}
+
+
+class TransformBatch:
+    """``jobs``: list of ``(op, transform, inputs, outputs)`` with torch tensors.
+
+    Inputs and outputs are caller-owned device tensors in the boundary
+    layouts; they must stay alive and keep their storage for the batch's
+    lifetime (the work list holds their addresses).
+    """
+
+    def __init__(self, jobs):
+        self._lib = _native.load()
+        self._keep = []
+        arr = (_native.Job * len(jobs))()
+        precision = None
+        for k, (op, tr, ins, outs) in enumerate(jobs):
+            if op not in OPS:
+                raise ValueError(f"unknown op {op!r}; expected one of {sorted(OPS)}")
+            if not isinstance(tr, SphericalHarmonicTransform):
+                raise ValueError("each job needs a SphericalHarmonicTransform")
+            code, nin, nout = OPS[op]
+            ins = tuple(ins) if isinstance(ins, (tuple, list)) else (ins,)
+            outs = tuple(outs) if isinstance(outs, (tuple, list)) else (outs,)
+            if len(ins) != nin or len(outs) != nout:
+                raise ValueError(f"{op} takes {nin} input(s) and {nout} output(s)")
+            want_in = tr.spectral_shape if op in _SPECTRAL_IN else tr.grid.shape
+            want_out = tr.grid.shape if op in _SPECTRAL_IN else tr.spectral_shape
+            lead = tuple(ins[0].shape[:-2])
+            for t in ins:
+                if tuple(t.shape[-2:]) != want_in or tuple(t.shape[:-2]) != lead:
+                    raise ValueError(f"{op} input has shape {tuple(t.shape)}")
+                if not t.is_contiguous():
+                    raise ValueError(f"{op} inputs must be contiguous")
+            for t in outs:
+                if tuple(t.shape[-2:]) != want_out or tuple(t.shape[:-2]) != lead:
+                    raise ValueError(f"{op} output has shape {tuple(t.shape)}")
+                if not t.is_contiguous():
+                    raise ValueError(f"{op} outputs must be contiguous")
+            if precision is None:
+                precision = tr.backend.precision
+            elif precision != tr.backend.precision:
+                raise ValueError("all transforms of a batch must share one precision")
+            count = SphericalHarmonicTransform._count(ins[0])
+            ptr = lambda seq, i: seq[i].data_ptr() if i < len(seq) else None
+            arr[k] = _native.Job(code, tr.plan_handle.value, count, ptr(ins, 0), ptr(ins, 1),
+                                 ptr(outs, 0), ptr(outs, 1))
+            self._keep.append((tr, ins, outs))
+        handle = ctypes.c_void_p()
+        _native.check(self._lib.arwen_sht_batch_create(arr, len(jobs), ctypes.byref(handle)))
+        self._handle = handle
+        self._backend = jobs[0][1].backend
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h:
+            try:
+                self._lib.arwen_sht_batch_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    def run(self):
+        _native.check(self._lib.arwen_sht_batch_run(self._handle, self._backend.stream()))
+
+    def capture(self):
+        _native.check(self._lib.arwen_sht_batch_capture(self._handle, self._backend.stream()))
+
+    def launch(self):
+        _native.check(self._lib.arwen_sht_batch_launch(self._handle, self._backend.stream()))
+
+    def set_grid(self, gemm_blocks: int):
+        _native.check(self._lib.arwen_sht_batch_set_grid(self._handle, int(gemm_blocks)))
+
+    def work(self) -> tuple[float, float]:
+        """(Legendre FLOPs, FFT bytes) per run, algorithmic."""
+        f = ctypes.c_double()
+        b = ctypes.c_double()
+        _native.check(self._lib.arwen_sht_batch_work(self._handle, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value

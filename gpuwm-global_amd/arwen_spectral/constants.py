@@ -1,0 +1,4 @@
+"""Physical constants of the transform (reference spectral/constants.py:4)."""
+from __future__ import annotations
+
+EARTH_RADIUS_M = 6_371_220.0

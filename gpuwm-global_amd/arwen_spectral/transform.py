@@ -1,0 +1,290 @@
+"""Triangular spherical-harmonic analysis/synthesis on a Gaussian grid, on MI355X.
+
+Drop-in for the reference's ``SphericalHarmonicTransform``
+(src/arwen_global/spectral/transform.py:632-1863): same constructor, same
+layouts -- grid ``(*lead, nlat, nlon)`` real, spectral ``(*lead, T+1, T+1)``
+complex indexed ``[n, m]`` -- same method names, identity rules and error
+types.  Arrays are torch tensors on the current ROCm device; numpy inputs
+are copied there.  Every transform is one call into ``libarwen_sht.so``.
+"""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import json
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from .backend import Backend, get_backend
+from .constants import EARTH_RADIUS_M
+from .grid import GaussianGrid
+
+DEFAULT_BAND = 32  # reference legendre.py:36; accepted for API compatibility
+
+
+def _vp(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+@dataclass
+class SphericalHarmonicTransform:
+    grid: GaussianGrid
+    backend: Backend
+    tensor_core_contractions: bool = False
+    legendre_band: int = DEFAULT_BAND
+    streaming: bool = False
+    latitude_bands: int = 1
+    order_partition: tuple = ()
+    _plan: ctypes.c_void_p = field(default=None, init=False, repr=False)
+
+    def __post_init__(self) -> None:
+        if self.tensor_core_contractions:
+            raise ValueError(
+                "tensor_core_contractions=True selects TF32 on CUDA; on this engine "
+                "choose precision='bf16x6' or 'float32' instead")
+        if int(self.legendre_band) < 1:
+            raise ValueError(f"legendre_band must be >= 1, got {self.legendre_band}")
+        if int(self.latitude_bands) < 1:
+            raise ValueError(f"latitude_bands must be >= 1, got {self.latitude_bands}")
+        if self.streaming:
+            raise ValueError(
+                "streaming=True is not built: on 288 GB of HBM the T767 tables "
+                "are resident (about 4 GB)")
+        if tuple(self.order_partition or ()):
+            raise ValueError("order_partition is a multi-device axis; this engine runs on one GPU")
+        lib = _native.load()
+        g = self.grid
+        mu = np.ascontiguousarray(g.sin_lat, dtype=np.float64)
+        w = np.ascontiguousarray(g.quadrature_weights, dtype=np.float64)
+        desc = _native.PlanDesc(int(g.truncation), int(g.nlat), int(g.nlon), float(g.radius_m),
+                                self.backend.precision_code, _native.dptr(mu), _native.dptr(w))
+        plan = ctypes.c_void_p()
+        _native.check(lib.arwen_sht_plan_create(ctypes.byref(desc), ctypes.byref(plan)))
+        self._plan = plan
+        self._lib = lib
+        torch = self.backend.xp
+        n = np.arange(g.truncation + 1, dtype=np.float64)
+        self._eigen = -(n * (n + 1.0)) / (g.radius_m * g.radius_m)
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan:
+            try:
+                self._lib.arwen_sht_plan_destroy(plan)
+            except Exception:
+                pass
+            self._plan = None
+
+    # ---- construction (reference transform.py:953-1023) -----------------
+    @classmethod
+    def create(cls, truncation: int, *, nlat=None, nlon=None, dealias_factor: float = 1.5,
+               radius_m=None, grid: GaussianGrid | None = None, backend: str = "hip",
+               precision: str = "float32", tensor_core_contractions: bool = False,
+               legendre_band: int = DEFAULT_BAND, streaming: bool = False,
+               latitude_bands: int = 1, order_partition: tuple = ()):
+        if grid is not None:
+            if nlat is not None or nlon is not None or radius_m is not None \
+                    or dealias_factor != 1.5:
+                raise ValueError(
+                    "grid= carries its own shape and radius: nlat, nlon, "
+                    "dealias_factor and radius_m cannot be combined with it")
+            grid = grid.truncated_to(truncation)
+        else:
+            kwargs = {} if radius_m is None else {"radius_m": radius_m}
+            grid = GaussianGrid.create(truncation, nlat=nlat, nlon=nlon,
+                                       dealias_factor=dealias_factor, **kwargs)
+        return cls(grid, get_backend(backend, precision),
+                   tensor_core_contractions=tensor_core_contractions,
+                   legendre_band=legendre_band, streaming=streaming,
+                   latitude_bands=latitude_bands, order_partition=tuple(order_partition or ()))
+
+    # ---- shapes and identity (transform.py:1044-1104) -------------------
+    @property
+    def truncation(self) -> int:
+        return self.grid.truncation
+
+    @property
+    def spectral_shape(self) -> tuple[int, int]:
+        n = self.truncation + 1
+        return n, n
+
+    @property
+    def plan_handle(self) -> ctypes.c_void_p:
+        return self._plan
+
+    @property
+    def geometry_identity(self) -> dict:
+        return {
+            "truncation": int(self.truncation),
+            "nlat": int(self.grid.nlat),
+            "nlon": int(self.grid.nlon),
+            "radius_m": float(self.grid.radius_m),
+            "normalization": "complex-orthonormal-condon-shortley",
+            "latitude_grid": "gauss-legendre-in-sin-latitude",
+        }
+
+    @property
+    def geometry_hash(self) -> str:
+        return hashlib.sha256(json.dumps(self.geometry_identity, sort_keys=True,
+                                         separators=(",", ":")).encode()).hexdigest()
+
+    @property
+    def identity(self) -> dict:
+        """Geometry plus arithmetic: a precision mode other than float32 joins
+        the identity, as the reference joins TF32 (transform.py:1074-1098)."""
+        payload = {**self.geometry_identity, "backend": self.backend.name,
+                   "float_dtype": "float32"}
+        if self.backend.precision != "float32":
+            payload["legendre_precision"] = self.backend.precision
+        if int(self.legendre_band) != DEFAULT_BAND:
+            payload["legendre_band"] = int(self.legendre_band)
+        return payload
+
+    @property
+    def identity_hash(self) -> str:
+        return hashlib.sha256(json.dumps(self.identity, sort_keys=True,
+                                         separators=(",", ":")).encode()).hexdigest()
+
+    @property
+    def legendre_table_nbytes(self) -> dict:
+        total = int(self._lib.arwen_sht_plan_table_bytes(self._plan))
+        sixth = total // 6  # P, dP/dphi, P*m/cos in the synthesis and analysis layouts
+        return {"basis": sixth, "analysis": sixth, "derivative": 2 * sixth,
+                "zonal": 2 * sixth, "scratch": 0}
+
+    # ---- validation -----------------------------------------------------
+    def _grid_tensor(self, field):
+        f = self.backend.asarray(field, dtype=self.backend.float_dtype).contiguous()
+        if tuple(f.shape[-2:]) != self.grid.shape:
+            raise ValueError(
+                f"grid field has trailing shape {tuple(f.shape[-2:])}, expected {self.grid.shape}")
+        return f
+
+    def _spectral_tensor(self, coeff):
+        c = self.backend.asarray(coeff, dtype=self.backend.complex_dtype).contiguous()
+        if tuple(c.shape[-2:]) != self.spectral_shape:
+            raise ValueError(
+                f"spectral field has trailing shape {tuple(c.shape[-2:])}, "
+                f"expected {self.spectral_shape}")
+        return c
+
+    @staticmethod
+    def _count(t) -> int:
+        return int(np.prod(t.shape[:-2], dtype=np.int64)) if t.dim() > 2 else 1
+
+    def _empty_grid(self, lead):
+        torch = self.backend.xp
+        return torch.empty((*lead, *self.grid.shape), dtype=self.backend.float_dtype,
+                           device=self.backend.device())
+
+    def _empty_spectral(self, lead):
+        torch = self.backend.xp
+        return torch.empty((*lead, *self.spectral_shape), dtype=self.backend.complex_dtype,
+                           device=self.backend.device())
+
+    # ---- transforms -----------------------------------------------------
+    def forward(self, field):
+        """Analysis, grid -> spectral (transform.py:1486-1490)."""
+        f = self._grid_tensor(field)
+        out = self._empty_spectral(f.shape[:-2])
+        _native.check(self._lib.arwen_sht_analysis(self._plan, _vp(f), _vp(out), self._count(f),
+                                                   self.backend.stream()))
+        return out
+
+    def _synth(self, fn, coeff):
+        c = self._spectral_tensor(coeff)
+        out = self._empty_grid(c.shape[:-2])
+        _native.check(fn(self._plan, _vp(c), _vp(out), self._count(c), self.backend.stream()))
+        return out
+
+    def inverse(self, coeff):
+        """Synthesis, spectral -> grid (transform.py:1742-1746)."""
+        return self._synth(self._lib.arwen_sht_synthesis, coeff)
+
+    def inverse_zonal_derivative(self, coeff):
+        """inverse(c * i m) (transform.py:1752-1760)."""
+        return self._synth(self._lib.arwen_sht_zonal_derivative, coeff)
+
+    def inverse_meridional_derivative(self, coeff):
+        """d field / d latitude in grid space (transform.py:1748-1750)."""
+        return self._synth(self._lib.arwen_sht_meridional_derivative, coeff)
+
+    def gradient(self, coeff):
+        """(east, north) derivatives per metre (transform.py:1762-1768)."""
+        c = self._spectral_tensor(coeff)
+        east = self._empty_grid(c.shape[:-2])
+        north = self._empty_grid(c.shape[:-2])
+        _native.check(self._lib.arwen_sht_gradient(self._plan, _vp(c), _vp(east), _vp(north),
+                                                   self._count(c), self.backend.stream()))
+        return east, north
+
+    def _spectral_op(self, op, coeff):
+        c = self._spectral_tensor(coeff)
+        out = self._empty_spectral(c.shape[:-2])
+        _native.check(self._lib.arwen_sht_spectral_op(self._plan, op, _vp(c), _vp(out),
+                                                      self._count(c), self.backend.stream()))
+        return out
+
+    def project(self, coeff):
+        """Zero above the triangle, m = 0 real; a fresh array (transform.py:1124-1144)."""
+        return self._spectral_op(_native.SPECTRAL_PROJECT, coeff)
+
+    def laplacian(self, coeff):
+        return self._spectral_op(_native.SPECTRAL_LAPLACIAN, coeff)
+
+    def inverse_laplacian(self, coeff):
+        return self._spectral_op(_native.SPECTRAL_INVERSE_LAPLACIAN, coeff)
+
+    # ---- small helpers (transform.py:1813-1835) -------------------------
+    def zeros(self, *leading: int):
+        torch = self.backend.xp
+        return torch.zeros((*leading, *self.spectral_shape), dtype=self.backend.complex_dtype,
+                           device=self.backend.device())
+
+    def constant_coeff(self, value: float):
+        out = self.zeros()
+        out[0, 0] = float(value) * math.sqrt(4.0 * math.pi)
+        return out
+
+    def add_grid_constant(self, coeff, value: float):
+        out = self._spectral_tensor(coeff).clone()
+        out[..., 0, 0] += float(value) * math.sqrt(4.0 * math.pi)
+        return out
+
+    def spectral_mean_square(self, coeff) -> float:
+        c = self.backend.to_numpy(coeff)
+        if tuple(c.shape[-2:]) != self.spectral_shape:
+            raise ValueError(f"spectral field has trailing shape {c.shape[-2:]}, "
+                             f"expected {self.spectral_shape}")
+        w = np.ones(self.truncation + 1)
+        w[1:] = 2.0
+        total = np.sum(np.abs(c) ** 2 * w[None, :], axis=(-2, -1))
+        return float(np.asarray(total).mean() / (4.0 * np.pi))
+
+    def power_by_degree(self, coeff) -> np.ndarray:
+        c = self.backend.to_numpy(coeff)
+        w = np.ones(self.truncation + 1)
+        w[1:] = 2.0
+        return np.sum(np.abs(c) ** 2 * w[None, :], axis=-1)
+
+    def transform_check(self, *, seed: int = 0, max_degree: int | None = None) -> dict:
+        """Round-trip and Parseval gate (reference transform.py:1837-1863)."""
+        rng = np.random.default_rng(seed)
+        t = self.truncation if max_degree is None else min(self.truncation, int(max_degree))
+        coeff = np.zeros(self.spectral_shape, dtype=np.complex128)
+        for n in range(t + 1):
+            for m in range(n + 1):
+                coeff[n, m] = rng.normal() + (0.0j if m == 0 else 1j * rng.normal())
+        grid = self.inverse(coeff)
+        back = self.backend.to_numpy(self.forward(grid))
+        scale = max(1.0, float(np.max(np.abs(coeff))))
+        error = float(np.max(np.abs(back - coeff)) / scale)
+        spatial = self.grid.global_mean(self.backend.to_numpy(grid).astype(np.float64) ** 2)
+        spectral = float(np.sum(np.abs(coeff) ** 2 * np.where(
+            np.arange(self.truncation + 1)[None, :] == 0, 1.0, 2.0)) / (4.0 * np.pi))
+        parseval = abs(spatial - spectral) / max(1.0, abs(spectral))
+        return {"truncation": self.truncation, "nlat": self.grid.nlat, "nlon": self.grid.nlon,
+                "roundtrip_relative_linf": error, "parseval_relative_error": float(parseval)}

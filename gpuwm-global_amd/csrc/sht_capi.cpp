@@ -1,0 +1,809 @@
+// C ABI of the spherical-harmonic transform engine (include/arwen_sht.h).
+//
+// A plan owns one truncation's device state: Gauss nodes and weights, the
+// three Legendre tables in both GEMM layouts, the FFT twiddles and the
+// per-row scale vectors.  A batch owns the workspaces, the grouped-GEMM work
+// list of its jobs and, once captured, a hipGraph of its launches.
+#include "arwen_sht.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "sht_kernels.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Failure {
+  int code;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) {
+  g_error = msg;
+  throw Failure{code};
+}
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    fail(e == hipErrorOutOfMemory ? ARWEN_SHT_ENOMEM : ARWEN_SHT_EHIP,
+         std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return ARWEN_SHT_OK;
+  } catch (const Failure& x) {
+    return x.code;
+  } catch (const std::bad_alloc&) {
+    g_error = "host allocation failed";
+    return ARWEN_SHT_ENOMEM;
+  } catch (...) {
+    g_error = "unexpected C++ exception";
+    return ARWEN_SHT_EHIP;
+  }
+}
+
+// Owning device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n, bool zero = true) { alloc(n, zero); }
+  void alloc(size_t n, bool zero = true) {
+    release();
+    bytes = n;
+    if (n == 0) return;
+    hip_check(hipMalloc(&p, n), "hipMalloc");
+    if (zero) hip_check(hipMemset(p, 0, n), "hipMemset");
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+template <class T>
+void upload(DevBuf& buf, const std::vector<T>& host) {
+  buf.alloc(host.size() * sizeof(T), false);
+  if (!host.empty())
+    hip_check(hipMemcpy(buf.p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice),
+              "hipMemcpy H2D");
+}
+
+std::vector<int> factor_radices(int n) {
+  std::vector<int> r;
+  while (n % 4 == 0) { r.push_back(4); n /= 4; }
+  while (n % 2 == 0) { r.push_back(2); n /= 2; }
+  while (n % 3 == 0) { r.push_back(3); n /= 3; }
+  for (int p = 5; n > 1; p += 2) {
+    while (n % p == 0) { r.push_back(p); n /= p; }
+    if (p * p > n && n > 1) { r.push_back(n); n = 1; }
+  }
+  return r;
+}
+
+}  // namespace
+
+enum TableKind { kP = 0, kD = 1, kPm = 2 };
+
+struct arwen_sht_batch;
+
+struct arwen_sht_plan {
+  int device = 0;
+  int T = 0, nlat = 0, nlon = 0, JH = 0, JHP = 0, MB = 0;
+  double radius = 0;
+  int precision = 0;
+  std::vector<int> koff;  // 2(T+1)+1 entries
+  long long tab_elems = 0;
+  int ldp = 0;            // packed floats per column
+  int ldw = 0;            // waist floats per column
+  std::vector<double> mu, w;
+  DevBuf d_mu, d_w, d_koff, d_tw;
+  DevBuf tab_syn[3], tab_ana[3];
+  DevBuf scale_fold, scale_fold_vec, scale_one, scale_inv_a;
+  sht::FftPlan fft{};
+  int fft_rp = 4;
+  // single-operation batch cache (most recent first)
+  std::list<std::pair<std::tuple<int, int, const void*, const void*, void*, void*>,
+                      arwen_sht_batch*>>
+      cache;
+  ~arwen_sht_plan();
+
+  int kp(int m, int p) const { return koff[2 * m + p + 1] - koff[2 * m + p]; }
+  size_t table_bytes() const {
+    size_t per = precision == ARWEN_SHT_FP32 ? 4 : 6;
+    return 6 * (size_t)tab_elems * per;
+  }
+  const void* table(int kind, bool syn, long long* piece) const {
+    *piece = tab_elems;
+    return syn ? tab_syn[kind].p : tab_ana[kind].p;
+  }
+};
+
+namespace {
+
+void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
+  if (!d) fail(ARWEN_SHT_EINVAL, "plan descriptor is NULL");
+  const int T = d->truncation;
+  if (T < 1) fail(ARWEN_SHT_EINVAL, "truncation must be >= 1, got " + std::to_string(T));
+  if (d->nlat < T + 1)
+    fail(ARWEN_SHT_EINVAL, "nlat=" + std::to_string(d->nlat) + " is too small for T" +
+                               std::to_string(T) + "; need at least " + std::to_string(T + 1));
+  if (d->nlon % 2 || d->nlon < 2 * T + 2)
+    fail(ARWEN_SHT_EINVAL, "nlon must be even and >= 2T+2, got " + std::to_string(d->nlon));
+  if (!(d->radius_m > 0) || !std::isfinite(d->radius_m))
+    fail(ARWEN_SHT_EINVAL, "radius_m must be finite and positive");
+  if (d->precision != ARWEN_SHT_FP32 && d->precision != ARWEN_SHT_BF16X3 &&
+      d->precision != ARWEN_SHT_BF16X6)
+    fail(ARWEN_SHT_EINVAL, "precision must be ARWEN_SHT_FP32, _BF16X3 or _BF16X6");
+  if ((d->sin_lat == nullptr) != (d->weights == nullptr))
+    fail(ARWEN_SHT_EINVAL, "sin_lat and weights must be given together");
+  hip_check(hipGetDevice(&P->device), "hipGetDevice");
+  P->T = T;
+  P->nlat = d->nlat;
+  P->nlon = d->nlon;
+  P->radius = d->radius_m;
+  P->precision = d->precision;
+  P->JH = (d->nlat + 1) / 2;
+  P->JHP = SHT_ROUND_UP(P->JH, sht::kPad);
+  P->MB = (T + 1 + sht::kMT - 1) / sht::kMT;
+  P->koff.assign(2 * (T + 1) + 1, 0);
+  for (int m = 0; m <= T; ++m)
+    for (int p = 0; p < 2; ++p)
+      P->koff[2 * m + p + 1] = P->koff[2 * m + p] + sht::k_padded(T, m, p);
+  P->ldp = P->koff.back();
+  P->tab_elems = (long long)P->ldp * P->JHP;
+  P->ldw = P->MB * 2 * P->JHP * sht::kMT;
+
+  // Nodes and weights.
+  P->mu.resize(P->nlat);
+  P->w.resize(P->nlat);
+  P->d_mu.alloc(P->nlat * sizeof(double), false);
+  P->d_w.alloc(P->nlat * sizeof(double), false);
+  if (d->sin_lat) {
+    for (int j = 0; j < P->nlat; ++j) {
+      P->mu[j] = d->sin_lat[j];
+      P->w[j] = d->weights[j];
+    }
+    for (int j = 0; j < P->nlat; ++j)
+      if (P->mu[j] != -P->mu[P->nlat - 1 - j])
+        fail(ARWEN_SHT_EINVAL, "sin_lat must be ascending and symmetric about the equator");
+    hip_check(hipMemcpy(P->d_mu.p, P->mu.data(), P->nlat * 8, hipMemcpyHostToDevice), "copy mu");
+    hip_check(hipMemcpy(P->d_w.p, P->w.data(), P->nlat * 8, hipMemcpyHostToDevice), "copy w");
+  } else {
+    hip_check(sht_launch_gauss_newton(P->nlat, P->d_mu.as<double>(), P->d_w.as<double>(), 0),
+              "gauss_newton launch");
+    hip_check(hipMemcpy(P->mu.data(), P->d_mu.p, P->nlat * 8, hipMemcpyDeviceToHost), "read mu");
+    hip_check(hipMemcpy(P->w.data(), P->d_w.p, P->nlat * 8, hipMemcpyDeviceToHost), "read w");
+  }
+  upload(P->d_koff, P->koff);
+
+  // Legendre tables.
+  sht::TableOut out{};
+  out.elems = P->tab_elems;
+  const bool fp32 = P->precision == ARWEN_SHT_FP32;
+  for (int k = 0; k < 3; ++k) {
+    size_t bytes = (size_t)P->tab_elems * (fp32 ? 4 : 6);
+    P->tab_syn[k].alloc(bytes, true);
+    P->tab_ana[k].alloc(bytes, true);
+    if (fp32) {
+      out.f32_syn[k] = P->tab_syn[k].as<float>();
+      out.f32_ana[k] = P->tab_ana[k].as<float>();
+    } else {
+      out.bf_syn[k] = P->tab_syn[k].as<unsigned short>();
+      out.bf_ana[k] = P->tab_ana[k].as<unsigned short>();
+    }
+  }
+  hip_check(sht_launch_legendre_tables(T, P->nlat, P->JH, P->JHP, P->d_mu.as<double>(),
+                                       P->d_koff.as<int>(), &out, 0),
+            "legendre_tables launch");
+
+  // FFT plan.
+  std::vector<int> rad = factor_radices(P->nlon);
+  if ((int)rad.size() > sht::kMaxStages) fail(ARWEN_SHT_EINVAL, "nlon has too many factors");
+  P->fft.N = P->nlon;
+  P->fft.nstages = (int)rad.size();
+  for (size_t s = 0; s < rad.size(); ++s) P->fft.radix[s] = rad[s];
+  std::vector<float2> tw(P->nlon);
+  for (int k = 0; k < P->nlon; ++k) {
+    double ang = -2.0 * M_PI * (double)k / (double)P->nlon;
+    tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  upload(P->d_tw, tw);
+  P->fft.tw = P->d_tw.as<float2>();
+  // Rows per FFT block: as many as fit 64 KiB of LDS (vector fold is the largest).
+  P->fft_rp = 4;
+  while (P->fft_rp > 1 && sht_fold_lds_bytes(P->nlon, T, 1, P->fft_rp) > 65536) P->fft_rp /= 2;
+
+  // Per folded-row scales.
+  std::vector<float> sf(P->JHP, 0.f), sfv(P->JHP, 0.f), one(P->JHP, 1.f), inva(P->JHP, 0.f);
+  for (int i = 0; i < P->JH; ++i) {
+    const int jn = P->nlat - 1 - i;
+    double wq = P->w[jn];
+    if (jn == i) wq *= 0.5;  // the equator row is its own mirror
+    sf[i] = (float)(2.0 * M_PI * wq / P->nlon);
+    sfv[i] = (float)(2.0 * M_PI * wq / (P->radius * P->nlon));
+    inva[i] = (float)(1.0 / P->radius);
+  }
+  upload(P->scale_fold, sf);
+  upload(P->scale_fold_vec, sfv);
+  upload(P->scale_one, one);
+  upload(P->scale_inv_a, inva);
+  hip_check(hipDeviceSynchronize(), "plan build");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batches
+// ---------------------------------------------------------------------------
+
+struct JobState {
+  arwen_sht_job job;
+  DevBuf packed_a, packed_b;  // synthesis inputs (packed coefficients)
+  DevBuf waist_a, waist_b;    // synthesis output / analysis input
+  DevBuf packed_out;          // analysis output
+  int ncols = 0;
+};
+
+struct arwen_sht_batch {
+  int device = 0;
+  int precision = 0;
+  std::vector<std::unique_ptr<JobState>> jobs;
+  std::vector<sht::PackArgs> packs;
+  std::vector<sht::UnfoldArgs> unfolds;
+  std::vector<sht::FoldArgs> folds;
+  std::vector<sht::UnpackArgs> unpacks;
+  DevBuf syn_probs, syn_tiles, ana_probs, ana_tiles;
+  int syn_ntiles = 0, ana_ntiles = 0;
+  int gemm_blocks = 0;  // 0 = auto
+  double legendre_flops = 0, fft_bytes = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~arwen_sht_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+arwen_sht_plan::~arwen_sht_plan() {
+  for (auto& e : cache) delete e.second;
+}
+
+namespace {
+
+bool is_synthesis(int op) { return op >= ARWEN_SHT_OP_SYNTHESIS && op <= ARWEN_SHT_OP_UV_FROM_VORDIV; }
+
+struct ProbBuilder {
+  std::vector<sht::GemmProb> probs;
+  std::vector<std::pair<long long, int4>> tiles;  // (work, tile)
+  void add(const sht::GemmProb& pr) {
+    int id = (int)probs.size();
+    probs.push_back(pr);
+    long long k = 0;
+    for (int s = 0; s < pr.nseg; ++s) k += pr.seg[s].K;
+    for (int r0 = 0; r0 < pr.R; r0 += 64)
+      for (int c0 = 0; c0 < pr.N; c0 += 64) {
+        long long rows = std::min(64, pr.R - r0), cols = std::min(64, pr.N - c0);
+        tiles.push_back({k * rows * cols, int4{id, r0, c0, 0}});
+      }
+  }
+  void finish(DevBuf& dprobs, DevBuf& dtiles, int* ntiles) {
+    std::stable_sort(tiles.begin(), tiles.end(),
+                     [](const auto& a, const auto& b) { return a.first > b.first; });
+    std::vector<int4> t;
+    t.reserve(tiles.size());
+    for (auto& x : tiles) t.push_back(x.second);
+    upload(dprobs, probs);
+    upload(dtiles, t);
+    *ntiles = (int)t.size();
+  }
+};
+
+// Data-side segment on the waist layout W[c][m/8][p][i][m%8].
+sht::GemmSeg waist_seg(const arwen_sht_plan* P, const float* W, int m, int p) {
+  sht::GemmSeg s{};
+  s.B = W + ((size_t)((m / sht::kMT) * 2 + p) * P->JHP) * sht::kMT + (m % sht::kMT);
+  s.bks = sht::kMT;
+  s.bcs = P->ldw;
+  s.K = P->JHP;
+  return s;
+}
+// Data-side segment on the packed layout C[c][koff(m,p) + n_idx].
+sht::GemmSeg packed_seg(const arwen_sht_plan* P, const float* C, int m, int p) {
+  sht::GemmSeg s{};
+  s.B = C + P->koff[2 * m + p];
+  s.bks = 1;
+  s.bcs = P->ldp;
+  s.K = P->kp(m, p);
+  return s;
+}
+void set_table(const arwen_sht_plan* P, sht::GemmSeg& s, int kind, bool syn, int m, int p) {
+  long long piece;
+  const char* base = static_cast<const char*>(P->table(kind, syn, &piece));
+  const size_t esz = P->precision == ARWEN_SHT_FP32 ? 4 : 2;
+  s.A = base + (size_t)P->koff[2 * m + p] * P->JHP * esz;
+  s.piece = piece;
+  s.lda = syn ? P->kp(m, p) : P->JHP;
+}
+
+void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
+  if (njobs <= 0 || !jobs) fail(ARWEN_SHT_EINVAL, "a batch needs at least one job");
+  hip_check(hipGetDevice(&B->device), "hipGetDevice");
+  B->precision = jobs[0].plan ? jobs[0].plan->precision : 0;
+  ProbBuilder syn, ana;
+  for (int j = 0; j < njobs; ++j) {
+    const arwen_sht_job& J = jobs[j];
+    const arwen_sht_plan* P = J.plan;
+    if (!P) fail(ARWEN_SHT_EINVAL, "job " + std::to_string(j) + " has no plan");
+    if (P->precision != B->precision)
+      fail(ARWEN_SHT_EINVAL, "all plans of a batch must share one precision");
+    if (P->device != B->device)
+      fail(ARWEN_SHT_EINVAL, "job " + std::to_string(j) + "'s plan belongs to another device");
+    if (J.count < 0) fail(ARWEN_SHT_EINVAL, "job count must be >= 0");
+    const bool vec = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV || J.op == ARWEN_SHT_OP_VORDIV_FROM_UV;
+    const bool two_out = vec || J.op == ARWEN_SHT_OP_GRADIENT;
+    if (J.op < ARWEN_SHT_OP_SYNTHESIS || J.op > ARWEN_SHT_OP_VORDIV_FROM_UV)
+      fail(ARWEN_SHT_EINVAL, "unknown op " + std::to_string(J.op));
+    if (J.count > 0 && (!J.in0 || !J.out0 || (vec && !J.in1) || (two_out && !J.out1)))
+      fail(ARWEN_SHT_EINVAL, "job " + std::to_string(j) + " is missing a buffer");
+    auto js = std::make_unique<JobState>();
+    js->job = J;
+    const int n = J.count;
+    const int T = P->T;
+    const double pcf = 4.0 * (P->nlat / 2.0) * (T + 1.0) * (T + 2.0) / 2.0;  // FLOP per PC
+    const double fftb = 4.0 * P->nlat * P->nlon + 8.0 * P->nlat * (T + 1.0);
+    if (n == 0) { B->jobs.push_back(std::move(js)); continue; }
+    if (is_synthesis(J.op)) {
+      // ---- pack ----
+      const int ncol = vec ? 4 : 2;
+      js->ncols = ncol * n;
+      js->packed_a.alloc((size_t)js->ncols * P->ldp * 4);
+      if (vec) js->packed_b.alloc((size_t)js->ncols * P->ldp * 4);
+      js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
+      const bool grad = J.op == ARWEN_SHT_OP_GRADIENT;
+      if (grad) js->waist_b.alloc((size_t)js->ncols * P->ldw * 4);
+      sht::PackArgs pa{};
+      pa.koff = P->d_koff.as<int>();
+      pa.T = T;
+      pa.mtiles = (T + 1 + 31) / 32;
+      pa.ntiles = (P->kp(0, 0) + 31) / 32;
+      pa.job.in0 = static_cast<const float2*>(J.in0);
+      pa.job.in1 = static_cast<const float2*>(J.in1);
+      pa.job.out0 = js->packed_a.as<float>();
+      pa.job.out1 = js->packed_b.as<float>();
+      pa.job.nfields = n;
+      pa.job.col0 = 0;
+      pa.job.ldp = P->ldp;
+      pa.job.radius = (float)P->radius;
+      pa.job.mode = J.op == ARWEN_SHT_OP_ZONAL_DERIVATIVE ? sht::kIM
+                    : vec                                 ? sht::kVector
+                                                          : sht::kPlain;
+      B->packs.push_back(pa);
+      // ---- GEMM problems: output parity q over all orders ----
+      for (int m = 0; m <= T; ++m)
+        for (int q = 0; q < 2; ++q) {
+          sht::GemmProb pr{};
+          pr.R = P->JHP;
+          pr.N = js->ncols;
+          pr.crs = sht::kMT;
+          pr.ccs = P->ldw;
+          const size_t coff = ((size_t)((m / sht::kMT) * 2 + q) * P->JHP) * sht::kMT + m % sht::kMT;
+          if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
+            // U,V = -dP/dphi (psi) + i P m/cos (chi), i P m/cos (psi) + dP/dphi (chi)
+            pr.nseg = 2;
+            pr.seg[0] = packed_seg(P, js->packed_a.as<float>(), m, 1 - q);
+            set_table(P, pr.seg[0], kD, true, m, 1 - q);
+            pr.seg[1] = packed_seg(P, js->packed_b.as<float>(), m, q);
+            set_table(P, pr.seg[1], kPm, true, m, q);
+            pr.C = js->waist_a.as<float>() + coff;
+            syn.add(pr);
+          } else if (J.op == ARWEN_SHT_OP_MERIDIONAL_DERIVATIVE) {
+            pr.nseg = 1;
+            pr.seg[0] = packed_seg(P, js->packed_a.as<float>(), m, 1 - q);
+            set_table(P, pr.seg[0], kD, true, m, 1 - q);
+            pr.C = js->waist_a.as<float>() + coff;
+            syn.add(pr);
+          } else if (grad) {
+            sht::GemmProb pe = pr, pn = pr;
+            pe.nseg = 1;
+            pe.seg[0] = packed_seg(P, js->packed_a.as<float>(), m, q);
+            set_table(P, pe.seg[0], kPm, true, m, q);
+            pe.C = js->waist_a.as<float>() + coff;
+            syn.add(pe);
+            pn.nseg = 1;
+            pn.seg[0] = packed_seg(P, js->packed_a.as<float>(), m, 1 - q);
+            set_table(P, pn.seg[0], kD, true, m, 1 - q);
+            pn.C = js->waist_b.as<float>() + coff;
+            syn.add(pn);
+          } else {  // SYNTHESIS, ZONAL_DERIVATIVE
+            pr.nseg = 1;
+            pr.seg[0] = packed_seg(P, js->packed_a.as<float>(), m, q);
+            set_table(P, pr.seg[0], kP, true, m, q);
+            pr.C = js->waist_a.as<float>() + coff;
+            syn.add(pr);
+          }
+        }
+      // algorithmic FLOPs: one plane contraction per table application
+      {
+        double pcs = (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) ? 4.0 * n
+                     : grad                                ? 2.0 * n
+                                                           : 1.0 * n;
+        B->legendre_flops += pcs * pcf;
+      }
+      // ---- inverse FFTs ----
+      auto unfold = [&](const DevBuf& waist, void* grid, int col0, int fstride, const DevBuf& sc,
+                        int imul) {
+        sht::UnfoldArgs ua{};
+        ua.fft = P->fft;
+        ua.T = T;
+        ua.nlat = P->nlat;
+        ua.JH = P->JH;
+        ua.JHP = P->JHP;
+        ua.rp = P->fft_rp;
+        ua.ngroups = (P->JH + ua.rp - 1) / ua.rp;
+        ua.job.waist = waist.as<float>();
+        ua.job.grid = static_cast<float*>(grid);
+        ua.job.rowscale = sc.as<float>();
+        ua.job.nfields = n;
+        ua.job.col0 = col0;
+        ua.job.fstride = fstride;
+        ua.job.ldw = P->ldw;
+        ua.job.imul = imul;
+        B->unfolds.push_back(ua);
+        B->fft_bytes += n * fftb;
+      };
+      if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
+        unfold(js->waist_a, J.out0, 0, 4, P->scale_inv_a, 0);
+        unfold(js->waist_a, J.out1, 2, 4, P->scale_inv_a, 0);
+      } else if (grad) {
+        unfold(js->waist_a, J.out0, 0, 2, P->scale_inv_a, 1);
+        unfold(js->waist_b, J.out1, 0, 2, P->scale_inv_a, 0);
+      } else {
+        unfold(js->waist_a, J.out0, 0, 2, P->scale_one, 0);
+      }
+    } else {
+      // ---- FFT + fold ----
+      const int ncol = vec ? 4 : 2;
+      js->ncols = ncol * n;
+      js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
+      if (vec) js->waist_b.alloc((size_t)js->ncols * P->ldw * 4);
+      js->packed_out.alloc((size_t)js->ncols * P->ldp * 4);
+      sht::FoldArgs fa{};
+      fa.fft = P->fft;
+      fa.T = T;
+      fa.nlat = P->nlat;
+      fa.JH = P->JH;
+      fa.JHP = P->JHP;
+      fa.rp = P->fft_rp;
+      fa.ngroups = P->JHP / fa.rp;
+      fa.job.grid0 = static_cast<const float*>(J.in0);
+      fa.job.grid1 = static_cast<const float*>(J.in1);
+      fa.job.waist0 = js->waist_a.as<float>();
+      fa.job.waist1 = js->waist_b.as<float>();
+      fa.job.rowscale = vec ? P->scale_fold_vec.as<float>() : P->scale_fold.as<float>();
+      fa.job.nfields = n;
+      fa.job.col0 = 0;
+      fa.job.ldw = P->ldw;
+      fa.job.vector = vec ? 1 : 0;
+      B->folds.push_back(fa);
+      B->fft_bytes += (vec ? 2.0 : 1.0) * n * fftb;
+      for (int m = 0; m <= T; ++m)
+        for (int q = 0; q < 2; ++q) {
+          sht::GemmProb pr{};
+          pr.R = P->kp(m, q);
+          pr.N = js->ncols;
+          pr.crs = 1;
+          pr.ccs = P->ldp;
+          pr.C = js->packed_out.as<float>() + P->koff[2 * m + q];
+          if (vec) {
+            // zeta, D = Pm^T [iV, iU]_q + dP^T [U, -V]_{1-q}
+            pr.nseg = 2;
+            pr.seg[0] = waist_seg(P, js->waist_a.as<float>(), m, q);
+            set_table(P, pr.seg[0], kPm, false, m, q);
+            pr.seg[1] = waist_seg(P, js->waist_b.as<float>(), m, 1 - q);
+            set_table(P, pr.seg[1], kD, false, m, q);
+          } else {
+            pr.nseg = 1;
+            pr.seg[0] = waist_seg(P, js->waist_a.as<float>(), m, q);
+            set_table(P, pr.seg[0], kP, false, m, q);
+          }
+          ana.add(pr);
+        }
+      B->legendre_flops += (vec ? 4.0 : 1.0) * n * pcf;
+      sht::UnpackArgs ua{};
+      ua.koff = P->d_koff.as<int>();
+      ua.T = T;
+      ua.tiles = (T + 1 + 31) / 32;
+      ua.job.in = js->packed_out.as<float>();
+      ua.job.out0 = static_cast<float2*>(J.out0);
+      ua.job.out1 = static_cast<float2*>(J.out1);
+      ua.job.nfields = n;
+      ua.job.col0 = 0;
+      ua.job.ldp = P->ldp;
+      ua.job.vector = vec ? 1 : 0;
+      B->unpacks.push_back(ua);
+    }
+    B->jobs.push_back(std::move(js));
+  }
+  syn.finish(B->syn_probs, B->syn_tiles, &B->syn_ntiles);
+  ana.finish(B->ana_probs, B->ana_tiles, &B->ana_ntiles);
+  hip_check(hipDeviceSynchronize(), "batch setup");
+}
+
+int gemm_grid(const arwen_sht_batch* B, int ntiles) {
+  if (B->gemm_blocks > 0) return std::min(B->gemm_blocks, std::max(ntiles, 1));
+  return std::min(ntiles, 256 * 8);
+}
+
+void run_batch(arwen_sht_batch* B, hipStream_t s) {
+  int dev;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev != B->device) fail(ARWEN_SHT_EINVAL, "batch used on a device other than its own");
+  for (auto& a : B->packs) hip_check(sht_launch_pack(&a, s), "pack launch");
+  hip_check(sht_launch_legendre_gemm(B->precision, B->syn_probs.as<sht::GemmProb>(),
+                                     B->syn_tiles.as<int4>(), B->syn_ntiles,
+                                     gemm_grid(B, B->syn_ntiles), s),
+            "synthesis GEMM launch");
+  for (auto& a : B->unfolds) hip_check(sht_launch_irfft_unfold(&a, s), "irfft launch");
+  for (auto& a : B->folds) hip_check(sht_launch_rfft_fold(&a, s), "rfft launch");
+  hip_check(sht_launch_legendre_gemm(B->precision, B->ana_probs.as<sht::GemmProb>(),
+                                     B->ana_tiles.as<int4>(), B->ana_ntiles,
+                                     gemm_grid(B, B->ana_ntiles), s),
+            "analysis GEMM launch");
+  for (auto& a : B->unpacks) hip_check(sht_launch_unpack(&a, s), "unpack launch");
+}
+
+int single_op(arwen_sht_plan* P, int op, const void* in0, const void* in1, void* out0,
+              void* out1, int count, void* stream) {
+  return guarded([&] {
+    if (!P) fail(ARWEN_SHT_EINVAL, "plan is NULL");
+    if (count < 0) fail(ARWEN_SHT_EINVAL, "count must be >= 0");
+    if (count == 0) return;
+    auto key = std::make_tuple(op, count, in0, in1, out0, out1);
+    arwen_sht_batch* B = nullptr;
+    for (auto it = P->cache.begin(); it != P->cache.end(); ++it) {
+      if (it->first == key) {
+        B = it->second;
+        P->cache.splice(P->cache.begin(), P->cache, it);
+        break;
+      }
+    }
+    if (!B) {
+      arwen_sht_job J{op, P, count, in0, in1, out0, out1};
+      auto nb = std::make_unique<arwen_sht_batch>();
+      plan_batch(nb.get(), &J, 1);
+      B = nb.release();
+      P->cache.push_front({key, B});
+      while (P->cache.size() > 16) {
+        hip_check(hipDeviceSynchronize(), "cache eviction");
+        delete P->cache.back().second;
+        P->cache.pop_back();
+      }
+    }
+    run_batch(B, static_cast<hipStream_t>(stream));
+  });
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Exported C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* arwen_sht_last_error(void) { return g_error.c_str(); }
+
+int arwen_sht_plan_create(const arwen_sht_plan_desc* desc, arwen_sht_plan** out) {
+  return guarded([&] {
+    if (!out) fail(ARWEN_SHT_EINVAL, "out is NULL");
+    *out = nullptr;
+    auto P = std::make_unique<arwen_sht_plan>();
+    build_plan(P.get(), desc);
+    *out = P.release();
+  });
+}
+
+int arwen_sht_plan_destroy(arwen_sht_plan* plan) {
+  return guarded([&] {
+    if (!plan) return;
+    (void)hipDeviceSynchronize();
+    delete plan;
+  });
+}
+
+int arwen_sht_plan_grid(const arwen_sht_plan* P, double* sin_lat, double* weights) {
+  return guarded([&] {
+    if (!P) fail(ARWEN_SHT_EINVAL, "plan is NULL");
+    if (sin_lat) std::memcpy(sin_lat, P->mu.data(), P->nlat * sizeof(double));
+    if (weights) std::memcpy(weights, P->w.data(), P->nlat * sizeof(double));
+  });
+}
+
+size_t arwen_sht_plan_table_bytes(const arwen_sht_plan* P) { return P ? P->table_bytes() : 0; }
+
+int arwen_sht_gauss_nodes(int nlat, double* sin_lat, double* weights) {
+  return guarded([&] {
+    if (nlat < 1) fail(ARWEN_SHT_EINVAL, "nlat must be >= 1, got " + std::to_string(nlat));
+    if (!sin_lat || !weights) fail(ARWEN_SHT_EINVAL, "sin_lat and weights are required");
+    DevBuf mu((size_t)nlat * 8, false), w((size_t)nlat * 8, false);
+    hip_check(sht_launch_gauss_newton(nlat, mu.as<double>(), w.as<double>(), 0),
+              "gauss_newton launch");
+    hip_check(hipMemcpy(sin_lat, mu.p, (size_t)nlat * 8, hipMemcpyDeviceToHost), "read nodes");
+    hip_check(hipMemcpy(weights, w.p, (size_t)nlat * 8, hipMemcpyDeviceToHost), "read weights");
+  });
+}
+
+int arwen_sht_plan_table_f64(const arwen_sht_plan* P, int kind, double* out) {
+  return guarded([&] {
+    if (!P || !out) fail(ARWEN_SHT_EINVAL, "plan and out are required");
+    if (kind < 0 || kind > 2) fail(ARWEN_SHT_EINVAL, "kind must be 0 (P), 1 (dP) or 2 (P*m/cos)");
+    DevBuf tab((size_t)P->tab_elems * 8, true);
+    sht::TableOut t{};
+    t.elems = P->tab_elems;
+    t.f64_syn[kind] = tab.as<double>();
+    hip_check(sht_launch_legendre_tables(P->T, P->nlat, P->JH, P->JHP, P->d_mu.as<double>(),
+                                         P->d_koff.as<int>(), &t, 0),
+              "legendre_tables launch");
+    std::vector<double> h(P->tab_elems);
+    hip_check(hipMemcpy(h.data(), tab.p, h.size() * 8, hipMemcpyDeviceToHost), "read table");
+    const int T1 = P->T + 1;
+    std::fill(out, out + (size_t)T1 * T1 * P->JH, 0.0);
+    for (int m = 0; m <= P->T; ++m)
+      for (int n = m; n <= P->T; ++n) {
+        const int p = (n - m) & 1, idx = (n - m) >> 1;
+        const size_t base = (size_t)P->koff[2 * m + p] * P->JHP;
+        for (int i = 0; i < P->JH; ++i)
+          out[((size_t)m * T1 + n) * P->JH + i] = h[base + (size_t)i * P->kp(m, p) + idx];
+      }
+  });
+}
+
+int arwen_sht_analysis(arwen_sht_plan* P, const float* grid, void* coeff, int count, void* s) {
+  return single_op(P, ARWEN_SHT_OP_ANALYSIS, grid, nullptr, coeff, nullptr, count, s);
+}
+int arwen_sht_synthesis(arwen_sht_plan* P, const void* coeff, float* grid, int count, void* s) {
+  return single_op(P, ARWEN_SHT_OP_SYNTHESIS, coeff, nullptr, grid, nullptr, count, s);
+}
+int arwen_sht_zonal_derivative(arwen_sht_plan* P, const void* coeff, float* grid, int count,
+                               void* s) {
+  return single_op(P, ARWEN_SHT_OP_ZONAL_DERIVATIVE, coeff, nullptr, grid, nullptr, count, s);
+}
+int arwen_sht_meridional_derivative(arwen_sht_plan* P, const void* coeff, float* grid,
+                                    int count, void* s) {
+  return single_op(P, ARWEN_SHT_OP_MERIDIONAL_DERIVATIVE, coeff, nullptr, grid, nullptr, count,
+                   s);
+}
+int arwen_sht_gradient(arwen_sht_plan* P, const void* coeff, float* east, float* north,
+                       int count, void* s) {
+  return single_op(P, ARWEN_SHT_OP_GRADIENT, coeff, nullptr, east, north, count, s);
+}
+int arwen_sht_uv_from_vordiv(arwen_sht_plan* P, const void* vort, const void* div, float* u,
+                             float* v, int count, void* s) {
+  return single_op(P, ARWEN_SHT_OP_UV_FROM_VORDIV, vort, div, u, v, count, s);
+}
+int arwen_sht_vordiv_from_uv(arwen_sht_plan* P, const float* u, const float* v, void* vort,
+                             void* div, int count, void* s) {
+  return single_op(P, ARWEN_SHT_OP_VORDIV_FROM_UV, u, v, vort, div, count, s);
+}
+
+int arwen_sht_spectral_op(arwen_sht_plan* P, int op, const void* in, void* out, int count,
+                          void* s) {
+  return guarded([&] {
+    if (!P) fail(ARWEN_SHT_EINVAL, "plan is NULL");
+    if (op < 0 || op > 2) fail(ARWEN_SHT_EINVAL, "op must be 0 (project), 1 or 2");
+    if (count < 0) fail(ARWEN_SHT_EINVAL, "count must be >= 0");
+    if (count == 0) return;
+    if (!in || !out) fail(ARWEN_SHT_EINVAL, "in and out are required");
+    long long total = (long long)count * (P->T + 1) * (P->T + 1);
+    hip_check(sht_launch_spectral_elementwise(static_cast<const float2*>(in),
+                                              static_cast<float2*>(out), total, P->T, op,
+                                              (float)P->radius, static_cast<hipStream_t>(s)),
+              "spectral op launch");
+  });
+}
+
+int arwen_sht_batch_create(const arwen_sht_job* jobs, int njobs, arwen_sht_batch** out) {
+  return guarded([&] {
+    if (!out) fail(ARWEN_SHT_EINVAL, "out is NULL");
+    *out = nullptr;
+    auto B = std::make_unique<arwen_sht_batch>();
+    plan_batch(B.get(), jobs, njobs);
+    *out = B.release();
+  });
+}
+
+int arwen_sht_batch_destroy(arwen_sht_batch* B) {
+  return guarded([&] {
+    if (!B) return;
+    (void)hipDeviceSynchronize();
+    delete B;
+  });
+}
+
+int arwen_sht_batch_run(arwen_sht_batch* B, void* stream) {
+  return guarded([&] {
+    if (!B) fail(ARWEN_SHT_EINVAL, "batch is NULL");
+    run_batch(B, static_cast<hipStream_t>(stream));
+  });
+}
+
+int arwen_sht_batch_capture(arwen_sht_batch* B, void* stream) {
+  return guarded([&] {
+    if (!B) fail(ARWEN_SHT_EINVAL, "batch is NULL");
+    if (B->exec) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    bool own = false;
+    if (s == nullptr) {  // the null stream cannot be captured
+      hip_check(hipStreamCreate(&s), "hipStreamCreate");
+      own = true;
+    }
+    hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+    try {
+      run_batch(B, s);
+    } catch (...) {
+      hipGraph_t g;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      if (own) (void)hipStreamDestroy(s);
+      throw;
+    }
+    hip_check(hipStreamEndCapture(s, &B->graph), "end capture");
+    hip_check(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0), "graph instantiate");
+    if (own) (void)hipStreamDestroy(s);
+  });
+}
+
+int arwen_sht_batch_launch(arwen_sht_batch* B, void* stream) {
+  int rc = arwen_sht_batch_capture(B, stream);
+  if (rc != ARWEN_SHT_OK) return rc;
+  return guarded([&] {
+    hip_check(hipGraphLaunch(B->exec, static_cast<hipStream_t>(stream)), "graph launch");
+  });
+}
+
+int arwen_sht_batch_set_grid(arwen_sht_batch* B, int gemm_blocks) {
+  return guarded([&] {
+    if (!B) fail(ARWEN_SHT_EINVAL, "batch is NULL");
+    if (gemm_blocks < 0) fail(ARWEN_SHT_EINVAL, "gemm_blocks must be >= 0");
+    B->gemm_blocks = gemm_blocks;
+    if (B->exec) {
+      (void)hipGraphExecDestroy(B->exec);
+      B->exec = nullptr;
+    }
+    if (B->graph) {
+      (void)hipGraphDestroy(B->graph);
+      B->graph = nullptr;
+    }
+  });
+}
+
+int arwen_sht_batch_work(const arwen_sht_batch* B, double* flops, double* bytes) {
+  return guarded([&] {
+    if (!B) fail(ARWEN_SHT_EINVAL, "batch is NULL");
+    if (flops) *flops = B->legendre_flops;
+    if (bytes) *bytes = B->fft_bytes;
+  });
+}
+
+}  // extern "C"

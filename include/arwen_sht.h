@@ -1,0 +1,166 @@
+/*
+ * arwen_sht.h -- C ABI of the MI355X spherical-harmonic transform engine.
+ *
+ * Drop-in boundary for the reference's transform layer
+ * (src/arwen_global/spectral/transform.py, vector.py).  Every data pointer
+ * is DEVICE memory owned by the caller, in the reference's array layouts:
+ *
+ *   grid field      (*lead, nlat, nlon)   float32, C-contiguous, latitudes
+ *                                         ascending (south to north)
+ *   spectral field  (*lead, T+1, T+1)     complex64 (interleaved re, im),
+ *                                         indexed [n, m], zero above the
+ *                                         triangle on output, m = 0 real
+ *                                         after analysis/projection
+ *
+ * "count" is the flattened lead size (fields x levels x members).  Every
+ * operation is enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ * and returns without synchronizing.  Functions return ARWEN_SHT_OK or an
+ * error code; arwen_sht_last_error() gives this thread's last message.
+ * No exception crosses the ABI.  A plan belongs to the device that was
+ * current when it was created; call operations with that device current.
+ */
+#ifndef ARWEN_SHT_H
+#define ARWEN_SHT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define ARWEN_SHT_OK 0
+#define ARWEN_SHT_EINVAL 1 /* bad argument: Python raises ValueError   */
+#define ARWEN_SHT_EHIP 2   /* HIP runtime error: Python raises RuntimeError */
+#define ARWEN_SHT_ENOMEM 3 /* device allocation failed                 */
+
+/* Legendre GEMM precision. */
+#define ARWEN_SHT_FP32 1   /* v_mfma_f32_16x16x4_f32 on fp32 tables            */
+#define ARWEN_SHT_BF16X3 2 /* bf16 hi+lo pairs, 3 MFMA products                 */
+#define ARWEN_SHT_BF16X6 3 /* bf16 hi+lo+residual pieces, 6 MFMA products       */
+
+/* Operation codes of a batch job (arwen_sht_job.op). */
+#define ARWEN_SHT_OP_SYNTHESIS 1           /* in0 coeff -> out0 grid                 */
+#define ARWEN_SHT_OP_ZONAL_DERIVATIVE 2    /* in0 coeff -> out0 grid of inverse(c*i*m) */
+#define ARWEN_SHT_OP_MERIDIONAL_DERIVATIVE 3 /* in0 coeff -> out0 grid of d/dphi     */
+#define ARWEN_SHT_OP_GRADIENT 4            /* in0 coeff -> out0 east, out1 north     */
+#define ARWEN_SHT_OP_UV_FROM_VORDIV 5      /* in0 vort, in1 div -> out0 u, out1 v    */
+#define ARWEN_SHT_OP_ANALYSIS 6            /* in0 grid -> out0 coeff                 */
+#define ARWEN_SHT_OP_VORDIV_FROM_UV 7      /* in0 u, in1 v -> out0 vort, out1 div    */
+
+typedef struct arwen_sht_plan arwen_sht_plan;
+typedef struct arwen_sht_batch arwen_sht_batch;
+
+typedef struct arwen_sht_plan_desc {
+  int truncation;         /* T >= 1                                          */
+  int nlat;               /* Gaussian latitudes, >= T+1                      */
+  int nlon;               /* even, >= 2T+2                                   */
+  double radius_m;        /* sphere radius (reference constants.py:4)        */
+  int precision;          /* ARWEN_SHT_FP32 / _BF16X3 / _BF16X6              */
+  const double* sin_lat;  /* optional HOST nodes, ascending; NULL: the device
+                             Newton iteration computes them                  */
+  const double* weights;  /* HOST quadrature weights (with sin_lat)          */
+} arwen_sht_plan_desc;
+
+/* Build a plan: Gauss nodes and weights (FP64 Newton on the device) and the
+ * parity-folded Legendre tables P, dP/dphi and P*m/cos(phi) (FP64 recurrence
+ * on the device, stored in the plan's precision).  Replaces
+ * SphericalHarmonicTransform.create/__post_init__/_build_tables
+ * (reference transform.py:728-786, 837-933, 953-1023) and
+ * GaussianGrid.create (grid.py:96-126). */
+int arwen_sht_plan_create(const arwen_sht_plan_desc* desc, arwen_sht_plan** out);
+int arwen_sht_plan_destroy(arwen_sht_plan* plan);
+
+/* Host copies of the plan's nodes (ascending sin latitude) and weights,
+ * nlat each.  Replaces GaussianGrid.sin_lat / quadrature_weights
+ * (reference grid.py:112-125). */
+int arwen_sht_plan_grid(const arwen_sht_plan* plan, double* sin_lat, double* weights);
+
+/* Gauss-Legendre nodes (ascending sin latitude) and weights of `nlat` points
+ * by FP64 Newton iteration on the current device, copied to HOST arrays of
+ * nlat doubles.  Replaces numpy.polynomial.legendre.leggauss as called by
+ * GaussianGrid.create (reference grid.py:112). */
+int arwen_sht_gauss_nodes(int nlat, double* sin_lat, double* weights);
+
+/* Resident device bytes of the plan's Legendre tables.  Replaces
+ * legendre_table_nbytes (reference transform.py:935-951). */
+size_t arwen_sht_plan_table_bytes(const arwen_sht_plan* plan);
+
+/* FP64 Legendre table (kind 0 = P, 1 = dP/dphi, 2 = P*m/cos) for the folded
+ * north rows, written to HOST memory `out` as out[m][n][i] (zero for n < m),
+ * (T+1) x (T+1) x ceil(nlat/2) doubles.  Exposes the device recurrence that
+ * replaces legendre_bands / legendre_derivative_block (reference
+ * legendre.py:76-146) so it can be checked against them. */
+int arwen_sht_plan_table_f64(const arwen_sht_plan* plan, int kind, double* out);
+
+/* Single operations (one grouped-GEMM launch each; workspaces cached per
+ * shape and pointers inside the plan). */
+/* forward (reference transform.py:1486-1490) */
+int arwen_sht_analysis(arwen_sht_plan* plan, const float* grid, void* coeff, int count,
+                       void* stream);
+/* inverse (transform.py:1742-1746) */
+int arwen_sht_synthesis(arwen_sht_plan* plan, const void* coeff, float* grid, int count,
+                        void* stream);
+/* inverse_zonal_derivative (transform.py:1752-1760) */
+int arwen_sht_zonal_derivative(arwen_sht_plan* plan, const void* coeff, float* grid, int count,
+                               void* stream);
+/* inverse_meridional_derivative (transform.py:1748-1750) */
+int arwen_sht_meridional_derivative(arwen_sht_plan* plan, const void* coeff, float* grid,
+                                    int count, void* stream);
+/* gradient (transform.py:1762-1768): east = d/dlambda / (a cos), north = d/dphi / a */
+int arwen_sht_gradient(arwen_sht_plan* plan, const void* coeff, float* east, float* north,
+                       int count, void* stream);
+/* VorticityDivergenceOperator.wind_from_vordiv (reference vector.py:36-52) */
+int arwen_sht_uv_from_vordiv(arwen_sht_plan* plan, const void* vort, const void* div, float* u,
+                             float* v, int count, void* stream);
+/* VorticityDivergenceOperator.vordiv_from_wind (reference vector.py:54-73, 113-134) */
+int arwen_sht_vordiv_from_uv(arwen_sht_plan* plan, const float* u, const float* v, void* vort,
+                             void* div, int count, void* stream);
+
+/* Elementwise spectral operators on (count, T+1, T+1) complex64:
+ * op 0 = project (transform.py:1124-1144), 1 = laplacian (:1798-1802),
+ * 2 = inverse_laplacian (:1804-1811).  `in` may equal `out`. */
+int arwen_sht_spectral_op(arwen_sht_plan* plan, int op, const void* in, void* out, int count,
+                          void* stream);
+
+/* A batch: an ordered list of jobs executed as
+ *   pack -> ONE grouped synthesis GEMM -> inverse FFTs        (synthesis jobs)
+ *   FFTs -> ONE grouped analysis GEMM -> unpack               (analysis jobs)
+ * so every field, level, ensemble member and truncation of the batch rides one
+ * persistent launch per direction.  Analysis jobs may read the grid outputs
+ * of synthesis jobs of the same batch (a round trip).  Jobs may mix plans of
+ * different truncations; all plans must share one precision.  No reference
+ * counterpart: the reference steps fields in chunks and members serially
+ * (reference da/ensemble.py:437-455). */
+typedef struct arwen_sht_job {
+  int op;                 /* ARWEN_SHT_OP_* */
+  arwen_sht_plan* plan;
+  int count;              /* fields (scalar ops) or levels (vector ops) */
+  const void* in0;
+  const void* in1;
+  void* out0;
+  void* out1;
+} arwen_sht_job;
+
+int arwen_sht_batch_create(const arwen_sht_job* jobs, int njobs, arwen_sht_batch** out);
+int arwen_sht_batch_destroy(arwen_sht_batch* batch);
+/* Enqueue the batch's kernels on `stream`. */
+int arwen_sht_batch_run(arwen_sht_batch* batch, void* stream);
+/* Capture the batch into a hipGraph (once); later runs launch the graph. */
+int arwen_sht_batch_capture(arwen_sht_batch* batch, void* stream);
+/* Launch the captured graph (captures first if needed). */
+int arwen_sht_batch_launch(arwen_sht_batch* batch, void* stream);
+/* GEMM schedule knobs: tiles per launch grid (0 = auto).  Used by the
+ * autotuner; takes effect at the next run (graphs are re-captured). */
+int arwen_sht_batch_set_grid(arwen_sht_batch* batch, int gemm_blocks);
+/* Algorithmic work of the batch: Legendre FLOPs (parity folded, triangular)
+ * and FFT bytes (grid + waist once each) per run. */
+int arwen_sht_batch_work(const arwen_sht_batch* batch, double* legendre_flops, double* fft_bytes);
+
+const char* arwen_sht_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARWEN_SHT_H */

@@ -1,0 +1,183 @@
+"""HIP engine vs the FP64 oracle, through the C ABI (needs an MI355X).
+
+Inputs are seeded; the oracle runs the reference's numpy algorithm in
+float64.  The error measure is max|engine - oracle| / max|oracle| over the
+compared array.  Tolerances per precision mode:
+  float32 2e-6, bf16x6 2e-6, bf16x3 6e-5
+(float32 rounding of the data and tables alone gives a few 1e-7 at T63).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle.sht import OracleTransform, red_spectrum
+
+from _helpers import hip_transform, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"float32": 2e-6, "bf16x6": 2e-6, "bf16x3": 6e-5}
+PRECISIONS = ["float32", "bf16x6", "bf16x3"]
+RECORD = {}
+
+
+def _record(name, value):
+    RECORD[name] = value
+    out = Path(os.environ.get("ARWEN_SHT_RECORD", "gpurun_out/parity_errors.json"))
+    if out.parent.exists():
+        out.write_text(json.dumps(RECORD, indent=1, sort_keys=True))
+
+
+@pytest.fixture(scope="module")
+def oracles():
+    return {
+        21: OracleTransform.create(21),                       # 33 x 66: odd nlat, radix 11
+        63: OracleTransform.create(63, dealias_factor=1.0),   # 64 x 128 linear grid
+    }
+
+
+def test_device_gauss_nodes_match_leggauss():
+    from arwen_spectral.grid import gauss_nodes
+
+    for nlat in (1, 2, 3, 33, 64, 96, 384, 768, 1536):
+        mu, w = gauss_nodes(nlat)
+        ref_mu, ref_w = np.polynomial.legendre.leggauss(nlat)
+        assert np.all(np.diff(mu) > 0) or nlat == 1
+        assert np.array_equal(mu, -mu[::-1])
+        dmu = float(np.max(np.abs(mu - ref_mu)))
+        dw = float(np.max(np.abs(w - ref_w) / ref_w))
+        _record(f"nodes{nlat}", [dmu, dw])
+        assert dmu < 1e-15, (nlat, dmu)
+        assert dw < 1e-13, (nlat, dw)
+        assert abs(w.sum() - 2.0) < 1e-13
+
+
+@pytest.mark.parametrize("T", [21, 63])
+def test_device_fp64_tables_match_the_oracle_recurrence(oracles, T):
+    import ctypes
+
+    from arwen_spectral import _native
+    from oracle.sht import derivative_column, legendre_columns
+
+    o = oracles[T]
+    tr = hip_transform(o, "float32")
+    jh = (o.grid.nlat + 1) // 2
+    north = o.grid.sin_lat[o.grid.nlat - 1 - np.arange(jh)]
+    lib = _native.load()
+    for kind in (0, 1, 2):
+        out = np.zeros((T + 1, T + 1, jh))
+        _native.check(lib.arwen_sht_plan_table_f64(tr.plan_handle, kind, _native.dptr(out)))
+        want = np.zeros_like(out)
+        for m, col in legendre_columns(T, north):
+            if kind == 0:
+                want[m, m:] = col
+            elif kind == 1:
+                want[m, m:] = derivative_column(m, north, col)
+            else:
+                want[m, m:] = col * m / np.sqrt(1.0 - north * north)[None, :]
+        exact = bool(np.array_equal(out, want)) if kind < 2 else None
+        err = rel_err(out, want)
+        _record(f"table{T}_kind{kind}", [err, exact])
+        assert err < 1e-14, (kind, err)
+        if kind < 2:
+            assert exact, f"kind {kind}: device FP64 recurrence differs from the oracle's bits"
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("T", [21, 63])
+def test_scalar_operations_match_oracle(oracles, T, precision):
+    o = oracles[T]
+    tr = hip_transform(o, precision)
+    tol = TOL[precision]
+    lead = (3,)
+    coeff = o.project(red_spectrum(T, lead, seed=T, scale=1.0, mean=0.5))
+    rng = np.random.default_rng(T)
+    field = rng.standard_normal((*lead, *o.grid.shape))
+
+    err = rel_err(tr.inverse(coeff), o.inverse(coeff))
+    _record(f"inverse{T}_{precision}", err)
+    assert err < tol
+    err = rel_err(tr.forward(field), o.forward(field))
+    _record(f"forward{T}_{precision}", err)
+    assert err < tol
+    err = rel_err(tr.inverse_zonal_derivative(coeff), o.inverse_zonal_derivative(coeff))
+    _record(f"zonal{T}_{precision}", err)
+    assert err < tol
+    err = rel_err(tr.inverse_meridional_derivative(coeff), o.inverse_meridional_derivative(coeff))
+    _record(f"meridional{T}_{precision}", err)
+    assert err < tol
+    east, north = tr.gradient(coeff)
+    oe, on = o.gradient(coeff)
+    err = max(rel_err(east, oe), rel_err(north, on))
+    _record(f"gradient{T}_{precision}", err)
+    assert err < tol
+    # single plane and a 2-D lead
+    err = rel_err(tr.inverse(coeff[0]), o.inverse(coeff[0]))
+    assert err < tol
+    f2 = field.reshape(1, 3, *o.grid.shape)
+    err = rel_err(tr.forward(f2), o.forward(f2))
+    assert err < tol
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("T", [21, 63])
+def test_vector_operations_match_oracle(oracles, T, precision):
+    from arwen_spectral import VorticityDivergenceOperator
+
+    o = oracles[T]
+    tr = hip_transform(o, precision)
+    op = VorticityDivergenceOperator(tr)
+    tol = TOL[precision]
+    zeta = o.project(red_spectrum(T, (2,), seed=10 + T, scale=1e-5))
+    div = o.project(red_spectrum(T, (2,), seed=20 + T, scale=1e-5))
+    u, v = op.wind_from_vordiv(zeta, div)
+    ou, ov = o.wind_from_vordiv(zeta, div)
+    err = max(rel_err(u, ou), rel_err(v, ov))
+    _record(f"wind{T}_{precision}", err)
+    assert err < tol
+    rng = np.random.default_rng(T)
+    gu = rng.standard_normal((2, *o.grid.shape)) * 10
+    gv = rng.standard_normal((2, *o.grid.shape)) * 10
+    z, d = op.vordiv_from_wind(gu, gv)
+    oz, od = o.vordiv_from_wind(gu, gv)
+    err = max(rel_err(z, oz), rel_err(d, od))
+    _record(f"vordiv{T}_{precision}", err)
+    assert err < tol
+    zz = z.cpu().numpy()
+    assert zz[0, 0, 0] == 0.0 and d.cpu().numpy()[0, 0, 0] == 0.0
+
+
+@pytest.mark.parametrize("T", [21, 63])
+def test_spectral_elementwise_operations(oracles, T):
+    o = oracles[T]
+    tr = hip_transform(o, "float32")
+    rng = np.random.default_rng(3)
+    c = (rng.standard_normal((2, T + 1, T + 1)) + 1j * rng.standard_normal((2, T + 1, T + 1)))
+    assert rel_err(tr.project(c), o.project(c)) < 1e-7
+    assert rel_err(tr.laplacian(o.project(c)), o.laplacian(o.project(c))) < 1e-6
+    assert rel_err(tr.inverse_laplacian(c), o.inverse_laplacian(c)) < 1e-6
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_t383_roundtrip_and_parity(precision):
+    o = OracleTransform.create(383, dealias_factor=1.0)
+    tr = hip_transform(o, precision)
+    tol = TOL[precision]
+    coeff = red_spectrum(383, (2,), seed=5, scale=1.0)
+    grid = tr.inverse(coeff)
+    err = rel_err(grid, o.inverse(coeff))
+    _record(f"inverse383_{precision}", err)
+    assert err < tol
+    back = tr.forward(grid)
+    err = rel_err(back, coeff)
+    _record(f"roundtrip383_{precision}", err)
+    assert err < 2 * tol
+    check = tr.transform_check(seed=1)
+    _record(f"check383_{precision}", check)
+    assert check["roundtrip_relative_linf"] < (1e-6 if precision != "bf16x3" else 1e-4)
