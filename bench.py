@@ -1,0 +1,313 @@
+#!/usr/bin/env python3
+"""Spherical-harmonic round trips per second on one MI355X.
+
+Default workload (the headline metric): T383 on the 768 x 384 linear Gaussian
+grid, 121 fields -- 40 levels of vorticity and divergence, 40 of temperature
+and ln ps.  One step is one round trip:
+
+    synthesis  vorticity/divergence -> u, v ; temperature, ln ps -> grid
+    analysis   u, v -> vorticity/divergence ; temperature, ln ps -> spectral
+
+run as one TransformBatch (one grouped Legendre GEMM launch per direction)
+replayed from a hipGraph, inputs resident in HBM.  ``value`` is the split-BF16
+(bf16x6) round trips/s; the same workload in FP32 MFMA is timed beside it.
+
+    python bench.py [--steps K] [--warmup W] [--workload t383|t383x32|mixed]
+                    [--precision bf16x6|bf16x3|float32] [--no-cpu-baseline]
+
+Prints one JSON line.  Only ``cpu_baseline()`` touches the FP64 oracle: it
+times the oracle (the reference's numpy algorithm) on a bounded sample of the
+same fields on this host's cores and returns its outputs for the error check.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "gpuwm-global_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "T383 121-field SH round trips/s: split-BF16 MFMA vs FP32, rel err ≤1e-6"
+MFMA_PEAK = {"bf16x6": 2516.6, "bf16x3": 2516.6, "float32": 157.3}   # dense TFLOP/s
+PRODUCTS = {"bf16x6": 6, "bf16x3": 3, "float32": 1}
+HBM_PEAK_GBS = 8000.0
+LEVELS = 40
+CLASS_SCALE = {"vorticity": 1e-5, "divergence": 2e-6, "temperature": 10.0, "lnps": 1e-2}
+
+
+def red_spectrum(truncation, lead, seed, scale):
+    """Seeded red-spectrum coefficients: scale * n^-1.5 * (N + iN)/sqrt(2),
+    1 <= n <= T, m <= n, m = 0 real, zero above the triangle and at n = 0
+    (temperature and ln ps are anomalies about fixed reference values)."""
+    t1 = truncation + 1
+    rng = np.random.Generator(np.random.PCG64(seed))
+    re = rng.standard_normal((*lead, t1, t1), dtype=np.float64)
+    im = rng.standard_normal((*lead, t1, t1), dtype=np.float64)
+    n = np.arange(t1, dtype=np.float64)
+    amp = np.zeros(t1)
+    amp[1:] = n[1:] ** -1.5
+    c = (re + 1j * im) * (scale * amp / math.sqrt(2.0))[:, None]
+    c *= np.tri(t1, dtype=bool)
+    c[..., :, 0] = c[..., :, 0].real
+    return c
+
+
+def member_fields(truncation, members, seed0=0):
+    """(vorticity, divergence, scalars) per member; member 0 is the control,
+    member k adds a 1% seeded red-spectrum perturbation."""
+    out = {}
+    seeds = {"vorticity": 11, "divergence": 12, "temperature": 13, "lnps": 14}
+    for name, s in seeds.items():
+        lev = 1 if name == "lnps" else LEVELS
+        ctrl = red_spectrum(truncation, (lev,), seed0 + s, CLASS_SCALE[name])
+        mem = [ctrl]
+        for k in range(1, members):
+            mem.append(ctrl + 0.01 * red_spectrum(truncation, (lev,), seed0 + 1000 * k + s,
+                                                  CLASS_SCALE[name]))
+        out[name] = np.stack(mem)  # (members, lev, T+1, T+1)
+    return out
+
+
+class RoundTrip:
+    """The device buffers and the batch of one round-trip workload."""
+
+    def __init__(self, specs, precision, torch):
+        from arwen_spectral import SphericalHarmonicTransform, TransformBatch
+
+        self.parts = []
+        jobs = []
+        dev = torch.device("cuda", torch.cuda.current_device())
+        for spec in specs:
+            tr = SphericalHarmonicTransform.create(spec["T"], dealias_factor=1.0, precision=precision)
+            f = spec["fields"]
+            c64 = lambda a: torch.from_numpy(a.astype(np.complex64)).to(dev)
+            zeta = c64(f["vorticity"]).contiguous()
+            div = c64(f["divergence"]).contiguous()
+            scal = c64(np.concatenate([f["temperature"], f["lnps"]], axis=1)).contiguous()
+            M = zeta.shape[0]
+            g = lambda n: torch.empty((M, n, *tr.grid.shape), dtype=torch.float32, device=dev)
+            s = lambda n: torch.empty((M, n, *tr.spectral_shape), dtype=torch.complex64, device=dev)
+            u, v, grid = g(LEVELS), g(LEVELS), g(LEVELS + 1)
+            z2, d2, s2 = s(LEVELS), s(LEVELS), s(LEVELS + 1)
+            jobs += [("wind", tr, (zeta, div), (u, v)), ("inverse", tr, scal, grid),
+                     ("vordiv", tr, (u, v), (z2, d2)), ("forward", tr, grid, s2)]
+            self.parts.append(dict(tr=tr, zeta=zeta, div=div, scal=scal, u=u, v=v, grid=grid,
+                                   z2=z2, d2=d2, s2=s2, spec=spec))
+        self.batch = TransformBatch(jobs)
+        self.flops, self.fft_bytes = self.batch.work()
+
+
+def time_steps(rt, steps, warmup, torch):
+    rt.batch.capture()
+    for _ in range(warmup):
+        rt.batch.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rt.batch.launch()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def class_errors(rt):
+    """Round-trip error per field class, max|c' - c| / max|c|, over every field
+    of the first and last member of each truncation."""
+    out = {}
+    for part in rt.parts:
+        M = part["zeta"].shape[0]
+        sel = sorted({0, M - 1})
+        pairs = {"vorticity": (part["z2"], part["zeta"]), "divergence": (part["d2"], part["div"]),
+                 "temperature": (part["s2"][:, :LEVELS], part["scal"][:, :LEVELS]),
+                 "lnps": (part["s2"][:, LEVELS:], part["scal"][:, LEVELS:])}
+        for name, (have, want) in pairs.items():
+            h = have[sel].cpu().numpy().astype(np.complex128)
+            w = part["spec"]["fields"][name][sel]
+            e = float(np.max(np.abs(h - w)) / np.max(np.abs(w)))
+            key = f"T{part['spec']['T']}_{name}"
+            out[key] = max(out.get(key, 0.0), e)
+    return out
+
+
+def cpu_baseline(part, sample_levels=4):
+    """FP64 oracle (reference numpy algorithm) on a bounded sample of the same
+    workload: the first `sample_levels` levels of each class plus ln ps, one
+    member.  Returns the timing entry and the oracle's outputs."""
+    sys.path.insert(0, str(ROOT))
+    from oracle.sht import OracleTransform
+
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:  # pragma: no cover
+        threadpool_limits = None
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    T = part["spec"]["T"]
+    f = part["spec"]["fields"]
+    L = sample_levels
+    zeta, div = f["vorticity"][0, :L], f["divergence"][0, :L]
+    scal = np.concatenate([f["temperature"][0, :L], f["lnps"][0]], axis=0)
+    ctx = threadpool_limits(cores) if threadpool_limits else None
+    try:
+        o = OracleTransform.create(T, dealias_factor=1.0)
+        times = []
+        for rep in range(3):  # first call, then two steady calls
+            t0 = time.perf_counter()
+            u, v = o.wind_from_vordiv(zeta, div)
+            g = o.inverse(scal)
+            z2, d2 = o.vordiv_from_wind(u, v)
+            s2 = o.forward(g)
+            times.append(time.perf_counter() - t0)
+    finally:
+        if ctx is not None:
+            ctx.__exit__(None, None, None)
+    fields_sample = 3 * L + 1
+    steady = float(np.mean(times[1:]))
+    fields_total = 3 * LEVELS + 1
+    value = (fields_sample / fields_total) / steady  # 121-field round trips/s
+    entry = {"value": value, "unit": "round trips/s", "cores": cores, "kind": "port",
+             "sample": f"FP64 numpy oracle (reference per-order BLAS algorithm), T{T}, "
+                       f"{fields_sample} of {fields_total} fields ({L} levels of vorticity, "
+                       f"divergence, temperature + ln ps), steady mean of 2 after a first "
+                       f"call ({times[0]:.2f} s), scaled linearly to {fields_total} fields; "
+                       f"sample round trip {steady:.2f} s"}
+    return entry, dict(u=u, v=v, grid=g, zeta=z2, div=d2, scal=s2, L=L)
+
+
+def oracle_errors(part, ref):
+    L = ref["L"]
+    def e(have, want):
+        h = have.cpu().numpy().astype(np.float64 if np.isrealobj(want) else np.complex128)
+        return float(np.max(np.abs(h - want)) / np.max(np.abs(want)))
+    return {
+        "u": e(part["u"][0, :L], ref["u"]), "v": e(part["v"][0, :L], ref["v"]),
+        "temperature_grid": e(part["grid"][0, :L], ref["grid"][:L]),
+        "lnps_grid": e(part["grid"][0, LEVELS:], ref["grid"][L:]),
+        "vorticity": e(part["z2"][0, :L], ref["zeta"]), "divergence": e(part["d2"][0, :L], ref["div"]),
+        "temperature": e(part["s2"][0, :L], ref["scal"][:L]),
+        "lnps": e(part["s2"][0, LEVELS:], ref["scal"][L:]),
+    }
+
+
+def roofline(rt, phases, precision):
+    """Dominant kernel of the round trip: Legendre GEMM (MFMA-bound) or FFT (HBM-bound)."""
+    gemm_ms = phases["synthesis_gemm"] + phases["analysis_gemm"]
+    fft_ms = phases["inverse_fft"] + phases["forward_fft"]
+    s = PRODUCTS[precision]
+    if gemm_ms >= fft_ms:
+        achieved = s * rt.flops / (gemm_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "kernel": "legendre_gemm (synthesis + analysis launches)",
+                "achieved": achieved, "peak": MFMA_PEAK[precision], "unit": "TFLOP/s",
+                "frac": achieved / MFMA_PEAK[precision], "traffic": None,
+                "fp32_equivalent_tflops": rt.flops / (gemm_ms * 1e-3) / 1e12,
+                "mfma_products_per_fma": s}
+    achieved = rt.fft_bytes / (fft_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "rfft_fold + irfft_unfold", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+
+
+def workload_specs(name):
+    if name == "t383":
+        return [dict(T=383, members=1)]
+    if name == "t383x32":
+        return [dict(T=383, members=32)]
+    if name == "mixed":
+        return [dict(T=767, members=4), dict(T=383, members=8), dict(T=127, members=16)]
+    raise SystemExit(f"unknown workload {name}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="t383")
+    ap.add_argument("--precision", default="bf16x6")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the transform "
+              "shards by members or fields with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    torch.cuda.set_device(0)
+    specs = workload_specs(args.workload)
+    for sp in specs:
+        sp["fields"] = member_fields(sp["T"], sp["members"])
+    nfields = sum(sp["members"] * (3 * LEVELS + 1) for sp in specs)
+    print(f"[bench] workload {args.workload}: {nfields} fields, precision {args.precision}",
+          file=sys.stderr, flush=True)
+
+    rt = RoundTrip(specs, args.precision, torch)
+    sec = time_steps(rt, args.steps, args.warmup, torch)
+    phases = rt.batch.profile(reps=3)
+    errors = class_errors(rt)
+    print(f"[bench] {args.precision}: {sec * 1e3:.3f} ms/step, phases {phases}", file=sys.stderr,
+          flush=True)
+
+    fp32 = None
+    if not args.no_fp32 and args.precision != "float32":
+        rt32 = RoundTrip(specs, "float32", torch)
+        sec32 = time_steps(rt32, args.steps, args.warmup, torch)
+        ph32 = rt32.batch.profile(reps=3)
+        fp32 = {"value": 1.0 / sec32, "ms_per_step": sec32 * 1e3, "phases_ms": ph32,
+                "roundtrip_error": class_errors(rt32), "roofline": roofline(rt32, ph32, "float32")}
+        print(f"[bench] float32: {sec32 * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+        del rt32
+
+    cpu = None
+    oracle_err = None
+    if not args.no_cpu_baseline:
+        part = next(p for p in rt.parts if p["spec"]["T"] == 383) if any(
+            p["spec"]["T"] == 383 for p in rt.parts) else rt.parts[0]
+        cpu, ref = cpu_baseline(part)
+        oracle_err = oracle_errors(part, ref)
+        print(f"[bench] cpu baseline {cpu['value']:.4f} round trips/s on {cpu['cores']} cores",
+              file=sys.stderr, flush=True)
+
+    members = sum(sp["members"] for sp in specs)
+    result = {
+        "metric": METRIC,
+        "value": 1.0 / sec,
+        "unit": "round trips/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16x6": "bf16x6 (three bf16 pieces, fp32 accumulate)",
+                  "bf16x3": "bf16x3 (bf16 hi+lo, fp32 accumulate)",
+                  "float32": "fp32"}[args.precision],
+        "data": "synthetic: seeded red-spectrum coefficients (n^-1.5), temperature and ln ps as "
+                "anomalies about fixed references",
+        "config": {"workload": "T383 768x384 linear Gaussian grid, 121 fields per member (40 "
+                               "vorticity, 40 divergence, 40 temperature, ln ps): synthesis "
+                               "incl. u,v from vorticity/divergence, then analysis incl. "
+                               "vorticity/divergence from u,v" if args.workload == "t383"
+                   else args.workload,
+                   "members": members, "fields": nfields, "precision": args.precision,
+                   "specs": [{k: v for k, v in sp.items() if k != "fields"} for sp in specs]},
+        "phases_ms": phases,
+        "roundtrip_error": errors,
+        "error_vs_fp64_oracle": oracle_err,
+        "fp32": fp32,
+        "speedup_vs_fp32": (fp32["ms_per_step"] / (sec * 1e3)) if fp32 else None,
+        "legendre_gflop_per_step": rt.flops / 1e9,
+        "fft_mb_per_step": rt.fft_bytes / 1e6,
+        "roofline": roofline(rt, phases, args.precision),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

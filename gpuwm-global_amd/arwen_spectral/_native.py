@@ -33,6 +33,7 @@ EXPORTS = (
     "arwen_sht_vordiv_from_uv", "arwen_sht_spectral_op", "arwen_sht_batch_create",
     "arwen_sht_batch_destroy", "arwen_sht_batch_run", "arwen_sht_batch_capture",
     "arwen_sht_batch_launch", "arwen_sht_batch_set_grid", "arwen_sht_batch_work",
+    "arwen_sht_batch_profile",
     "arwen_sht_last_error",
 )
 
@@ -71,6 +72,12 @@ def load():
     with _lock:
         if _lib is not None:
             return _lib
+        # torch's bundled HIP runtime carries the same SONAME as the one the
+        # library links (libamdhip64.so.7); loading torch first makes the
+        # dynamic linker bind the library to that one runtime instead of
+        # mapping a second HIP runtime into the process.
+        import torch  # noqa: F401
+
         if not LIB_PATH.exists():
             raise RuntimeError(
                 f"the HIP engine library {LIB_PATH} is not built; run "
@@ -102,6 +109,7 @@ def load():
             "arwen_sht_batch_launch": (I, [P, P]),
             "arwen_sht_batch_set_grid": (I, [P, I]),
             "arwen_sht_batch_work": (I, [P, D, D]),
+            "arwen_sht_batch_profile": (I, [P, P, I, D]),
             "arwen_sht_last_error": (ctypes.c_char_p, []),
         }
         for name, (res, args) in sigs.items():

@@ -105,6 +105,18 @@ class TransformBatch:
     def set_grid(self, gemm_blocks: int):
         _native.check(self._lib.arwen_sht_batch_set_grid(self._handle, int(gemm_blocks)))
 
+    PHASES = ("pack", "synthesis_gemm", "inverse_fft", "forward_fft", "analysis_gemm", "unpack")
+
+    def profile(self, reps: int = 5) -> dict:
+        """Mean milliseconds per phase over ``reps`` runs, HIP events on the
+        current stream (synchronizes)."""
+        import numpy as np
+
+        ms = np.zeros(len(self.PHASES), dtype=np.float64)
+        _native.check(self._lib.arwen_sht_batch_profile(self._handle, self._backend.stream(),
+                                                        int(reps), _native.dptr(ms)))
+        return dict(zip(self.PHASES, ms.tolist()))
+
     def work(self) -> tuple[float, float]:
         """(Legendre FLOPs, FFT bytes) per run, algorithmic."""
         f = ctypes.c_double()

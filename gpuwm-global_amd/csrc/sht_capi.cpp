@@ -556,22 +556,36 @@ int gemm_grid(const arwen_sht_batch* B, int ntiles) {
   return std::min(ntiles, 256 * 8);
 }
 
-void run_batch(arwen_sht_batch* B, hipStream_t s) {
+// Phases of a batch run, in order; `ev` (optional) gets one event before
+// each phase and one after the last.
+constexpr int kPhases = 6;  // pack, synthesis GEMM, inverse FFT, FFT, analysis GEMM, unpack
+
+void run_batch(arwen_sht_batch* B, hipStream_t s, hipEvent_t* ev = nullptr) {
   int dev;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   if (dev != B->device) fail(ARWEN_SHT_EINVAL, "batch used on a device other than its own");
+  auto mark = [&](int k) {
+    if (ev) hip_check(hipEventRecord(ev[k], s), "hipEventRecord");
+  };
+  mark(0);
   for (auto& a : B->packs) hip_check(sht_launch_pack(&a, s), "pack launch");
+  mark(1);
   hip_check(sht_launch_legendre_gemm(B->precision, B->syn_probs.as<sht::GemmProb>(),
                                      B->syn_tiles.as<int4>(), B->syn_ntiles,
                                      gemm_grid(B, B->syn_ntiles), s),
             "synthesis GEMM launch");
+  mark(2);
   for (auto& a : B->unfolds) hip_check(sht_launch_irfft_unfold(&a, s), "irfft launch");
+  mark(3);
   for (auto& a : B->folds) hip_check(sht_launch_rfft_fold(&a, s), "rfft launch");
+  mark(4);
   hip_check(sht_launch_legendre_gemm(B->precision, B->ana_probs.as<sht::GemmProb>(),
                                      B->ana_tiles.as<int4>(), B->ana_ntiles,
                                      gemm_grid(B, B->ana_ntiles), s),
             "analysis GEMM launch");
+  mark(5);
   for (auto& a : B->unpacks) hip_check(sht_launch_unpack(&a, s), "unpack launch");
+  mark(6);
 }
 
 int single_op(arwen_sht_plan* P, int op, const void* in0, const void* in1, void* out0,
@@ -779,6 +793,33 @@ int arwen_sht_batch_launch(arwen_sht_batch* B, void* stream) {
   if (rc != ARWEN_SHT_OK) return rc;
   return guarded([&] {
     hip_check(hipGraphLaunch(B->exec, static_cast<hipStream_t>(stream)), "graph launch");
+  });
+}
+
+int arwen_sht_batch_profile(arwen_sht_batch* B, void* stream, int reps, double* ms) {
+  return guarded([&] {
+    if (!B || !ms) fail(ARWEN_SHT_EINVAL, "batch and ms are required");
+    if (reps < 1) fail(ARWEN_SHT_EINVAL, "reps must be >= 1");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::vector<hipEvent_t> ev((size_t)(kPhases + 1) * reps, nullptr);
+    for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    try {
+      for (int r = 0; r < reps; ++r) run_batch(B, s, &ev[(size_t)r * (kPhases + 1)]);
+      hip_check(hipEventSynchronize(ev.back()), "hipEventSynchronize");
+      for (int k = 0; k < kPhases; ++k) ms[k] = 0.0;
+      for (int r = 0; r < reps; ++r)
+        for (int k = 0; k < kPhases; ++k) {
+          float t = 0.f;
+          hip_check(hipEventElapsedTime(&t, ev[(size_t)r * (kPhases + 1) + k],
+                                        ev[(size_t)r * (kPhases + 1) + k + 1]),
+                    "hipEventElapsedTime");
+          ms[k] += t / reps;
+        }
+    } catch (...) {
+      for (auto& e : ev) (void)hipEventDestroy(e);
+      throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
   });
 }
 

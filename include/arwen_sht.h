@@ -150,6 +150,12 @@ int arwen_sht_batch_run(arwen_sht_batch* batch, void* stream);
 int arwen_sht_batch_capture(arwen_sht_batch* batch, void* stream);
 /* Launch the captured graph (captures first if needed). */
 int arwen_sht_batch_launch(arwen_sht_batch* batch, void* stream);
+/* Run the batch `reps` times with HIP events recorded on `stream` around each
+ * phase and return the mean milliseconds per phase in ms[6]: pack, synthesis
+ * GEMM, inverse FFTs, forward FFTs, analysis GEMM, unpack.  Synchronizes on
+ * the last event.  Measurement hook; no reference counterpart (the
+ * reference's StepProfiler times whole sections, profile.py:87-330). */
+int arwen_sht_batch_profile(arwen_sht_batch* batch, void* stream, int reps, double* ms);
 /* GEMM schedule knobs: tiles per launch grid (0 = auto).  Used by the
  * autotuner; takes effect at the next run (graphs are re-captured). */
 int arwen_sht_batch_set_grid(arwen_sht_batch* batch, int gemm_blocks);
