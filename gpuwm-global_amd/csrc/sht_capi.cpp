@@ -122,7 +122,7 @@ struct arwen_sht_plan {
   DevBuf tab_syn[3], tab_ana[3];
   DevBuf scale_fold, scale_fold_vec, scale_one, scale_inv_a;
   sht::FftPlan fft{};
-  int fft_rp = 4;
+  int rp_fold = 4, rp_fold_vec = 4, rp_unfold = 4;  // folded rows (wavefronts) per FFT block
   // single-operation batch cache (most recent first)
   std::list<std::pair<std::tuple<int, int, const void*, const void*, void*, void*>,
                       arwen_sht_batch*>>
@@ -231,9 +231,24 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   }
   upload(P->d_tw, tw);
   P->fft.tw = P->d_tw.as<float2>();
-  // Rows per FFT block: as many as fit 64 KiB of LDS (vector fold is the largest).
-  P->fft_rp = 4;
-  while (P->fft_rp > 1 && sht_fold_lds_bytes(P->nlon, T, 1, P->fft_rp) > 65536) P->fft_rp /= 2;
+  // Wavefronts (folded rows) per FFT block: as many as fit the LDS budget,
+  // at most 80 KiB so two blocks share a CU.
+  {
+    int lds_max = 65536;
+    (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, P->device);
+    const size_t budget = std::min<size_t>((size_t)lds_max, 81920);
+    auto pick = [&](auto bytes) {
+      int rp = 4;
+      while (rp > 1 && bytes(rp) > budget) rp /= 2;
+      if (bytes(rp) > budget)
+        fail(ARWEN_SHT_EINVAL, "nlon=" + std::to_string(P->nlon) + " needs more LDS than the device has");
+      return rp;
+    };
+    P->rp_fold = pick([&](int rp) { return sht_fold_lds_bytes(P->nlon, T, 0, rp); });
+    P->rp_fold_vec = pick([&](int rp) { return sht_fold_lds_bytes(P->nlon, T, 1, rp); });
+    P->rp_unfold = pick([&](int rp) { return sht_unfold_lds_bytes(P->nlon, T, rp); });
+    hip_check(sht_fft_allow_lds(budget), "FFT LDS attribute");
+  }
 
   // Per folded-row scales.
   std::vector<float> sf(P->JHP, 0.f), sfv(P->JHP, 0.f), one(P->JHP, 1.f), inva(P->JHP, 0.f);
@@ -294,64 +309,124 @@ namespace {
 
 bool is_synthesis(int op) { return op >= ARWEN_SHT_OP_SYNTHESIS && op <= ARWEN_SHT_OP_UV_FROM_VORDIV; }
 
-struct ProbBuilder {
-  std::vector<sht::GemmProb> probs;
+// Collects the grouped-GEMM work list of one direction.
+struct GroupBuilder {
+  std::vector<sht::GemmGroup> groups;
   std::vector<std::pair<long long, int4>> tiles;  // (work, tile)
-  void add(const sht::GemmProb& pr) {
-    int id = (int)probs.size();
-    probs.push_back(pr);
-    long long k = 0;
-    for (int s = 0; s < pr.nseg; ++s) k += pr.seg[s].K;
-    for (int r0 = 0; r0 < pr.R; r0 += 64)
-      for (int c0 = 0; c0 < pr.N; c0 += 64) {
-        long long rows = std::min(64, pr.R - r0), cols = std::min(64, pr.N - c0);
-        tiles.push_back({k * rows * cols, int4{id, r0, c0, 0}});
+  void add(sht::GemmGroup g) {
+    g.Rmax = 0;
+    long long kwork = 0;
+    for (int sg = 0; sg < 2; ++sg) g.Kmax[sg] = 0;
+    for (int mm = 0; mm < g.mcount; ++mm) {
+      g.Rmax = std::max(g.Rmax, g.R[mm]);
+      for (int sg = 0; sg < g.nseg; ++sg) {
+        g.Kmax[sg] = std::max(g.Kmax[sg], g.K[sg][mm]);
+        kwork += (long long)g.K[sg][mm] * g.R[mm];
       }
+    }
+    const int id = (int)groups.size();
+    groups.push_back(g);
+    const int TR = sht_gemm_tile_rows(), TC = sht_gemm_tile_cols();
+    for (int r0 = 0; r0 < g.Rmax; r0 += TR)
+      for (int c0 = 0; c0 < g.N; c0 += TC)
+        tiles.push_back({kwork * std::min(TC, g.N - c0), int4{id, r0, c0, 0}});
   }
-  void finish(DevBuf& dprobs, DevBuf& dtiles, int* ntiles) {
+  void finish(DevBuf& dgroups, DevBuf& dtiles, int* ntiles) {
+    // longest first: the grid-stride walk then balances the triangular schedule
     std::stable_sort(tiles.begin(), tiles.end(),
                      [](const auto& a, const auto& b) { return a.first > b.first; });
     std::vector<int4> t;
     t.reserve(tiles.size());
     for (auto& x : tiles) t.push_back(x.second);
-    upload(dprobs, probs);
+    upload(dgroups, groups);
     upload(dtiles, t);
     *ntiles = (int)t.size();
   }
 };
 
-// Data-side segment on the waist layout W[c][m/8][p][i][m%8].
-sht::GemmSeg waist_seg(const arwen_sht_plan* P, const float* W, int m, int p) {
-  sht::GemmSeg s{};
-  s.B = W + ((size_t)((m / sht::kMT) * 2 + p) * P->JHP) * sht::kMT + (m % sht::kMT);
-  s.bks = sht::kMT;
-  s.bcs = P->ldw;
-  s.K = P->JHP;
-  return s;
-}
-// Data-side segment on the packed layout C[c][koff(m,p) + n_idx].
-sht::GemmSeg packed_seg(const arwen_sht_plan* P, const float* C, int m, int p) {
-  sht::GemmSeg s{};
-  s.B = C + P->koff[2 * m + p];
-  s.bks = 1;
-  s.bcs = P->ldp;
-  s.K = P->kp(m, p);
-  return s;
-}
-void set_table(const arwen_sht_plan* P, sht::GemmSeg& s, int kind, bool syn, int m, int p) {
+const void* table_block(const arwen_sht_plan* P, int kind, bool syn, int m, int p, int* lda) {
   long long piece;
   const char* base = static_cast<const char*>(P->table(kind, syn, &piece));
   const size_t esz = P->precision == ARWEN_SHT_FP32 ? 4 : 2;
-  s.A = base + (size_t)P->koff[2 * m + p] * P->JHP * esz;
-  s.piece = piece;
-  s.lda = syn ? P->kp(m, p) : P->JHP;
+  *lda = syn ? P->kp(m, p) : P->JHP;
+  return base + (size_t)P->koff[2 * m + p] * P->JHP * esz;
+}
+
+// Waist block (m-tile of m, parity p) of the layout W[c][m/8][p][i][m%8].
+float* waist_block(const arwen_sht_plan* P, float* W, int m, int p) {
+  return W + ((size_t)((m / sht::kMT) * 2 + p) * P->JHP) * sht::kMT;
+}
+
+// Synthesis group: segments (table kind, parity, packed data) per order.
+struct SynSeg {
+  int kind;
+  int pflip;  // data/table parity = q ^ pflip
+  const float* packed;
+};
+
+void add_syn_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int ncols,
+                   float* waist, const SynSeg* segs, int nseg) {
+  sht::GemmGroup g{};
+  g.nseg = nseg;
+  g.N = ncols;
+  g.mcount = std::min(sht::kMT, P->T + 1 - mb * sht::kMT);
+  g.piece = P->tab_elems;
+  g.ccs = P->ldw;
+  g.C[0] = waist_block(P, waist, mb * sht::kMT, q);
+  for (int sg = 0; sg < nseg; ++sg) {
+    g.bks[sg] = 1;
+    g.bcs[sg] = P->ldp;
+  }
+  for (int mm = 0; mm < g.mcount; ++mm) {
+    const int m = mb * sht::kMT + mm;
+    g.R[mm] = P->JHP;
+    for (int sg = 0; sg < nseg; ++sg) {
+      const int p = q ^ segs[sg].pflip;
+      g.A[sg][mm] = table_block(P, segs[sg].kind, true, m, p, &g.lda[sg][mm]);
+      g.B[sg][mm] = segs[sg].packed + P->koff[2 * m + p];
+      g.K[sg][mm] = P->kp(m, p);
+    }
+  }
+  b.add(g);
+}
+
+// Analysis group: segments (table kind, waist, waist parity flip) per order.
+struct AnaSeg {
+  int kind;
+  int pflip;
+  float* waist;
+};
+
+void add_ana_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int ncols,
+                   float* packed_out, const AnaSeg* segs, int nseg) {
+  sht::GemmGroup g{};
+  g.nseg = nseg;
+  g.N = ncols;
+  g.mcount = std::min(sht::kMT, P->T + 1 - mb * sht::kMT);
+  g.piece = P->tab_elems;
+  g.ccs = P->ldp;
+  for (int sg = 0; sg < nseg; ++sg) {
+    g.bks[sg] = sht::kMT;
+    g.bcs[sg] = P->ldw;
+    g.B[sg][0] = waist_block(P, segs[sg].waist, mb * sht::kMT, q ^ segs[sg].pflip);
+  }
+  for (int mm = 0; mm < g.mcount; ++mm) {
+    const int m = mb * sht::kMT + mm;
+    g.R[mm] = P->kp(m, q);
+    g.C[mm] = packed_out + P->koff[2 * m + q];
+    for (int sg = 0; sg < nseg; ++sg) {
+      g.A[sg][mm] = table_block(P, segs[sg].kind, false, m, q, &g.lda[sg][mm]);
+      g.K[sg][mm] = P->JHP;
+    }
+  }
+  b.add(g);
 }
 
 void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
   if (njobs <= 0 || !jobs) fail(ARWEN_SHT_EINVAL, "a batch needs at least one job");
   hip_check(hipGetDevice(&B->device), "hipGetDevice");
   B->precision = jobs[0].plan ? jobs[0].plan->precision : 0;
-  ProbBuilder syn, ana;
+  GroupBuilder syn, ana;
   for (int j = 0; j < njobs; ++j) {
     const arwen_sht_job& J = jobs[j];
     const arwen_sht_plan* P = J.plan;
@@ -400,48 +475,25 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
                     : vec                                 ? sht::kVector
                                                           : sht::kPlain;
       B->packs.push_back(pa);
-      // ---- GEMM problems: output parity q over all orders ----
-      for (int m = 0; m <= T; ++m)
+      // ---- grouped GEMM: one group per (m-tile, output parity) ----
+      for (int mb = 0; mb < P->MB; ++mb)
         for (int q = 0; q < 2; ++q) {
-          sht::GemmProb pr{};
-          pr.R = P->JHP;
-          pr.N = js->ncols;
-          pr.crs = sht::kMT;
-          pr.ccs = P->ldw;
-          const size_t coff = ((size_t)((m / sht::kMT) * 2 + q) * P->JHP) * sht::kMT + m % sht::kMT;
           if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
-            // U,V = -dP/dphi (psi) + i P m/cos (chi), i P m/cos (psi) + dP/dphi (chi)
-            pr.nseg = 2;
-            pr.seg[0] = packed_seg(P, js->packed_a.as<float>(), m, 1 - q);
-            set_table(P, pr.seg[0], kD, true, m, 1 - q);
-            pr.seg[1] = packed_seg(P, js->packed_b.as<float>(), m, q);
-            set_table(P, pr.seg[1], kPm, true, m, q);
-            pr.C = js->waist_a.as<float>() + coff;
-            syn.add(pr);
+            // U,V = -dP/dphi (psi) + i P m/cos (chi), i P m/cos (psi) + dP/dphi (chi):
+            // dP/dphi flips the parity, so its segment reads the other parity.
+            SynSeg sg[2] = {{kD, 1, js->packed_a.as<float>()}, {kPm, 0, js->packed_b.as<float>()}};
+            add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 2);
           } else if (J.op == ARWEN_SHT_OP_MERIDIONAL_DERIVATIVE) {
-            pr.nseg = 1;
-            pr.seg[0] = packed_seg(P, js->packed_a.as<float>(), m, 1 - q);
-            set_table(P, pr.seg[0], kD, true, m, 1 - q);
-            pr.C = js->waist_a.as<float>() + coff;
-            syn.add(pr);
+            SynSeg sg[1] = {{kD, 1, js->packed_a.as<float>()}};
+            add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
           } else if (grad) {
-            sht::GemmProb pe = pr, pn = pr;
-            pe.nseg = 1;
-            pe.seg[0] = packed_seg(P, js->packed_a.as<float>(), m, q);
-            set_table(P, pe.seg[0], kPm, true, m, q);
-            pe.C = js->waist_a.as<float>() + coff;
-            syn.add(pe);
-            pn.nseg = 1;
-            pn.seg[0] = packed_seg(P, js->packed_a.as<float>(), m, 1 - q);
-            set_table(P, pn.seg[0], kD, true, m, 1 - q);
-            pn.C = js->waist_b.as<float>() + coff;
-            syn.add(pn);
+            SynSeg se[1] = {{kPm, 0, js->packed_a.as<float>()}};
+            add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), se, 1);
+            SynSeg sn[1] = {{kD, 1, js->packed_a.as<float>()}};
+            add_syn_group(syn, P, mb, q, js->ncols, js->waist_b.as<float>(), sn, 1);
           } else {  // SYNTHESIS, ZONAL_DERIVATIVE
-            pr.nseg = 1;
-            pr.seg[0] = packed_seg(P, js->packed_a.as<float>(), m, q);
-            set_table(P, pr.seg[0], kP, true, m, q);
-            pr.C = js->waist_a.as<float>() + coff;
-            syn.add(pr);
+            SynSeg sg[1] = {{kP, 0, js->packed_a.as<float>()}};
+            add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
           }
         }
       // algorithmic FLOPs: one plane contraction per table application
@@ -460,7 +512,7 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
         ua.nlat = P->nlat;
         ua.JH = P->JH;
         ua.JHP = P->JHP;
-        ua.rp = P->fft_rp;
+        ua.rp = P->rp_unfold;
         ua.ngroups = (P->JH + ua.rp - 1) / ua.rp;
         ua.job.waist = waist.as<float>();
         ua.job.grid = static_cast<float*>(grid);
@@ -495,7 +547,7 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       fa.nlat = P->nlat;
       fa.JH = P->JH;
       fa.JHP = P->JHP;
-      fa.rp = P->fft_rp;
+      fa.rp = vec ? P->rp_fold_vec : P->rp_fold;
       fa.ngroups = P->JHP / fa.rp;
       fa.job.grid0 = static_cast<const float*>(J.in0);
       fa.job.grid1 = static_cast<const float*>(J.in1);
@@ -508,27 +560,16 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       fa.job.vector = vec ? 1 : 0;
       B->folds.push_back(fa);
       B->fft_bytes += (vec ? 2.0 : 1.0) * n * fftb;
-      for (int m = 0; m <= T; ++m)
+      for (int mb = 0; mb < P->MB; ++mb)
         for (int q = 0; q < 2; ++q) {
-          sht::GemmProb pr{};
-          pr.R = P->kp(m, q);
-          pr.N = js->ncols;
-          pr.crs = 1;
-          pr.ccs = P->ldp;
-          pr.C = js->packed_out.as<float>() + P->koff[2 * m + q];
           if (vec) {
             // zeta, D = Pm^T [iV, iU]_q + dP^T [U, -V]_{1-q}
-            pr.nseg = 2;
-            pr.seg[0] = waist_seg(P, js->waist_a.as<float>(), m, q);
-            set_table(P, pr.seg[0], kPm, false, m, q);
-            pr.seg[1] = waist_seg(P, js->waist_b.as<float>(), m, 1 - q);
-            set_table(P, pr.seg[1], kD, false, m, q);
+            AnaSeg sg[2] = {{kPm, 0, js->waist_a.as<float>()}, {kD, 1, js->waist_b.as<float>()}};
+            add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 2);
           } else {
-            pr.nseg = 1;
-            pr.seg[0] = waist_seg(P, js->waist_a.as<float>(), m, q);
-            set_table(P, pr.seg[0], kP, false, m, q);
+            AnaSeg sg[1] = {{kP, 0, js->waist_a.as<float>()}};
+            add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 1);
           }
-          ana.add(pr);
         }
       B->legendre_flops += (vec ? 4.0 : 1.0) * n * pcf;
       sht::UnpackArgs ua{};
@@ -570,7 +611,7 @@ void run_batch(arwen_sht_batch* B, hipStream_t s, hipEvent_t* ev = nullptr) {
   mark(0);
   for (auto& a : B->packs) hip_check(sht_launch_pack(&a, s), "pack launch");
   mark(1);
-  hip_check(sht_launch_legendre_gemm(B->precision, B->syn_probs.as<sht::GemmProb>(),
+  hip_check(sht_launch_legendre_gemm(B->precision, 0, B->syn_probs.as<sht::GemmGroup>(),
                                      B->syn_tiles.as<int4>(), B->syn_ntiles,
                                      gemm_grid(B, B->syn_ntiles), s),
             "synthesis GEMM launch");
@@ -579,7 +620,7 @@ void run_batch(arwen_sht_batch* B, hipStream_t s, hipEvent_t* ev = nullptr) {
   mark(3);
   for (auto& a : B->folds) hip_check(sht_launch_rfft_fold(&a, s), "rfft launch");
   mark(4);
-  hip_check(sht_launch_legendre_gemm(B->precision, B->ana_probs.as<sht::GemmProb>(),
+  hip_check(sht_launch_legendre_gemm(B->precision, 1, B->ana_probs.as<sht::GemmGroup>(),
                                      B->ana_tiles.as<int4>(), B->ana_ntiles,
                                      gemm_grid(B, B->ana_ntiles), s),
             "analysis GEMM launch");

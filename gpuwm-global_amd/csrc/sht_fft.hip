@@ -2,10 +2,12 @@
 // (synthesis).
 //
 // The waist is stored m-tiled:  W[c][m/8][p][i][m%8]  (fp32), per data
-// column c (2*field + re/im), so that four consecutive folded rows of one
-// m-tile are one 128-byte run.  A block handles rp (<= 4) consecutive folded
-// rows of one field (or of one u/v pair): it stages their waist tile in LDS
-// and runs one complex FFT per mirror pair.
+// column c (2*field + re/im).  A block holds `rp` wavefronts; wavefront w owns
+// folded row i0 + w (the mirror pair north = nlat-1-i, south = i) and runs
+// its own FFT in its own LDS slice, so rp FFTs advance together between the
+// block barriers of the Stockham passes.  Neighbouring wavefronts write
+// neighbouring 32-byte runs of each m-tile, which meet in L2 as 128-byte
+// lines.
 //
 // Two real rows ride one complex FFT of length N = nlon:
 //   z = north + i*south,  X_N[m] = (Z[m] + conj Z[N-m]) / 2,
@@ -23,6 +25,7 @@
 
 namespace sht {
 
+constexpr int kWave = 64;
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -32,12 +35,11 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
 // Twiddle e^{sign * 2 pi i k / N} from the forward table.
-__device__ __forceinline__ float2 twiddle(const float2* tw, int k, int N, int sign) {
-  float2 t = tw[k % N];
+__device__ __forceinline__ float2 twiddle(const float2* tw, int k, int sign) {
+  float2 t = tw[k];
   return sign < 0 ? t : cconj(t);
 }
 
-// One Stockham pass of radix R (compile-time for 2/3/4, runtime otherwise).
 template <int R>
 __device__ __forceinline__ void dft_small(float2* v, int sign) {
   if constexpr (R == 2) {
@@ -47,7 +49,7 @@ __device__ __forceinline__ void dft_small(float2* v, int sign) {
   } else if constexpr (R == 4) {
     float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
     float2 a2 = cadd(v[1], v[3]), a3 = csub(v[1], v[3]);
-    // multiply a3 by -i (forward) or +i (inverse)
+    // a3 times -i (forward) or +i (inverse)
     float2 a3r = sign < 0 ? make_float2(a3.y, -a3.x) : make_float2(-a3.y, a3.x);
     v[0] = cadd(a0, a2);
     v[2] = csub(a0, a2);
@@ -66,18 +68,19 @@ __device__ __forceinline__ void dft_small(float2* v, int sign) {
   }
 }
 
+// One Stockham pass of radix R over one wavefront's N points.
 template <int R>
 __device__ void stockham_pass(const float2* __restrict__ in, float2* __restrict__ out, int N,
-                              int Ns, const float2* tw, int sign) {
+                              int Ns, const float2* tw, int sign, int lane) {
   const int nb = N / R;
-  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+  const int tstep = N / (Ns * R);
+  for (int j = lane; j < nb; j += kWave) {
     const int k = j % Ns;
-    const int tstep = N / (Ns * R);
     float2 v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       float2 x = in[j + r * nb];
-      v[r] = (r == 0 || k == 0) ? x : cmul(x, twiddle(tw, r * k * tstep, N, sign));
+      v[r] = (r == 0 || k == 0) ? x : cmul(x, twiddle(tw, r * k * tstep, sign));
     }
     dft_small<R>(v, sign);
     const int d = (j / Ns) * Ns * R + k;
@@ -87,19 +90,20 @@ __device__ void stockham_pass(const float2* __restrict__ in, float2* __restrict_
 }
 
 __device__ void stockham_pass_generic(const float2* __restrict__ in, float2* __restrict__ out,
-                                      int N, int Ns, int R, const float2* tw, int sign) {
+                                      int N, int Ns, int R, const float2* tw, int sign,
+                                      int lane) {
   const int nb = N / R;
   const int rstep = N / R;  // twiddle stride of the R-point DFT
-  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+  const int tstep = N / (Ns * R);
+  for (int j = lane; j < nb; j += kWave) {
     const int k = j % Ns;
-    const int tstep = N / (Ns * R);
     const int d = (j / Ns) * Ns * R + k;
     for (int q = 0; q < R; ++q) {
       float2 acc = make_float2(0.f, 0.f);
       for (int r = 0; r < R; ++r) {
         float2 x = in[j + r * nb];
-        if (r > 0 && k > 0) x = cmul(x, twiddle(tw, r * k * tstep, N, sign));
-        if (r > 0 && q > 0) x = cmul(x, twiddle(tw, ((r * q) % R) * rstep, N, sign));
+        if (r > 0 && k > 0) x = cmul(x, twiddle(tw, r * k * tstep, sign));
+        if (r > 0 && q > 0) x = cmul(x, twiddle(tw, ((r * q) % R) * rstep, sign));
         acc = cadd(acc, x);
       }
       out[d + q * Ns] = acc;
@@ -107,17 +111,19 @@ __device__ void stockham_pass_generic(const float2* __restrict__ in, float2* __r
   }
 }
 
-// Full FFT of N points held in buf[0]; returns the buffer holding the result.
-__device__ float2* block_fft(float2* buf0, float2* buf1, const FftPlan& plan, int sign) {
+// FFT of N points in buf0 (one wavefront's slice); every wavefront of the
+// block runs the same pass sequence, so block barriers separate the passes.
+// Returns the buffer holding the result.
+__device__ float2* wave_fft(float2* buf0, float2* buf1, const FftPlan& plan, int sign, int lane) {
   float2* in = buf0;
   float2* out = buf1;
   int Ns = 1;
   for (int s = 0; s < plan.nstages; ++s) {
     const int R = plan.radix[s];
-    if (R == 4) stockham_pass<4>(in, out, plan.N, Ns, plan.tw, sign);
-    else if (R == 2) stockham_pass<2>(in, out, plan.N, Ns, plan.tw, sign);
-    else if (R == 3) stockham_pass<3>(in, out, plan.N, Ns, plan.tw, sign);
-    else stockham_pass_generic(in, out, plan.N, Ns, R, plan.tw, sign);
+    if (R == 4) stockham_pass<4>(in, out, plan.N, Ns, plan.tw, sign, lane);
+    else if (R == 2) stockham_pass<2>(in, out, plan.N, Ns, plan.tw, sign, lane);
+    else if (R == 3) stockham_pass<3>(in, out, plan.N, Ns, plan.tw, sign, lane);
+    else stockham_pass_generic(in, out, plan.N, Ns, R, plan.tw, sign, lane);
     __syncthreads();
     Ns *= R;
     float2* t = in;
@@ -127,98 +133,94 @@ __device__ float2* block_fft(float2* buf0, float2* buf1, const FftPlan& plan, in
   return in;
 }
 
+__device__ __forceinline__ size_t waist_index(int c, int ldw, int JHP, int m, int p, int i) {
+  return (size_t)c * ldw + ((size_t)((m / kMT) * 2 + p) * JHP + i) * kMT + (m % kMT);
+}
+
 // ---------------------------------------------------------------------------
 // Analysis: grid rows -> folded, weighted waist.
 // ---------------------------------------------------------------------------
-
-
-
-
-// Fold-stage LDS: two FFT buffers of N points and, per row of the group,
-// E/O for m = 0..T: [field-in-pair][p][q][m] complex.
-__global__ void __launch_bounds__(kFftThreads)
-rfft_fold_kernel(FoldArgs a) {
+// LDS per wavefront: two N-point buffers, plus a third holding the u
+// spectrum while v is transformed (vector jobs).
+__global__ void __launch_bounds__(kFftThreads) rfft_fold_kernel(FoldArgs a) {
   extern __shared__ float2 lds[];
   const int N = a.fft.N;
   const int T1 = a.T + 1;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
   const int group = blockIdx.x % a.ngroups;
   const int item = blockIdx.x / a.ngroups;  // field (scalar) or level (vector)
-  if (item >= a.job.nfields) return;
+  if (item >= a.job.nfields) return;        // block-uniform
   const int nsrc = a.job.vector ? 2 : 1;
-  float2* buf0 = lds;
-  float2* buf1 = lds + N;
-  float2* eo = lds + 2 * N;  // [src][p][q][m]
-  const int i0 = group * a.rp;
+  const int nbuf = a.job.vector ? 3 : 2;
+  float2* base = lds + (size_t)w * nbuf * N;
+  const int i = group * a.rp + w;
+  const bool live = i < a.JH;
   const size_t plane = (size_t)a.nlat * N;
+  const float s = live ? a.job.rowscale[i] : 0.f;
 
+  float2* Zsrc[2] = {nullptr, nullptr};
   for (int src = 0; src < nsrc; ++src) {
-    const float* g = (src == 0 ? a.job.grid0 : a.job.grid1) + (size_t)item * plane;
-    for (int q = 0; q < a.rp; ++q) {
-      const int i = i0 + q;
-      float2* eoq = eo + ((size_t)(src * 2) * a.rp + q) * T1;
-      float2* eoq1 = eo + ((size_t)(src * 2 + 1) * a.rp + q) * T1;
-      if (i >= a.JH) {  // padding rows of the waist are written as zeros
-        for (int m = threadIdx.x; m < T1; m += blockDim.x) {
-          eoq[m] = make_float2(0.f, 0.f);
-          eoq1[m] = make_float2(0.f, 0.f);
-        }
-        continue;
-      }
+    float2* b0 = (src == 0) ? base : base + N;
+    float2* b1 = (src == 0) ? base + N : base + 2 * N;
+    if (live) {
+      const float* g = (src == 0 ? a.job.grid0 : a.job.grid1) + (size_t)item * plane;
       const float* rn = g + (size_t)(a.nlat - 1 - i) * N;
       const float* rs = g + (size_t)i * N;
-      for (int k = threadIdx.x; k < N; k += blockDim.x) buf0[k] = make_float2(rn[k], rs[k]);
+      for (int k = lane; k < N; k += kWave) b0[k] = make_float2(rn[k], rs[k]);
+    }
+    __syncthreads();
+    float2* Z = wave_fft(b0, b1, a.fft, -1, lane);
+    if (src == 0 && nsrc == 2 && Z != base) {
+      // keep the u spectrum in buffer 0 while v is transformed in 1 and 2
+      for (int k = lane; k < N; k += kWave) base[k] = Z[k];
+      Z = base;
       __syncthreads();
-      float2* Z = block_fft(buf0, buf1, a.fft, -1);
-      const float s = a.job.rowscale[i];
-      for (int m = threadIdx.x; m < T1; m += blockDim.x) {
+    }
+    Zsrc[src] = Z;
+  }
+  // Fold and write: E = (X_N + X_S) s, O = (X_N - X_S) s for m = 0..T,
+  // zeros for the padding rows and the padding orders of the last m-tile.
+  const int Tpad = ((T1 + kMT - 1) / kMT) * kMT;
+  for (int m = lane; m < Tpad; m += kWave) {
+    float2 e[2], o[2];
+#pragma unroll
+    for (int src = 0; src < 2; ++src) {
+      e[src] = make_float2(0.f, 0.f);
+      o[src] = make_float2(0.f, 0.f);
+      if (src < nsrc && live && m < T1) {
+        const float2* Z = Zsrc[src];
         float2 zm = Z[m];
         float2 zc = cconj(Z[(N - m) % N]);
-        // X_N = (zm + zc)/2, X_S = (zm - zc)/(2i)
         float2 xn = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
         float2 d = csub(zm, zc);
         float2 xs = make_float2(0.5f * d.y, -0.5f * d.x);
-        eoq[m] = make_float2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
-        eoq1[m] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
+        e[src] = make_float2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
+        o[src] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
       }
-      __syncthreads();
     }
-  }
-  // Write the group's tile: W[c][mb][p][i0..i0+3][0..7].
-  const int MB = (T1 + kMT - 1) / kMT;
-  const int ncolv = a.job.vector ? 4 : 2;
-  const int per_c = MB * 2 * a.rp * kMT;  // floats per column in this tile
-  const int narr = a.job.vector ? 2 : 1;
-  for (int arr = 0; arr < narr; ++arr) {
-    float* W = arr == 0 ? a.job.waist0 : a.job.waist1;
-    for (int e = threadIdx.x; e < ncolv * per_c; e += blockDim.x) {
-      const int cl = e / per_c;
-      int r = e % per_c;
-      const int mm = r % kMT;
-      r /= kMT;
-      const int q = r % a.rp;
-      r /= a.rp;
-      const int p = r % 2;
-      const int mb = r / 2;
-      const int m = mb * kMT + mm;
-      float val = 0.f;
-      if (m < T1) {
-        if (!a.job.vector) {
-          float2 v = eo[((size_t)p * a.rp + q) * T1 + m];
-          val = (cl == 0) ? v.x : v.y;
-        } else {
-          // data columns of level l: arrangement a = [-V.im, V.re, -U.im, U.re] (fold p),
-          // arrangement b = [U.re, U.im, -V.re, -V.im] (fold p).
-          float2 U = eo[((size_t)(0 * 2 + p) * a.rp + q) * T1 + m];
-          float2 V = eo[((size_t)(1 * 2 + p) * a.rp + q) * T1 + m];
-          if (arr == 0) {
-            val = cl == 0 ? -V.y : cl == 1 ? V.x : cl == 2 ? -U.y : U.x;
-          } else {
-            val = cl == 0 ? U.x : cl == 1 ? U.y : cl == 2 ? -V.x : -V.y;
-          }
+    if (!a.job.vector) {
+      const int c = a.job.col0 + 2 * item;
+      a.job.waist0[waist_index(c, a.job.ldw, a.JHP, m, 0, i)] = e[0].x;
+      a.job.waist0[waist_index(c + 1, a.job.ldw, a.JHP, m, 0, i)] = e[0].y;
+      a.job.waist0[waist_index(c, a.job.ldw, a.JHP, m, 1, i)] = o[0].x;
+      a.job.waist0[waist_index(c + 1, a.job.ldw, a.JHP, m, 1, i)] = o[0].y;
+    } else {
+      // level columns: arrangement a = [-V.im, V.re, -U.im, U.re],
+      //                arrangement b = [U.re, U.im, -V.re, -V.im]   (fold p)
+      const int c = a.job.col0 + 4 * item;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const float2 U = p == 0 ? e[0] : o[0];
+        const float2 V = p == 0 ? e[1] : o[1];
+        const float va[4] = {-V.y, V.x, -U.y, U.x};
+        const float vb[4] = {U.x, U.y, -V.x, -V.y};
+#pragma unroll
+        for (int cl = 0; cl < 4; ++cl) {
+          a.job.waist0[waist_index(c + cl, a.job.ldw, a.JHP, m, p, i)] = va[cl];
+          a.job.waist1[waist_index(c + cl, a.job.ldw, a.JHP, m, p, i)] = vb[cl];
         }
       }
-      const size_t c = (size_t)a.job.col0 + (size_t)item * ncolv + cl;
-      W[c * a.job.ldw + ((size_t)(mb * 2 + p) * a.JHP + i0 + q) * kMT + mm] = val;
     }
   }
 }
@@ -226,90 +228,74 @@ rfft_fold_kernel(FoldArgs a) {
 // ---------------------------------------------------------------------------
 // Synthesis: folded waist -> grid rows.
 // ---------------------------------------------------------------------------
-
-
-
-
-__global__ void __launch_bounds__(kFftThreads)
-irfft_unfold_kernel(UnfoldArgs a) {
+__global__ void __launch_bounds__(kFftThreads) irfft_unfold_kernel(UnfoldArgs a) {
   extern __shared__ float2 lds[];
   const int N = a.fft.N;
   const int T1 = a.T + 1;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
   const int group = blockIdx.x % a.ngroups;
   const int f = blockIdx.x / a.ngroups;
-  if (f >= a.job.nfields) return;
-  float2* buf0 = lds;
-  float2* buf1 = lds + N;
-  float2* eo = lds + 2 * N;  // [p][q][m]
-  const int i0 = group * a.rp;
-  const int MB = (T1 + kMT - 1) / kMT;
-  // Stage the waist tile (coalesced 128-byte runs).
-  const int per_c = MB * 2 * a.rp * kMT;
-  for (int e = threadIdx.x; e < 2 * per_c; e += blockDim.x) {
-    const int cl = e / per_c;
-    int r = e % per_c;
-    const int mm = r % kMT;
-    r /= kMT;
-    const int q = r % a.rp;
-    r /= a.rp;
-    const int p = r % 2;
-    const int mb = r / 2;
-    const int m = mb * kMT + mm;
-    const size_t c = (size_t)a.job.col0 + (size_t)a.job.fstride * f + cl;
-    float v = a.job.waist[c * a.job.ldw + ((size_t)(mb * 2 + p) * a.JHP + i0 + q) * kMT + mm];
-    if (m < T1) {
-      float* dst = reinterpret_cast<float*>(&eo[((size_t)p * a.rp + q) * T1 + m]);
-      dst[cl] = v;
-    }
-  }
-  __syncthreads();
-  const size_t plane = (size_t)a.nlat * N;
-  for (int q = 0; q < a.rp; ++q) {
-    const int i = i0 + q;
-    if (i >= a.JH) break;
+  if (f >= a.job.nfields) return;  // block-uniform
+  float2* b0 = lds + (size_t)w * 2 * N;
+  float2* b1 = b0 + N;
+  const int i = group * a.rp + w;
+  const bool live = i < a.JH;
+  const int c = a.job.col0 + a.job.fstride * f;
+  if (live) {
     const float s = a.job.rowscale[i];
-    // A = north spectrum (E + O), B = south spectrum (E - O), m <= T, zero above.
-    for (int k = threadIdx.x; k < N; k += blockDim.x) {
-      float2 z = make_float2(0.f, 0.f);
+    // Z[k]: A = north spectrum (E + O), B = south spectrum (E - O), m <= T
+    for (int k = lane; k < N; k += kWave) {
       int m = k;
       bool lower = true;
-      if (k > N / 2) { m = N - k; lower = false; }
+      if (k > N / 2) {
+        m = N - k;
+        lower = false;
+      }
+      float2 z = make_float2(0.f, 0.f);
       if (m < T1) {
-        float2 E = eo[((size_t)0 * a.rp + q) * T1 + m];
-        float2 O = eo[((size_t)1 * a.rp + q) * T1 + m];
+        const float* W = a.job.waist;
+        float2 E = make_float2(W[waist_index(c, a.job.ldw, a.JHP, m, 0, i)],
+                               W[waist_index(c + 1, a.job.ldw, a.JHP, m, 0, i)]);
+        float2 O = make_float2(W[waist_index(c, a.job.ldw, a.JHP, m, 1, i)],
+                               W[waist_index(c + 1, a.job.ldw, a.JHP, m, 1, i)]);
         float2 A = make_float2((E.x + O.x) * s, (E.y + O.y) * s);
         float2 B = make_float2((E.x - O.x) * s, (E.y - O.y) * s);
         if (a.job.imul) {
           A = make_float2(-A.y, A.x);
           B = make_float2(-B.y, B.x);
         }
-        if (m == 0) { A.y = 0.f; B.y = 0.f; }
-        if (lower) {
-          z = make_float2(A.x - B.y, A.y + B.x);  // A + iB
-        } else {
-          z = make_float2(A.x + B.y, -A.y + B.x); // conj(A) + i conj(B)
+        if (m == 0) {
+          A.y = 0.f;
+          B.y = 0.f;
         }
+        z = lower ? make_float2(A.x - B.y, A.y + B.x)     // A + iB
+                  : make_float2(A.x + B.y, -A.y + B.x);   // conj(A) + i conj(B)
       }
-      buf0[k] = z;
+      b0[k] = z;
     }
-    __syncthreads();
-    float2* x = block_fft(buf0, buf1, a.fft, +1);
+  }
+  __syncthreads();
+  float2* x = wave_fft(b0, b1, a.fft, +1, lane);
+  if (live) {
+    const size_t plane = (size_t)a.nlat * N;
     float* rn = a.job.grid + (size_t)f * plane + (size_t)(a.nlat - 1 - i) * N;
     float* rs = a.job.grid + (size_t)f * plane + (size_t)i * N;
-    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+    for (int k = lane; k < N; k += kWave) {
       float2 v = x[k];
       rn[k] = v.x;
       rs[k] = v.y;
     }
-    __syncthreads();
   }
 }
 
 size_t fold_lds_bytes(int N, int T, int vector, int rp) {
-  return (size_t)(2 * N + (vector ? 2 : 1) * 2 * rp * (T + 1)) * sizeof(float2);
+  (void)T;
+  return (size_t)rp * (vector ? 3 : 2) * N * sizeof(float2);
 }
 size_t unfold_lds_bytes(int N, int T, int rp) {
-  return (size_t)(2 * N + 2 * rp * (T + 1)) * sizeof(float2);
+  (void)T;
+  return (size_t)rp * 2 * N * sizeof(float2);
 }
 
 }  // namespace sht
@@ -318,7 +304,7 @@ extern "C" hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s
   size_t lds = sht::fold_lds_bytes(a->fft.N, a->T, a->job.vector, a->rp);
   int blocks = a->ngroups * a->job.nfields;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(sht::rfft_fold_kernel, dim3(blocks), dim3(sht::kFftThreads), lds, s, *a);
+  hipLaunchKernelGGL(sht::rfft_fold_kernel, dim3(blocks), dim3(sht::kWave * a->rp), lds, s, *a);
   return hipGetLastError();
 }
 
@@ -326,8 +312,17 @@ extern "C" hipError_t sht_launch_irfft_unfold(const sht::UnfoldArgs* a, hipStrea
   size_t lds = sht::unfold_lds_bytes(a->fft.N, a->T, a->rp);
   int blocks = a->ngroups * a->job.nfields;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(sht::irfft_unfold_kernel, dim3(blocks), dim3(sht::kFftThreads), lds, s, *a);
+  hipLaunchKernelGGL(sht::irfft_unfold_kernel, dim3(blocks), dim3(sht::kWave * a->rp), lds, s,
+                     *a);
   return hipGetLastError();
+}
+
+extern "C" hipError_t sht_fft_allow_lds(size_t bytes) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(sht::rfft_fold_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(sht::irfft_unfold_kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 extern "C" size_t sht_fold_lds_bytes(int N, int T, int vector, int rp) {

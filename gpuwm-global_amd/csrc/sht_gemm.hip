@@ -1,16 +1,24 @@
 // Grouped Legendre GEMM over the triangular m-schedule (one persistent launch).
 //
-// Every Legendre contraction of a batch -- every job, order m, output parity
-// and tile -- is one entry of a work list; each workgroup walks the list
-// with a grid stride, so one launch covers all fields, levels, members and
-// truncations of the batch.  A problem is
-//     C[r][c] = sum_seg sum_k A_seg[r][k] * B_seg[k][c]
+// Every Legendre contraction of a batch is an entry of a work list: a group
+// of up to eight consecutive orders m (one m-tile of the waist layout) for
+// one job and one output parity, times one row/column tile.  Each workgroup
+// walks the list with a grid stride, so one launch covers all fields, levels,
+// members and truncations of the batch.  Per order m the group holds
+//     C_m[r][c] = sum_seg sum_k A_seg,m[r][k] * B_seg,m[k][c]
 // with A a Legendre table block (k contiguous, rows and k zero padded to 32)
-// and B a data block (fp32, arbitrary k and c strides).  Two segments
-// express the vector operators (P*m/cos and dP/dphi against differently
-// arranged data) as one accumulation.
+// and B a data block.  Two segments express the vector operators (P*m/cos
+// and dP/dphi against differently arranged data) as one accumulation.
 //
-// Precision modes (template):
+// Direction SYN: B = packed coefficients (k = degree, contiguous per order),
+//   C = waist W[c][m/8][p][i][m%8]; the eight orders' accumulators are kept
+//   in registers, so each lane stores 4 latitudes x 8 orders = 128
+//   contiguous bytes per column.
+// Direction ANA: B = waist (k = latitude); each lane loads 8 latitudes x 8
+//   orders = 256 contiguous bytes per column and picks each order's fragment
+//   in registers; C = packed coefficients (4 contiguous degrees per lane).
+//
+// Precision modes:
 //   FP32   v_mfma_f32_16x16x4_f32, fp32 tables and data.
 //   BF16X3 v_mfma_f32_16x16x32_bf16 on hi+lo pieces: a0b0 + a0b1 + a1b0.
 //   BF16X6 three pieces each: a0b0 + (a0b1 + a1b0) + (a0b2 + a1b1 + a2b0),
@@ -19,10 +27,10 @@
 // Tables are pre-split on the device from FP64 (sht_tables.hip); data are
 // split here, in registers, by round-to-nearest-even conversion.
 //
-// Tile: 256 threads = 4 waves, block tile 64 rows x 64 columns, each wave a
-// 32x32 quadrant as 2x2 MFMA 16x16 tiles.  Lane l = (l & 15) row/column,
-// (l >> 4) selects a run of 8 consecutive k; A and B fragments use the same
-// k mapping, so each MFMA sums matching k.
+// Tile: 256 threads = 4 wavefronts in a 2 x 2 arrangement, each wavefront
+// RT x CT MFMA 16x16 tiles.  Lane l: (l & 15) is the row/column, (l >> 4)
+// selects a run of 8 consecutive k; A and B fragments use the same k
+// mapping, so each MFMA sums matching k.
 #include "sht_kernels.h"
 
 namespace sht {
@@ -31,13 +39,14 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(8))) unsigned short u16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 
-
-
-
+constexpr int kRT = 2;  // MFMA row tiles per wavefront
+constexpr int kCT = 1;  // MFMA column tiles per wavefront
+constexpr int kWR = 2;  // wavefront rows per block
+constexpr int kWC = 2;  // wavefront columns per block
 
 struct GemmWork {
-  const GemmProb* probs;
-  const int4* tiles;    // {prob, r0, c0, unused}
+  const GemmGroup* groups;
+  const int4* tiles;  // {group, r0, c0, unused}
   int ntiles;
 };
 
@@ -68,151 +77,203 @@ __device__ __forceinline__ void split_pieces(const float (&x)[8], u16x8 (&out)[3
   }
 }
 
-__device__ __forceinline__ void load_b8(const float* __restrict__ B, int bks, float (&x)[8]) {
-  if (bks == 1) {
-    const f32x4* p = reinterpret_cast<const f32x4*>(B);
-    f32x4 u = p[0], v = p[1];
-    x[0] = u[0]; x[1] = u[1]; x[2] = u[2]; x[3] = u[3];
-    x[4] = v[0]; x[5] = v[1]; x[6] = v[2]; x[7] = v[3];
-  } else {
-#pragma unroll
-    for (int s = 0; s < 8; ++s) x[s] = B[(long long)s * bks];
-  }
-}
+#define SHT_MFMA_BF(ACC, X, Y)                                                                 \
+  ACC = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, X),                \
+                                                __builtin_bit_cast(bf16x8, Y), ACC, 0, 0, 0)
 
-template <int MODE>
+template <int MODE, int DIR>
 __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int li = lane & 15;
-  const int kg = lane >> 4;  // k run: k0 + 8*kg .. +7
+  const int kg = lane >> 4;
   constexpr int NP = MODE == kBF16X6 ? 3 : 2;
+  constexpr bool SPLIT = MODE != kFP32;
 
   for (int t = blockIdx.x; t < w.ntiles; t += gridDim.x) {
     const int4 tile = w.tiles[t];
-    const GemmProb& P = w.probs[tile.x];
-    const int rbase = tile.y + 32 * (wave >> 1);
-    const int cbase = tile.z + 32 * (wave & 1);
-    if (rbase >= P.R || cbase >= P.N) continue;  // wave-uniform
+    const GemmGroup& G = w.groups[tile.x];
+    const int rbase = tile.y + 16 * kRT * (wave / kWC);
+    const int cbase = tile.z + 16 * kCT * (wave % kWC);
+    if (rbase >= G.Rmax || cbase >= G.N) continue;  // wave-uniform
 
-    f32x4 acc[2][2], acc2[2][2];
+    f32x4 acc[kMT][kRT][kCT];
+    f32x4 acc2[kMT][kRT][kCT];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int mm = 0; mm < kMT; ++mm)
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-        acc2[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-      }
+      for (int rb = 0; rb < kRT; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < kCT; ++cb) {
+          acc[mm][rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+          acc2[mm][rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
 
-    for (int sgi = 0; sgi < P.nseg; ++sgi) {
-      const GemmSeg S = P.seg[sgi];
-      for (int k0 = 0; k0 < S.K; k0 += 32) {
+    for (int sg = 0; sg < G.nseg; ++sg) {
+      const int bks = G.bks[sg];
+      const int bcs = G.bcs[sg];
+      for (int k0 = 0; k0 < G.Kmax[sg]; k0 += 32) {
         const int kk = k0 + 8 * kg;
-        // B fragments (two 16-column blocks), fp32.
-        float bx[2][8];
+        // ANA: one 256-byte load per column covers all eight orders.
+        float wb[kCT][64];
+        if constexpr (DIR == 1) {
 #pragma unroll
-        for (int cb = 0; cb < 2; ++cb) {
-          const int col = cbase + 16 * cb + li;
-          if (col < P.N) {
-            load_b8(S.B + (long long)kk * S.bks + (long long)col * S.bcs, S.bks, bx[cb]);
-          } else {
+          for (int cb = 0; cb < kCT; ++cb) {
+            const int col = cbase + 16 * cb + li;
+            if (col < G.N) {
+              const f32x4* p = reinterpret_cast<const f32x4*>(
+                  G.B[sg][0] + (long long)col * bcs + (long long)kk * kMT);
 #pragma unroll
-            for (int s = 0; s < 8; ++s) bx[cb][s] = 0.f;
+              for (int v = 0; v < 16; ++v) {
+                f32x4 x = p[v];
+                wb[cb][4 * v] = x[0];
+                wb[cb][4 * v + 1] = x[1];
+                wb[cb][4 * v + 2] = x[2];
+                wb[cb][4 * v + 3] = x[3];
+              }
+            } else {
+#pragma unroll
+              for (int v = 0; v < 64; ++v) wb[cb][v] = 0.f;
+            }
           }
         }
-        if constexpr (MODE == kFP32) {
-          float ax[2][8];
 #pragma unroll
-          for (int rb = 0; rb < 2; ++rb) {
-            const float* ap = reinterpret_cast<const float*>(S.A) +
-                              (long long)(rbase + 16 * rb + li) * S.lda + kk;
-            const f32x4* p = reinterpret_cast<const f32x4*>(ap);
-            f32x4 u = p[0], v = p[1];
-            ax[rb][0] = u[0]; ax[rb][1] = u[1]; ax[rb][2] = u[2]; ax[rb][3] = u[3];
-            ax[rb][4] = v[0]; ax[rb][5] = v[1]; ax[rb][6] = v[2]; ax[rb][7] = v[3];
-          }
+        for (int mm = 0; mm < kMT; ++mm) {
+          if (mm >= G.mcount || k0 >= G.K[sg][mm] || rbase >= G.R[mm]) continue;
+          // B fragments for this order
+          float bx[kCT][8];
 #pragma unroll
-          for (int s = 0; s < 8; ++s)
+          for (int cb = 0; cb < kCT; ++cb) {
+            if constexpr (DIR == 1) {
 #pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
+              for (int s = 0; s < 8; ++s) bx[cb][s] = wb[cb][s * kMT + mm];
+            } else {
+              const int col = cbase + 16 * cb + li;
+              if (col < G.N) {
+                const f32x4* p = reinterpret_cast<const f32x4*>(
+                    G.B[sg][mm] + (long long)col * bcs + (long long)kk * bks);
+                f32x4 u = p[0], v = p[1];
+                bx[cb][0] = u[0]; bx[cb][1] = u[1]; bx[cb][2] = u[2]; bx[cb][3] = u[3];
+                bx[cb][4] = v[0]; bx[cb][5] = v[1]; bx[cb][6] = v[2]; bx[cb][7] = v[3];
+              } else {
 #pragma unroll
-              for (int cb = 0; cb < 2; ++cb)
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[rb][s], bx[cb][s],
-                                                                   acc[rb][cb], 0, 0, 0);
-        } else {
-          u16x8 bp[2][3];
-#pragma unroll
-          for (int cb = 0; cb < 2; ++cb) split_pieces<NP>(bx[cb], bp[cb]);
-          u16x8 ap[2][3];
-#pragma unroll
-          for (int rb = 0; rb < 2; ++rb) {
-            const unsigned short* a0 = reinterpret_cast<const unsigned short*>(S.A) +
-                                       (long long)(rbase + 16 * rb + li) * S.lda + kk;
-#pragma unroll
-            for (int pc = 0; pc < NP; ++pc)
-              ap[rb][pc] = *reinterpret_cast<const u16x8*>(a0 + pc * S.piece);
-          }
-#pragma unroll
-          for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb) {
-#define SHT_MFMA(ACC, X, Y) \
-  ACC = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, X), \
-                                                __builtin_bit_cast(bf16x8, Y), ACC, 0, 0, 0)
-              if constexpr (MODE == kBF16X6) {
-                SHT_MFMA(acc2[rb][cb], ap[rb][2], bp[cb][0]);
-                SHT_MFMA(acc2[rb][cb], ap[rb][1], bp[cb][1]);
-                SHT_MFMA(acc2[rb][cb], ap[rb][0], bp[cb][2]);
+                for (int s = 0; s < 8; ++s) bx[cb][s] = 0.f;
               }
-              SHT_MFMA(acc2[rb][cb], ap[rb][1], bp[cb][0]);
-              SHT_MFMA(acc2[rb][cb], ap[rb][0], bp[cb][1]);
-              SHT_MFMA(acc[rb][cb], ap[rb][0], bp[cb][0]);
-#undef SHT_MFMA
             }
+          }
+          const int lda = G.lda[sg][mm];
+          if constexpr (!SPLIT) {
+            float ax[kRT][8];
+#pragma unroll
+            for (int rb = 0; rb < kRT; ++rb) {
+              const f32x4* p = reinterpret_cast<const f32x4*>(
+                  reinterpret_cast<const float*>(G.A[sg][mm]) +
+                  (long long)(rbase + 16 * rb + li) * lda + kk);
+              f32x4 u = p[0], v = p[1];
+              ax[rb][0] = u[0]; ax[rb][1] = u[1]; ax[rb][2] = u[2]; ax[rb][3] = u[3];
+              ax[rb][4] = v[0]; ax[rb][5] = v[1]; ax[rb][6] = v[2]; ax[rb][7] = v[3];
+            }
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+#pragma unroll
+              for (int rb = 0; rb < kRT; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < kCT; ++cb)
+                  acc[mm][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                      ax[rb][s], bx[cb][s], acc[mm][rb][cb], 0, 0, 0);
+          } else {
+            u16x8 bp[kCT][3];
+#pragma unroll
+            for (int cb = 0; cb < kCT; ++cb) split_pieces<NP>(bx[cb], bp[cb]);
+            u16x8 ap[kRT][3];
+#pragma unroll
+            for (int rb = 0; rb < kRT; ++rb) {
+              const unsigned short* a0 = reinterpret_cast<const unsigned short*>(G.A[sg][mm]) +
+                                         (long long)(rbase + 16 * rb + li) * lda + kk;
+#pragma unroll
+              for (int pc = 0; pc < NP; ++pc)
+                ap[rb][pc] = *reinterpret_cast<const u16x8*>(a0 + pc * G.piece);
+            }
+#pragma unroll
+            for (int rb = 0; rb < kRT; ++rb)
+#pragma unroll
+              for (int cb = 0; cb < kCT; ++cb) {
+                if constexpr (MODE == kBF16X6) {
+                  SHT_MFMA_BF(acc2[mm][rb][cb], ap[rb][2], bp[cb][0]);
+                  SHT_MFMA_BF(acc2[mm][rb][cb], ap[rb][1], bp[cb][1]);
+                  SHT_MFMA_BF(acc2[mm][rb][cb], ap[rb][0], bp[cb][2]);
+                }
+                SHT_MFMA_BF(acc2[mm][rb][cb], ap[rb][1], bp[cb][0]);
+                SHT_MFMA_BF(acc2[mm][rb][cb], ap[rb][0], bp[cb][1]);
+                SHT_MFMA_BF(acc[mm][rb][cb], ap[rb][0], bp[cb][0]);
+              }
+          }
         }
       }
     }
-    // Store: lane holds rows 4*kg + q (q = 0..3) of column li in each 16x16 tile.
+
+    // Epilogue.  Lane holds rows 4*kg + q (q = 0..3) of column li per 16x16 tile.
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
+    for (int rb = 0; rb < kRT; ++rb)
 #pragma unroll
-      for (int cb = 0; cb < 2; ++cb) {
+      for (int cb = 0; cb < kCT; ++cb) {
         const int col = cbase + 16 * cb + li;
-        if (col >= P.N) continue;
+        if (col >= G.N) continue;
         const int row = rbase + 16 * rb + 4 * kg;
-        f32x4 v = acc[rb][cb];
-        if constexpr (MODE != kFP32) v = v + acc2[rb][cb];
-        float* cp = P.C + (long long)row * P.crs + (long long)col * P.ccs;
-        if (P.crs == 1) {
-          *reinterpret_cast<f32x4*>(cp) = v;
+        if constexpr (DIR == 0) {
+          // waist: rows row..row+3 x orders 0..7 are 32 contiguous floats
+          float* cp = G.C[0] + (long long)col * G.ccs + (long long)row * kMT;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            float o[kMT];
+#pragma unroll
+            for (int mm = 0; mm < kMT; ++mm) {
+              f32x4 v = acc[mm][rb][cb];
+              if constexpr (SPLIT) v = v + acc2[mm][rb][cb];
+              o[mm] = (mm < G.mcount) ? v[q] : 0.f;
+            }
+            f32x4* d = reinterpret_cast<f32x4*>(cp + q * kMT);
+            d[0] = f32x4{o[0], o[1], o[2], o[3]};
+            d[1] = f32x4{o[4], o[5], o[6], o[7]};
+          }
         } else {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) cp[(long long)q * P.crs] = v[q];
+          for (int mm = 0; mm < kMT; ++mm) {
+            if (mm >= G.mcount || row >= G.R[mm]) continue;
+            f32x4 v = acc[mm][rb][cb];
+            if constexpr (SPLIT) v = v + acc2[mm][rb][cb];
+            *reinterpret_cast<f32x4*>(G.C[mm] + (long long)col * G.ccs + row) = v;
+          }
         }
       }
   }
 }
 
+#undef SHT_MFMA_BF
+
 }  // namespace sht
 
-extern "C" hipError_t sht_launch_legendre_gemm(int mode, const sht::GemmProb* probs,
+extern "C" hipError_t sht_launch_legendre_gemm(int mode, int dir, const sht::GemmGroup* groups,
                                                const int4* tiles, int ntiles, int grid,
                                                hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
-  sht::GemmWork w{probs, tiles, ntiles};
+  sht::GemmWork w{groups, tiles, ntiles};
   dim3 g(grid), b(256);
-  switch (mode) {
-    case sht::kFP32:
-      hipLaunchKernelGGL(sht::legendre_gemm_kernel<sht::kFP32>, g, b, 0, s, w);
-      break;
-    case sht::kBF16X3:
-      hipLaunchKernelGGL(sht::legendre_gemm_kernel<sht::kBF16X3>, g, b, 0, s, w);
-      break;
-    case sht::kBF16X6:
-      hipLaunchKernelGGL(sht::legendre_gemm_kernel<sht::kBF16X6>, g, b, 0, s, w);
-      break;
-    default:
-      return hipErrorInvalidValue;
+#define SHT_LAUNCH(M, D) hipLaunchKernelGGL((sht::legendre_gemm_kernel<M, D>), g, b, 0, s, w)
+  if (dir == 0) {
+    if (mode == sht::kFP32) SHT_LAUNCH(sht::kFP32, 0);
+    else if (mode == sht::kBF16X3) SHT_LAUNCH(sht::kBF16X3, 0);
+    else if (mode == sht::kBF16X6) SHT_LAUNCH(sht::kBF16X6, 0);
+    else return hipErrorInvalidValue;
+  } else {
+    if (mode == sht::kFP32) SHT_LAUNCH(sht::kFP32, 1);
+    else if (mode == sht::kBF16X3) SHT_LAUNCH(sht::kBF16X3, 1);
+    else if (mode == sht::kBF16X6) SHT_LAUNCH(sht::kBF16X6, 1);
+    else return hipErrorInvalidValue;
   }
+#undef SHT_LAUNCH
   return hipGetLastError();
 }
+
+extern "C" int sht_gemm_tile_rows() { return 16 * sht::kRT * sht::kWR; }
+extern "C" int sht_gemm_tile_cols() { return 16 * sht::kCT * sht::kWC; }

@@ -69,22 +69,20 @@ struct UnfoldArgs {
 };
 
 // ---- grouped GEMM (sht_gemm.hip) ----
-struct GemmSeg {
-  const void* A;              // A[r][k] at A + r*lda + k (fp32 or bf16 piece 0)
-  const float* B;             // B[k][c] at B + k*bks + c*bcs
-  long long piece;            // bf16 piece stride (elements) of A
-  int K;                      // multiple of 32
-  int lda;
-  int bks, bcs;
-};
-
-struct GemmProb {
-  GemmSeg seg[2];
-  int nseg;
-  int R;                      // rows to compute (multiple of 32)
-  int N;                      // columns
-  float* C;                   // C[r][c] at C + r*crs + c*ccs
-  int crs, ccs;
+// One work group: up to kMT consecutive orders m of one job and one output
+// parity.  Per order: C_m = sum_seg A_seg,m x B_seg,m (see sht_gemm.hip).
+struct GemmGroup {
+  const void* A[2][kMT];      // table block of (seg, m): A[r][k] at A + r*lda + k
+  const float* B[2][kMT];     // SYN: packed block of (seg, m); ANA: waist block (m-tile, parity)
+  float* C[kMT];              // SYN: C[0] = waist block (m-tile, parity); ANA: packed block of m
+  long long piece;            // bf16 piece stride (elements) of the tables
+  int K[2][kMT];              // k extent per (seg, m), multiple of 32; 0 = no work
+  int lda[2][kMT];
+  int R[kMT];                 // rows per m (multiple of 32)
+  int Kmax[2];
+  int bks[2], bcs[2];         // B strides (k, column); ANA reads the waist's m-tiled k
+  int nseg, N, Rmax, mcount;  // segments, columns, max rows, orders in the tile
+  int ccs;                    // C column stride
 };
 
 // ---- spectral data movement (sht_pack.hip) ----
@@ -136,8 +134,11 @@ hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s);
 hipError_t sht_launch_irfft_unfold(const sht::UnfoldArgs* a, hipStream_t s);
 size_t sht_fold_lds_bytes(int N, int T, int vector, int rp);
 size_t sht_unfold_lds_bytes(int N, int T, int rp);
-hipError_t sht_launch_legendre_gemm(int mode, const sht::GemmProb* probs, const int4* tiles,
-                                    int ntiles, int grid, hipStream_t s);
+hipError_t sht_fft_allow_lds(size_t bytes);
+hipError_t sht_launch_legendre_gemm(int mode, int dir, const sht::GemmGroup* groups,
+                                    const int4* tiles, int ntiles, int grid, hipStream_t s);
+int sht_gemm_tile_rows();
+int sht_gemm_tile_cols();
 hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s);
 hipError_t sht_launch_unpack(const sht::UnpackArgs* a, hipStream_t s);
 hipError_t sht_launch_spectral_elementwise(const float2* in, float2* out, long long total, int T,
