@@ -311,6 +311,7 @@ bool is_synthesis(int op) { return op >= ARWEN_SHT_OP_SYNTHESIS && op <= ARWEN_S
 
 // Collects the grouped-GEMM work list of one direction.
 struct GroupBuilder {
+  int dir = 0;  // 0 synthesis, 1 analysis
   std::vector<sht::GemmGroup> groups;
   std::vector<std::pair<long long, int4>> tiles;  // (work, tile)
   void add(sht::GemmGroup g) {
@@ -326,7 +327,7 @@ struct GroupBuilder {
     }
     const int id = (int)groups.size();
     groups.push_back(g);
-    const int TR = sht_gemm_tile_rows(), TC = sht_gemm_tile_cols();
+    const int TR = sht_gemm_tile_rows(dir), TC = sht_gemm_tile_cols(dir);
     for (int r0 = 0; r0 < g.Rmax; r0 += TR)
       for (int c0 = 0; c0 < g.N; c0 += TC)
         tiles.push_back({kwork * std::min(TC, g.N - c0), int4{id, r0, c0, 0}});
@@ -427,6 +428,7 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
   hip_check(hipGetDevice(&B->device), "hipGetDevice");
   B->precision = jobs[0].plan ? jobs[0].plan->precision : 0;
   GroupBuilder syn, ana;
+  ana.dir = 1;
   for (int j = 0; j < njobs; ++j) {
     const arwen_sht_job& J = jobs[j];
     const arwen_sht_plan* P = J.plan;

@@ -1,4 +1,5 @@
-// Grouped Legendre GEMM over the triangular m-schedule (one persistent launch).
+// Grouped Legendre GEMM over the triangular m-schedule (one persistent launch
+// per direction).
 //
 // Every Legendre contraction of a batch is an entry of a work list: a group
 // of up to eight consecutive orders m (one m-tile of the waist layout) for
@@ -10,13 +11,20 @@
 // and B a data block.  Two segments express the vector operators (P*m/cos
 // and dP/dphi against differently arranged data) as one accumulation.
 //
-// Direction SYN: B = packed coefficients (k = degree, contiguous per order),
-//   C = waist W[c][m/8][p][i][m%8]; the eight orders' accumulators are kept
-//   in registers, so each lane stores 4 latitudes x 8 orders = 128
-//   contiguous bytes per column.
-// Direction ANA: B = waist (k = latitude); each lane loads 8 latitudes x 8
-//   orders = 256 contiguous bytes per column and picks each order's fragment
-//   in registers; C = packed coefficients (4 contiguous degrees per lane).
+// Both directions stream 32-deep k-steps through double-buffered LDS: all
+// 256 threads fetch the next step's A and B tiles into registers while the
+// four wavefronts run the current step's MFMAs, then store them and meet at
+// one barrier per step.
+//
+// SYN (synthesis): rows = folded latitudes, k = degrees (packed, contiguous
+//   per order), C = waist W[c][m/8][p][i][m%8].  Steps run order by order;
+//   an order's tile is stored when its last step is done.  Block tile 64 x 64,
+//   each wavefront 32 x 32 (2 x 2 MFMA 16x16 tiles).
+// ANA (analysis): rows = degrees, k = folded latitudes from the waist, whose
+//   m-tile layout puts eight orders' values for one latitude side by side:
+//   one 32-latitude B tile (1 KB per column) serves all eight orders, so the
+//   loop runs k outer, order inner, with eight accumulator sets.  Block tile
+//   64 x 32, each wavefront 32 x 16.
 //
 // Precision modes:
 //   FP32   v_mfma_f32_16x16x4_f32, fp32 tables and data.
@@ -25,12 +33,9 @@
 //          every term down to 2^-16 relative; the five small products
 //          accumulate apart from a0b0 and are added once at the end.
 // Tables are pre-split on the device from FP64 (sht_tables.hip); data are
-// split here, in registers, by round-to-nearest-even conversion.
-//
-// Tile: 256 threads = 4 wavefronts in a 2 x 2 arrangement, each wavefront
-// RT x CT MFMA 16x16 tiles.  Lane l: (l & 15) is the row/column, (l >> 4)
-// selects a run of 8 consecutive k; A and B fragments use the same k
-// mapping, so each MFMA sums matching k.
+// split when their fragments are read from LDS, by round-to-nearest-even
+// conversion.  Lane l of a 16x16 MFMA: (l & 15) is the row/column, (l >> 4)
+// selects a run of 8 consecutive k; A and B fragments use the same k mapping.
 #include "sht_kernels.h"
 
 namespace sht {
@@ -39,10 +44,14 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(8))) unsigned short u16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 
-constexpr int kRT = 2;  // MFMA row tiles per wavefront
-constexpr int kCT = 1;  // MFMA column tiles per wavefront
-constexpr int kWR = 2;  // wavefront rows per block
-constexpr int kWC = 2;  // wavefront columns per block
+constexpr int kBK = 32;       // k per step
+constexpr int kPadF = 4;      // LDS row padding, floats
+constexpr int kPadH = 8;      // LDS row padding, bf16
+constexpr int kLdF = kBK + kPadF;  // fp32 LDS row stride
+constexpr int kLdH = kBK + kPadH;  // bf16 LDS row stride
+
+constexpr int kSynBM = 64, kSynBN = 64;
+constexpr int kAnaBM = 64, kAnaBN = 32;
 
 struct GemmWork {
   const GemmGroup* groups;
@@ -81,171 +90,390 @@ __device__ __forceinline__ void split_pieces(const float (&x)[8], u16x8 (&out)[3
   ACC = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, X),                \
                                                 __builtin_bit_cast(bf16x8, Y), ACC, 0, 0, 0)
 
-template <int MODE, int DIR>
-__global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int li = lane & 15;
-  const int kg = lane >> 4;
+// A tile staging: BM rows x 32 k.  FP32: one 8-float run per thread and row
+// (BM*4 runs); bf16: one 8-element run per thread, row and piece.
+template <int MODE, int BM>
+struct ATile {
+  static constexpr int NP = MODE == kBF16X6 ? 3 : (MODE == kBF16X3 ? 2 : 1);
+  static constexpr int RUNS = BM * 4;                 // 8-element runs per piece
+  static constexpr int PER_THREAD = RUNS / 256;       // runs per thread per piece
+  static constexpr int LDS_ELEMS = MODE == kFP32 ? BM * kLdF : NP * BM * kLdH;
+  f32x4 f[PER_THREAD][2];
+  u16x8 h[PER_THREAD][3];
+
+  __device__ __forceinline__ void load(const void* A, int lda, long long piece, int r0, int R,
+                                       int k0) {
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      const int run = threadIdx.x + 256 * u;
+      const int row = run >> 2;
+      const int kq = (run & 3) * 8;
+      const bool ok = r0 + row < R;
+      if constexpr (MODE == kFP32) {
+        f[u][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+        f[u][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+        if (ok) {
+          const f32x4* p = reinterpret_cast<const f32x4*>(
+              reinterpret_cast<const float*>(A) + (long long)(r0 + row) * lda + k0 + kq);
+          f[u][0] = p[0];
+          f[u][1] = p[1];
+        }
+      } else {
+#pragma unroll
+        for (int pc = 0; pc < NP; ++pc) h[u][pc] = u16x8{};
+        if (ok) {
+          const unsigned short* a0 = reinterpret_cast<const unsigned short*>(A) +
+                                     (long long)(r0 + row) * lda + k0 + kq;
+#pragma unroll
+          for (int pc = 0; pc < NP; ++pc) h[u][pc] = *reinterpret_cast<const u16x8*>(a0 + pc * piece);
+        }
+      }
+    }
+  }
+  __device__ __forceinline__ void store(float* sf, unsigned short* sh) const {
+#pragma unroll
+    for (int u = 0; u < PER_THREAD; ++u) {
+      const int run = threadIdx.x + 256 * u;
+      const int row = run >> 2;
+      const int kq = (run & 3) * 8;
+      if constexpr (MODE == kFP32) {
+        f32x4* d = reinterpret_cast<f32x4*>(sf + row * kLdF + kq);
+        d[0] = f[u][0];
+        d[1] = f[u][1];
+      } else {
+#pragma unroll
+        for (int pc = 0; pc < NP; ++pc)
+          *reinterpret_cast<u16x8*>(sh + (pc * BM + row) * kLdH + kq) = h[u][pc];
+      }
+    }
+  }
+};
+
+// Fragment reads from LDS.
+template <int MODE, int BM>
+__device__ __forceinline__ void read_a(const float* sf, const unsigned short* sh, int row, int kg,
+                                       float (&ax)[8], u16x8 (&ap)[3]) {
   constexpr int NP = MODE == kBF16X6 ? 3 : 2;
-  constexpr bool SPLIT = MODE != kFP32;
+  if constexpr (MODE == kFP32) {
+    const f32x4* p = reinterpret_cast<const f32x4*>(sf + row * kLdF + 8 * kg);
+    f32x4 u = p[0], v = p[1];
+    ax[0] = u[0]; ax[1] = u[1]; ax[2] = u[2]; ax[3] = u[3];
+    ax[4] = v[0]; ax[5] = v[1]; ax[6] = v[2]; ax[7] = v[3];
+  } else {
+#pragma unroll
+    for (int pc = 0; pc < NP; ++pc)
+      ap[pc] = *reinterpret_cast<const u16x8*>(sh + (pc * BM + row) * kLdH + 8 * kg);
+  }
+}
+
+__device__ __forceinline__ void read_b(const float* sb, int col, int kg, float (&bx)[8]) {
+  const f32x4* p = reinterpret_cast<const f32x4*>(sb + col * kLdF + 8 * kg);
+  f32x4 u = p[0], v = p[1];
+  bx[0] = u[0]; bx[1] = u[1]; bx[2] = u[2]; bx[3] = u[3];
+  bx[4] = v[0]; bx[5] = v[1]; bx[6] = v[2]; bx[7] = v[3];
+}
+
+// acc (+acc2) += A-fragment x B-fragment for one 16x16 tile.
+template <int MODE>
+__device__ __forceinline__ void mfma_tile(f32x4& acc, f32x4& acc2, const float (&ax)[8],
+                                          const u16x8 (&ap)[3], const float (&bx)[8],
+                                          const u16x8 (&bp)[3]) {
+  if constexpr (MODE == kFP32) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[s], bx[s], acc, 0, 0, 0);
+  } else {
+    if constexpr (MODE == kBF16X6) {
+      SHT_MFMA_BF(acc2, ap[2], bp[0]);
+      SHT_MFMA_BF(acc2, ap[1], bp[1]);
+      SHT_MFMA_BF(acc2, ap[0], bp[2]);
+    }
+    SHT_MFMA_BF(acc2, ap[1], bp[0]);
+    SHT_MFMA_BF(acc2, ap[0], bp[1]);
+    SHT_MFMA_BF(acc, ap[0], bp[0]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Synthesis kernel
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void __launch_bounds__(256) legendre_syn_kernel(GemmWork w) {
+  constexpr int BM = kSynBM, BN = kSynBN;
+  constexpr int NP = MODE == kBF16X6 ? 3 : 2;
+  using AT = ATile<MODE, BM>;
+  __shared__ __attribute__((aligned(16))) float sAf[2][MODE == kFP32 ? AT::LDS_ELEMS : 4];
+  __shared__ __attribute__((aligned(16))) unsigned short sAh[2][MODE == kFP32 ? 8 : AT::LDS_ELEMS];
+  __shared__ __attribute__((aligned(16))) float sB[2][BN * kLdF];
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, kg = lane >> 4;
+  const int wr = wave >> 1, wc = wave & 1;
 
   for (int t = blockIdx.x; t < w.ntiles; t += gridDim.x) {
     const int4 tile = w.tiles[t];
     const GemmGroup& G = w.groups[tile.x];
-    const int rbase = tile.y + 16 * kRT * (wave / kWC);
-    const int cbase = tile.z + 16 * kCT * (wave % kWC);
-    if (rbase >= G.Rmax || cbase >= G.N) continue;  // wave-uniform
+    const int r0 = tile.y, c0 = tile.z;
+    // step cursor: (mm, sg, k0); every order has K >= 32 for every segment
+    int mm = 0, sg = 0, k0 = 0;
+    // advance to the next step; returns false past the end
+    auto advance = [&](int& m_, int& s_, int& k_) {
+      k_ += kBK;
+      while (m_ < G.mcount && k_ >= G.K[s_][m_]) {
+        k_ = 0;
+        if (++s_ >= G.nseg) {
+          s_ = 0;
+          ++m_;
+        }
+      }
+      return m_ < G.mcount;
+    };
 
-    f32x4 acc[kMT][kRT][kCT];
-    f32x4 acc2[kMT][kRT][kCT];
-#pragma unroll
-    for (int mm = 0; mm < kMT; ++mm)
-#pragma unroll
-      for (int rb = 0; rb < kRT; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < kCT; ++cb) {
-          acc[mm][rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-          acc2[mm][rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
+    AT at;
+    // B staging: 64 cols x 32 k = 256 runs of 8 floats, one per thread
+    f32x4 bv[2];
+    auto load_b = [&](int m_, int s_, int k_) {
+      const int col = threadIdx.x >> 2;
+      const int kq = (threadIdx.x & 3) * 8;
+      bv[0] = f32x4{0.f, 0.f, 0.f, 0.f};
+      bv[1] = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (c0 + col < G.N) {
+        const f32x4* p = reinterpret_cast<const f32x4*>(
+            G.B[s_][m_] + (long long)(c0 + col) * G.bcs[s_] + k_ + kq);
+        bv[0] = p[0];
+        bv[1] = p[1];
+      }
+    };
+    auto store_b = [&](float* sb) {
+      const int col = threadIdx.x >> 2;
+      const int kq = (threadIdx.x & 3) * 8;
+      f32x4* d = reinterpret_cast<f32x4*>(sb + col * kLdF + kq);
+      d[0] = bv[0];
+      d[1] = bv[1];
+    };
 
-    for (int sg = 0; sg < G.nseg; ++sg) {
-      const int bks = G.bks[sg];
-      const int bcs = G.bcs[sg];
-      for (int k0 = 0; k0 < G.Kmax[sg]; k0 += 32) {
-        const int kk = k0 + 8 * kg;
-        // ANA: one 256-byte load per column covers all eight orders.
-        float wb[kCT][64];
-        if constexpr (DIR == 1) {
+    // prologue
+    at.load(G.A[sg][mm], G.lda[sg][mm], G.piece, r0, G.R[mm], k0);
+    load_b(mm, sg, k0);
+    at.store(sAf[0], sAh[0]);
+    store_b(sB[0]);
+    __syncthreads();
+
+    f32x4 acc[2][2], acc2[2][2];
 #pragma unroll
-          for (int cb = 0; cb < kCT; ++cb) {
-            const int col = cbase + 16 * cb + li;
-            if (col < G.N) {
-              const f32x4* p = reinterpret_cast<const f32x4*>(
-                  G.B[sg][0] + (long long)col * bcs + (long long)kk * kMT);
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-              for (int v = 0; v < 16; ++v) {
-                f32x4 x = p[v];
-                wb[cb][4 * v] = x[0];
-                wb[cb][4 * v + 1] = x[1];
-                wb[cb][4 * v + 2] = x[2];
-                wb[cb][4 * v + 3] = x[3];
-              }
-            } else {
+      for (int b = 0; b < 2; ++b) {
+        acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc2[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+
+    int buf = 0;
+    bool more = true;
+    while (more) {
+      int nm = mm, ns = sg, nk = k0;
+      const bool has_next = advance(nm, ns, nk);
+      if (has_next) {
+        at.load(G.A[ns][nm], G.lda[ns][nm], G.piece, r0, G.R[nm], nk);
+        load_b(nm, ns, nk);
+      }
+      // compute current step
+      {
+        const float* sa = sAf[buf];
+        const unsigned short* sh = sAh[buf];
+        const float* sb = sB[buf];
+        float ax[2][8], bx[2][8];
+        u16x8 ap[2][3], bp[2][3];
 #pragma unroll
-              for (int v = 0; v < 64; ++v) wb[cb][v] = 0.f;
-            }
-          }
+        for (int rb = 0; rb < 2; ++rb) read_a<MODE, BM>(sa, sh, wr * 32 + rb * 16 + li, kg, ax[rb], ap[rb]);
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+          read_b(sb, wc * 32 + cb * 16 + li, kg, bx[cb]);
+          if constexpr (MODE != kFP32) split_pieces<NP>(bx[cb], bp[cb]);
         }
 #pragma unroll
-        for (int mm = 0; mm < kMT; ++mm) {
-          if (mm >= G.mcount || k0 >= G.K[sg][mm] || rbase >= G.R[mm]) continue;
-          // B fragments for this order
-          float bx[kCT][8];
+        for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-          for (int cb = 0; cb < kCT; ++cb) {
-            if constexpr (DIR == 1) {
+          for (int cb = 0; cb < 2; ++cb)
+            mfma_tile<MODE>(acc[rb][cb], acc2[rb][cb], ax[rb], ap[rb], bx[cb], bp[cb]);
+      }
+      // order finished: store its tile into the waist and restart the accumulators
+      if (!has_next || nm != mm) {
 #pragma unroll
-              for (int s = 0; s < 8; ++s) bx[cb][s] = wb[cb][s * kMT + mm];
-            } else {
-              const int col = cbase + 16 * cb + li;
-              if (col < G.N) {
-                const f32x4* p = reinterpret_cast<const f32x4*>(
-                    G.B[sg][mm] + (long long)col * bcs + (long long)kk * bks);
-                f32x4 u = p[0], v = p[1];
-                bx[cb][0] = u[0]; bx[cb][1] = u[1]; bx[cb][2] = u[2]; bx[cb][3] = u[3];
-                bx[cb][4] = v[0]; bx[cb][5] = v[1]; bx[cb][6] = v[2]; bx[cb][7] = v[3];
-              } else {
+        for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-                for (int s = 0; s < 8; ++s) bx[cb][s] = 0.f;
-              }
+          for (int cb = 0; cb < 2; ++cb) {
+            const int col = c0 + wc * 32 + cb * 16 + li;
+            const int row = r0 + wr * 32 + rb * 16 + 4 * kg;
+            f32x4 v = acc[rb][cb];
+            if constexpr (MODE != kFP32) v = v + acc2[rb][cb];
+            if (col < G.N && row < G.R[mm]) {
+              float* cp = G.C[0] + (long long)col * G.ccs + (long long)row * kMT + mm;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) cp[q * kMT] = v[q];
             }
+            acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+            acc2[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
           }
-          const int lda = G.lda[sg][mm];
-          if constexpr (!SPLIT) {
-            float ax[kRT][8];
-#pragma unroll
-            for (int rb = 0; rb < kRT; ++rb) {
-              const f32x4* p = reinterpret_cast<const f32x4*>(
-                  reinterpret_cast<const float*>(G.A[sg][mm]) +
-                  (long long)(rbase + 16 * rb + li) * lda + kk);
-              f32x4 u = p[0], v = p[1];
-              ax[rb][0] = u[0]; ax[rb][1] = u[1]; ax[rb][2] = u[2]; ax[rb][3] = u[3];
-              ax[rb][4] = v[0]; ax[rb][5] = v[1]; ax[rb][6] = v[2]; ax[rb][7] = v[3];
-            }
-#pragma unroll
-            for (int s = 0; s < 8; ++s)
-#pragma unroll
-              for (int rb = 0; rb < kRT; ++rb)
-#pragma unroll
-                for (int cb = 0; cb < kCT; ++cb)
-                  acc[mm][rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                      ax[rb][s], bx[cb][s], acc[mm][rb][cb], 0, 0, 0);
-          } else {
-            u16x8 bp[kCT][3];
-#pragma unroll
-            for (int cb = 0; cb < kCT; ++cb) split_pieces<NP>(bx[cb], bp[cb]);
-            u16x8 ap[kRT][3];
-#pragma unroll
-            for (int rb = 0; rb < kRT; ++rb) {
-              const unsigned short* a0 = reinterpret_cast<const unsigned short*>(G.A[sg][mm]) +
-                                         (long long)(rbase + 16 * rb + li) * lda + kk;
-#pragma unroll
-              for (int pc = 0; pc < NP; ++pc)
-                ap[rb][pc] = *reinterpret_cast<const u16x8*>(a0 + pc * G.piece);
-            }
-#pragma unroll
-            for (int rb = 0; rb < kRT; ++rb)
-#pragma unroll
-              for (int cb = 0; cb < kCT; ++cb) {
-                if constexpr (MODE == kBF16X6) {
-                  SHT_MFMA_BF(acc2[mm][rb][cb], ap[rb][2], bp[cb][0]);
-                  SHT_MFMA_BF(acc2[mm][rb][cb], ap[rb][1], bp[cb][1]);
-                  SHT_MFMA_BF(acc2[mm][rb][cb], ap[rb][0], bp[cb][2]);
-                }
-                SHT_MFMA_BF(acc2[mm][rb][cb], ap[rb][1], bp[cb][0]);
-                SHT_MFMA_BF(acc2[mm][rb][cb], ap[rb][0], bp[cb][1]);
-                SHT_MFMA_BF(acc[mm][rb][cb], ap[rb][0], bp[cb][0]);
-              }
-          }
-        }
+      }
+      if (has_next) {
+        at.store(sAf[buf ^ 1], sAh[buf ^ 1]);
+        store_b(sB[buf ^ 1]);
+      }
+      __syncthreads();
+      buf ^= 1;
+      mm = nm;
+      sg = ns;
+      k0 = nk;
+      more = has_next;
+    }
+    // orders of the m-tile beyond T+1 (last tile only): zero their waist slots
+    if (G.mcount < kMT) {
+      for (int e = threadIdx.x; e < BM * BN; e += 256) {
+        const int col = c0 + e / BM, row = r0 + e % BM;
+        if (col < G.N && row < G.Rmax)
+          for (int z = G.mcount; z < kMT; ++z)
+            G.C[0][(long long)col * G.ccs + (long long)row * kMT + z] = 0.f;
       }
     }
+  }
+}
 
-    // Epilogue.  Lane holds rows 4*kg + q (q = 0..3) of column li per 16x16 tile.
+// ---------------------------------------------------------------------------
+// Analysis kernel
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void __launch_bounds__(256) legendre_ana_kernel(GemmWork w) {
+  constexpr int BM = kAnaBM, BN = kAnaBN;
+  constexpr int NP = MODE == kBF16X6 ? 3 : 2;
+  using AT = ATile<MODE, BM>;
+  __shared__ __attribute__((aligned(16))) float sAf[2][MODE == kFP32 ? AT::LDS_ELEMS : 4];
+  __shared__ __attribute__((aligned(16))) unsigned short sAh[2][MODE == kFP32 ? 8 : AT::LDS_ELEMS];
+  __shared__ __attribute__((aligned(16))) float sB[kMT][BN * kLdF];  // [order][col][k]
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, kg = lane >> 4;
+  const int wr = wave >> 1, wc = wave & 1;
+
+  for (int t = blockIdx.x; t < w.ntiles; t += gridDim.x) {
+    const int4 tile = w.tiles[t];
+    const GemmGroup& G = w.groups[tile.x];
+    const int r0 = tile.y, c0 = tile.z;
+    // orders whose rows reach this tile
+    int mlive = 0;
+    for (int z = 0; z < G.mcount; ++z)
+      if (r0 < G.R[z]) mlive = z + 1;
+    if (mlive == 0) continue;
+    // B tile of one k-group: 32 cols x 32 latitudes x 8 orders (1 KB per column)
+    // = 2048 runs of 4 floats, 8 per thread.
+    f32x4 bv[8];
+    auto load_b = [&](int s_, int k_) {
 #pragma unroll
-    for (int rb = 0; rb < kRT; ++rb)
+      for (int u = 0; u < 8; ++u) {
+        const int run = threadIdx.x + 256 * u;  // col = run / 64, within col: 64 runs
+        const int col = run >> 6;
+        const int r = run & 63;                 // latitude k_ + r/2, orders (r&1)*4 .. +3
+        bv[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+        if (c0 + col < G.N)
+          bv[u] = *reinterpret_cast<const f32x4*>(
+              G.B[s_][0] + (long long)(c0 + col) * G.bcs[s_] + (long long)k_ * kMT + r * 4);
+      }
+    };
+    auto store_b = [&]() {
 #pragma unroll
-      for (int cb = 0; cb < kCT; ++cb) {
-        const int col = cbase + 16 * cb + li;
-        if (col >= G.N) continue;
-        const int row = rbase + 16 * rb + 4 * kg;
-        if constexpr (DIR == 0) {
-          // waist: rows row..row+3 x orders 0..7 are 32 contiguous floats
-          float* cp = G.C[0] + (long long)col * G.ccs + (long long)row * kMT;
+      for (int u = 0; u < 8; ++u) {
+        const int run = threadIdx.x + 256 * u;
+        const int col = run >> 6;
+        const int r = run & 63;
+        const int i = r >> 1, z0 = (r & 1) * 4;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            float o[kMT];
+        for (int q = 0; q < 4; ++q) sB[z0 + q][col * kLdF + i] = bv[u][q];
+      }
+    };
+    AT at;
+    // step cursor: (sg, k0, z) with z over live orders
+    int sg = 0, k0 = 0, z = 0;
+    auto advance = [&](int& s_, int& k_, int& z_, bool& newk) {
+      newk = false;
+      if (++z_ < mlive) return true;
+      z_ = 0;
+      newk = true;
+      k_ += kBK;
+      if (k_ < G.Kmax[s_]) return true;
+      k_ = 0;
+      return ++s_ < G.nseg;
+    };
+
+    f32x4 acc[kMT][2], acc2[kMT][2];
 #pragma unroll
-            for (int mm = 0; mm < kMT; ++mm) {
-              f32x4 v = acc[mm][rb][cb];
-              if constexpr (SPLIT) v = v + acc2[mm][rb][cb];
-              o[mm] = (mm < G.mcount) ? v[q] : 0.f;
-            }
-            f32x4* d = reinterpret_cast<f32x4*>(cp + q * kMT);
-            d[0] = f32x4{o[0], o[1], o[2], o[3]};
-            d[1] = f32x4{o[4], o[5], o[6], o[7]};
-          }
-        } else {
+    for (int a = 0; a < kMT; ++a)
 #pragma unroll
-          for (int mm = 0; mm < kMT; ++mm) {
-            if (mm >= G.mcount || row >= G.R[mm]) continue;
-            f32x4 v = acc[mm][rb][cb];
-            if constexpr (SPLIT) v = v + acc2[mm][rb][cb];
-            *reinterpret_cast<f32x4*>(G.C[mm] + (long long)col * G.ccs + row) = v;
-          }
+      for (int b = 0; b < 2; ++b) {
+        acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc2[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+
+    // prologue
+    load_b(sg, k0);
+    at.load(G.A[sg][z], G.lda[sg][z], G.piece, r0, G.R[z], k0);
+    store_b();
+    at.store(sAf[0], sAh[0]);
+    __syncthreads();
+
+    int buf = 0;
+    bool more = true;
+    while (more) {
+      int ns = sg, nk = k0, nz = z;
+      bool newk;
+      const bool has_next = advance(ns, nk, nz, newk);
+      if (has_next) {
+        at.load(G.A[ns][nz], G.lda[ns][nz], G.piece, r0, G.R[nz], nk);
+        if (newk) load_b(ns, nk);
+      }
+      {
+        const float* sa = sAf[buf];
+        const unsigned short* sh = sAh[buf];
+        const float* sb = sB[z];
+        float ax[2][8], bx[8];
+        u16x8 ap[2][3], bp[3];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) read_a<MODE, BM>(sa, sh, wr * 32 + rb * 16 + li, kg, ax[rb], ap[rb]);
+        read_b(sb, wc * 16 + li, kg, bx);
+        if constexpr (MODE != kFP32) split_pieces<NP>(bx, bp);
+#pragma unroll
+        for (int zz = 0; zz < kMT; ++zz) {
+          if (zz != z) continue;
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb) mfma_tile<MODE>(acc[zz][rb], acc2[zz][rb], ax[rb], ap[rb], bx, bp);
         }
       }
+      if (has_next) {
+        if (newk) {
+          __syncthreads();  // every wavefront is done with this k-group's B tile
+          store_b();
+        }
+        at.store(sAf[buf ^ 1], sAh[buf ^ 1]);
+      }
+      __syncthreads();
+      buf ^= 1;
+      sg = ns;
+      k0 = nk;
+      z = nz;
+      more = has_next;
+    }
+    // store every live order's rows
+#pragma unroll
+    for (int zz = 0; zz < kMT; ++zz) {
+      if (zz >= mlive) continue;
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const int col = c0 + wc * 16 + li;
+        const int row = r0 + wr * 32 + rb * 16 + 4 * kg;
+        if (col >= G.N || row >= G.R[zz]) continue;
+        f32x4 v = acc[zz][rb];
+        if constexpr (MODE != kFP32) v = v + acc2[zz][rb];
+        *reinterpret_cast<f32x4*>(G.C[zz] + (long long)col * G.ccs + row) = v;
+      }
+    }
   }
 }
 
@@ -259,21 +487,21 @@ extern "C" hipError_t sht_launch_legendre_gemm(int mode, int dir, const sht::Gem
   if (ntiles <= 0) return hipSuccess;
   sht::GemmWork w{groups, tiles, ntiles};
   dim3 g(grid), b(256);
-#define SHT_LAUNCH(M, D) hipLaunchKernelGGL((sht::legendre_gemm_kernel<M, D>), g, b, 0, s, w)
+#define SHT_LAUNCH(K, M) hipLaunchKernelGGL((sht::K<M>), g, b, 0, s, w)
   if (dir == 0) {
-    if (mode == sht::kFP32) SHT_LAUNCH(sht::kFP32, 0);
-    else if (mode == sht::kBF16X3) SHT_LAUNCH(sht::kBF16X3, 0);
-    else if (mode == sht::kBF16X6) SHT_LAUNCH(sht::kBF16X6, 0);
+    if (mode == sht::kFP32) SHT_LAUNCH(legendre_syn_kernel, sht::kFP32);
+    else if (mode == sht::kBF16X3) SHT_LAUNCH(legendre_syn_kernel, sht::kBF16X3);
+    else if (mode == sht::kBF16X6) SHT_LAUNCH(legendre_syn_kernel, sht::kBF16X6);
     else return hipErrorInvalidValue;
   } else {
-    if (mode == sht::kFP32) SHT_LAUNCH(sht::kFP32, 1);
-    else if (mode == sht::kBF16X3) SHT_LAUNCH(sht::kBF16X3, 1);
-    else if (mode == sht::kBF16X6) SHT_LAUNCH(sht::kBF16X6, 1);
+    if (mode == sht::kFP32) SHT_LAUNCH(legendre_ana_kernel, sht::kFP32);
+    else if (mode == sht::kBF16X3) SHT_LAUNCH(legendre_ana_kernel, sht::kBF16X3);
+    else if (mode == sht::kBF16X6) SHT_LAUNCH(legendre_ana_kernel, sht::kBF16X6);
     else return hipErrorInvalidValue;
   }
 #undef SHT_LAUNCH
   return hipGetLastError();
 }
 
-extern "C" int sht_gemm_tile_rows() { return 16 * sht::kRT * sht::kWR; }
-extern "C" int sht_gemm_tile_cols() { return 16 * sht::kCT * sht::kWC; }
+extern "C" int sht_gemm_tile_rows(int dir) { return dir == 0 ? sht::kSynBM : sht::kAnaBM; }
+extern "C" int sht_gemm_tile_cols(int dir) { return dir == 0 ? sht::kSynBN : sht::kAnaBN; }

@@ -137,8 +137,8 @@ size_t sht_unfold_lds_bytes(int N, int T, int rp);
 hipError_t sht_fft_allow_lds(size_t bytes);
 hipError_t sht_launch_legendre_gemm(int mode, int dir, const sht::GemmGroup* groups,
                                     const int4* tiles, int ntiles, int grid, hipStream_t s);
-int sht_gemm_tile_rows();
-int sht_gemm_tile_cols();
+int sht_gemm_tile_rows(int dir);
+int sht_gemm_tile_cols(int dir);
 hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s);
 hipError_t sht_launch_unpack(const sht::UnpackArgs* a, hipStream_t s);
 hipError_t sht_launch_spectral_elementwise(const float2* in, float2* out, long long total, int T,
