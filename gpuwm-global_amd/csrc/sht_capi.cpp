@@ -123,6 +123,7 @@ struct arwen_sht_plan {
   DevBuf scale_fold, scale_fold_vec, scale_one, scale_inv_a;
   sht::FftPlan fft{};
   int rp_fold = 4, rp_fold_vec = 4, rp_unfold = 4;  // folded rows (wavefronts) per FFT block
+  int fft_gpb = 2;                                   // row groups walked by one FFT block
   // single-operation batch cache (most recent first)
   std::list<std::pair<std::tuple<int, int, const void*, const void*, void*, void*>,
                       arwen_sht_batch*>>
@@ -309,6 +310,8 @@ namespace {
 
 bool is_synthesis(int op) { return op >= ARWEN_SHT_OP_SYNTHESIS && op <= ARWEN_SHT_OP_UV_FROM_VORDIV; }
 
+constexpr int kXcds = 8;  // MI355X: 8 XCDs x 32 CUs
+
 // Collects the grouped-GEMM work list of one direction.
 struct GroupBuilder {
   int dir = 0;  // 0 synthesis, 1 analysis
@@ -332,13 +335,38 @@ struct GroupBuilder {
       for (int c0 = 0; c0 < g.N; c0 += TC)
         tiles.push_back({kwork * std::min(TC, g.N - c0), int4{id, r0, c0, 0}});
   }
+  // XCD-aware order.  Workgroups are dispatched round-robin over the 8 XCDs
+  // and the kernels walk the list with a grid stride that is a multiple of 8,
+  // so list position t always runs on XCD t % 8.  Every tile of one group
+  // (same orders, same tables, same data columns) goes to one XCD, where its
+  // tiles run side by side and share that XCD's L2; groups are balanced over
+  // the XCDs longest first.  Short lists are padded with idle tiles (-1).
   void finish(DevBuf& dgroups, DevBuf& dtiles, int* ntiles) {
-    // longest first: the grid-stride walk then balances the triangular schedule
-    std::stable_sort(tiles.begin(), tiles.end(),
-                     [](const auto& a, const auto& b) { return a.first > b.first; });
+    const int nx = kXcds;
+    std::vector<long long> gwork(groups.size(), 0);
+    std::vector<std::vector<std::pair<long long, int4>>> per_group(groups.size());
+    for (auto& x : tiles) {
+      gwork[x.second.x] += x.first;
+      per_group[x.second.x].push_back(x);
+    }
+    std::vector<int> order(groups.size());
+    for (size_t g = 0; g < order.size(); ++g) order[g] = (int)g;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return gwork[a] > gwork[b]; });
+    std::vector<long long> load(nx, 0);
+    std::vector<std::vector<int4>> lists(nx);
+    for (int g : order) {
+      int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[x] += gwork[g];
+      for (auto& tt : per_group[g]) lists[x].push_back(tt.second);
+    }
+    size_t longest = 0;
+    for (auto& l : lists) longest = std::max(longest, l.size());
     std::vector<int4> t;
-    t.reserve(tiles.size());
-    for (auto& x : tiles) t.push_back(x.second);
+    t.reserve(longest * nx);
+    for (size_t j = 0; j < longest; ++j)
+      for (int x = 0; x < nx; ++x)
+        t.push_back(j < lists[x].size() ? lists[x][j] : int4{-1, 0, 0, 0});
     upload(dgroups, groups);
     upload(dtiles, t);
     *ntiles = (int)t.size();
@@ -516,6 +544,8 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
         ua.JHP = P->JHP;
         ua.rp = P->rp_unfold;
         ua.ngroups = (P->JH + ua.rp - 1) / ua.rp;
+        ua.gpb = P->fft_gpb;
+        ua.nchunks = (ua.ngroups + ua.gpb - 1) / ua.gpb;
         ua.job.waist = waist.as<float>();
         ua.job.grid = static_cast<float*>(grid);
         ua.job.rowscale = sc.as<float>();
@@ -551,6 +581,8 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       fa.JHP = P->JHP;
       fa.rp = vec ? P->rp_fold_vec : P->rp_fold;
       fa.ngroups = P->JHP / fa.rp;
+      fa.gpb = P->fft_gpb;
+      fa.nchunks = (fa.ngroups + fa.gpb - 1) / fa.gpb;
       fa.job.grid0 = static_cast<const float*>(J.in0);
       fa.job.grid1 = static_cast<const float*>(J.in1);
       fa.job.waist0 = js->waist_a.as<float>();
@@ -594,9 +626,12 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
   hip_check(hipDeviceSynchronize(), "batch setup");
 }
 
+// Persistent grid: a multiple of the XCD count (see GroupBuilder::finish).
 int gemm_grid(const arwen_sht_batch* B, int ntiles) {
-  if (B->gemm_blocks > 0) return std::min(B->gemm_blocks, std::max(ntiles, 1));
-  return std::min(ntiles, 256 * 8);
+  int g = B->gemm_blocks > 0 ? B->gemm_blocks : 256 * 8;
+  g = std::min(g, ntiles);
+  if (g >= kXcds) g -= g % kXcds;
+  return std::max(g, 1);
 }
 
 // Phases of a batch run, in order; `ev` (optional) gets one event before

@@ -114,16 +114,17 @@ __device__ void stockham_pass_generic(const float2* __restrict__ in, float2* __r
 // FFT of N points in buf0 (one wavefront's slice); every wavefront of the
 // block runs the same pass sequence, so block barriers separate the passes.
 // Returns the buffer holding the result.
-__device__ float2* wave_fft(float2* buf0, float2* buf1, const FftPlan& plan, int sign, int lane) {
+__device__ float2* wave_fft(float2* buf0, float2* buf1, const FftPlan& plan, const float2* tw,
+                            int sign, int lane) {
   float2* in = buf0;
   float2* out = buf1;
   int Ns = 1;
   for (int s = 0; s < plan.nstages; ++s) {
     const int R = plan.radix[s];
-    if (R == 4) stockham_pass<4>(in, out, plan.N, Ns, plan.tw, sign, lane);
-    else if (R == 2) stockham_pass<2>(in, out, plan.N, Ns, plan.tw, sign, lane);
-    else if (R == 3) stockham_pass<3>(in, out, plan.N, Ns, plan.tw, sign, lane);
-    else stockham_pass_generic(in, out, plan.N, Ns, R, plan.tw, sign, lane);
+    if (R == 4) stockham_pass<4>(in, out, plan.N, Ns, tw, sign, lane);
+    else if (R == 2) stockham_pass<2>(in, out, plan.N, Ns, tw, sign, lane);
+    else if (R == 3) stockham_pass<3>(in, out, plan.N, Ns, tw, sign, lane);
+    else stockham_pass_generic(in, out, plan.N, Ns, R, tw, sign, lane);
     __syncthreads();
     Ns *= R;
     float2* t = in;
@@ -140,189 +141,325 @@ __device__ __forceinline__ size_t waist_index(int c, int ldw, int JHP, int m, in
 // ---------------------------------------------------------------------------
 // Analysis: grid rows -> folded, weighted waist.
 // ---------------------------------------------------------------------------
-// LDS per wavefront: two N-point buffers, plus a third holding the u
-// spectrum while v is transformed (vector jobs).
+// A block walks `gpb` consecutive row groups of one field; each wavefront
+// fetches its next mirror pair's rows into registers (PF floats per lane and
+// row, PF * 64 >= N) while it transforms the current pair.  PF = 0 reads the
+// rows straight into LDS instead.  LDS: the N twiddles, then per wavefront
+// two N-point buffers plus a third holding the u spectrum while v is
+// transformed (vector jobs).
+template <int PF>
 __global__ void __launch_bounds__(kFftThreads) rfft_fold_kernel(FoldArgs a) {
   extern __shared__ float2 lds[];
   const int N = a.fft.N;
   const int T1 = a.T + 1;
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
-  const int group = blockIdx.x % a.ngroups;
-  const int item = blockIdx.x / a.ngroups;  // field (scalar) or level (vector)
+  const int chunk = blockIdx.x % a.nchunks;
+  const int item = blockIdx.x / a.nchunks;  // field (scalar) or level (vector)
   if (item >= a.job.nfields) return;        // block-uniform
   const int nsrc = a.job.vector ? 2 : 1;
   const int nbuf = a.job.vector ? 3 : 2;
-  float2* base = lds + (size_t)w * nbuf * N;
-  const int i = group * a.rp + w;
-  const bool live = i < a.JH;
+  float2* tw = lds;
+  float2* base = lds + N + (size_t)w * nbuf * N;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
   const size_t plane = (size_t)a.nlat * N;
-  const float s = live ? a.job.rowscale[i] : 0.f;
-
-  float2* Zsrc[2] = {nullptr, nullptr};
-  for (int src = 0; src < nsrc; ++src) {
-    float2* b0 = (src == 0) ? base : base + N;
-    float2* b1 = (src == 0) ? base + N : base + 2 * N;
-    if (live) {
-      const float* g = (src == 0 ? a.job.grid0 : a.job.grid1) + (size_t)item * plane;
-      const float* rn = g + (size_t)(a.nlat - 1 - i) * N;
-      const float* rs = g + (size_t)i * N;
-      for (int k = lane; k < N; k += kWave) b0[k] = make_float2(rn[k], rs[k]);
-    }
-    __syncthreads();
-    float2* Z = wave_fft(b0, b1, a.fft, -1, lane);
-    if (src == 0 && nsrc == 2 && Z != base) {
-      // keep the u spectrum in buffer 0 while v is transformed in 1 and 2
-      for (int k = lane; k < N; k += kWave) base[k] = Z[k];
-      Z = base;
-      __syncthreads();
-    }
-    Zsrc[src] = Z;
-  }
-  // Fold and write: E = (X_N + X_S) s, O = (X_N - X_S) s for m = 0..T,
-  // zeros for the padding rows and the padding orders of the last m-tile.
+  const float* gsrc[2] = {a.job.grid0 + (size_t)item * plane,
+                          a.job.vector ? a.job.grid1 + (size_t)item * plane : nullptr};
+  const int g0 = chunk * a.gpb;
+  const int g1 = min(g0 + a.gpb, a.ngroups);
   const int Tpad = ((T1 + kMT - 1) / kMT) * kMT;
-  for (int m = lane; m < Tpad; m += kWave) {
-    float2 e[2], o[2];
+
+  float rn[2][PF > 0 ? PF : 1], rs[2][PF > 0 ? PF : 1];
+  auto fetch = [&](int g) {
+    if constexpr (PF > 0) {
+      const int i = g * a.rp + w;
+      if (i >= a.JH) return;
 #pragma unroll
-    for (int src = 0; src < 2; ++src) {
-      e[src] = make_float2(0.f, 0.f);
-      o[src] = make_float2(0.f, 0.f);
-      if (src < nsrc && live && m < T1) {
-        const float2* Z = Zsrc[src];
-        float2 zm = Z[m];
-        float2 zc = cconj(Z[(N - m) % N]);
-        float2 xn = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
-        float2 d = csub(zm, zc);
-        float2 xs = make_float2(0.5f * d.y, -0.5f * d.x);
-        e[src] = make_float2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
-        o[src] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
-      }
-    }
-    if (!a.job.vector) {
-      const int c = a.job.col0 + 2 * item;
-      a.job.waist0[waist_index(c, a.job.ldw, a.JHP, m, 0, i)] = e[0].x;
-      a.job.waist0[waist_index(c + 1, a.job.ldw, a.JHP, m, 0, i)] = e[0].y;
-      a.job.waist0[waist_index(c, a.job.ldw, a.JHP, m, 1, i)] = o[0].x;
-      a.job.waist0[waist_index(c + 1, a.job.ldw, a.JHP, m, 1, i)] = o[0].y;
-    } else {
-      // level columns: arrangement a = [-V.im, V.re, -U.im, U.re],
-      //                arrangement b = [U.re, U.im, -V.re, -V.im]   (fold p)
-      const int c = a.job.col0 + 4 * item;
+      for (int src = 0; src < 2; ++src) {
+        if (src >= nsrc) break;
+        const float* n_ = gsrc[src] + (size_t)(a.nlat - 1 - i) * N;
+        const float* s_ = gsrc[src] + (size_t)i * N;
 #pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const float2 U = p == 0 ? e[0] : o[0];
-        const float2 V = p == 0 ? e[1] : o[1];
-        const float va[4] = {-V.y, V.x, -U.y, U.x};
-        const float vb[4] = {U.x, U.y, -V.x, -V.y};
-#pragma unroll
-        for (int cl = 0; cl < 4; ++cl) {
-          a.job.waist0[waist_index(c + cl, a.job.ldw, a.JHP, m, p, i)] = va[cl];
-          a.job.waist1[waist_index(c + cl, a.job.ldw, a.JHP, m, p, i)] = vb[cl];
+        for (int j = 0; j < PF; ++j) {
+          const int k = lane + kWave * j;
+          rn[src][j] = k < N ? n_[k] : 0.f;
+          rs[src][j] = k < N ? s_[k] : 0.f;
         }
       }
     }
+  };
+  auto put = [&](int g, int src, float2* b) {
+    const int i = g * a.rp + w;
+    if (i >= a.JH) return;
+    if constexpr (PF > 0) {
+#pragma unroll
+      for (int j = 0; j < PF; ++j) {
+        const int k = lane + kWave * j;
+        if (k < N) b[k] = make_float2(rn[src][j], rs[src][j]);
+      }
+    } else {
+      const float* n_ = gsrc[src] + (size_t)(a.nlat - 1 - i) * N;
+      const float* s_ = gsrc[src] + (size_t)i * N;
+      for (int k = lane; k < N; k += kWave) b[k] = make_float2(n_[k], s_[k]);
+    }
+  };
+
+  fetch(g0);
+  for (int g = g0; g < g1; ++g) {
+    const int i = g * a.rp + w;
+    const bool live = i < a.JH;
+    const float s = live ? a.job.rowscale[i] : 0.f;
+    put(g, 0, base);
+    float vreg[PF > 0 ? PF : 1], vsreg[PF > 0 ? PF : 1];  // v rows wait in registers
+    if constexpr (PF > 0) {
+#pragma unroll
+      for (int j = 0; j < PF; ++j) {
+        vreg[j] = rn[1][j];
+        vsreg[j] = rs[1][j];
+      }
+    }
+    if (g + 1 < g1) fetch(g + 1);  // next pair's rows load during these FFTs
+    __syncthreads();
+    float2* Zsrc[2] = {nullptr, nullptr};
+    float2* Z = wave_fft(base, base + N, a.fft, tw, -1, lane);
+    Zsrc[0] = Z;
+    if (nsrc == 2) {
+      if (Z != base) {
+        // keep the u spectrum in buffer 0 while v is transformed in 1 and 2
+        for (int k = lane; k < N; k += kWave) base[k] = Z[k];
+        Zsrc[0] = base;
+      }
+      if (live) {
+        if constexpr (PF > 0) {
+#pragma unroll
+          for (int j = 0; j < PF; ++j) {
+            const int k = lane + kWave * j;
+            if (k < N) base[N + k] = make_float2(vreg[j], vsreg[j]);
+          }
+        } else {
+          put(g, 1, base + N);
+        }
+      }
+      __syncthreads();
+      Zsrc[1] = wave_fft(base + N, base + 2 * N, a.fft, tw, -1, lane);
+    }
+    // Fold and write: E = (X_N + X_S) s, O = (X_N - X_S) s for m = 0..T,
+    // zeros for the padding rows and the padding orders of the last m-tile.
+    for (int m = lane; m < Tpad; m += kWave) {
+      float2 e[2], o[2];
+#pragma unroll
+      for (int src = 0; src < 2; ++src) {
+        e[src] = make_float2(0.f, 0.f);
+        o[src] = make_float2(0.f, 0.f);
+        if (src < nsrc && live && m < T1) {
+          const float2* Zs = Zsrc[src];
+          float2 zm = Zs[m];
+          float2 zc = cconj(Zs[(N - m) % N]);
+          float2 xn = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
+          float2 d = csub(zm, zc);
+          float2 xs = make_float2(0.5f * d.y, -0.5f * d.x);
+          e[src] = make_float2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
+          o[src] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
+        }
+      }
+      if (!a.job.vector) {
+        const int c = a.job.col0 + 2 * item;
+        a.job.waist0[waist_index(c, a.job.ldw, a.JHP, m, 0, i)] = e[0].x;
+        a.job.waist0[waist_index(c + 1, a.job.ldw, a.JHP, m, 0, i)] = e[0].y;
+        a.job.waist0[waist_index(c, a.job.ldw, a.JHP, m, 1, i)] = o[0].x;
+        a.job.waist0[waist_index(c + 1, a.job.ldw, a.JHP, m, 1, i)] = o[0].y;
+      } else {
+        // level columns: arrangement a = [-V.im, V.re, -U.im, U.re],
+        //                arrangement b = [U.re, U.im, -V.re, -V.im]   (fold p)
+        const int c = a.job.col0 + 4 * item;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const float2 U = p == 0 ? e[0] : o[0];
+          const float2 V = p == 0 ? e[1] : o[1];
+          const float va[4] = {-V.y, V.x, -U.y, U.x};
+          const float vb[4] = {U.x, U.y, -V.x, -V.y};
+#pragma unroll
+          for (int cl = 0; cl < 4; ++cl) {
+            a.job.waist0[waist_index(c + cl, a.job.ldw, a.JHP, m, p, i)] = va[cl];
+            a.job.waist1[waist_index(c + cl, a.job.ldw, a.JHP, m, p, i)] = vb[cl];
+          }
+        }
+      }
+    }
+    __syncthreads();  // buffers are rewritten by the next group
   }
 }
 
 // ---------------------------------------------------------------------------
 // Synthesis: folded waist -> grid rows.
 // ---------------------------------------------------------------------------
+// A block walks `gpb` consecutive row groups of one field; each wavefront
+// fetches its next row's E/O values (PF per lane and quantity) while it
+// transforms the current one.  LDS: the N twiddles, then two N-point buffers
+// per wavefront.
+template <int PF>
 __global__ void __launch_bounds__(kFftThreads) irfft_unfold_kernel(UnfoldArgs a) {
   extern __shared__ float2 lds[];
   const int N = a.fft.N;
   const int T1 = a.T + 1;
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
-  const int group = blockIdx.x % a.ngroups;
-  const int f = blockIdx.x / a.ngroups;
+  const int chunk = blockIdx.x % a.nchunks;
+  const int f = blockIdx.x / a.nchunks;
   if (f >= a.job.nfields) return;  // block-uniform
-  float2* b0 = lds + (size_t)w * 2 * N;
+  float2* tw = lds;
+  float2* b0 = lds + N + (size_t)w * 2 * N;
   float2* b1 = b0 + N;
-  const int i = group * a.rp + w;
-  const bool live = i < a.JH;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
   const int c = a.job.col0 + a.job.fstride * f;
-  if (live) {
-    const float s = a.job.rowscale[i];
-    // Z[k]: A = north spectrum (E + O), B = south spectrum (E - O), m <= T
-    for (int k = lane; k < N; k += kWave) {
-      int m = k;
-      bool lower = true;
-      if (k > N / 2) {
-        m = N - k;
-        lower = false;
+  const float* W = a.job.waist;
+  const int g0 = chunk * a.gpb;
+  const int g1 = min(g0 + a.gpb, a.ngroups);
+  // orders per lane, m = lane + 64 j < T+1 <= N/2: half the row depth
+  constexpr int MP = PF > 0 ? PF / 2 : 1;
+  float er[MP], ei[MP], orr[MP], oi[MP];
+  auto fetch = [&](int g) {
+    if constexpr (PF > 0) {
+      const int i = g * a.rp + w;
+      if (i >= a.JH) return;
+#pragma unroll
+      for (int j = 0; j < MP; ++j) {
+        const int m = lane + kWave * j;
+        const bool ok = m < T1;
+        er[j] = ok ? W[waist_index(c, a.job.ldw, a.JHP, m, 0, i)] : 0.f;
+        ei[j] = ok ? W[waist_index(c + 1, a.job.ldw, a.JHP, m, 0, i)] : 0.f;
+        orr[j] = ok ? W[waist_index(c, a.job.ldw, a.JHP, m, 1, i)] : 0.f;
+        oi[j] = ok ? W[waist_index(c + 1, a.job.ldw, a.JHP, m, 1, i)] : 0.f;
       }
-      float2 z = make_float2(0.f, 0.f);
-      if (m < T1) {
-        const float* W = a.job.waist;
-        float2 E = make_float2(W[waist_index(c, a.job.ldw, a.JHP, m, 0, i)],
-                               W[waist_index(c + 1, a.job.ldw, a.JHP, m, 0, i)]);
-        float2 O = make_float2(W[waist_index(c, a.job.ldw, a.JHP, m, 1, i)],
-                               W[waist_index(c + 1, a.job.ldw, a.JHP, m, 1, i)]);
-        float2 A = make_float2((E.x + O.x) * s, (E.y + O.y) * s);
-        float2 B = make_float2((E.x - O.x) * s, (E.y - O.y) * s);
-        if (a.job.imul) {
-          A = make_float2(-A.y, A.x);
-          B = make_float2(-B.y, B.x);
+    }
+  };
+  // spectrum value Z at order m from E, O
+  auto zval = [&](float2 E, float2 O, float s, int m, bool lower) {
+    float2 A = make_float2((E.x + O.x) * s, (E.y + O.y) * s);
+    float2 B = make_float2((E.x - O.x) * s, (E.y - O.y) * s);
+    if (a.job.imul) {
+      A = make_float2(-A.y, A.x);
+      B = make_float2(-B.y, B.x);
+    }
+    if (m == 0) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    return lower ? make_float2(A.x - B.y, A.y + B.x)     // A + iB
+                 : make_float2(A.x + B.y, -A.y + B.x);   // conj(A) + i conj(B)
+  };
+
+  fetch(g0);
+  for (int g = g0; g < g1; ++g) {
+    const int i = g * a.rp + w;
+    const bool live = i < a.JH;
+    if (live) {
+      const float s = a.job.rowscale[i];
+      // Z[k] = A + iB at k = m <= N/2, conj(A) + i conj(B) at k = N - m; zero above T
+      for (int k = lane; k < N; k += kWave) b0[k] = make_float2(0.f, 0.f);
+      if constexpr (PF > 0) {
+#pragma unroll
+        for (int j = 0; j < MP; ++j) {
+          const int m = lane + kWave * j;
+          if (m < T1) {
+            const float2 E = make_float2(er[j], ei[j]), O = make_float2(orr[j], oi[j]);
+            b0[m] = zval(E, O, s, m, true);
+            if (m > 0) b0[N - m] = zval(E, O, s, m, false);
+          }
         }
-        if (m == 0) {
-          A.y = 0.f;
-          B.y = 0.f;
+      } else {
+        for (int m = lane; m < T1; m += kWave) {
+          const float2 E = make_float2(W[waist_index(c, a.job.ldw, a.JHP, m, 0, i)],
+                                       W[waist_index(c + 1, a.job.ldw, a.JHP, m, 0, i)]);
+          const float2 O = make_float2(W[waist_index(c, a.job.ldw, a.JHP, m, 1, i)],
+                                       W[waist_index(c + 1, a.job.ldw, a.JHP, m, 1, i)]);
+          b0[m] = zval(E, O, s, m, true);
+          if (m > 0) b0[N - m] = zval(E, O, s, m, false);
         }
-        z = lower ? make_float2(A.x - B.y, A.y + B.x)     // A + iB
-                  : make_float2(A.x + B.y, -A.y + B.x);   // conj(A) + i conj(B)
       }
-      b0[k] = z;
     }
-  }
-  __syncthreads();
-  float2* x = wave_fft(b0, b1, a.fft, +1, lane);
-  if (live) {
-    const size_t plane = (size_t)a.nlat * N;
-    float* rn = a.job.grid + (size_t)f * plane + (size_t)(a.nlat - 1 - i) * N;
-    float* rs = a.job.grid + (size_t)f * plane + (size_t)i * N;
-    for (int k = lane; k < N; k += kWave) {
-      float2 v = x[k];
-      rn[k] = v.x;
-      rs[k] = v.y;
+    if (g + 1 < g1) fetch(g + 1);  // next row's E/O load during this FFT
+    __syncthreads();
+    float2* x = wave_fft(b0, b1, a.fft, tw, +1, lane);
+    if (live) {
+      const size_t plane = (size_t)a.nlat * N;
+      float* rn = a.job.grid + (size_t)f * plane + (size_t)(a.nlat - 1 - i) * N;
+      float* rs = a.job.grid + (size_t)f * plane + (size_t)i * N;
+      for (int k = lane; k < N; k += kWave) {
+        float2 v = x[k];
+        rn[k] = v.x;
+        rs[k] = v.y;
+      }
     }
+    __syncthreads();  // buffers are rewritten by the next group
   }
 }
 
 size_t fold_lds_bytes(int N, int T, int vector, int rp) {
   (void)T;
-  return (size_t)rp * (vector ? 3 : 2) * N * sizeof(float2);
+  return (size_t)(1 + rp * (vector ? 3 : 2)) * N * sizeof(float2);
 }
 size_t unfold_lds_bytes(int N, int T, int rp) {
   (void)T;
-  return (size_t)rp * 2 * N * sizeof(float2);
+  return (size_t)(1 + rp * 2) * N * sizeof(float2);
+}
+
+// Register prefetch depth per lane: PF * 64 >= N, or 0 (no prefetch) above 1536.
+int fft_pf(int N) {
+  const int need = (N + kWave - 1) / kWave;
+  const int opts[] = {2, 4, 8, 12, 16, 24};
+  for (int o : opts)
+    if (need <= o) return o;
+  return 0;
 }
 
 }  // namespace sht
 
+#define SHT_PF_DISPATCH(KERNEL, PF, ...)                                 \
+  switch (PF) {                                                          \
+    case 2: hipLaunchKernelGGL(sht::KERNEL<2>, __VA_ARGS__); break;      \
+    case 4: hipLaunchKernelGGL(sht::KERNEL<4>, __VA_ARGS__); break;      \
+    case 8: hipLaunchKernelGGL(sht::KERNEL<8>, __VA_ARGS__); break;      \
+    case 12: hipLaunchKernelGGL(sht::KERNEL<12>, __VA_ARGS__); break;    \
+    case 16: hipLaunchKernelGGL(sht::KERNEL<16>, __VA_ARGS__); break;    \
+    case 24: hipLaunchKernelGGL(sht::KERNEL<24>, __VA_ARGS__); break;    \
+    default: hipLaunchKernelGGL(sht::KERNEL<0>, __VA_ARGS__); break;     \
+  }
+
 extern "C" hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s) {
   size_t lds = sht::fold_lds_bytes(a->fft.N, a->T, a->job.vector, a->rp);
-  int blocks = a->ngroups * a->job.nfields;
+  int blocks = a->nchunks * a->job.nfields;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(sht::rfft_fold_kernel, dim3(blocks), dim3(sht::kWave * a->rp), lds, s, *a);
+  SHT_PF_DISPATCH(rfft_fold_kernel, sht::fft_pf(a->fft.N), dim3(blocks), dim3(sht::kWave * a->rp),
+                  lds, s, *a);
   return hipGetLastError();
 }
 
 extern "C" hipError_t sht_launch_irfft_unfold(const sht::UnfoldArgs* a, hipStream_t s) {
   size_t lds = sht::unfold_lds_bytes(a->fft.N, a->T, a->rp);
-  int blocks = a->ngroups * a->job.nfields;
+  int blocks = a->nchunks * a->job.nfields;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(sht::irfft_unfold_kernel, dim3(blocks), dim3(sht::kWave * a->rp), lds, s,
-                     *a);
+  SHT_PF_DISPATCH(irfft_unfold_kernel, sht::fft_pf(a->fft.N), dim3(blocks),
+                  dim3(sht::kWave * a->rp), lds, s, *a);
   return hipGetLastError();
 }
 
-extern "C" hipError_t sht_fft_allow_lds(size_t bytes) {
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(sht::rfft_fold_kernel),
+template <int PF>
+static hipError_t allow_lds_one(size_t bytes) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(sht::rfft_fold_kernel<PF>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(sht::irfft_unfold_kernel),
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(sht::irfft_unfold_kernel<PF>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+extern "C" hipError_t sht_fft_allow_lds(size_t bytes) {
+  hipError_t e = hipSuccess;
+  for (hipError_t r : {allow_lds_one<0>(bytes), allow_lds_one<2>(bytes), allow_lds_one<4>(bytes),
+                       allow_lds_one<8>(bytes), allow_lds_one<12>(bytes), allow_lds_one<16>(bytes),
+                       allow_lds_one<24>(bytes)})
+    if (r != hipSuccess) e = r;
+  return e;
 }
 
 extern "C" size_t sht_fold_lds_bytes(int N, int T, int vector, int rp) {

@@ -17,9 +17,11 @@
 // one barrier per step.
 //
 // SYN (synthesis): rows = folded latitudes, k = degrees (packed, contiguous
-//   per order), C = waist W[c][m/8][p][i][m%8].  Steps run order by order;
-//   an order's tile is stored when its last step is done.  Block tile 64 x 64,
-//   each wavefront 32 x 32 (2 x 2 MFMA 16x16 tiles).
+//   per order), C = waist W[c][m/8][p][i][m%8].  The loop runs k outer,
+//   order inner, with eight accumulator sets, so the finished tile is stored
+//   as 4 latitudes x 8 orders = 128 contiguous bytes per lane and column
+//   (storing each order separately writes every waist line eight times).
+//   Block tile 64 x 32, each wavefront 32 x 16.
 // ANA (analysis): rows = degrees, k = folded latitudes from the waist, whose
 //   m-tile layout puts eight orders' values for one latitude side by side:
 //   one 32-latitude B tile (1 KB per column) serves all eight orders, so the
@@ -50,7 +52,7 @@ constexpr int kPadH = 8;      // LDS row padding, bf16
 constexpr int kLdF = kBK + kPadF;  // fp32 LDS row stride
 constexpr int kLdH = kBK + kPadH;  // bf16 LDS row stride
 
-constexpr int kSynBM = 64, kSynBN = 64;
+constexpr int kSynBM = 64, kSynBN = 32;
 constexpr int kAnaBM = 64, kAnaBN = 32;
 
 struct GemmWork {
@@ -196,6 +198,10 @@ __device__ __forceinline__ void mfma_tile(f32x4& acc, f32x4& acc2, const float (
 // ---------------------------------------------------------------------------
 // Synthesis kernel
 // ---------------------------------------------------------------------------
+// k outer, order inner, eight accumulator sets: when the tile is done each
+// lane holds 4 latitudes x 8 orders of one column, which the waist layout
+// stores as 128 contiguous bytes.  Orders whose packed block is shorter than
+// the current k skip the step.
 template <int MODE>
 __global__ void __launch_bounds__(256) legendre_syn_kernel(GemmWork w) {
   constexpr int BM = kSynBM, BN = kSynBN;
@@ -211,109 +217,84 @@ __global__ void __launch_bounds__(256) legendre_syn_kernel(GemmWork w) {
 
   for (int t = blockIdx.x; t < w.ntiles; t += gridDim.x) {
     const int4 tile = w.tiles[t];
+    if (tile.x < 0) continue;  // idle padding of the XCD-interleaved list
     const GemmGroup& G = w.groups[tile.x];
     const int r0 = tile.y, c0 = tile.z;
-    // step cursor: (mm, sg, k0); every order has K >= 32 for every segment
-    int mm = 0, sg = 0, k0 = 0;
-    // advance to the next step; returns false past the end
-    auto advance = [&](int& m_, int& s_, int& k_) {
-      k_ += kBK;
-      while (m_ < G.mcount && k_ >= G.K[s_][m_]) {
-        k_ = 0;
-        if (++s_ >= G.nseg) {
-          s_ = 0;
-          ++m_;
-        }
-      }
-      return m_ < G.mcount;
-    };
-
-    AT at;
-    // B staging: 64 cols x 32 k = 256 runs of 8 floats, one per thread
-    f32x4 bv[2];
-    auto load_b = [&](int m_, int s_, int k_) {
-      const int col = threadIdx.x >> 2;
-      const int kq = (threadIdx.x & 3) * 8;
-      bv[0] = f32x4{0.f, 0.f, 0.f, 0.f};
-      bv[1] = f32x4{0.f, 0.f, 0.f, 0.f};
-      if (c0 + col < G.N) {
-        const f32x4* p = reinterpret_cast<const f32x4*>(
-            G.B[s_][m_] + (long long)(c0 + col) * G.bcs[s_] + k_ + kq);
-        bv[0] = p[0];
-        bv[1] = p[1];
-      }
+    // B staging: 32 cols x 32 k = 256 runs of 4 floats, one per thread
+    f32x4 bv;
+    auto load_b = [&](int s_, int m_, int k_) {
+      const int col = threadIdx.x >> 3;
+      const int kq = (threadIdx.x & 7) * 4;
+      bv = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (c0 + col < G.N)
+        bv = *reinterpret_cast<const f32x4*>(G.B[s_][m_] + (long long)(c0 + col) * G.bcs[s_] +
+                                             k_ + kq);
     };
     auto store_b = [&](float* sb) {
-      const int col = threadIdx.x >> 2;
-      const int kq = (threadIdx.x & 3) * 8;
-      f32x4* d = reinterpret_cast<f32x4*>(sb + col * kLdF + kq);
-      d[0] = bv[0];
-      d[1] = bv[1];
+      const int col = threadIdx.x >> 3;
+      const int kq = (threadIdx.x & 7) * 4;
+      *reinterpret_cast<f32x4*>(sb + col * kLdF + kq) = bv;
     };
+    // step cursor (sg, k0, z): the orders z with K[sg][z] > k0, k0 < Kmax[sg]
+    auto first_z = [&](int s_, int k_, int z_) {
+      while (z_ < G.mcount && k_ >= G.K[s_][z_]) ++z_;
+      return z_;
+    };
+    auto advance = [&](int& s_, int& k_, int& z_) {
+      z_ = first_z(s_, k_, z_ + 1);
+      while (z_ >= G.mcount) {
+        k_ += kBK;
+        if (k_ >= G.Kmax[s_]) {
+          k_ = 0;
+          if (++s_ >= G.nseg) return false;
+        }
+        z_ = first_z(s_, k_, 0);
+      }
+      return true;
+    };
+    int sg = 0, k0 = 0, z = first_z(0, 0, 0);
+    AT at;
 
-    // prologue
-    at.load(G.A[sg][mm], G.lda[sg][mm], G.piece, r0, G.R[mm], k0);
-    load_b(mm, sg, k0);
-    at.store(sAf[0], sAh[0]);
-    store_b(sB[0]);
-    __syncthreads();
-
-    f32x4 acc[2][2], acc2[2][2];
+    f32x4 acc[kMT][2], acc2[kMT][2];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < kMT; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
         acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
         acc2[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
       }
 
+    // prologue
+    at.load(G.A[sg][z], G.lda[sg][z], G.piece, r0, G.R[z], k0);
+    load_b(sg, z, k0);
+    at.store(sAf[0], sAh[0]);
+    store_b(sB[0]);
+    __syncthreads();
+
     int buf = 0;
     bool more = true;
     while (more) {
-      int nm = mm, ns = sg, nk = k0;
-      const bool has_next = advance(nm, ns, nk);
+      int ns = sg, nk = k0, nz = z;
+      const bool has_next = advance(ns, nk, nz);
       if (has_next) {
-        at.load(G.A[ns][nm], G.lda[ns][nm], G.piece, r0, G.R[nm], nk);
-        load_b(nm, ns, nk);
+        at.load(G.A[ns][nz], G.lda[ns][nz], G.piece, r0, G.R[nz], nk);
+        load_b(ns, nz, nk);
       }
-      // compute current step
       {
         const float* sa = sAf[buf];
         const unsigned short* sh = sAh[buf];
-        const float* sb = sB[buf];
-        float ax[2][8], bx[2][8];
-        u16x8 ap[2][3], bp[2][3];
+        float ax[2][8], bx[8];
+        u16x8 ap[2][3], bp[3];
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb) read_a<MODE, BM>(sa, sh, wr * 32 + rb * 16 + li, kg, ax[rb], ap[rb]);
+        read_b(sB[buf], wc * 16 + li, kg, bx);
+        if constexpr (MODE != kFP32) split_pieces<NP>(bx, bp);
 #pragma unroll
-        for (int cb = 0; cb < 2; ++cb) {
-          read_b(sb, wc * 32 + cb * 16 + li, kg, bx[cb]);
-          if constexpr (MODE != kFP32) split_pieces<NP>(bx[cb], bp[cb]);
+        for (int zz = 0; zz < kMT; ++zz) {
+          if (zz != z) continue;
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb) mfma_tile<MODE>(acc[zz][rb], acc2[zz][rb], ax[rb], ap[rb], bx, bp);
         }
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-          for (int cb = 0; cb < 2; ++cb)
-            mfma_tile<MODE>(acc[rb][cb], acc2[rb][cb], ax[rb], ap[rb], bx[cb], bp[cb]);
-      }
-      // order finished: store its tile into the waist and restart the accumulators
-      if (!has_next || nm != mm) {
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-          for (int cb = 0; cb < 2; ++cb) {
-            const int col = c0 + wc * 32 + cb * 16 + li;
-            const int row = r0 + wr * 32 + rb * 16 + 4 * kg;
-            f32x4 v = acc[rb][cb];
-            if constexpr (MODE != kFP32) v = v + acc2[rb][cb];
-            if (col < G.N && row < G.R[mm]) {
-              float* cp = G.C[0] + (long long)col * G.ccs + (long long)row * kMT + mm;
-#pragma unroll
-              for (int q = 0; q < 4; ++q) cp[q * kMT] = v[q];
-            }
-            acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-            acc2[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-          }
       }
       if (has_next) {
         at.store(sAf[buf ^ 1], sAh[buf ^ 1]);
@@ -321,18 +302,31 @@ __global__ void __launch_bounds__(256) legendre_syn_kernel(GemmWork w) {
       }
       __syncthreads();
       buf ^= 1;
-      mm = nm;
       sg = ns;
       k0 = nk;
+      z = nz;
       more = has_next;
     }
-    // orders of the m-tile beyond T+1 (last tile only): zero their waist slots
-    if (G.mcount < kMT) {
-      for (int e = threadIdx.x; e < BM * BN; e += 256) {
-        const int col = c0 + e / BM, row = r0 + e % BM;
-        if (col < G.N && row < G.Rmax)
-          for (int z = G.mcount; z < kMT; ++z)
-            G.C[0][(long long)col * G.ccs + (long long)row * kMT + z] = 0.f;
+    // Store: rows row..row+3 x orders 0..7 are 32 contiguous floats of the
+    // waist (orders beyond T+1 in the last tile are written as zeros).
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      const int col = c0 + wc * 16 + li;
+      const int row = r0 + wr * 32 + rb * 16 + 4 * kg;
+      if (col >= G.N || row >= G.Rmax) continue;
+      float* cp = G.C[0] + (long long)col * G.ccs + (long long)row * kMT;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float o[kMT];
+#pragma unroll
+        for (int zz = 0; zz < kMT; ++zz) {
+          f32x4 v = acc[zz][rb];
+          if constexpr (MODE != kFP32) v = v + acc2[zz][rb];
+          o[zz] = v[q];
+        }
+        f32x4* d = reinterpret_cast<f32x4*>(cp + q * kMT);
+        d[0] = f32x4{o[0], o[1], o[2], o[3]};
+        d[1] = f32x4{o[4], o[5], o[6], o[7]};
       }
     }
   }
@@ -356,6 +350,7 @@ __global__ void __launch_bounds__(256) legendre_ana_kernel(GemmWork w) {
 
   for (int t = blockIdx.x; t < w.ntiles; t += gridDim.x) {
     const int4 tile = w.tiles[t];
+    if (tile.x < 0) continue;  // idle padding of the XCD-interleaved list
     const GemmGroup& G = w.groups[tile.x];
     const int r0 = tile.y, c0 = tile.z;
     // orders whose rows reach this tile

@@ -44,8 +44,10 @@ struct FoldJob {
 struct FoldArgs {
   FftPlan fft;
   int T, nlat, JH, JHP;
-  int rp;                     // folded rows per block (1, 2 or 4)
+  int rp;                     // folded rows (wavefronts) per block (1, 2 or 4)
   int ngroups;                // folded-row groups covering JHP rows
+  int gpb;                    // groups per block
+  int nchunks;                // ceil(ngroups / gpb)
   FoldJob job;
 };
 
@@ -65,6 +67,8 @@ struct UnfoldArgs {
   int T, nlat, JH, JHP;
   int rp;
   int ngroups;                // ceil(JH / rp)
+  int gpb;                    // groups per block
+  int nchunks;                // ceil(ngroups / gpb)
   UnfoldJob job;
 };
 
