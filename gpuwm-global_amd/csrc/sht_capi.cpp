@@ -290,7 +290,7 @@ struct arwen_sht_batch {
   std::vector<sht::UnfoldArgs> unfolds;
   std::vector<sht::FoldArgs> folds;
   std::vector<sht::UnpackArgs> unpacks;
-  DevBuf syn_probs, syn_tiles, ana_probs, ana_tiles;
+  DevBuf syn_panels, syn_steps, syn_tiles, ana_panels, ana_steps, ana_tiles;
   int syn_ntiles = 0, ana_ntiles = 0;
   int gemm_blocks = 0;  // 0 = auto
   double legendre_flops = 0, fft_bytes = 0;
@@ -312,76 +312,144 @@ bool is_synthesis(int op) { return op >= ARWEN_SHT_OP_SYNTHESIS && op <= ARWEN_S
 
 constexpr int kXcds = 8;  // MI355X: 8 XCDs x 32 CUs
 
-// Collects the grouped-GEMM work list of one direction.
+// One work group of the grouped GEMM before it is cut into panels: up to kMT
+// consecutive orders of one job and one output parity, with one or two
+// segments (table kind, data) accumulated into the same output.
+struct HostGroup {
+  int nseg = 0, N = 0, mcount = 0;
+  const char* A[2][sht::kMT] = {};  // table block of (seg, order), tiled layout
+  int K[2][sht::kMT] = {};          // k extent: SYN degrees KP(m, p); ANA JHP
+  int Arows[2][sht::kMT] = {};      // rows of the table block: SYN JHP; ANA KP(m, q)
+  const char* B[2][sht::kMT] = {};  // SYN packed block of (seg, order); ANA waist block of seg
+  float* C[sht::kMT] = {};          // SYN C[0] = waist block; ANA packed block of the order
+  int R[sht::kMT] = {};             // output rows per order (SYN JHP; ANA KP(m, q))
+  int bcs = 0, ccs = 0;             // column strides (floats)
+  long long piece_bytes = 0;
+  size_t esz = 4;                   // table element bytes
+};
+
+// Collects the grouped-GEMM work list of one direction: panels (group x row
+// tile), their k-step lists and the (panel, column tile) work items.
 struct GroupBuilder {
   int dir = 0;  // 0 synthesis, 1 analysis
-  std::vector<sht::GemmGroup> groups;
-  std::vector<std::pair<long long, int4>> tiles;  // (work, tile)
-  void add(sht::GemmGroup g) {
-    g.Rmax = 0;
-    long long kwork = 0;
-    for (int sg = 0; sg < 2; ++sg) g.Kmax[sg] = 0;
-    for (int mm = 0; mm < g.mcount; ++mm) {
-      g.Rmax = std::max(g.Rmax, g.R[mm]);
-      for (int sg = 0; sg < g.nseg; ++sg) {
-        g.Kmax[sg] = std::max(g.Kmax[sg], g.K[sg][mm]);
-        kwork += (long long)g.K[sg][mm] * g.R[mm];
-      }
-    }
-    const int id = (int)groups.size();
-    groups.push_back(g);
+  std::vector<sht::GemmPanel> panels;
+  std::vector<sht::GemmStep> steps;
+  struct Item { long long work; int group; int2 tile; };
+  std::vector<Item> items;
+  int ngroups = 0;
+
+  void add(const HostGroup& g) {
     const int TR = sht_gemm_tile_rows(dir), TC = sht_gemm_tile_cols(dir);
-    for (int r0 = 0; r0 < g.Rmax; r0 += TR)
+    const int gid = ngroups++;
+    int Rmax = 0;
+    for (int z = 0; z < g.mcount; ++z) Rmax = std::max(Rmax, g.R[z]);
+    for (int r0 = 0; r0 < Rmax; r0 += TR) {
+      sht::GemmPanel pn{};
+      pn.step0 = (int)steps.size();
+      pn.N = g.N;
+      pn.bcs = g.bcs;
+      pn.ccs = g.ccs;
+      pn.piece_bytes = g.piece_bytes;
+      for (int z = 0; z < sht::kMT; ++z) {
+        pn.rows[z] = z < g.mcount ? std::max(0, std::min(TR, g.R[z] - r0)) : 0;
+        if (dir == 0)
+          pn.c[z] = z == 0 ? (unsigned long long)(g.C[0] + (size_t)r0 * sht::kMT) : 0ull;
+        else
+          pn.c[z] = z < g.mcount ? (unsigned long long)(g.C[z] + r0) : 0ull;
+      }
+      for (int sg = 0; sg < g.nseg; ++sg) {
+        int kmax = 0;
+        for (int z = 0; z < g.mcount; ++z) kmax = std::max(kmax, g.K[sg][z]);
+        for (int k0 = 0; k0 < kmax; k0 += 32) {
+          bool first = true;
+          for (int z = 0; z < g.mcount; ++z) {
+            if (k0 >= g.K[sg][z]) continue;
+            if (dir == 1 && g.Arows[sg][z] <= r0) continue;  // order ends above this tile
+            sht::GemmStep st{};
+            const int chunk_rows = g.Arows[sg][z];
+            st.a = (unsigned long long)(g.A[sg][z] +
+                                        ((size_t)k0 * chunk_rows + (size_t)r0 * 32) * g.esz);
+            st.a_bytes = (unsigned)(std::min(TR, chunk_rows - r0) * 32 * g.esz);
+            st.z = z;
+            if (dir == 0) {
+              st.b = (unsigned long long)(g.B[sg][z] + (size_t)k0 * 4);
+              st.b_klen = g.K[sg][z] - k0;
+            } else if (first) {
+              st.b = (unsigned long long)(g.B[sg][0] + (size_t)k0 * sht::kMT * 4);
+              st.b_klen = g.K[sg][z] - k0;
+            }
+            first = false;
+            steps.push_back(st);
+          }
+        }
+      }
+      pn.nsteps = (int)steps.size() - pn.step0;
+      if (pn.nsteps == 0) {  // nothing reaches this row tile
+        steps.resize(pn.step0);
+        continue;
+      }
+      // two null steps: the kernel's loads run two steps ahead
+      sht::GemmStep null{};
+      null.a = steps[pn.step0].a;
+      null.b = steps[pn.step0].b ? steps[pn.step0].b : steps[pn.step0].a;
+      steps.push_back(null);
+      steps.push_back(null);
+      const int pid = (int)panels.size();
+      panels.push_back(pn);
       for (int c0 = 0; c0 < g.N; c0 += TC)
-        tiles.push_back({kwork * std::min(TC, g.N - c0), int4{id, r0, c0, 0}});
+        items.push_back({(long long)pn.nsteps * std::min(TC, g.N - c0), gid, int2{pid, c0}});
+    }
   }
   // XCD-aware order.  Workgroups are dispatched round-robin over the 8 XCDs
   // and the kernels walk the list with a grid stride that is a multiple of 8,
-  // so list position t always runs on XCD t % 8.  Every tile of one group
-  // (same orders, same tables, same data columns) goes to one XCD, where its
-  // tiles run side by side and share that XCD's L2; groups are balanced over
-  // the XCDs longest first.  Short lists are padded with idle tiles (-1).
-  void finish(DevBuf& dgroups, DevBuf& dtiles, int* ntiles) {
+  // so list position t always runs on XCD t % 8.  Every work item of one
+  // group (same orders, same tables, same data columns) goes to one XCD,
+  // where its tiles run side by side and share that XCD's L2; groups are
+  // balanced over the XCDs longest first.  Short lists are padded with idle
+  // items (-1).
+  void finish(DevBuf& dpanels, DevBuf& dsteps, DevBuf& dtiles, int* ntiles) {
     const int nx = kXcds;
-    std::vector<long long> gwork(groups.size(), 0);
-    std::vector<std::vector<std::pair<long long, int4>>> per_group(groups.size());
-    for (auto& x : tiles) {
-      gwork[x.second.x] += x.first;
-      per_group[x.second.x].push_back(x);
+    std::vector<long long> gwork(ngroups, 0);
+    std::vector<std::vector<int2>> per_group(ngroups);
+    for (auto& x : items) {
+      gwork[x.group] += x.work;
+      per_group[x.group].push_back(x.tile);
     }
-    std::vector<int> order(groups.size());
-    for (size_t g = 0; g < order.size(); ++g) order[g] = (int)g;
+    std::vector<int> order(ngroups);
+    for (int g = 0; g < ngroups; ++g) order[g] = g;
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return gwork[a] > gwork[b]; });
     std::vector<long long> load(nx, 0);
-    std::vector<std::vector<int4>> lists(nx);
+    std::vector<std::vector<int2>> lists(nx);
     for (int g : order) {
+      if (per_group[g].empty()) continue;
       int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
       load[x] += gwork[g];
-      for (auto& tt : per_group[g]) lists[x].push_back(tt.second);
+      for (auto& tt : per_group[g]) lists[x].push_back(tt);
     }
     size_t longest = 0;
     for (auto& l : lists) longest = std::max(longest, l.size());
-    std::vector<int4> t;
+    std::vector<int2> t;
     t.reserve(longest * nx);
     for (size_t j = 0; j < longest; ++j)
-      for (int x = 0; x < nx; ++x)
-        t.push_back(j < lists[x].size() ? lists[x][j] : int4{-1, 0, 0, 0});
-    upload(dgroups, groups);
+      for (int x = 0; x < nx; ++x) t.push_back(j < lists[x].size() ? lists[x][j] : int2{-1, 0});
+    upload(dpanels, panels);
+    upload(dsteps, steps);
     upload(dtiles, t);
     *ntiles = (int)t.size();
   }
 };
 
-const void* table_block(const arwen_sht_plan* P, int kind, bool syn, int m, int p, int* lda) {
+const char* table_block(const arwen_sht_plan* P, int kind, bool syn, int m, int p,
+                        long long* piece_bytes, size_t* esz) {
   long long piece;
   const char* base = static_cast<const char*>(P->table(kind, syn, &piece));
-  const size_t esz = P->precision == ARWEN_SHT_FP32 ? 4 : 2;
-  *lda = syn ? P->kp(m, p) : P->JHP;
-  return base + (size_t)P->koff[2 * m + p] * P->JHP * esz;
+  *esz = P->precision == ARWEN_SHT_FP32 ? 4 : 2;
+  *piece_bytes = piece * (long long)*esz;
+  return base + (size_t)P->koff[2 * m + p] * P->JHP * *esz;
 }
 
-// Waist block (m-tile of m, parity p) of the layout W[c][m/8][p][i][m%8].
+// Waist block (m-tile of m, parity p) of the layout W[c][m/kMT][p][i][m%kMT].
 float* waist_block(const arwen_sht_plan* P, float* W, int m, int p) {
   return W + ((size_t)((m / sht::kMT) * 2 + p) * P->JHP) * sht::kMT;
 }
@@ -395,24 +463,21 @@ struct SynSeg {
 
 void add_syn_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int ncols,
                    float* waist, const SynSeg* segs, int nseg) {
-  sht::GemmGroup g{};
+  HostGroup g;
   g.nseg = nseg;
   g.N = ncols;
   g.mcount = std::min(sht::kMT, P->T + 1 - mb * sht::kMT);
-  g.piece = P->tab_elems;
   g.ccs = P->ldw;
+  g.bcs = P->ldp;
   g.C[0] = waist_block(P, waist, mb * sht::kMT, q);
-  for (int sg = 0; sg < nseg; ++sg) {
-    g.bks[sg] = 1;
-    g.bcs[sg] = P->ldp;
-  }
   for (int mm = 0; mm < g.mcount; ++mm) {
     const int m = mb * sht::kMT + mm;
     g.R[mm] = P->JHP;
     for (int sg = 0; sg < nseg; ++sg) {
       const int p = q ^ segs[sg].pflip;
-      g.A[sg][mm] = table_block(P, segs[sg].kind, true, m, p, &g.lda[sg][mm]);
-      g.B[sg][mm] = segs[sg].packed + P->koff[2 * m + p];
+      g.A[sg][mm] = table_block(P, segs[sg].kind, true, m, p, &g.piece_bytes, &g.esz);
+      g.Arows[sg][mm] = P->JHP;
+      g.B[sg][mm] = reinterpret_cast<const char*>(segs[sg].packed + P->koff[2 * m + p]);
       g.K[sg][mm] = P->kp(m, p);
     }
   }
@@ -428,23 +493,22 @@ struct AnaSeg {
 
 void add_ana_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int ncols,
                    float* packed_out, const AnaSeg* segs, int nseg) {
-  sht::GemmGroup g{};
+  HostGroup g;
   g.nseg = nseg;
   g.N = ncols;
   g.mcount = std::min(sht::kMT, P->T + 1 - mb * sht::kMT);
-  g.piece = P->tab_elems;
   g.ccs = P->ldp;
-  for (int sg = 0; sg < nseg; ++sg) {
-    g.bks[sg] = sht::kMT;
-    g.bcs[sg] = P->ldw;
-    g.B[sg][0] = waist_block(P, segs[sg].waist, mb * sht::kMT, q ^ segs[sg].pflip);
-  }
+  g.bcs = P->ldw;
+  for (int sg = 0; sg < nseg; ++sg)
+    g.B[sg][0] = reinterpret_cast<const char*>(
+        waist_block(P, segs[sg].waist, mb * sht::kMT, q ^ segs[sg].pflip));
   for (int mm = 0; mm < g.mcount; ++mm) {
     const int m = mb * sht::kMT + mm;
     g.R[mm] = P->kp(m, q);
     g.C[mm] = packed_out + P->koff[2 * m + q];
     for (int sg = 0; sg < nseg; ++sg) {
-      g.A[sg][mm] = table_block(P, segs[sg].kind, false, m, q, &g.lda[sg][mm]);
+      g.A[sg][mm] = table_block(P, segs[sg].kind, false, m, q, &g.piece_bytes, &g.esz);
+      g.Arows[sg][mm] = P->kp(m, q);
       g.K[sg][mm] = P->JHP;
     }
   }
@@ -621,8 +685,8 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
     }
     B->jobs.push_back(std::move(js));
   }
-  syn.finish(B->syn_probs, B->syn_tiles, &B->syn_ntiles);
-  ana.finish(B->ana_probs, B->ana_tiles, &B->ana_ntiles);
+  syn.finish(B->syn_panels, B->syn_steps, B->syn_tiles, &B->syn_ntiles);
+  ana.finish(B->ana_panels, B->ana_steps, B->ana_tiles, &B->ana_ntiles);
   hip_check(hipDeviceSynchronize(), "batch setup");
 }
 
@@ -648,18 +712,18 @@ void run_batch(arwen_sht_batch* B, hipStream_t s, hipEvent_t* ev = nullptr) {
   mark(0);
   for (auto& a : B->packs) hip_check(sht_launch_pack(&a, s), "pack launch");
   mark(1);
-  hip_check(sht_launch_legendre_gemm(B->precision, 0, B->syn_probs.as<sht::GemmGroup>(),
-                                     B->syn_tiles.as<int4>(), B->syn_ntiles,
-                                     gemm_grid(B, B->syn_ntiles), s),
+  hip_check(sht_launch_legendre_gemm(B->precision, 0, B->syn_panels.as<sht::GemmPanel>(),
+                                     B->syn_steps.as<sht::GemmStep>(), B->syn_tiles.as<int2>(),
+                                     B->syn_ntiles, gemm_grid(B, B->syn_ntiles), s),
             "synthesis GEMM launch");
   mark(2);
   for (auto& a : B->unfolds) hip_check(sht_launch_irfft_unfold(&a, s), "irfft launch");
   mark(3);
   for (auto& a : B->folds) hip_check(sht_launch_rfft_fold(&a, s), "rfft launch");
   mark(4);
-  hip_check(sht_launch_legendre_gemm(B->precision, 1, B->ana_probs.as<sht::GemmGroup>(),
-                                     B->ana_tiles.as<int4>(), B->ana_ntiles,
-                                     gemm_grid(B, B->ana_ntiles), s),
+  hip_check(sht_launch_legendre_gemm(B->precision, 1, B->ana_panels.as<sht::GemmPanel>(),
+                                     B->ana_steps.as<sht::GemmStep>(), B->ana_tiles.as<int2>(),
+                                     B->ana_ntiles, gemm_grid(B, B->ana_ntiles), s),
             "analysis GEMM launch");
   mark(5);
   for (auto& a : B->unpacks) hip_check(sht_launch_unpack(&a, s), "unpack launch");

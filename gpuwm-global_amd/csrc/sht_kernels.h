@@ -6,11 +6,24 @@
 
 namespace sht {
 
-constexpr int kMT = 8;           // m-tile of the waist layout W[c][m/8][p][i][m%8]
+constexpr int kMT = 4;           // m-tile of the waist layout W[c][m/4][p][i][m%4]
 constexpr int kFftThreads = 256;
 constexpr int kMaxStages = 16;
 
 // ---- tables (sht_tables.hip) ----
+// The block of order m and parity p starts at koff[2m+p] * JHP elements, holds
+// JHP folded latitudes i x KP = kp(m,p) degrees (n - m)/2, both multiples of
+// 32, and is stored in 32-wide k-chunks so that one GEMM k-step of any 64 rows
+// is one contiguous run:
+//   synthesis (rows i, k = degree):   chunk k/32, then i, then k%32
+//   analysis  (rows degree, k = i):   chunk i/32, then degree, then i%32
+__host__ __device__ inline long long syn_tiled(int JHP, int i, int nidx) {
+  return (long long)(nidx >> 5) * JHP * 32 + (long long)i * 32 + (nidx & 31);
+}
+__host__ __device__ inline long long ana_tiled(int KP, int i, int nidx) {
+  return (long long)(i >> 5) * KP * 32 + (long long)nidx * 32 + (i & 31);
+}
+
 struct TableOut {
   // One entry per kind (0 = P, 1 = dP/dphi, 2 = P*m/cos), NULL to skip.
   float* f32_syn[3];
@@ -73,21 +86,31 @@ struct UnfoldArgs {
 };
 
 // ---- grouped GEMM (sht_gemm.hip) ----
-// One work group: up to kMT consecutive orders m of one job and one output
-// parity.  Per order: C_m = sum_seg A_seg,m x B_seg,m (see sht_gemm.hip).
-struct GemmGroup {
-  const void* A[2][kMT];      // table block of (seg, m): A[r][k] at A + r*lda + k
-  const float* B[2][kMT];     // SYN: packed block of (seg, m); ANA: waist block (m-tile, parity)
-  float* C[kMT];              // SYN: C[0] = waist block (m-tile, parity); ANA: packed block of m
-  long long piece;            // bf16 piece stride (elements) of the tables
-  int K[2][kMT];              // k extent per (seg, m), multiple of 32; 0 = no work
-  int lda[2][kMT];
-  int R[kMT];                 // rows per m (multiple of 32)
-  int Kmax[2];
-  int bks[2], bcs[2];         // B strides (k, column); ANA reads the waist's m-tiled k
-  int nseg, N, Rmax, mcount;  // segments, columns, max rows, orders in the tile
-  int ccs;                    // C column stride
+// A panel is one row tile of one work group: up to kMT consecutive orders m
+// of one job and one output parity.  Its k-steps (one order, 32 k each) are
+// listed on the host, in the order the kernel runs them; the kernel reads
+// panels and steps through the scalar cache.  Work items are (panel, first
+// column).  All addresses are byte addresses.
+struct GemmStep {
+  unsigned long long a;       // A tile: rows r0.. of the panel, this k-chunk (tiled table)
+  unsigned long long b;       // B at column 0: SYN packed block + k0; ANA waist block + k0
+  unsigned int a_bytes;       // valid bytes from a (rows past the order's end read 0)
+  int b_klen;                 // valid k from b (SYN: K - k0; ANA: JHP - k0); 0 = no B load
+  int z;                      // order within the m-tile
+  int pad;
 };
+
+struct GemmPanel {
+  unsigned long long c[kMT];  // SYN: c[0] = waist block + r0 rows; ANA: packed block of order z + r0
+  long long piece_bytes;      // bf16 piece stride of the tables
+  int step0, nsteps;          // steps[step0 .. step0+nsteps), followed by two null steps
+  int N;                      // data columns
+  int bcs;                    // B column stride, floats
+  int ccs;                    // C column stride, floats
+  int rows[kMT];              // rows of C written per order (<= tile rows)
+  int pad;
+};
+static_assert(sizeof(GemmStep) == 32, "GemmStep is one 8-dword scalar load");
 
 // ---- spectral data movement (sht_pack.hip) ----
 enum PackMode : int { kPlain = 0, kIM = 1, kICoef = 2, kVector = 3 };
@@ -139,8 +162,9 @@ hipError_t sht_launch_irfft_unfold(const sht::UnfoldArgs* a, hipStream_t s);
 size_t sht_fold_lds_bytes(int N, int T, int vector, int rp);
 size_t sht_unfold_lds_bytes(int N, int T, int rp);
 hipError_t sht_fft_allow_lds(size_t bytes);
-hipError_t sht_launch_legendre_gemm(int mode, int dir, const sht::GemmGroup* groups,
-                                    const int4* tiles, int ntiles, int grid, hipStream_t s);
+hipError_t sht_launch_legendre_gemm(int mode, int dir, const sht::GemmPanel* panels,
+                                    const sht::GemmStep* steps, const int2* tiles, int ntiles,
+                                    int grid, hipStream_t s);
 int sht_gemm_tile_rows(int dir);
 int sht_gemm_tile_cols(int dir);
 hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s);

@@ -86,11 +86,13 @@ __device__ static inline float bf16_value(unsigned short b) {
 
 
 
-__device__ static inline void store_value(const TableOut& out, int kind, long long syn_idx,
-                                          long long ana_idx, double v) {
+// f32 and bf16 tables use the k-chunk tiled layouts; the FP64 copy (tests)
+// keeps the plain synthesis layout, row i then degree.
+__device__ static inline void store_value(const TableOut& out, int kind, long long plain_idx,
+                                          long long syn_idx, long long ana_idx, double v) {
   if (out.f32_syn[kind]) out.f32_syn[kind][syn_idx] = (float)v;
   if (out.f32_ana[kind]) out.f32_ana[kind][ana_idx] = (float)v;
-  if (out.f64_syn[kind]) out.f64_syn[kind][syn_idx] = v;
+  if (out.f64_syn[kind]) out.f64_syn[kind][plain_idx] = v;
   if (out.bf_syn[kind] || out.bf_ana[kind]) {
     unsigned short b0 = bf16_bits((float)v);
     double r1 = v - (double)bf16_value(b0);
@@ -152,11 +154,12 @@ __global__ void legendre_tables_kernel(int T, int nlat, int JH, int JHP,
     const int nidx = (n - m) >> 1;
     const long long base = (long long)koff[2 * m + par] * JHP;
     const int kp = koff[2 * m + par + 1] - koff[2 * m + par];
-    const long long syn_idx = base + (long long)i * kp + nidx;
-    const long long ana_idx = base + (long long)nidx * JHP + i;
-    store_value(out, 0, syn_idx, ana_idx, p);
-    store_value(out, 1, syn_idx, ana_idx, d);
-    store_value(out, 2, syn_idx, ana_idx, pm);
+    const long long plain_idx = base + (long long)i * kp + nidx;
+    const long long syn_idx = base + syn_tiled(JHP, i, nidx);
+    const long long ana_idx = base + ana_tiled(kp, i, nidx);
+    store_value(out, 0, plain_idx, syn_idx, ana_idx, p);
+    store_value(out, 1, plain_idx, syn_idx, ana_idx, d);
+    store_value(out, 2, plain_idx, syn_idx, ana_idx, pm);
     p2 = p1;
     p1 = p;
   }
