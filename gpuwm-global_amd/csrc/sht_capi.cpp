@@ -276,8 +276,8 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
 
 struct JobState {
   arwen_sht_job job;
-  DevBuf packed_a, packed_b;  // synthesis inputs (packed coefficients)
-  DevBuf waist_a, waist_b;    // synthesis output / analysis input
+  DevBuf packed_a;            // synthesis input (packed coefficients)
+  DevBuf waist_a, waist_b;    // synthesis output (b: gradient north) / analysis input
   DevBuf packed_out;          // analysis output
   int ncols = 0;
 };
@@ -323,6 +323,7 @@ struct HostGroup {
   const char* B[2][sht::kMT] = {};  // SYN packed block of (seg, order); ANA waist block of seg
   float* C[sht::kMT] = {};          // SYN C[0] = waist block; ANA packed block of the order
   int R[sht::kMT] = {};             // output rows per order (SYN JHP; ANA KP(m, q))
+  int cmap[2] = {0, 0};             // B column map per segment (GemmStep::cmap)
   int bcs = 0, ccs = 0;             // column strides (floats)
   long long piece_bytes = 0;
   size_t esz = 4;                   // table element bytes
@@ -371,6 +372,7 @@ struct GroupBuilder {
                                         ((size_t)k0 * chunk_rows + (size_t)r0 * 32) * g.esz);
             st.a_bytes = (unsigned)(std::min(TR, chunk_rows - r0) * 32 * g.esz);
             st.z = z;
+            st.cmap = g.cmap[sg];
             if (dir == 0) {
               st.b = (unsigned long long)(g.B[sg][z] + (size_t)k0 * 4);
               st.b_klen = g.K[sg][z] - k0;
@@ -454,11 +456,18 @@ float* waist_block(const arwen_sht_plan* P, float* W, int m, int p) {
   return W + ((size_t)((m / sht::kMT) * 2 + p) * P->JHP) * sht::kMT;
 }
 
-// Synthesis group: segments (table kind, parity, packed data) per order.
+// Column maps of the vector operators' second data arrangement (see
+// GemmStep::cmap): synthesis [i chi, i psi] = [-a3, a2, a1, -a0] of the packed
+// [-psi, chi]; analysis [U, -V] = [a3, -a2, -a1, a0] of the waist [iV, iU].
+constexpr int kMapSynVec = 1 | (1 << 1) | (1 << 4);
+constexpr int kMapAnaVec = 1 | (1 << 2) | (1 << 3);
+
+// Synthesis group: segments (table kind, parity, packed data, column map).
 struct SynSeg {
   int kind;
   int pflip;  // data/table parity = q ^ pflip
   const float* packed;
+  int cmap;
 };
 
 void add_syn_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int ncols,
@@ -470,6 +479,7 @@ void add_syn_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int 
   g.ccs = P->ldw;
   g.bcs = P->ldp;
   g.C[0] = waist_block(P, waist, mb * sht::kMT, q);
+  for (int sg = 0; sg < nseg; ++sg) g.cmap[sg] = segs[sg].cmap;
   for (int mm = 0; mm < g.mcount; ++mm) {
     const int m = mb * sht::kMT + mm;
     g.R[mm] = P->JHP;
@@ -484,11 +494,12 @@ void add_syn_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int 
   b.add(g);
 }
 
-// Analysis group: segments (table kind, waist, waist parity flip) per order.
+// Analysis group: segments (table kind, waist parity flip, waist, column map).
 struct AnaSeg {
   int kind;
   int pflip;
   float* waist;
+  int cmap;
 };
 
 void add_ana_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int ncols,
@@ -499,9 +510,11 @@ void add_ana_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int 
   g.mcount = std::min(sht::kMT, P->T + 1 - mb * sht::kMT);
   g.ccs = P->ldp;
   g.bcs = P->ldw;
-  for (int sg = 0; sg < nseg; ++sg)
+  for (int sg = 0; sg < nseg; ++sg) {
     g.B[sg][0] = reinterpret_cast<const char*>(
         waist_block(P, segs[sg].waist, mb * sht::kMT, q ^ segs[sg].pflip));
+    g.cmap[sg] = segs[sg].cmap;
+  }
   for (int mm = 0; mm < g.mcount; ++mm) {
     const int m = mb * sht::kMT + mm;
     g.R[mm] = P->kp(m, q);
@@ -548,7 +561,6 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       const int ncol = vec ? 4 : 2;
       js->ncols = ncol * n;
       js->packed_a.alloc((size_t)js->ncols * P->ldp * 4);
-      if (vec) js->packed_b.alloc((size_t)js->ncols * P->ldp * 4);
       js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
       const bool grad = J.op == ARWEN_SHT_OP_GRADIENT;
       if (grad) js->waist_b.alloc((size_t)js->ncols * P->ldw * 4);
@@ -560,7 +572,6 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       pa.job.in0 = static_cast<const float2*>(J.in0);
       pa.job.in1 = static_cast<const float2*>(J.in1);
       pa.job.out0 = js->packed_a.as<float>();
-      pa.job.out1 = js->packed_b.as<float>();
       pa.job.nfields = n;
       pa.job.col0 = 0;
       pa.job.ldp = P->ldp;
@@ -575,18 +586,19 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
           if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
             // U,V = -dP/dphi (psi) + i P m/cos (chi), i P m/cos (psi) + dP/dphi (chi):
             // dP/dphi flips the parity, so its segment reads the other parity.
-            SynSeg sg[2] = {{kD, 1, js->packed_a.as<float>()}, {kPm, 0, js->packed_b.as<float>()}};
+            SynSeg sg[2] = {{kD, 1, js->packed_a.as<float>(), 0},
+                            {kPm, 0, js->packed_a.as<float>(), kMapSynVec}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 2);
           } else if (J.op == ARWEN_SHT_OP_MERIDIONAL_DERIVATIVE) {
-            SynSeg sg[1] = {{kD, 1, js->packed_a.as<float>()}};
+            SynSeg sg[1] = {{kD, 1, js->packed_a.as<float>(), 0}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
           } else if (grad) {
-            SynSeg se[1] = {{kPm, 0, js->packed_a.as<float>()}};
+            SynSeg se[1] = {{kPm, 0, js->packed_a.as<float>(), 0}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), se, 1);
-            SynSeg sn[1] = {{kD, 1, js->packed_a.as<float>()}};
+            SynSeg sn[1] = {{kD, 1, js->packed_a.as<float>(), 0}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_b.as<float>(), sn, 1);
           } else {  // SYNTHESIS, ZONAL_DERIVATIVE
-            SynSeg sg[1] = {{kP, 0, js->packed_a.as<float>()}};
+            SynSeg sg[1] = {{kP, 0, js->packed_a.as<float>(), 0}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
           }
         }
@@ -635,7 +647,6 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       const int ncol = vec ? 4 : 2;
       js->ncols = ncol * n;
       js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
-      if (vec) js->waist_b.alloc((size_t)js->ncols * P->ldw * 4);
       js->packed_out.alloc((size_t)js->ncols * P->ldp * 4);
       sht::FoldArgs fa{};
       fa.fft = P->fft;
@@ -650,7 +661,6 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       fa.job.grid0 = static_cast<const float*>(J.in0);
       fa.job.grid1 = static_cast<const float*>(J.in1);
       fa.job.waist0 = js->waist_a.as<float>();
-      fa.job.waist1 = js->waist_b.as<float>();
       fa.job.rowscale = vec ? P->scale_fold_vec.as<float>() : P->scale_fold.as<float>();
       fa.job.nfields = n;
       fa.job.col0 = 0;
@@ -662,10 +672,11 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
         for (int q = 0; q < 2; ++q) {
           if (vec) {
             // zeta, D = Pm^T [iV, iU]_q + dP^T [U, -V]_{1-q}
-            AnaSeg sg[2] = {{kPm, 0, js->waist_a.as<float>()}, {kD, 1, js->waist_b.as<float>()}};
+            AnaSeg sg[2] = {{kPm, 0, js->waist_a.as<float>(), 0},
+                            {kD, 1, js->waist_a.as<float>(), kMapAnaVec}};
             add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 2);
           } else {
-            AnaSeg sg[1] = {{kP, 0, js->waist_a.as<float>()}};
+            AnaSeg sg[1] = {{kP, 0, js->waist_a.as<float>(), 0}};
             add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 1);
           }
         }

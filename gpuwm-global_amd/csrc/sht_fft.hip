@@ -269,20 +269,18 @@ __global__ void __launch_bounds__(kFftThreads) rfft_fold_kernel(FoldArgs a) {
         a.job.waist0[waist_index(c, a.job.ldw, a.JHP, m, 1, i)] = o[0].x;
         a.job.waist0[waist_index(c + 1, a.job.ldw, a.JHP, m, 1, i)] = o[0].y;
       } else {
-        // level columns: arrangement a = [-V.im, V.re, -U.im, U.re],
-        //                arrangement b = [U.re, U.im, -V.re, -V.im]   (fold p)
+        // level columns [iV, iU] = [-V.im, V.re, -U.im, U.re] (fold p); the
+        // dP/dphi segment reads [U, -V] = [a3, -a2, -a1, a0] of the same
+        // columns through the GEMM's column map.
         const int c = a.job.col0 + 4 * item;
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
           const float2 U = p == 0 ? e[0] : o[0];
           const float2 V = p == 0 ? e[1] : o[1];
           const float va[4] = {-V.y, V.x, -U.y, U.x};
-          const float vb[4] = {U.x, U.y, -V.x, -V.y};
 #pragma unroll
-          for (int cl = 0; cl < 4; ++cl) {
+          for (int cl = 0; cl < 4; ++cl)
             a.job.waist0[waist_index(c + cl, a.job.ldw, a.JHP, m, p, i)] = va[cl];
-            a.job.waist1[waist_index(c + cl, a.job.ldw, a.JHP, m, p, i)] = vb[cl];
-          }
         }
       }
     }

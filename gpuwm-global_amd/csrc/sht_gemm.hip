@@ -20,6 +20,10 @@
 // ends the step.  Each panel's list ends in two null steps (extent 0), so
 // every load is issued unconditionally.
 //
+// A step may read its B columns through a map (reverse each group of four,
+// negate some of them): the vector operators' second arrangement of the data
+// is such a map of the first, so only one copy is stored.
+//
 // SYN (synthesis): rows = folded latitudes, k = degrees, C = waist
 //   W[c][m/4][p][i][m%4].  Each lane keeps four order accumulators; the tile
 //   is stored as 4 latitudes x 4 orders = 64 contiguous bytes per lane.
@@ -186,7 +190,8 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
         const unsigned bytes =
             s.b_klen > 0 ? (unsigned)(((ncol - 1) * P.bcs + s.b_klen) * 4) : 0u;
         const __amdgpu_buffer_rsrc_t r = make_rsrc(s.b + coloff, bytes);
-        B[0] = load4(r, (unsigned)((tid >> 3) * P.bcs + (tid & 7) * 4) * 4u);
+        const int src = (tid >> 3) ^ ((s.cmap & 1) ? 3 : 0);
+        B[0] = load4(r, (unsigned)(src * P.bcs + (tid & 7) * 4) * 4u);
       } else {
         const unsigned bytes =
             s.b_klen > 0 ? (unsigned)(((ncol - 1) * P.bcs + s.b_klen * kMT) * 4) : 0u;
@@ -194,7 +199,8 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int q = tid + 256 * u;  // column q >> 4, latitude pair q & 15
-          const unsigned off = (unsigned)((q >> 4) * P.bcs + (q & 15) * 2 * kMT) * 4u;
+          const int src = (q >> 4) ^ ((s.cmap & 1) ? 3 : 0);
+          const unsigned off = (unsigned)(src * P.bcs + (q & 15) * 2 * kMT) * 4u;
           B[2 * u] = load4(r, off);
           B[2 * u + 1] = load4(r, off + 16u);
         }
@@ -215,18 +221,20 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
               A[pc];
       }
     };
-    // B stage -> LDS (split into pieces for the bf16 modes)
-    auto put_b = [&](const f32x4(&B)[TR::B_LOADS], int buf) {
+    // B stage -> LDS (split into pieces for the bf16 modes); `cmap` bits
+    // 1..4 negate columns 0..3 of each group of four.
+    auto put_b = [&](const f32x4(&B)[TR::B_LOADS], int buf, int cmap) {
       if constexpr (DIR == 0) {
         const int col = tid >> 3, kq = (tid & 7) * 4;
+        const f32x4 v = ((cmap >> (1 + (col & 3))) & 1) ? -B[0] : B[0];
         if constexpr (F32) {
-          *reinterpret_cast<f32x4*>(&sBf[buf][col * kLdF + kq]) = B[0];
+          *reinterpret_cast<f32x4*>(&sBf[buf][col * kLdF + kq]) = v;
         } else {
           u16x4 pv[3];
 #pragma unroll
           for (int s = 0; s < 4; ++s) {
             unsigned short p[3];
-            split1<NP>(B[0][s], p);
+            split1<NP>(v[s], p);
 #pragma unroll
             for (int pc = 0; pc < NP; ++pc) pv[pc][s] = p[pc];
           }
@@ -240,15 +248,17 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
         for (int u = 0; u < 2; ++u) {
           const int q = tid + 256 * u;
           const int col = q >> 4, k2 = (q & 15) * 2;
+          const bool neg = (cmap >> (1 + (col & 3))) & 1;
+          const f32x4 v0 = neg ? -B[2 * u] : B[2 * u], v1 = neg ? -B[2 * u + 1] : B[2 * u + 1];
 #pragma unroll
           for (int z = 0; z < kMT; ++z) {
             if constexpr (F32) {
               *reinterpret_cast<float2*>(&sBf[0][(z * kBN + col) * kLdF + k2]) =
-                  float2{B[2 * u][z], B[2 * u + 1][z]};
+                  float2{v0[z], v1[z]};
             } else {
               unsigned short p0[3], p1[3];
-              split1<NP>(B[2 * u][z], p0);
-              split1<NP>(B[2 * u + 1][z], p1);
+              split1<NP>(v0[z], p0);
+              split1<NP>(v1[z], p1);
 #pragma unroll
               for (int pc = 0; pc < NP; ++pc)
                 *reinterpret_cast<u16x2*>(&sBh[0][((pc * kMT + z) * kBN + col) * kLdH + k2]) =
@@ -321,7 +331,7 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
     issue(ra[0], rb[0], 0);
     issue(ra[1], rb[1], 1);
     put_a(ra[0], 0);
-    if (DIR == 0 || steps[step0].b_klen > 0) put_b(rb[0], 0);
+    if (DIR == 0 || steps[step0].b_klen > 0) put_b(rb[0], 0, steps[step0].cmap);
     __syncthreads();
 
     // ---- main loop, two steps per trip (register stage = step parity) ----
@@ -330,12 +340,13 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
       issue(ra[s], rb[s], j + 2);
       compute(s, steps[step0 + j].z);
       if (j + 1 < nsteps) {
+        const GemmStep nx = steps[step0 + j + 1];
         if constexpr (DIR == 0) {
-          put_b(rb[s ^ 1], s ^ 1);
+          put_b(rb[s ^ 1], s ^ 1, nx.cmap);
         } else {
-          if (steps[step0 + j + 1].b_klen > 0) {
+          if (nx.b_klen > 0) {
             __syncthreads();  // every wavefront is done with the resident B tile
-            put_b(rb[s ^ 1], 0);
+            put_b(rb[s ^ 1], 0, nx.cmap);
           }
         }
         put_a(ra[s ^ 1], s ^ 1);

@@ -45,8 +45,7 @@ struct FftPlan {
 struct FoldJob {
   const float* grid0;         // field rows at grid0 + f*nlat*nlon (u for vector jobs)
   const float* grid1;         // v rows for vector jobs, else NULL
-  float* waist0;              // scalar: waist; vector: arrangement a = [iV, iU]
-  float* waist1;              // vector: arrangement b = [U, -V]
+  float* waist0;              // scalar: waist; vector: [iV, iU] per level
   const float* rowscale;      // per folded row: 1/nlon x quadrature weight (x 1/a)
   int nfields;                // scalar fields, or u/v level pairs
   int col0;                   // first data column of this job in the waist
@@ -97,7 +96,8 @@ struct GemmStep {
   unsigned int a_bytes;       // valid bytes from a (rows past the order's end read 0)
   int b_klen;                 // valid k from b (SYN: K - k0; ANA: JHP - k0); 0 = no B load
   int z;                      // order within the m-tile
-  int pad;
+  int cmap;                   // B column map: 0 = as stored; else bit 0 reverses each
+                              // group of 4 columns and bit 1+c negates column c of a group
 };
 
 struct GemmPanel {
@@ -118,8 +118,7 @@ enum PackMode : int { kPlain = 0, kIM = 1, kICoef = 2, kVector = 3 };
 struct PackJob {
   const float2* in0;          // [f][n][m]; vorticity for VECTOR
   const float2* in1;          // divergence for VECTOR
-  float* out0;                // packed (arrangement a for VECTOR)
-  float* out1;                // arrangement b for VECTOR
+  float* out0;                // packed ([-psi, chi] per level for VECTOR)
   int nfields;                // fields, or levels for VECTOR
   int col0;
   int ldp;                    // floats per packed column

@@ -12,9 +12,10 @@
 //   ICOEF   i*c                  (zonal derivative against the P*m/cos table)
 //   VECTOR  psi = zeta/lambda_n, chi = D/lambda_n, lambda_n = -n(n+1)/a^2,
 //           n = 0 -> 0, m = 0 realified (reference transform.py:1804-1811),
-//           written as two arrangements per level:
-//             a = [-psi, chi]       (against dP/dphi)
-//             b = [i chi, i psi]    (against P*m/cos)
+//           written as the four columns [-psi, chi] per level.  The P*m/cos
+//           segment needs [i chi, i psi] = [-a3, a2, a1, -a0] of those
+//           columns; the GEMM reads that arrangement from the same block
+//           through a column map instead of a second copy.
 #include "sht_kernels.h"
 
 namespace sht {
@@ -68,7 +69,6 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
     const int n = m + p + 2 * nidx;
     const int r = mi + p + 2 * tj;  // row in the tile
     float o[4] = {0.f, 0.f, 0.f, 0.f};
-    float ob[4] = {0.f, 0.f, 0.f, 0.f};
     if (n <= T) {
       float2 c = tile[0][r][mi];
       if (a.job.mode == kPlain) {
@@ -87,13 +87,11 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
           if (m == 0) { psi.y = 0.f; chi.y = 0.f; }
         }
         o[0] = -psi.x; o[1] = -psi.y; o[2] = chi.x; o[3] = chi.y;
-        ob[0] = -chi.y; ob[1] = chi.x; ob[2] = -psi.y; ob[3] = psi.x;
       }
     }
     for (int cl = 0; cl < ncol; ++cl) {
       const size_t col = (size_t)a.job.col0 + (size_t)f * ncol + cl;
       a.job.out0[col * a.job.ldp + kbeg + nidx] = o[cl];
-      if (vec) a.job.out1[col * a.job.ldp + kbeg + nidx] = ob[cl];
     }
   }
 }
