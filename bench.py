@@ -78,7 +78,7 @@ def member_fields(truncation, members, seed0=0):
 class RoundTrip:
     """The device buffers and the batch of one round-trip workload."""
 
-    def __init__(self, specs, precision, torch):
+    def __init__(self, specs, precision, torch, tune=True, split_acc=None):
         from arwen_spectral import SphericalHarmonicTransform, TransformBatch
 
         self.parts = []
@@ -101,6 +101,10 @@ class RoundTrip:
             self.parts.append(dict(tr=tr, zeta=zeta, div=div, scal=scal, u=u, v=v, grid=grid,
                                    z2=z2, d2=d2, s2=s2, spec=spec))
         self.batch = TransformBatch(jobs)
+        if split_acc is not None:
+            self.batch.set_config(split_acc=split_acc)
+        # the autotuner picks each direction's column tile and grid by timing them
+        self.config = self.batch.autotune(reps=3) if tune else self.batch.config()
         self.flops, self.fft_bytes = self.batch.work()
 
 
@@ -230,6 +234,9 @@ def main():
     ap.add_argument("--precision", default="bf16x6")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--split-acc", type=int, default=None,
+                    help="split modes: 1 = small products in their own accumulators")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the transform "
@@ -245,19 +252,20 @@ def main():
     print(f"[bench] workload {args.workload}: {nfields} fields, precision {args.precision}",
           file=sys.stderr, flush=True)
 
-    rt = RoundTrip(specs, args.precision, torch)
+    rt = RoundTrip(specs, args.precision, torch, not args.no_autotune, args.split_acc)
     sec = time_steps(rt, args.steps, args.warmup, torch)
     phases = rt.batch.profile(reps=3)
     errors = class_errors(rt)
-    print(f"[bench] {args.precision}: {sec * 1e3:.3f} ms/step, phases {phases}", file=sys.stderr,
-          flush=True)
+    print(f"[bench] {args.precision}: {sec * 1e3:.3f} ms/step, phases {phases}, "
+          f"gemm config {rt.config}", file=sys.stderr, flush=True)
 
     fp32 = None
     if not args.no_fp32 and args.precision != "float32":
-        rt32 = RoundTrip(specs, "float32", torch)
+        rt32 = RoundTrip(specs, "float32", torch, not args.no_autotune)
         sec32 = time_steps(rt32, args.steps, args.warmup, torch)
         ph32 = rt32.batch.profile(reps=3)
         fp32 = {"value": 1.0 / sec32, "ms_per_step": sec32 * 1e3, "phases_ms": ph32,
+                "gemm_config": rt32.config,
                 "roundtrip_error": class_errors(rt32), "roofline": roofline(rt32, ph32, "float32")}
         print(f"[bench] float32: {sec32 * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
         del rt32
@@ -297,6 +305,7 @@ def main():
                    "members": members, "fields": nfields, "precision": args.precision,
                    "specs": [{k: v for k, v in sp.items() if k != "fields"} for sp in specs]},
         "phases_ms": phases,
+        "gemm_config": rt.config,
         "roundtrip_error": errors,
         "error_vs_fp64_oracle": oracle_err,
         "fp32": fp32,

@@ -32,8 +32,8 @@ EXPORTS = (
     "arwen_sht_meridional_derivative", "arwen_sht_gradient", "arwen_sht_uv_from_vordiv",
     "arwen_sht_vordiv_from_uv", "arwen_sht_spectral_op", "arwen_sht_batch_create",
     "arwen_sht_batch_destroy", "arwen_sht_batch_run", "arwen_sht_batch_capture",
-    "arwen_sht_batch_launch", "arwen_sht_batch_set_grid", "arwen_sht_batch_work",
-    "arwen_sht_batch_profile",
+    "arwen_sht_batch_launch", "arwen_sht_batch_get_config", "arwen_sht_batch_set_config",
+    "arwen_sht_batch_autotune", "arwen_sht_batch_work", "arwen_sht_batch_profile",
     "arwen_sht_last_error",
 )
 
@@ -48,6 +48,19 @@ class PlanDesc(ctypes.Structure):
         ("sin_lat", ctypes.POINTER(ctypes.c_double)),
         ("weights", ctypes.POINTER(ctypes.c_double)),
     ]
+
+
+class GemmConfig(ctypes.Structure):
+    """``arwen_sht_gemm_config``: per direction (0 synthesis, 1 analysis)."""
+
+    _fields_ = [
+        ("cols", ctypes.c_int * 2),
+        ("grid", ctypes.c_int * 2),
+        ("split_acc", ctypes.c_int),
+    ]
+
+    def as_dict(self) -> dict:
+        return {"cols": list(self.cols), "grid": list(self.grid), "split_acc": self.split_acc}
 
 
 class Job(ctypes.Structure):
@@ -107,7 +120,9 @@ def load():
             "arwen_sht_batch_run": (I, [P, P]),
             "arwen_sht_batch_capture": (I, [P, P]),
             "arwen_sht_batch_launch": (I, [P, P]),
-            "arwen_sht_batch_set_grid": (I, [P, I]),
+            "arwen_sht_batch_get_config": (I, [P, ctypes.POINTER(GemmConfig)]),
+            "arwen_sht_batch_set_config": (I, [P, ctypes.POINTER(GemmConfig)]),
+            "arwen_sht_batch_autotune": (I, [P, P, I, ctypes.POINTER(GemmConfig)]),
             "arwen_sht_batch_work": (I, [P, D, D]),
             "arwen_sht_batch_profile": (I, [P, P, I, D]),
             "arwen_sht_last_error": (ctypes.c_char_p, []),

@@ -102,8 +102,32 @@ class TransformBatch:
     def launch(self):
         _native.check(self._lib.arwen_sht_batch_launch(self._handle, self._backend.stream()))
 
-    def set_grid(self, gemm_blocks: int):
-        _native.check(self._lib.arwen_sht_batch_set_grid(self._handle, int(gemm_blocks)))
+    def config(self) -> dict:
+        """The grouped-GEMM schedule: ``cols`` and ``grid`` per direction
+        (synthesis, analysis) and ``split_acc``."""
+        c = _native.GemmConfig()
+        _native.check(self._lib.arwen_sht_batch_get_config(self._handle, ctypes.byref(c)))
+        return c.as_dict()
+
+    def set_config(self, cols=None, grid=None, split_acc=None):
+        """Change the schedule; ``cols``/``grid`` are (synthesis, analysis) pairs."""
+        c = _native.GemmConfig()
+        _native.check(self._lib.arwen_sht_batch_get_config(self._handle, ctypes.byref(c)))
+        if cols is not None:
+            c.cols[0], c.cols[1] = (int(v) for v in cols)
+        if grid is not None:
+            c.grid[0], c.grid[1] = (int(v) for v in grid)
+        if split_acc is not None:
+            c.split_acc = int(split_acc)
+        _native.check(self._lib.arwen_sht_batch_set_config(self._handle, ctypes.byref(c)))
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Time every column tile and grid per direction on the current stream
+        and keep the fastest (synchronizes; overwrites intermediate buffers)."""
+        c = _native.GemmConfig()
+        _native.check(self._lib.arwen_sht_batch_autotune(self._handle, self._backend.stream(),
+                                                         int(reps), ctypes.byref(c)))
+        return c.as_dict()
 
     PHASES = ("pack", "synthesis_gemm", "inverse_fft", "forward_fft", "analysis_gemm", "unpack")
 

@@ -248,7 +248,10 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
     P->rp_fold = pick([&](int rp) { return sht_fold_lds_bytes(P->nlon, T, 0, rp); });
     P->rp_fold_vec = pick([&](int rp) { return sht_fold_lds_bytes(P->nlon, T, 1, rp); });
     P->rp_unfold = pick([&](int rp) { return sht_unfold_lds_bytes(P->nlon, T, rp); });
-    hip_check(sht_fft_allow_lds(budget), "FFT LDS attribute");
+    hip_check(sht_fft_allow_lds(budget, (size_t)lds_max), "FFT LDS attribute");
+    // wave-resident FFTs (N = 64 L) when their LDS fits the device
+    const int L = sht_fft_wave_L(P->nlon);
+    P->fft.wave_L = (L && sht_fft_wave_lds_bytes(P->nlon, 1) <= (size_t)lds_max) ? L : 0;
   }
 
   // Per folded-row scales.
@@ -274,6 +277,31 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
 // Batches
 // ---------------------------------------------------------------------------
 
+// One work group of the grouped GEMM before it is cut into panels: up to kMT
+// consecutive orders of one job and one output parity, with one or two
+// segments (table kind, data) accumulated into the same output.
+struct HostGroup {
+  int nseg = 0, N = 0, mcount = 0;
+  const char* A[2][sht::kMT] = {};  // table block of (seg, order), tiled layout
+  int K[2][sht::kMT] = {};          // k extent: SYN degrees KP(m, p); ANA JHP
+  int Arows[2][sht::kMT] = {};      // rows of the table block: SYN JHP; ANA KP(m, q)
+  const char* B[2][sht::kMT] = {};  // SYN packed block of (seg, order); ANA waist block of seg
+  float* C[sht::kMT] = {};          // SYN C[0] = waist block; ANA packed block of the order
+  int R[sht::kMT] = {};             // output rows per order (SYN JHP; ANA KP(m, q))
+  int cmap[2] = {0, 0};             // B column map per segment (GemmStep::cmap)
+  int bcs = 0, ccs = 0;             // column strides (floats)
+  long long piece_bytes = 0;        // table piece stride (split modes)
+  long long bpiece_bytes = 0;       // data piece stride (split modes)
+  size_t esz = 4;                   // table and data element bytes (4 fp32, 2 bf16 pieces)
+};
+
+// Device work lists of one GEMM direction, built for one column tile.
+struct GemmLists {
+  DevBuf panels, steps, tiles;
+  int ntiles = 0;
+  int bn = 0;
+};
+
 struct JobState {
   arwen_sht_job job;
   DevBuf packed_a;            // synthesis input (packed coefficients)
@@ -290,9 +318,9 @@ struct arwen_sht_batch {
   std::vector<sht::UnfoldArgs> unfolds;
   std::vector<sht::FoldArgs> folds;
   std::vector<sht::UnpackArgs> unpacks;
-  DevBuf syn_panels, syn_steps, syn_tiles, ana_panels, ana_steps, ana_tiles;
-  int syn_ntiles = 0, ana_ntiles = 0;
-  int gemm_blocks = 0;  // 0 = auto
+  std::vector<HostGroup> groups[2];  // GEMM work groups per direction (0 SYN, 1 ANA)
+  GemmLists lists[2];
+  arwen_sht_gemm_config config{{32, 32}, {0, 0}, 1};
   double legendre_flops = 0, fft_bytes = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -312,36 +340,22 @@ bool is_synthesis(int op) { return op >= ARWEN_SHT_OP_SYNTHESIS && op <= ARWEN_S
 
 constexpr int kXcds = 8;  // MI355X: 8 XCDs x 32 CUs
 
-// One work group of the grouped GEMM before it is cut into panels: up to kMT
-// consecutive orders of one job and one output parity, with one or two
-// segments (table kind, data) accumulated into the same output.
-struct HostGroup {
-  int nseg = 0, N = 0, mcount = 0;
-  const char* A[2][sht::kMT] = {};  // table block of (seg, order), tiled layout
-  int K[2][sht::kMT] = {};          // k extent: SYN degrees KP(m, p); ANA JHP
-  int Arows[2][sht::kMT] = {};      // rows of the table block: SYN JHP; ANA KP(m, q)
-  const char* B[2][sht::kMT] = {};  // SYN packed block of (seg, order); ANA waist block of seg
-  float* C[sht::kMT] = {};          // SYN C[0] = waist block; ANA packed block of the order
-  int R[sht::kMT] = {};             // output rows per order (SYN JHP; ANA KP(m, q))
-  int cmap[2] = {0, 0};             // B column map per segment (GemmStep::cmap)
-  int bcs = 0, ccs = 0;             // column strides (floats)
-  long long piece_bytes = 0;
-  size_t esz = 4;                   // table element bytes
-};
-
-// Collects the grouped-GEMM work list of one direction: panels (group x row
-// tile), their k-step lists and the (panel, column tile) work items.
-struct GroupBuilder {
-  int dir = 0;  // 0 synthesis, 1 analysis
+// Cuts the work groups of one direction into panels (group x 64-row tile),
+// their k-step lists and (panel, column tile) work items, in XCD-aware order.
+// Workgroups are dispatched round-robin over the 8 XCDs and the kernels walk
+// the list with a grid stride that is a multiple of 8, so list position t
+// always runs on XCD t % 8.  Every work item of one group (same orders, same
+// tables, same data columns) goes to one XCD, where its tiles run side by
+// side and share that XCD's L2; groups are balanced over the XCDs longest
+// first.  Short lists are padded with idle items (-1).
+void build_lists(int dir, const std::vector<HostGroup>& groups, int bn, GemmLists& out) {
+  const int TR = sht::kGemmBM;
   std::vector<sht::GemmPanel> panels;
   std::vector<sht::GemmStep> steps;
-  struct Item { long long work; int group; int2 tile; };
-  std::vector<Item> items;
-  int ngroups = 0;
-
-  void add(const HostGroup& g) {
-    const int TR = sht_gemm_tile_rows(dir), TC = sht_gemm_tile_cols(dir);
-    const int gid = ngroups++;
+  std::vector<long long> gwork(groups.size(), 0);
+  std::vector<std::vector<int2>> per_group(groups.size());
+  for (size_t gid = 0; gid < groups.size(); ++gid) {
+    const HostGroup& g = groups[gid];
     int Rmax = 0;
     for (int z = 0; z < g.mcount; ++z) Rmax = std::max(Rmax, g.R[z]);
     for (int r0 = 0; r0 < Rmax; r0 += TR) {
@@ -351,6 +365,7 @@ struct GroupBuilder {
       pn.bcs = g.bcs;
       pn.ccs = g.ccs;
       pn.piece_bytes = g.piece_bytes;
+      pn.bpiece_bytes = g.bpiece_bytes;
       for (int z = 0; z < sht::kMT; ++z) {
         pn.rows[z] = z < g.mcount ? std::max(0, std::min(TR, g.R[z] - r0)) : 0;
         if (dir == 0)
@@ -374,10 +389,10 @@ struct GroupBuilder {
             st.z = z;
             st.cmap = g.cmap[sg];
             if (dir == 0) {
-              st.b = (unsigned long long)(g.B[sg][z] + (size_t)k0 * 4);
+              st.b = (unsigned long long)(g.B[sg][z] + (size_t)k0 * g.esz);
               st.b_klen = g.K[sg][z] - k0;
             } else if (first) {
-              st.b = (unsigned long long)(g.B[sg][0] + (size_t)k0 * sht::kMT * 4);
+              st.b = (unsigned long long)(g.B[sg][0] + (size_t)k0 * sht::kMT * g.esz);
               st.b_klen = g.K[sg][z] - k0;
             }
             first = false;
@@ -398,49 +413,36 @@ struct GroupBuilder {
       steps.push_back(null);
       const int pid = (int)panels.size();
       panels.push_back(pn);
-      for (int c0 = 0; c0 < g.N; c0 += TC)
-        items.push_back({(long long)pn.nsteps * std::min(TC, g.N - c0), gid, int2{pid, c0}});
+      for (int c0 = 0; c0 < g.N; c0 += bn) {
+        gwork[gid] += (long long)pn.nsteps * std::min(bn, g.N - c0);
+        per_group[gid].push_back(int2{pid, c0});
+      }
     }
   }
-  // XCD-aware order.  Workgroups are dispatched round-robin over the 8 XCDs
-  // and the kernels walk the list with a grid stride that is a multiple of 8,
-  // so list position t always runs on XCD t % 8.  Every work item of one
-  // group (same orders, same tables, same data columns) goes to one XCD,
-  // where its tiles run side by side and share that XCD's L2; groups are
-  // balanced over the XCDs longest first.  Short lists are padded with idle
-  // items (-1).
-  void finish(DevBuf& dpanels, DevBuf& dsteps, DevBuf& dtiles, int* ntiles) {
-    const int nx = kXcds;
-    std::vector<long long> gwork(ngroups, 0);
-    std::vector<std::vector<int2>> per_group(ngroups);
-    for (auto& x : items) {
-      gwork[x.group] += x.work;
-      per_group[x.group].push_back(x.tile);
-    }
-    std::vector<int> order(ngroups);
-    for (int g = 0; g < ngroups; ++g) order[g] = g;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return gwork[a] > gwork[b]; });
-    std::vector<long long> load(nx, 0);
-    std::vector<std::vector<int2>> lists(nx);
-    for (int g : order) {
-      if (per_group[g].empty()) continue;
-      int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      load[x] += gwork[g];
-      for (auto& tt : per_group[g]) lists[x].push_back(tt);
-    }
-    size_t longest = 0;
-    for (auto& l : lists) longest = std::max(longest, l.size());
-    std::vector<int2> t;
-    t.reserve(longest * nx);
-    for (size_t j = 0; j < longest; ++j)
-      for (int x = 0; x < nx; ++x) t.push_back(j < lists[x].size() ? lists[x][j] : int2{-1, 0});
-    upload(dpanels, panels);
-    upload(dsteps, steps);
-    upload(dtiles, t);
-    *ntiles = (int)t.size();
+  const int nx = kXcds;
+  std::vector<int> order(groups.size());
+  for (size_t g = 0; g < order.size(); ++g) order[g] = (int)g;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return gwork[a] > gwork[b]; });
+  std::vector<long long> load(nx, 0);
+  std::vector<std::vector<int2>> lists(nx);
+  for (int g : order) {
+    if (per_group[g].empty()) continue;
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += gwork[g];
+    for (auto& tt : per_group[g]) lists[x].push_back(tt);
   }
-};
+  size_t longest = 0;
+  for (auto& l : lists) longest = std::max(longest, l.size());
+  std::vector<int2> t;
+  t.reserve(longest * nx);
+  for (size_t j = 0; j < longest; ++j)
+    for (int x = 0; x < nx; ++x) t.push_back(j < lists[x].size() ? lists[x][j] : int2{-1, 0});
+  upload(out.panels, panels);
+  upload(out.steps, steps);
+  upload(out.tiles, t);
+  out.ntiles = (int)t.size();
+  out.bn = bn;
+}
 
 const char* table_block(const arwen_sht_plan* P, int kind, bool syn, int m, int p,
                         long long* piece_bytes, size_t* esz) {
@@ -451,9 +453,10 @@ const char* table_block(const arwen_sht_plan* P, int kind, bool syn, int m, int 
   return base + (size_t)P->koff[2 * m + p] * P->JHP * *esz;
 }
 
-// Waist block (m-tile of m, parity p) of the layout W[c][m/kMT][p][i][m%kMT].
-float* waist_block(const arwen_sht_plan* P, float* W, int m, int p) {
-  return W + ((size_t)((m / sht::kMT) * 2 + p) * P->JHP) * sht::kMT;
+// Element offset of the waist block (m-tile of m, parity p) within a column,
+// layout W[c][m/kMT][p][i][m%kMT].
+size_t waist_offset(const arwen_sht_plan* P, int m, int p) {
+  return ((size_t)((m / sht::kMT) * 2 + p) * P->JHP) * sht::kMT;
 }
 
 // Column maps of the vector operators' second data arrangement (see
@@ -462,15 +465,23 @@ float* waist_block(const arwen_sht_plan* P, float* W, int m, int p) {
 constexpr int kMapSynVec = 1 | (1 << 1) | (1 << 4);
 constexpr int kMapAnaVec = 1 | (1 << 2) | (1 << 3);
 
+// GEMM data operand: base address, element bytes (4 fp32, 2 bf16 pieces) and
+// piece plane stride in bytes.
+struct DataRef {
+  const char* base;
+  size_t esz;
+  long long piece_bytes;
+};
+
 // Synthesis group: segments (table kind, parity, packed data, column map).
 struct SynSeg {
   int kind;
   int pflip;  // data/table parity = q ^ pflip
-  const float* packed;
+  DataRef packed;
   int cmap;
 };
 
-void add_syn_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int ncols,
+void add_syn_group(std::vector<HostGroup>& b, const arwen_sht_plan* P, int mb, int q, int ncols,
                    float* waist, const SynSeg* segs, int nseg) {
   HostGroup g;
   g.nseg = nseg;
@@ -478,7 +489,8 @@ void add_syn_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int 
   g.mcount = std::min(sht::kMT, P->T + 1 - mb * sht::kMT);
   g.ccs = P->ldw;
   g.bcs = P->ldp;
-  g.C[0] = waist_block(P, waist, mb * sht::kMT, q);
+  g.C[0] = waist + waist_offset(P, mb * sht::kMT, q);
+  g.bpiece_bytes = segs[0].packed.piece_bytes;
   for (int sg = 0; sg < nseg; ++sg) g.cmap[sg] = segs[sg].cmap;
   for (int mm = 0; mm < g.mcount; ++mm) {
     const int m = mb * sht::kMT + mm;
@@ -487,22 +499,22 @@ void add_syn_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int 
       const int p = q ^ segs[sg].pflip;
       g.A[sg][mm] = table_block(P, segs[sg].kind, true, m, p, &g.piece_bytes, &g.esz);
       g.Arows[sg][mm] = P->JHP;
-      g.B[sg][mm] = reinterpret_cast<const char*>(segs[sg].packed + P->koff[2 * m + p]);
+      g.B[sg][mm] = segs[sg].packed.base + (size_t)P->koff[2 * m + p] * segs[sg].packed.esz;
       g.K[sg][mm] = P->kp(m, p);
     }
   }
-  b.add(g);
+  b.push_back(g);
 }
 
 // Analysis group: segments (table kind, waist parity flip, waist, column map).
 struct AnaSeg {
   int kind;
   int pflip;
-  float* waist;
+  DataRef waist;
   int cmap;
 };
 
-void add_ana_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int ncols,
+void add_ana_group(std::vector<HostGroup>& b, const arwen_sht_plan* P, int mb, int q, int ncols,
                    float* packed_out, const AnaSeg* segs, int nseg) {
   HostGroup g;
   g.nseg = nseg;
@@ -510,9 +522,10 @@ void add_ana_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int 
   g.mcount = std::min(sht::kMT, P->T + 1 - mb * sht::kMT);
   g.ccs = P->ldp;
   g.bcs = P->ldw;
+  g.bpiece_bytes = segs[0].waist.piece_bytes;
   for (int sg = 0; sg < nseg; ++sg) {
-    g.B[sg][0] = reinterpret_cast<const char*>(
-        waist_block(P, segs[sg].waist, mb * sht::kMT, q ^ segs[sg].pflip));
+    g.B[sg][0] = segs[sg].waist.base +
+                 waist_offset(P, mb * sht::kMT, q ^ segs[sg].pflip) * segs[sg].waist.esz;
     g.cmap[sg] = segs[sg].cmap;
   }
   for (int mm = 0; mm < g.mcount; ++mm) {
@@ -525,15 +538,18 @@ void add_ana_group(GroupBuilder& b, const arwen_sht_plan* P, int mb, int q, int 
       g.K[sg][mm] = P->JHP;
     }
   }
-  b.add(g);
+  b.push_back(g);
 }
 
 void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
   if (njobs <= 0 || !jobs) fail(ARWEN_SHT_EINVAL, "a batch needs at least one job");
   hip_check(hipGetDevice(&B->device), "hipGetDevice");
   B->precision = jobs[0].plan ? jobs[0].plan->precision : 0;
-  GroupBuilder syn, ana;
-  ana.dir = 1;
+  // split modes keep GEMM data operands as bf16 pieces in planes
+  const int np = B->precision == ARWEN_SHT_BF16X6 ? 3 : B->precision == ARWEN_SHT_BF16X3 ? 2 : 0;
+  const size_t desz = np ? 2 : 4;
+  auto& syn = B->groups[0];
+  auto& ana = B->groups[1];
   for (int j = 0; j < njobs; ++j) {
     const arwen_sht_job& J = jobs[j];
     const arwen_sht_plan* P = J.plan;
@@ -560,7 +576,8 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       // ---- pack ----
       const int ncol = vec ? 4 : 2;
       js->ncols = ncol * n;
-      js->packed_a.alloc((size_t)js->ncols * P->ldp * 4);
+      const size_t pplane = (size_t)js->ncols * P->ldp;  // elements per piece plane
+      js->packed_a.alloc(pplane * desz * (np ? np : 1));
       js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
       const bool grad = J.op == ARWEN_SHT_OP_GRADIENT;
       if (grad) js->waist_b.alloc((size_t)js->ncols * P->ldw * 4);
@@ -572,6 +589,8 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       pa.job.in0 = static_cast<const float2*>(J.in0);
       pa.job.in1 = static_cast<const float2*>(J.in1);
       pa.job.out0 = js->packed_a.as<float>();
+      pa.job.split = np;
+      pa.job.pstride = (long long)pplane;
       pa.job.nfields = n;
       pa.job.col0 = 0;
       pa.job.ldp = P->ldp;
@@ -581,24 +600,24 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
                                                           : sht::kPlain;
       B->packs.push_back(pa);
       // ---- grouped GEMM: one group per (m-tile, output parity) ----
+      const DataRef pk{js->packed_a.as<char>(), desz, (long long)(pplane * desz)};
       for (int mb = 0; mb < P->MB; ++mb)
         for (int q = 0; q < 2; ++q) {
           if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
             // U,V = -dP/dphi (psi) + i P m/cos (chi), i P m/cos (psi) + dP/dphi (chi):
             // dP/dphi flips the parity, so its segment reads the other parity.
-            SynSeg sg[2] = {{kD, 1, js->packed_a.as<float>(), 0},
-                            {kPm, 0, js->packed_a.as<float>(), kMapSynVec}};
+            SynSeg sg[2] = {{kD, 1, pk, 0}, {kPm, 0, pk, kMapSynVec}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 2);
           } else if (J.op == ARWEN_SHT_OP_MERIDIONAL_DERIVATIVE) {
-            SynSeg sg[1] = {{kD, 1, js->packed_a.as<float>(), 0}};
+            SynSeg sg[1] = {{kD, 1, pk, 0}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
           } else if (grad) {
-            SynSeg se[1] = {{kPm, 0, js->packed_a.as<float>(), 0}};
+            SynSeg se[1] = {{kPm, 0, pk, 0}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), se, 1);
-            SynSeg sn[1] = {{kD, 1, js->packed_a.as<float>(), 0}};
+            SynSeg sn[1] = {{kD, 1, pk, 0}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_b.as<float>(), sn, 1);
           } else {  // SYNTHESIS, ZONAL_DERIVATIVE
-            SynSeg sg[1] = {{kP, 0, js->packed_a.as<float>(), 0}};
+            SynSeg sg[1] = {{kP, 0, pk, 0}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
           }
         }
@@ -646,7 +665,8 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       // ---- FFT + fold ----
       const int ncol = vec ? 4 : 2;
       js->ncols = ncol * n;
-      js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
+      const size_t wplane = (size_t)js->ncols * P->ldw;  // elements per piece plane
+      js->waist_a.alloc(wplane * desz * (np ? np : 1));
       js->packed_out.alloc((size_t)js->ncols * P->ldp * 4);
       sht::FoldArgs fa{};
       fa.fft = P->fft;
@@ -661,6 +681,8 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       fa.job.grid0 = static_cast<const float*>(J.in0);
       fa.job.grid1 = static_cast<const float*>(J.in1);
       fa.job.waist0 = js->waist_a.as<float>();
+      fa.job.split = np;
+      fa.job.pstride = (long long)wplane;
       fa.job.rowscale = vec ? P->scale_fold_vec.as<float>() : P->scale_fold.as<float>();
       fa.job.nfields = n;
       fa.job.col0 = 0;
@@ -668,15 +690,15 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       fa.job.vector = vec ? 1 : 0;
       B->folds.push_back(fa);
       B->fft_bytes += (vec ? 2.0 : 1.0) * n * fftb;
+      const DataRef wst{js->waist_a.as<char>(), desz, (long long)(wplane * desz)};
       for (int mb = 0; mb < P->MB; ++mb)
         for (int q = 0; q < 2; ++q) {
           if (vec) {
             // zeta, D = Pm^T [iV, iU]_q + dP^T [U, -V]_{1-q}
-            AnaSeg sg[2] = {{kPm, 0, js->waist_a.as<float>(), 0},
-                            {kD, 1, js->waist_a.as<float>(), kMapAnaVec}};
+            AnaSeg sg[2] = {{kPm, 0, wst, 0}, {kD, 1, wst, kMapAnaVec}};
             add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 2);
           } else {
-            AnaSeg sg[1] = {{kP, 0, js->waist_a.as<float>(), 0}};
+            AnaSeg sg[1] = {{kP, 0, wst, 0}};
             add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 1);
           }
         }
@@ -696,17 +718,32 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
     }
     B->jobs.push_back(std::move(js));
   }
-  syn.finish(B->syn_panels, B->syn_steps, B->syn_tiles, &B->syn_ntiles);
-  ana.finish(B->ana_panels, B->ana_steps, B->ana_tiles, &B->ana_ntiles);
+  for (int d = 0; d < 2; ++d) build_lists(d, B->groups[d], B->config.cols[d], B->lists[d]);
   hip_check(hipDeviceSynchronize(), "batch setup");
 }
 
-// Persistent grid: a multiple of the XCD count (see GroupBuilder::finish).
-int gemm_grid(const arwen_sht_batch* B, int ntiles) {
-  int g = B->gemm_blocks > 0 ? B->gemm_blocks : 256 * 8;
-  g = std::min(g, ntiles);
+// Persistent grid of direction d: a multiple of the XCD count (see
+// build_lists); 0 in the config = 8 workgroups per CU.
+int gemm_grid(const arwen_sht_batch* B, int d) {
+  int g = B->config.grid[d] > 0 ? B->config.grid[d] : 256 * 8;
+  g = std::min(g, std::max(B->lists[d].ntiles, 1));
   if (g >= kXcds) g -= g % kXcds;
   return std::max(g, 1);
+}
+
+hipError_t launch_gemm(const arwen_sht_batch* B, int d, hipStream_t s) {
+  const GemmLists& L = B->lists[d];
+  sht::GemmLaunch g{};
+  g.mode = B->precision;
+  g.dir = d;
+  g.bn = L.bn;
+  g.acc2 = B->config.split_acc;
+  g.panels = L.panels.as<sht::GemmPanel>();
+  g.steps = L.steps.as<sht::GemmStep>();
+  g.tiles = L.tiles.as<int2>();
+  g.ntiles = L.ntiles;
+  g.grid = gemm_grid(B, d);
+  return sht_launch_legendre_gemm(&g, s);
 }
 
 // Phases of a batch run, in order; `ev` (optional) gets one event before
@@ -723,22 +760,42 @@ void run_batch(arwen_sht_batch* B, hipStream_t s, hipEvent_t* ev = nullptr) {
   mark(0);
   for (auto& a : B->packs) hip_check(sht_launch_pack(&a, s), "pack launch");
   mark(1);
-  hip_check(sht_launch_legendre_gemm(B->precision, 0, B->syn_panels.as<sht::GemmPanel>(),
-                                     B->syn_steps.as<sht::GemmStep>(), B->syn_tiles.as<int2>(),
-                                     B->syn_ntiles, gemm_grid(B, B->syn_ntiles), s),
-            "synthesis GEMM launch");
+  hip_check(launch_gemm(B, 0, s), "synthesis GEMM launch");
   mark(2);
   for (auto& a : B->unfolds) hip_check(sht_launch_irfft_unfold(&a, s), "irfft launch");
   mark(3);
   for (auto& a : B->folds) hip_check(sht_launch_rfft_fold(&a, s), "rfft launch");
   mark(4);
-  hip_check(sht_launch_legendre_gemm(B->precision, 1, B->ana_panels.as<sht::GemmPanel>(),
-                                     B->ana_steps.as<sht::GemmStep>(), B->ana_tiles.as<int2>(),
-                                     B->ana_ntiles, gemm_grid(B, B->ana_ntiles), s),
-            "analysis GEMM launch");
+  hip_check(launch_gemm(B, 1, s), "analysis GEMM launch");
   mark(5);
   for (auto& a : B->unpacks) hip_check(sht_launch_unpack(&a, s), "unpack launch");
   mark(6);
+}
+
+void drop_graph(arwen_sht_batch* B) {
+  if (B->exec) {
+    (void)hipGraphExecDestroy(B->exec);
+    B->exec = nullptr;
+  }
+  if (B->graph) {
+    (void)hipGraphDestroy(B->graph);
+    B->graph = nullptr;
+  }
+}
+
+void apply_config(arwen_sht_batch* B, const arwen_sht_gemm_config& c) {
+  for (int d = 0; d < 2; ++d) {
+    if (c.cols[d] != 32 && c.cols[d] != 64) fail(ARWEN_SHT_EINVAL, "cols must be 32 or 64");
+    if (c.grid[d] < 0) fail(ARWEN_SHT_EINVAL, "grid must be >= 0");
+  }
+  if (c.split_acc != 0 && c.split_acc != 1) fail(ARWEN_SHT_EINVAL, "split_acc must be 0 or 1");
+  // the lists are rebuilt while no launch of this batch is in flight
+  hip_check(hipDeviceSynchronize(), "config change");
+  drop_graph(B);
+  for (int d = 0; d < 2; ++d)
+    if (B->lists[d].bn != c.cols[d]) build_lists(d, B->groups[d], c.cols[d], B->lists[d]);
+  B->config = c;
+  hip_check(hipDeviceSynchronize(), "config upload");
 }
 
 int single_op(arwen_sht_plan* P, int op, const void* in0, const void* in1, void* out0,
@@ -976,19 +1033,71 @@ int arwen_sht_batch_profile(arwen_sht_batch* B, void* stream, int reps, double* 
   });
 }
 
-int arwen_sht_batch_set_grid(arwen_sht_batch* B, int gemm_blocks) {
+
+int arwen_sht_batch_get_config(const arwen_sht_batch* B, arwen_sht_gemm_config* out) {
+  return guarded([&] {
+    if (!B || !out) fail(ARWEN_SHT_EINVAL, "batch and out are required");
+    *out = B->config;
+  });
+}
+
+int arwen_sht_batch_set_config(arwen_sht_batch* B, const arwen_sht_gemm_config* config) {
+  return guarded([&] {
+    if (!B || !config) fail(ARWEN_SHT_EINVAL, "batch and config are required");
+    apply_config(B, *config);
+  });
+}
+
+int arwen_sht_batch_autotune(arwen_sht_batch* B, void* stream, int reps,
+                             arwen_sht_gemm_config* out) {
   return guarded([&] {
     if (!B) fail(ARWEN_SHT_EINVAL, "batch is NULL");
-    if (gemm_blocks < 0) fail(ARWEN_SHT_EINVAL, "gemm_blocks must be >= 0");
-    B->gemm_blocks = gemm_blocks;
-    if (B->exec) {
-      (void)hipGraphExecDestroy(B->exec);
-      B->exec = nullptr;
+    if (reps < 1) fail(ARWEN_SHT_EINVAL, "reps must be >= 1");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    arwen_sht_gemm_config best = B->config;
+    try {
+      // Each direction is timed on its own, over every column tile and grid;
+      // the operands are the batch's own work buffers (their contents do not
+      // change the timing).  The split-accumulator choice is a precision
+      // setting and is kept as configured.
+      const int cols[2] = {32, 64};
+      const int grids[4] = {256 * 4, 256 * 8, 256 * 16, 256 * 32};
+      for (int d = 0; d < 2; ++d) {
+        if (B->lists[d].ntiles == 0) continue;
+        float best_ms = 0.f;
+        for (int bn : cols) {
+          for (int gr : grids) {
+            arwen_sht_gemm_config c = best;
+            c.cols[d] = bn;
+            c.grid[d] = gr;
+            apply_config(B, c);
+            hip_check(launch_gemm(B, d, s), "autotune warm-up");
+            hip_check(hipEventRecord(e0, s), "hipEventRecord");
+            for (int r = 0; r < reps; ++r) hip_check(launch_gemm(B, d, s), "autotune launch");
+            hip_check(hipEventRecord(e1, s), "hipEventRecord");
+            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            if (best_ms == 0.f || ms < best_ms) {
+              best_ms = ms;
+              best.cols[d] = bn;
+              best.grid[d] = gr;
+            }
+          }
+        }
+      }
+      apply_config(B, best);
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
     }
-    if (B->graph) {
-      (void)hipGraphDestroy(B->graph);
-      B->graph = nullptr;
-    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (out) *out = best;
   });
 }
 

@@ -1,13 +1,15 @@
 // Longitude real FFTs fused with the parity fold (analysis) and unfold
 // (synthesis).
 //
-// The waist is stored m-tiled:  W[c][m/8][p][i][m%8]  (fp32), per data
-// column c (2*field + re/im).  A block holds `rp` wavefronts; wavefront w owns
-// folded row i0 + w (the mirror pair north = nlat-1-i, south = i) and runs
-// its own FFT in its own LDS slice, so rp FFTs advance together between the
-// block barriers of the Stockham passes.  Neighbouring wavefronts write
-// neighbouring 32-byte runs of each m-tile, which meet in L2 as 128-byte
-// lines.
+// The waist is stored m-tiled:  W[c][m/4][p][i][m%4], per data column c
+// (2*field + re/im): fp32, or for the split-precision plans the analysis
+// waist as bf16 pieces in planes (the form the GEMM reads).
+//
+// A block holds `rp` wavefronts; wavefront w owns folded row i0 + w (the
+// mirror pair north = nlat-1-i, south = i) and runs its own FFT in its own
+// LDS slice, so rp FFTs advance together between the block barriers of the
+// Stockham passes.  Neighbouring wavefronts write neighbouring runs of each
+// m-tile (4 orders per latitude), which meet in L2 as whole sectors.
 //
 // Two real rows ride one complex FFT of length N = nlon:
 //   z = north + i*south,  X_N[m] = (Z[m] + conj Z[N-m]) / 2,
@@ -21,6 +23,9 @@
 // Stockham autosort, radix 4/2/3 butterflies and a generic-radix DFT for any
 // other prime factor, twiddles from a table of e^{-2 pi i k/N} computed in
 // FP64 and rounded once.
+#include <type_traits>
+#include <utility>
+
 #include "sht_kernels.h"
 
 namespace sht {
@@ -262,12 +267,20 @@ __global__ void __launch_bounds__(kFftThreads) rfft_fold_kernel(FoldArgs a) {
           o[src] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
         }
       }
+      auto put_w = [&](int c, int p, float v) {
+        const size_t idx = waist_index(c, a.job.ldw, a.JHP, m, p, i);
+        if (a.job.split == 0)
+          a.job.waist0[idx] = v;
+        else
+          store_bf16_pieces(reinterpret_cast<unsigned short*>(a.job.waist0) + idx,
+                            a.job.pstride, a.job.split, v);
+      };
       if (!a.job.vector) {
         const int c = a.job.col0 + 2 * item;
-        a.job.waist0[waist_index(c, a.job.ldw, a.JHP, m, 0, i)] = e[0].x;
-        a.job.waist0[waist_index(c + 1, a.job.ldw, a.JHP, m, 0, i)] = e[0].y;
-        a.job.waist0[waist_index(c, a.job.ldw, a.JHP, m, 1, i)] = o[0].x;
-        a.job.waist0[waist_index(c + 1, a.job.ldw, a.JHP, m, 1, i)] = o[0].y;
+        put_w(c, 0, e[0].x);
+        put_w(c + 1, 0, e[0].y);
+        put_w(c, 1, o[0].x);
+        put_w(c + 1, 1, o[0].y);
       } else {
         // level columns [iV, iU] = [-V.im, V.re, -U.im, U.re] (fold p); the
         // dP/dphi segment reads [U, -V] = [a3, -a2, -a1, a0] of the same
@@ -279,8 +292,7 @@ __global__ void __launch_bounds__(kFftThreads) rfft_fold_kernel(FoldArgs a) {
           const float2 V = p == 0 ? e[1] : o[1];
           const float va[4] = {-V.y, V.x, -U.y, U.x};
 #pragma unroll
-          for (int cl = 0; cl < 4; ++cl)
-            a.job.waist0[waist_index(c + cl, a.job.ldw, a.JHP, m, p, i)] = va[cl];
+          for (int cl = 0; cl < 4; ++cl) put_w(c + cl, p, va[cl]);
         }
       }
     }
@@ -393,6 +405,397 @@ __global__ void __launch_bounds__(kFftThreads) irfft_unfold_kernel(UnfoldArgs a)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-resident FFTs of N = 64 L points (L = 2, 3, 4, 6, 8, 12, 16, 24)
+// ---------------------------------------------------------------------------
+// One wavefront transforms F sequences on its own, with no block barrier:
+// with n = 64 n1 + n2 and k = k1 + L (c + 8 d), n2 = 8 a + b,
+//   A (registers): lane n2 holds x[64 n1 + n2], runs the L-point DFT over n1
+//                  and multiplies by w_N^(n2 k1);
+//   B (LDS transpose): the 8-point DFT over a of each (k1, b), times
+//                  w_64^(b c);
+//   C (LDS transpose): the 8-point DFT over b of each (k1, c), giving
+//                  X[k1 + L c + 8 L d], d = 0..7, stored in natural order.
+// The small DFTs are unrolled at compile time (mixed radix 4/3/2, constant
+// twiddles); the w_N and w_64 twiddles come from the block's copy of the FP64
+// table.  Each wavefront owns an LDS span of L rows of 72 points.
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double taylor_cos(double x) {
+  double t = 1.0, s = 1.0;
+  for (int n = 1; n < 40; ++n) {
+    t *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+    s += t;
+  }
+  return s;
+}
+constexpr double taylor_sin(double x) {
+  double t = x, s = x;
+  for (int n = 1; n < 40; ++n) {
+    t *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+    s += t;
+  }
+  return s;
+}
+// cos and sin of 2 pi j / 48, rounded once to fp32 (exact zeros snapped)
+struct Ring48 {
+  float c[48];
+  float s[48];
+};
+constexpr Ring48 make_ring48() {
+  Ring48 r{};
+  for (int j = 0; j < 48; ++j) {
+    const double x = kTwoPi * j / 48.0;
+    const double c = taylor_cos(x), sn = taylor_sin(x);
+    r.c[j] = (float)((c < 1e-12 && c > -1e-12) ? 0.0 : c);
+    r.s[j] = (float)((sn < 1e-12 && sn > -1e-12) ? 0.0 : sn);
+  }
+  return r;
+}
+constexpr Ring48 kRing48 = make_ring48();
+
+template <class Fn, int... I>
+__device__ __forceinline__ void static_for_impl(Fn&& fn, std::integer_sequence<int, I...>) {
+  (fn(std::integral_constant<int, I>{}), ...);
+}
+// fn(integral_constant<int, i>) for i = 0 .. N-1, unrolled at compile time
+template <int N, class Fn>
+__device__ __forceinline__ void static_for(Fn&& fn) {
+  static_for_impl(fn, std::make_integer_sequence<int, N>{});
+}
+
+// x * e^(S 2 pi i E / R), R | 48
+template <int E, int R, int S>
+__device__ __forceinline__ float2 rot(float2 x) {
+  constexpr int e = ((E % R) + R) % R;
+  if constexpr (e == 0) {
+    return x;
+  } else if constexpr (2 * e == R) {
+    return make_float2(-x.x, -x.y);
+  } else if constexpr (4 * e == R) {  // S i
+    return S > 0 ? make_float2(-x.y, x.x) : make_float2(x.y, -x.x);
+  } else if constexpr (4 * e == 3 * R) {  // -S i
+    return S > 0 ? make_float2(x.y, -x.x) : make_float2(-x.y, x.x);
+  } else {
+    constexpr int j = e * (48 / R);
+    constexpr float c = kRing48.c[j], sn = S * kRing48.s[j];
+    return make_float2(x.x * c - x.y * sn, x.x * sn + x.y * c);
+  }
+}
+
+// In-register DFT of R points, natural order in and out, sign S.
+template <int R, int S>
+__device__ __forceinline__ void dft(float2 (&v)[R]) {
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 3) {
+    constexpr float h = 0.86602540378443864676f * S;  // sin(S 2 pi / 3)
+    const float2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    const float2 m1 = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+    const float2 m2 = make_float2(-h * t2.y, h * t2.x);
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(m1, m2);
+    v[2] = csub(m1, m2);
+  } else if constexpr (R == 4) {
+    const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    const float2 a2 = cadd(v[1], v[3]), a3 = rot<1, 4, S>(csub(v[1], v[3]));
+    v[0] = cadd(a0, a2);
+    v[2] = csub(a0, a2);
+    v[1] = cadd(a1, a3);
+    v[3] = csub(a1, a3);
+  } else {
+    // n = Q n1 + n2, k = k1 + P k2:
+    // X[k1 + P k2] = sum_n2 w_Q^(n2 k2) w_R^(n2 k1) sum_n1 x[Q n1 + n2] w_P^(n1 k1)
+    constexpr int P = (R % 4 == 0) ? 4 : ((R % 3 == 0) ? 3 : 2);
+    constexpr int Q = R / P;
+    float2 y[R];
+    static_for<Q>([&](auto n2_) {
+      constexpr int n2 = decltype(n2_)::value;
+      float2 t[P];
+      static_for<P>([&](auto n1_) { t[n1_] = v[Q * decltype(n1_)::value + n2]; });
+      dft<P, S>(t);
+      static_for<P>([&](auto k1_) {
+        constexpr int k1 = decltype(k1_)::value;
+        y[n2 * P + k1] = rot<n2 * k1, R, S>(t[k1]);
+      });
+    });
+    static_for<P>([&](auto k1_) {
+      constexpr int k1 = decltype(k1_)::value;
+      float2 t[Q];
+      static_for<Q>([&](auto n2_) { t[n2_] = y[decltype(n2_)::value * P + k1]; });
+      dft<Q, S>(t);
+      static_for<Q>([&](auto k2_) { v[k1 + P * decltype(k2_)::value] = t[k2_]; });
+    });
+  }
+}
+
+// Order this wavefront's LDS accesses (LDS operations of one wavefront
+// complete in issue order once lgkmcnt drains).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kRowSpan = 72;  // LDS row of the transposes: 64 points + 8 pad
+template <int L>
+constexpr int wave_span() { return L * kRowSpan; }  // float2 per sequence (>= 64 L)
+
+// F sequences of N = 64 L points: v[f][n1] = x_f[64 n1 + lane] on entry; on
+// return buf[f * span + k] holds X_f[k], k < N.
+template <int L, int F, int S>
+__device__ __forceinline__ void wave_fft64(float2 (&v)[F][L], float2* buf, const float2* tw,
+                                           int lane) {
+  constexpr int SP = wave_span<L>();
+  static_for<F>([&](auto f_) {
+    constexpr int f = decltype(f_)::value;
+    dft<L, S>(v[f]);
+    static_for<L>([&](auto k1_) {
+      constexpr int k1 = decltype(k1_)::value;
+      float2 x = v[f][k1];
+      if constexpr (k1 > 0) x = cmul(x, twiddle(tw, lane * k1, S));
+      buf[f * SP + k1 * kRowSpan + lane] = x;
+    });
+  });
+  wave_sync();
+  constexpr int U = F * L * 8, RND = (U + 63) / 64;
+  float2 w[RND][8];
+  // B: units (f, k1, b)
+  static_for<RND>([&](auto r_) {
+    constexpr int r = decltype(r_)::value;
+    const int u = lane + 64 * r;
+    if (u < U) {
+      const int f = u / (8 * L), k1 = (u >> 3) % L, b = u & 7;
+      const float2* row = buf + f * SP + k1 * kRowSpan + b;
+      static_for<8>([&](auto a_) { w[r][a_] = row[8 * decltype(a_)::value]; });
+      dft<8, S>(w[r]);
+      static_for<8>([&](auto c_) {
+        constexpr int c = decltype(c_)::value;
+        if constexpr (c > 0) w[r][c] = cmul(w[r][c], twiddle(tw, b * c * L, S));
+      });
+    }
+  });
+  wave_sync();
+  static_for<RND>([&](auto r_) {
+    constexpr int r = decltype(r_)::value;
+    const int u = lane + 64 * r;
+    if (u < U) {
+      const int f = u / (8 * L), k1 = (u >> 3) % L, b = u & 7;
+      float2* row = buf + f * SP + k1 * kRowSpan + b;
+      static_for<8>([&](auto c_) { row[8 * decltype(c_)::value] = w[r][c_]; });
+    }
+  });
+  wave_sync();
+  // C: units (f, k1, c)
+  static_for<RND>([&](auto r_) {
+    constexpr int r = decltype(r_)::value;
+    const int u = lane + 64 * r;
+    if (u < U) {
+      const int f = u / (8 * L), k1 = (u >> 3) % L, c = u & 7;
+      const float2* row = buf + f * SP + k1 * kRowSpan + 8 * c;
+      static_for<8>([&](auto b_) { w[r][b_] = row[decltype(b_)::value]; });
+      dft<8, S>(w[r]);
+    }
+  });
+  wave_sync();
+  static_for<RND>([&](auto r_) {
+    constexpr int r = decltype(r_)::value;
+    const int u = lane + 64 * r;
+    if (u < U) {
+      const int f = u / (8 * L), k1 = (u >> 3) % L, c = u & 7;
+      float2* out = buf + f * SP + k1 + L * c;
+      static_for<8>([&](auto d_) { out[8 * L * decltype(d_)::value] = w[r][d_]; });
+    }
+  });
+  wave_sync();
+}
+
+constexpr int kWaveBlock = 4;  // wavefronts per block of the wave-resident kernels
+
+// Analysis: one wavefront per (field, folded row); F = 2 transforms u and v
+// of a vector level together.
+template <int L, int F>
+__global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave_kernel(FoldArgs a) {
+  extern __shared__ float2 lds[];
+  constexpr int N = 64 * L, SP = wave_span<L>();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  float2* tw = lds;
+  float2* buf = lds + N + (size_t)w * F * SP;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
+  __syncthreads();
+  const int item = blockIdx.x * kWaveBlock + w;
+  if (item >= a.job.nfields * a.JHP) return;  // no block barrier below
+  const int fld = item / a.JHP, i = item % a.JHP;
+  const bool live = i < a.JH;
+  const int T1 = a.T + 1;
+  const int Tpad = ((T1 + kMT - 1) / kMT) * kMT;
+  const size_t plane = (size_t)a.nlat * N;
+  if (live) {
+    float2 v[F][L];
+    static_for<F>([&](auto f_) {
+      constexpr int f = decltype(f_)::value;
+      const float* g = (f == 0 ? a.job.grid0 : a.job.grid1) + (size_t)fld * plane;
+      const float* rn = g + (size_t)(a.nlat - 1 - i) * N;
+      const float* rs = g + (size_t)i * N;
+      static_for<L>([&](auto n1_) {
+        constexpr int n1 = decltype(n1_)::value;
+        v[f][n1] = make_float2(rn[64 * n1 + lane], rs[64 * n1 + lane]);
+      });
+    });
+    wave_fft64<L, F, -1>(v, buf, tw, lane);
+  }
+  // Fold and write one m-tile (4 orders) per lane and round: per column and
+  // parity 4 contiguous values (16 bytes fp32, 8 bytes per bf16 piece).
+  const float s = live ? a.job.rowscale[i] : 0.f;
+  typedef __attribute__((ext_vector_type(4))) float f4;
+  typedef __attribute__((ext_vector_type(4))) unsigned short h4;
+  auto put4 = [&](int c, int p, int mt, f4 val) {
+    const size_t idx = waist_index(c, a.job.ldw, a.JHP, mt * kMT, p, i);
+    if (a.job.split == 0) {
+      *reinterpret_cast<f4*>(a.job.waist0 + idx) = val;
+    } else {
+      unsigned short* h = reinterpret_cast<unsigned short*>(a.job.waist0) + idx;
+      h4 p0, p1, p2;
+#pragma unroll
+      for (int z = 0; z < 4; ++z) {
+        const __bf16 b0 = (__bf16)val[z];
+        const float r = val[z] - (float)b0;
+        const __bf16 b1 = (__bf16)r;
+        p0[z] = __builtin_bit_cast(unsigned short, b0);
+        p1[z] = __builtin_bit_cast(unsigned short, b1);
+        p2[z] = __builtin_bit_cast(unsigned short, (__bf16)(r - (float)b1));
+      }
+      *reinterpret_cast<h4*>(h) = p0;
+      *reinterpret_cast<h4*>(h + a.job.pstride) = p1;
+      if (a.job.split > 2) *reinterpret_cast<h4*>(h + 2 * a.job.pstride) = p2;
+    }
+  };
+  static_assert(kMT == 4, "m-tile of four orders");
+  for (int mt = lane; mt * kMT < Tpad; mt += 64) {
+    // e[f][z], o[f][z]: folded values of order 4 mt + z
+    float2 e[2][4], o[2][4];
+#pragma unroll
+    for (int z = 0; z < 4; ++z) {
+      const int m = mt * kMT + z;
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        e[f][z] = make_float2(0.f, 0.f);
+        o[f][z] = make_float2(0.f, 0.f);
+        if (f < F && live && m < T1) {
+          const float2 zm = buf[f * SP + m];
+          const float2 zc = cconj(buf[f * SP + (N - m) % N]);
+          const float2 xn = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
+          const float2 d = csub(zm, zc);
+          const float2 xs = make_float2(0.5f * d.y, -0.5f * d.x);
+          e[f][z] = make_float2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
+          o[f][z] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
+        }
+      }
+    }
+    if constexpr (F == 1) {
+      const int c = a.job.col0 + 2 * fld;
+      put4(c, 0, mt, f4{e[0][0].x, e[0][1].x, e[0][2].x, e[0][3].x});
+      put4(c + 1, 0, mt, f4{e[0][0].y, e[0][1].y, e[0][2].y, e[0][3].y});
+      put4(c, 1, mt, f4{o[0][0].x, o[0][1].x, o[0][2].x, o[0][3].x});
+      put4(c + 1, 1, mt, f4{o[0][0].y, o[0][1].y, o[0][2].y, o[0][3].y});
+    } else {
+      // level columns [iV, iU] = [-V.im, V.re, -U.im, U.re] (fold p)
+      const int c = a.job.col0 + 4 * fld;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const float2(&U)[4] = p == 0 ? e[0] : o[0];
+        const float2(&V)[4] = p == 0 ? e[1] : o[1];
+        put4(c, p, mt, f4{-V[0].y, -V[1].y, -V[2].y, -V[3].y});
+        put4(c + 1, p, mt, f4{V[0].x, V[1].x, V[2].x, V[3].x});
+        put4(c + 2, p, mt, f4{-U[0].y, -U[1].y, -U[2].y, -U[3].y});
+        put4(c + 3, p, mt, f4{U[0].x, U[1].x, U[2].x, U[3].x});
+      }
+    }
+  }
+}
+
+// Synthesis: one wavefront per (field, folded row).
+template <int L>
+__global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(UnfoldArgs a) {
+  extern __shared__ float2 lds[];
+  constexpr int N = 64 * L, SP = wave_span<L>();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  float2* tw = lds;
+  float2* buf = lds + N + (size_t)w * SP;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
+  __syncthreads();
+  const int item = blockIdx.x * kWaveBlock + w;
+  if (item >= a.job.nfields * a.JH) return;  // no block barrier below
+  const int fld = item / a.JH, i = item % a.JH;
+  const int T1 = a.T + 1;
+  const int c = a.job.col0 + a.job.fstride * fld;
+  const float* W = a.job.waist;
+  const float s = a.job.rowscale[i];
+  // Z[k] = A + iB at k = m, conj(A) + i conj(B) at k = N - m; zero above T
+  for (int k = lane; k < N; k += 64) buf[k] = make_float2(0.f, 0.f);
+  wave_sync();
+  // one m-tile (4 orders) per lane and round: 16-byte waist reads
+  typedef __attribute__((ext_vector_type(4))) float f4;
+  static_assert(kMT == 4, "m-tile of four orders");
+  for (int mt = lane; mt * kMT < T1; mt += 64) {
+    const f4 er = *reinterpret_cast<const f4*>(W + waist_index(c, a.job.ldw, a.JHP, mt * kMT, 0, i));
+    const f4 ei = *reinterpret_cast<const f4*>(W + waist_index(c + 1, a.job.ldw, a.JHP, mt * kMT, 0, i));
+    const f4 orr = *reinterpret_cast<const f4*>(W + waist_index(c, a.job.ldw, a.JHP, mt * kMT, 1, i));
+    const f4 oi = *reinterpret_cast<const f4*>(W + waist_index(c + 1, a.job.ldw, a.JHP, mt * kMT, 1, i));
+#pragma unroll
+    for (int z = 0; z < 4; ++z) {
+      const int m = mt * kMT + z;
+      if (m >= T1) break;
+      float2 A = make_float2((er[z] + orr[z]) * s, (ei[z] + oi[z]) * s);
+      float2 B = make_float2((er[z] - orr[z]) * s, (ei[z] - oi[z]) * s);
+      if (a.job.imul) {
+        A = make_float2(-A.y, A.x);
+        B = make_float2(-B.y, B.x);
+      }
+      if (m == 0) {
+        A.y = 0.f;
+        B.y = 0.f;
+      }
+      buf[m] = make_float2(A.x - B.y, A.y + B.x);
+      if (m > 0) buf[N - m] = make_float2(A.x + B.y, -A.y + B.x);
+    }
+  }
+  wave_sync();
+  float2 v[1][L];
+  static_for<L>([&](auto n1_) { v[0][n1_] = buf[64 * decltype(n1_)::value + lane]; });
+  wave_sync();
+  wave_fft64<L, 1, +1>(v, buf, tw, lane);
+  const size_t plane = (size_t)a.nlat * N;
+  float* rn = a.job.grid + (size_t)fld * plane + (size_t)(a.nlat - 1 - i) * N;
+  float* rs = a.job.grid + (size_t)fld * plane + (size_t)i * N;
+  for (int k = lane; k < N; k += 64) {
+    const float2 x = buf[k];
+    rn[k] = x.x;
+    rs[k] = x.y;
+  }
+}
+
+// L of the wave-resident path for N, or 0 (generic path).
+int wave_L(int N) {
+  if (N % 64) return 0;
+  const int L = N / 64;
+  for (int l : {2, 3, 4, 6, 8, 12, 16, 24})
+    if (L == l) return L;
+  return 0;
+}
+size_t fold_wave_lds_bytes(int N, int vector) {
+  const int L = N / 64;
+  return ((size_t)N + (size_t)kWaveBlock * (vector ? 2 : 1) * L * kRowSpan) * sizeof(float2);
+}
+size_t unfold_wave_lds_bytes(int N) {
+  const int L = N / 64;
+  return ((size_t)N + (size_t)kWaveBlock * L * kRowSpan) * sizeof(float2);
+}
+
 size_t fold_lds_bytes(int N, int T, int vector, int rp) {
   (void)T;
   return (size_t)(1 + rp * (vector ? 3 : 2)) * N * sizeof(float2);
@@ -424,7 +827,34 @@ int fft_pf(int N) {
     default: hipLaunchKernelGGL(sht::KERNEL<0>, __VA_ARGS__); break;     \
   }
 
+#define SHT_L_DISPATCH(L, LAUNCH) \
+  switch (L) {                     \
+    case 2: LAUNCH(2); break;      \
+    case 3: LAUNCH(3); break;      \
+    case 4: LAUNCH(4); break;      \
+    case 6: LAUNCH(6); break;      \
+    case 8: LAUNCH(8); break;      \
+    case 12: LAUNCH(12); break;    \
+    case 16: LAUNCH(16); break;    \
+    case 24: LAUNCH(24); break;    \
+    default: return hipErrorInvalidValue; \
+  }
+
 extern "C" hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s) {
+  if (const int L = a->fft.wave_L) {
+    const int items = a->job.nfields * a->JHP;
+    if (items == 0) return hipSuccess;
+    const dim3 g((items + sht::kWaveBlock - 1) / sht::kWaveBlock), b(64 * sht::kWaveBlock);
+    const size_t lds = sht::fold_wave_lds_bytes(a->fft.N, a->job.vector);
+#define SHT_FOLD(LL)                                                                      \
+  if (a->job.vector)                                                                      \
+    hipLaunchKernelGGL((sht::rfft_fold_wave_kernel<LL, 2>), g, b, lds, s, *a);            \
+  else                                                                                    \
+    hipLaunchKernelGGL((sht::rfft_fold_wave_kernel<LL, 1>), g, b, lds, s, *a)
+    SHT_L_DISPATCH(L, SHT_FOLD)
+#undef SHT_FOLD
+    return hipGetLastError();
+  }
   size_t lds = sht::fold_lds_bytes(a->fft.N, a->T, a->job.vector, a->rp);
   int blocks = a->nchunks * a->job.nfields;
   if (blocks == 0) return hipSuccess;
@@ -434,6 +864,16 @@ extern "C" hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s
 }
 
 extern "C" hipError_t sht_launch_irfft_unfold(const sht::UnfoldArgs* a, hipStream_t s) {
+  if (const int L = a->fft.wave_L) {
+    const int items = a->job.nfields * a->JH;
+    if (items == 0) return hipSuccess;
+    const dim3 g((items + sht::kWaveBlock - 1) / sht::kWaveBlock), b(64 * sht::kWaveBlock);
+    const size_t lds = sht::unfold_wave_lds_bytes(a->fft.N);
+#define SHT_UNFOLD(LL) hipLaunchKernelGGL((sht::irfft_unfold_wave_kernel<LL>), g, b, lds, s, *a)
+    SHT_L_DISPATCH(L, SHT_UNFOLD)
+#undef SHT_UNFOLD
+    return hipGetLastError();
+  }
   size_t lds = sht::unfold_lds_bytes(a->fft.N, a->T, a->rp);
   int blocks = a->nchunks * a->job.nfields;
   if (blocks == 0) return hipSuccess;
@@ -451,8 +891,29 @@ static hipError_t allow_lds_one(size_t bytes) {
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-extern "C" hipError_t sht_fft_allow_lds(size_t bytes) {
+template <int L>
+static hipError_t allow_lds_wave(size_t bytes) {
+  for (const void* k : {reinterpret_cast<const void*>(sht::rfft_fold_wave_kernel<L, 1>),
+                        reinterpret_cast<const void*>(sht::rfft_fold_wave_kernel<L, 2>),
+                        reinterpret_cast<const void*>(sht::irfft_unfold_wave_kernel<L>)}) {
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+extern "C" int sht_fft_wave_L(int N) { return sht::wave_L(N); }
+extern "C" size_t sht_fft_wave_lds_bytes(int N, int vector) {
+  return vector >= 0 ? sht::fold_wave_lds_bytes(N, vector) : sht::unfold_wave_lds_bytes(N);
+}
+
+extern "C" hipError_t sht_fft_allow_lds(size_t bytes, size_t wave_bytes) {
   hipError_t e = hipSuccess;
+  const size_t wb = wave_bytes;
+  for (hipError_t r : {allow_lds_wave<2>(wb), allow_lds_wave<3>(wb), allow_lds_wave<4>(wb),
+                       allow_lds_wave<6>(wb), allow_lds_wave<8>(wb), allow_lds_wave<12>(wb),
+                       allow_lds_wave<16>(wb), allow_lds_wave<24>(wb)})
+    if (r != hipSuccess) e = r;
   for (hipError_t r : {allow_lds_one<0>(bytes), allow_lds_one<2>(bytes), allow_lds_one<4>(bytes),
                        allow_lds_one<8>(bytes), allow_lds_one<12>(bytes), allow_lds_one<16>(bytes),
                        allow_lds_one<24>(bytes)})

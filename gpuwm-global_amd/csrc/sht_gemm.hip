@@ -3,46 +3,53 @@
 //
 // Every Legendre contraction of a batch is a list of work items: a panel (a
 // group of up to kMT = 4 consecutive orders m of one job and one output
-// parity, times one 64-row tile) and a first column (32 columns per item).
-// Per order m the group holds
+// parity, times one 64-row tile) and a first column (BN = 32 or 64 columns per
+// item).  Per order m the group holds
 //     C_m[r][c] = sum_seg sum_k A_seg,m[r][k] * B_seg,m[k][c]
 // with A a Legendre table block and B a data block; two segments express the
 // vector operators (P*m/cos and dP/dphi against differently arranged data) as
 // one accumulation.  The host lists each panel's k-steps (one order, 32 k) in
 // execution order, with byte addresses and valid extents, so the kernel does
-// no schedule arithmetic: panels and steps come in through the scalar cache,
-// tiles through bounds-checked buffer loads (reads past an extent return 0,
-// which is also how row and column edges are handled).
-//
-// Pipeline: two register stages and two LDS buffers.  While the wavefronts
-// run step j's MFMAs out of LDS, the loads of steps j+1 and j+2 are in
-// flight; step j+1 is then written to the other LDS buffer and one barrier
-// ends the step.  Each panel's list ends in two null steps (extent 0), so
-// every load is issued unconditionally.
+// no schedule arithmetic: panels come in through the scalar cache, step
+// descriptors and tiles through bounds-checked buffer loads (reads past an
+// extent return 0, which is also how row and column edges are handled).
 //
 // A step may read its B columns through a map (reverse each group of four,
 // negate some of them): the vector operators' second arrangement of the data
 // is such a map of the first, so only one copy is stored.
+//
+// Pipeline: two register stages and two LDS buffers.  While the wavefronts
+// run step j's MFMAs out of LDS, the loads of steps j+1 and j+2 and the
+// descriptor of step j+3 are in flight; step j+1 is then written to the other
+// LDS buffer and one barrier ends the step.  No scalar-memory load is issued
+// inside the loop, so LDS waits never wait on one.  Each panel's list ends in
+// two null steps (extent 0), so every load is issued unconditionally.
+//
+// LDS tiles are unpadded rows of 32 k (128 bytes fp32, 64 bytes per bf16
+// piece) whose 16-byte chunks are XOR-swizzled by row, so that the MFMA
+// fragment reads (16 rows x one chunk) and the tile writes (whole rows) hit
+// distinct banks.
 //
 // SYN (synthesis): rows = folded latitudes, k = degrees, C = waist
 //   W[c][m/4][p][i][m%4].  Each lane keeps four order accumulators; the tile
 //   is stored as 4 latitudes x 4 orders = 64 contiguous bytes per lane.
 // ANA (analysis): rows = degrees, k = folded latitudes.  The waist puts the
 //   four orders of one latitude side by side, so one B tile (32 latitudes x
-//   32 columns x 4 orders) is loaded once per k-chunk and serves the steps
+//   BN columns x 4 orders) is loaded once per k-chunk and serves the steps
 //   of all four orders; steps flagged with a B extent start a new k-chunk.
 //
-// Block tile 64 x 32, four wavefronts of 32 x 16 (two 16x16 MFMA row blocks).
+// Block tile 64 x BN, four wavefronts of 32 x BN/2 (16x16 MFMA blocks).
 // Precision modes:
 //   FP32   v_mfma_f32_16x16x4_f32, fp32 tables and data.
 //   BF16X3 v_mfma_f32_16x16x32_bf16 on hi+lo pieces: a0b0 + a0b1 + a1b0.
 //   BF16X6 three pieces each: a0b0 + (a0b1 + a1b0) + (a0b2 + a1b1 + a2b0),
-//          every term down to 2^-16 relative; the five small products
-//          accumulate apart from a0b0 and are added once at the end.
-// Tables are pre-split on the device from FP64 (sht_tables.hip); data are
-// split into pieces (round-to-nearest-even) once per block, when a tile is
-// written to LDS.  Lane l of a 16x16 MFMA: (l & 15) is the row/column and
-// (l >> 4) selects a run of 8 consecutive k; A and B use the same k mapping.
+//          every term down to 2^-16 relative.  With ACC2 the five small
+//          products accumulate apart from a0b0 and are added at the end.
+// Tables are split on the device from FP64 (sht_tables.hip) and data by the
+// kernels that write them (sht_pack.hip, sht_fft.hip), each piece the
+// round-to-nearest-even bf16 of what the previous pieces leave.  Lane l of a
+// 16x16 MFMA: (l & 15) is the row/column and (l >> 4) selects the k chunk;
+// A and B use the same k mapping.
 #include <type_traits>
 
 #include "sht_kernels.h"
@@ -50,10 +57,9 @@
 namespace sht {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
-typedef __attribute__((ext_vector_type(8))) unsigned short u16x8;
-typedef __attribute__((ext_vector_type(4))) unsigned short u16x4;
-typedef __attribute__((ext_vector_type(2))) unsigned short u16x2;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+typedef __attribute__((ext_vector_type(2))) unsigned int u32x2;
 
 // Address spaces of the device pass (the host pass only parses the kernels).
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -64,11 +70,25 @@ typedef __attribute__((ext_vector_type(4))) float f32x4;
 #define SHT_GLOBAL
 #endif
 
-constexpr int kBK = 32;            // k per step
-constexpr int kBM = 64, kBN = 32;  // block tile
-constexpr int kLdF = kBK + 4;      // fp32 LDS row stride (floats)
-constexpr int kLdH = kBK + 8;      // bf16 LDS row stride (elements)
+constexpr int kBK = 32;  // k per step
 constexpr unsigned kRsrcWord3 = 0x00020000u;  // raw buffer, 32-bit data format
+
+// A step descriptor as loaded by every lane (two 16-byte vector loads); the
+// fields are uniform and move to scalar registers with readfirstlane.
+struct GemmDesc {
+  u32x4 lo, hi;  // {a lo, a hi, b lo, b hi}, {a_bytes, b_klen, z, cmap}
+  __device__ __forceinline__ GemmStep scalar() const {
+    auto rfl = [](unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); };
+    GemmStep s;
+    s.a = ((unsigned long long)rfl(lo[1]) << 32) | rfl(lo[0]);
+    s.b = ((unsigned long long)rfl(lo[3]) << 32) | rfl(lo[2]);
+    s.a_bytes = rfl(hi[0]);
+    s.b_klen = (int)rfl(hi[1]);
+    s.z = (int)rfl(hi[2]);
+    s.cmap = (int)rfl(hi[3]);
+    return s;
+  }
+};
 
 struct GemmWork {
   const GemmPanel* panels;
@@ -82,80 +102,78 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(unsigned long long a
   return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(addr), 0, (int)bytes,
                                            (int)kRsrcWord3);
 }
-__device__ __forceinline__ f32x4 load4(__amdgpu_buffer_rsrc_t r, unsigned off) {
-  return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+__device__ __forceinline__ u32x4 load16(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+}
+__device__ __forceinline__ u32x2 load8(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
 }
 
-__device__ __forceinline__ unsigned short f2bf(float f) {
-  __bf16 h = (__bf16)f;  // round to nearest even
-  return __builtin_bit_cast(unsigned short, h);
-}
-__device__ __forceinline__ float bf2f(unsigned short b) {
-  return __uint_as_float(((unsigned int)b) << 16);
-}
-
-// x = p0 + p1 (+ p2) in bf16 pieces, each the nearest bf16 of what is left.
-template <int NP>
-__device__ __forceinline__ void split1(float x, unsigned short (&p)[3]) {
-  p[0] = f2bf(x);
-  if constexpr (NP >= 2) {
-    const float r = x - bf2f(p[0]);
-    p[1] = f2bf(r);
-    if constexpr (NP >= 3) p[2] = f2bf(r - bf2f(p[1]));
-  }
+// 16-byte chunk swizzle of an LDS row: fp32 rows have 8 chunks, bf16 rows 4.
+template <bool F32>
+__device__ __forceinline__ int swz(int row, int chunk) {
+  if constexpr (F32) return chunk ^ ((row ^ (row >> 3)) & 7);
+  else return chunk ^ (((row >> 1) ^ (row >> 3)) & 3);
 }
 
+template <int MODE, int DIR, int BN>
+struct GemmCfg {
+  static constexpr bool F32 = MODE == kFP32;
+  static constexpr int NP = F32 ? 1 : (MODE == kBF16X6 ? 3 : 2);  // stored pieces
+  static constexpr int ESZ = F32 ? 4 : 2;   // element bytes of tables and data
+  static constexpr int ROWB = kBK * ESZ;    // LDS row bytes
+  static constexpr int CHS = ROWB / 16;     // 16-byte chunks per row
+  static constexpr int BZ = DIR == 0 ? 1 : kMT;
+  static constexpr int NB = BN / 32;        // 16-column MFMA blocks per wavefront
+  static constexpr int A_TILE = kGemmBM * ROWB;        // bytes per piece
+  static constexpr int B_TILE = BZ * BN * ROWB;        // bytes per piece
+  static constexpr int B_BUFS = DIR == 0 ? 2 : 1;      // ANA keeps one k-chunk resident
+  // register stage: 16-byte registers per thread
+  static constexpr int A_PER = A_TILE / (256 * 16);    // per piece
+  static constexpr int A_REGS = NP * A_PER;
+  static constexpr int B_BYTES = B_TILE / 256;         // per thread and piece
+  static constexpr bool B_HALF = B_BYTES == 8;         // one 8-byte load per piece
+  static constexpr int B_PER = B_HALF ? 1 : B_BYTES / 16;
+  static constexpr int B_REGS = NP * B_PER;
+  static_assert(A_PER >= 1 && B_BYTES >= 8, "tile shape");
+};
+
+template <int MODE>
+__device__ __forceinline__ void mfma_step(f32x4& acc, f32x4& acc2, const u32x4 (&a)[3],
+                                          const u32x4 (&b)[3]) {
 #define SHT_MFMA_BF(ACC, X, Y)                                                                 \
   ACC = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, X),                \
                                                 __builtin_bit_cast(bf16x8, Y), ACC, 0, 0, 0)
-
-// acc (+acc2) += A-fragment x B-fragment for one 16x16 tile.
-template <int MODE>
-__device__ __forceinline__ void mfma_tile(f32x4& acc, f32x4& acc2, const float (&ax)[8],
-                                          const u16x8 (&ap)[3], const float (&bx)[8],
-                                          const u16x8 (&bp)[3]) {
   if constexpr (MODE == kFP32) {
+    // a[0], b[0] hold k chunk kg, a[1], b[1] chunk kg + 4 (4 floats each)
 #pragma unroll
-    for (int s = 0; s < 8; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[s], bx[s], acc, 0, 0, 0);
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a[h][s]),
+                                                   __uint_as_float(b[h][s]), acc, 0, 0, 0);
   } else {
     if constexpr (MODE == kBF16X6) {
-      SHT_MFMA_BF(acc2, ap[2], bp[0]);
-      SHT_MFMA_BF(acc2, ap[1], bp[1]);
-      SHT_MFMA_BF(acc2, ap[0], bp[2]);
+      SHT_MFMA_BF(acc2, a[2], b[0]);
+      SHT_MFMA_BF(acc2, a[1], b[1]);
+      SHT_MFMA_BF(acc2, a[0], b[2]);
     }
-    SHT_MFMA_BF(acc2, ap[1], bp[0]);
-    SHT_MFMA_BF(acc2, ap[0], bp[1]);
-    SHT_MFMA_BF(acc, ap[0], bp[0]);
+    SHT_MFMA_BF(acc2, a[1], b[0]);
+    SHT_MFMA_BF(acc2, a[0], b[1]);
+    SHT_MFMA_BF(acc, a[0], b[0]);
   }
+#undef SHT_MFMA_BF
 }
 
-template <int MODE, int DIR>
-struct GemmTraits {
-  static constexpr bool kF32 = MODE == kFP32;
-  static constexpr int NP = MODE == kBF16X6 ? 3 : 2;  // pieces (bf16 modes)
-  static constexpr int A_LOADS = kF32 ? 2 : NP;       // 16-byte loads per thread per A tile
-  static constexpr int B_LOADS = DIR == 0 ? 1 : 4;    // per B tile (ANA: four orders)
-  static constexpr int BZ = DIR == 0 ? 1 : kMT;       // orders per B tile
-  // LDS, in elements of the storage type
-  static constexpr int A_F = kBM * kLdF;              // fp32 A tile
-  static constexpr int A_H = NP * kBM * kLdH;         // bf16 A tile, NP pieces
-  static constexpr int B_F = BZ * kBN * kLdF;
-  static constexpr int B_H = NP * BZ * kBN * kLdH;
-  static constexpr int B_BUFS = DIR == 0 ? 2 : 1;     // ANA keeps one k-chunk resident
-};
-
-template <int MODE, int DIR>
+template <int MODE, int DIR, int BN, bool ACC2>
 __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
-  using TR = GemmTraits<MODE, DIR>;
-  constexpr bool F32 = TR::kF32;
-  constexpr int NP = TR::NP;
-  __shared__ __attribute__((aligned(16))) float sAf[2][F32 ? TR::A_F : 4];
-  __shared__ __attribute__((aligned(16))) unsigned short sAh[2][F32 ? 8 : TR::A_H];
-  __shared__ __attribute__((aligned(16))) float sBf[TR::B_BUFS][F32 ? TR::B_F : 4];
-  __shared__ __attribute__((aligned(16))) unsigned short sBh[TR::B_BUFS][F32 ? 8 : TR::B_H];
+  using C = GemmCfg<MODE, DIR, BN>;
+  constexpr bool F32 = C::F32;
+  constexpr int NP = C::NP, ESZ = C::ESZ, ROWB = C::ROWB, CHS = C::CHS, NB = C::NB;
+  __shared__ __attribute__((aligned(16))) unsigned char sA[2][NP * C::A_TILE];
+  __shared__ __attribute__((aligned(16))) unsigned char sB[C::B_BUFS][NP * C::B_TILE];
 
   const SHT_CONST GemmPanel* panels = (const SHT_CONST GemmPanel*)w.panels;
-  const SHT_CONST GemmStep* steps = (const SHT_CONST GemmStep*)w.steps;
   const SHT_CONST int2* tiles = (const SHT_CONST int2*)w.tiles;
 
   const int tid = threadIdx.x;
@@ -168,190 +186,245 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
     if (tile.x < 0) continue;
     const GemmPanel P = panels[tile.x];
     const int c0 = tile.y;
-    const int ncol = min(kBN, P.N - c0);
-    const unsigned long long coloff = (unsigned long long)c0 * P.bcs * 4;
+    const int ncol = min(BN, P.N - c0);
+    const unsigned long long coloff = (unsigned long long)c0 * P.bcs * ESZ;
     const int step0 = P.step0, nsteps = P.nsteps;
 
-    // ---- stage registers and their loads ----
-    f32x4 ra[2][TR::A_LOADS], rb[2][TR::B_LOADS];
-    auto issue = [&](f32x4(&A)[TR::A_LOADS], f32x4(&B)[TR::B_LOADS], int j) {
-      const GemmStep s = steps[step0 + j];
-      if constexpr (F32) {
-        const __amdgpu_buffer_rsrc_t r = make_rsrc(s.a, s.a_bytes);
-#pragma unroll
-        for (int u = 0; u < 2; ++u) A[u] = load4(r, (unsigned)(tid + 256 * u) * 16u);
-      } else {
-#pragma unroll
-        for (int pc = 0; pc < NP; ++pc)
-          A[pc] = load4(make_rsrc(s.a + (unsigned long long)(pc * P.piece_bytes), s.a_bytes),
-                        (unsigned)tid * 16u);
-      }
-      if constexpr (DIR == 0) {
-        const unsigned bytes =
-            s.b_klen > 0 ? (unsigned)(((ncol - 1) * P.bcs + s.b_klen) * 4) : 0u;
-        const __amdgpu_buffer_rsrc_t r = make_rsrc(s.b + coloff, bytes);
-        const int src = (tid >> 3) ^ ((s.cmap & 1) ? 3 : 0);
-        B[0] = load4(r, (unsigned)(src * P.bcs + (tid & 7) * 4) * 4u);
-      } else {
-        const unsigned bytes =
-            s.b_klen > 0 ? (unsigned)(((ncol - 1) * P.bcs + s.b_klen * kMT) * 4) : 0u;
-        const __amdgpu_buffer_rsrc_t r = make_rsrc(s.b + coloff, bytes);
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int q = tid + 256 * u;  // column q >> 4, latitude pair q & 15
-          const int src = (q >> 4) ^ ((s.cmap & 1) ? 3 : 0);
-          const unsigned off = (unsigned)(src * P.bcs + (q & 15) * 2 * kMT) * 4u;
-          B[2 * u] = load4(r, off);
-          B[2 * u + 1] = load4(r, off + 16u);
-        }
-      }
+    // ---- register stages and their loads ----
+    u32x4 ra[2][C::A_REGS], rb[2][C::B_REGS];
+    // step descriptors are read with vector loads (the vmcnt queue, in order
+    // with the tile loads) one step before they are needed, and moved to
+    // scalar registers with readfirstlane
+    const __amdgpu_buffer_rsrc_t srs =
+        make_rsrc(reinterpret_cast<unsigned long long>(w.steps) + (unsigned long long)step0 * 32,
+                  (unsigned)(nsteps + 2) * 32u);
+    auto fetch_desc = [&](int j) {
+      const unsigned off = (unsigned)min(j, nsteps + 1) * 32u;
+      return GemmDesc{load16(srs, off), load16(srs, off + 16u)};
     };
-    // A stage -> LDS buffer
-    auto put_a = [&](const f32x4(&A)[TR::A_LOADS], int buf) {
-      if constexpr (F32) {
+    auto issue = [&](u32x4(&A)[C::A_REGS], u32x4(&B)[C::B_REGS], const GemmStep& s) {
+      // A: one contiguous tile per piece
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int q = tid + 256 * u;
-          *reinterpret_cast<f32x4*>(&sAf[buf][(q >> 3) * kLdF + (q & 7) * 4]) = A[u];
-        }
-      } else {
+      for (int pc = 0; pc < NP; ++pc) {
+        const __amdgpu_buffer_rsrc_t r =
+            make_rsrc(s.a + (unsigned long long)(pc * P.piece_bytes), s.a_bytes);
 #pragma unroll
-        for (int pc = 0; pc < NP; ++pc)
-          *reinterpret_cast<f32x4*>(&sAh[buf][(pc * kBM + (tid >> 2)) * kLdH + (tid & 3) * 8]) =
-              A[pc];
+        for (int u = 0; u < C::A_PER; ++u)
+          A[pc * C::A_PER + u] = load16(r, (unsigned)(tid + 256 * u) * 16u);
       }
-    };
-    // B stage -> LDS (split into pieces for the bf16 modes); `cmap` bits
-    // 1..4 negate columns 0..3 of each group of four.
-    auto put_b = [&](const f32x4(&B)[TR::B_LOADS], int buf, int cmap) {
+      const int rev = (s.cmap & 1) ? 3 : 0;
       if constexpr (DIR == 0) {
-        const int col = tid >> 3, kq = (tid & 7) * 4;
-        const f32x4 v = ((cmap >> (1 + (col & 3))) & 1) ? -B[0] : B[0];
-        if constexpr (F32) {
-          *reinterpret_cast<f32x4*>(&sBf[buf][col * kLdF + kq]) = v;
-        } else {
-          u16x4 pv[3];
+        // packed columns, k contiguous
+        const unsigned bytes =
+            s.b_klen > 0 ? (unsigned)(((ncol - 1) * P.bcs + s.b_klen) * ESZ) : 0u;
 #pragma unroll
-          for (int s = 0; s < 4; ++s) {
-            unsigned short p[3];
-            split1<NP>(v[s], p);
+        for (int pc = 0; pc < NP; ++pc) {
+          const __amdgpu_buffer_rsrc_t r =
+              make_rsrc(s.b + coloff + (unsigned long long)(pc * P.bpiece_bytes), bytes);
+          if constexpr (C::B_HALF) {
+            const int col = tid >> 3;
+            const u32x2 v = load8(r, (unsigned)((col ^ rev) * P.bcs * ESZ + (tid & 7) * 8));
+            B[pc] = u32x4{v[0], v[1], 0u, 0u};
+          } else {
 #pragma unroll
-            for (int pc = 0; pc < NP; ++pc) pv[pc][s] = p[pc];
-          }
-#pragma unroll
-          for (int pc = 0; pc < NP; ++pc)
-            *reinterpret_cast<u16x4*>(&sBh[buf][(pc * kBN + col) * kLdH + kq]) = pv[pc];
-        }
-      } else {
-        (void)buf;
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int q = tid + 256 * u;
-          const int col = q >> 4, k2 = (q & 15) * 2;
-          const bool neg = (cmap >> (1 + (col & 3))) & 1;
-          const f32x4 v0 = neg ? -B[2 * u] : B[2 * u], v1 = neg ? -B[2 * u + 1] : B[2 * u + 1];
-#pragma unroll
-          for (int z = 0; z < kMT; ++z) {
-            if constexpr (F32) {
-              *reinterpret_cast<float2*>(&sBf[0][(z * kBN + col) * kLdF + k2]) =
-                  float2{v0[z], v1[z]};
-            } else {
-              unsigned short p0[3], p1[3];
-              split1<NP>(v0[z], p0);
-              split1<NP>(v1[z], p1);
-#pragma unroll
-              for (int pc = 0; pc < NP; ++pc)
-                *reinterpret_cast<u16x2*>(&sBh[0][((pc * kMT + z) * kBN + col) * kLdH + k2]) =
-                    u16x2{p0[pc], p1[pc]};
+            for (int u = 0; u < C::B_PER; ++u) {
+              const int q = tid + 256 * u;
+              const int col = q / CHS;
+              B[pc * C::B_PER + u] =
+                  load16(r, (unsigned)((col ^ rev) * P.bcs * ESZ + (q % CHS) * 16));
             }
           }
         }
+      } else {
+        // k-chunk of the waist: per column 32 latitudes x 4 orders; a thread
+        // takes 4 latitudes (a quad) of one column
+        const unsigned bytes =
+            s.b_klen > 0 ? (unsigned)(((ncol - 1) * P.bcs + s.b_klen * kMT) * ESZ) : 0u;
+        constexpr int QB = 4 * kMT * ESZ;  // bytes of one quad: 64 fp32, 32 bf16
+        constexpr int PER = QB / 16;
+#pragma unroll
+        for (int pc = 0; pc < NP; ++pc) {
+          const __amdgpu_buffer_rsrc_t r =
+              make_rsrc(s.b + coloff + (unsigned long long)(pc * P.bpiece_bytes), bytes);
+#pragma unroll
+          for (int u = 0; u < BN / 32; ++u) {
+            const int q = tid + 256 * u;
+            const int col = q >> 3, quad = q & 7;
+            const unsigned off = (unsigned)((col ^ rev) * P.bcs * ESZ + quad * QB);
+#pragma unroll
+            for (int h = 0; h < PER; ++h)
+              B[(pc * (BN / 32) + u) * PER + h] = load16(r, off + 16u * h);
+          }
+        }
+      }
+    };
+    // A stage -> LDS buffer (swizzled rows)
+    auto put_a = [&](const u32x4(&A)[C::A_REGS], int buf) {
+#pragma unroll
+      for (int pc = 0; pc < NP; ++pc)
+#pragma unroll
+        for (int u = 0; u < C::A_PER; ++u) {
+          const int q = tid + 256 * u;
+          const int row = q / CHS, ch = q % CHS;
+          *reinterpret_cast<u32x4*>(
+              &sA[buf][pc * C::A_TILE + row * ROWB + swz<F32>(row, ch) * 16]) = A[pc * C::A_PER + u];
+        }
+    };
+    // B stage -> LDS; `cmap` bits 1..4 negate columns 0..3 of each group of four
+    auto put_b = [&](const u32x4(&B)[C::B_REGS], int buf, int cmap) {
+      constexpr unsigned kSign = F32 ? 0x80000000u : 0x80008000u;
+      if constexpr (DIR == 0) {
+        if constexpr (C::B_HALF) {
+          const int col = tid >> 3, e = tid & 7;
+          const unsigned flip = ((cmap >> (1 + (col & 3))) & 1) ? kSign : 0u;
+#pragma unroll
+          for (int pc = 0; pc < NP; ++pc)
+            *reinterpret_cast<u32x2*>(&sB[buf][pc * C::B_TILE + col * ROWB +
+                                               swz<F32>(col, e >> 1) * 16 + (e & 1) * 8]) =
+                u32x2{B[pc][0] ^ flip, B[pc][1] ^ flip};
+        } else {
+#pragma unroll
+          for (int pc = 0; pc < NP; ++pc)
+#pragma unroll
+            for (int u = 0; u < C::B_PER; ++u) {
+              const int q = tid + 256 * u;
+              const int col = q / CHS, ch = q % CHS;
+              const unsigned flip = ((cmap >> (1 + (col & 3))) & 1) ? kSign : 0u;
+              *reinterpret_cast<u32x4*>(
+                  &sB[buf][pc * C::B_TILE + col * ROWB + swz<F32>(col, ch) * 16]) =
+                  B[pc * C::B_PER + u] ^ flip;
+            }
+        }
+      } else {
+        (void)buf;
+        constexpr int PER = 4 * kMT * ESZ / 16;
+#pragma unroll
+        for (int pc = 0; pc < NP; ++pc)
+#pragma unroll
+          for (int u = 0; u < BN / 32; ++u) {
+            const int q = tid + 256 * u;
+            const int col = q >> 3, quad = q & 7;
+            const unsigned flip = ((cmap >> (1 + (col & 3))) & 1) ? kSign : 0u;
+            const u32x4* v = &B[(pc * (BN / 32) + u) * PER];
+#pragma unroll
+            for (int z = 0; z < kMT; ++z) {
+              unsigned char* row = &sB[0][pc * C::B_TILE + (z * BN + col) * ROWB];
+              if constexpr (F32) {
+                // v[h] = latitude 4 quad + h, four orders: chunk `quad` of row (z, col)
+                *reinterpret_cast<u32x4*>(row + swz<true>(col, quad) * 16) =
+                    u32x4{v[0][z], v[1][z], v[2][z], v[3][z]} ^ flip;
+              } else {
+                // v[h] = latitudes 4 quad + 2h, 2h+1 (four orders each, bf16 pairs)
+                const int sh = (z & 1) * 16;
+                const unsigned w0 = (v[0][z >> 1] >> sh) & 0xFFFFu;
+                const unsigned w1 = (v[0][2 + (z >> 1)] >> sh) & 0xFFFFu;
+                const unsigned w2 = (v[1][z >> 1] >> sh) & 0xFFFFu;
+                const unsigned w3 = (v[1][2 + (z >> 1)] >> sh) & 0xFFFFu;
+                *reinterpret_cast<u32x2*>(row + swz<false>(col, quad >> 1) * 16 + (quad & 1) * 8) =
+                    u32x2{(w0 | (w1 << 16)) ^ flip, (w2 | (w3 << 16)) ^ flip};
+              }
+            }
+          }
       }
     };
 
-    f32x4 acc[kMT][2], acc2[kMT][2];
+    f32x4 acc[kMT][2][NB], acc2[kMT][2][NB];
 #pragma unroll
     for (int a = 0; a < kMT; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-        acc2[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-      }
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int n = 0; n < NB; ++n) {
+          acc[a][b][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+          acc2[a][b][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
 
     // one step out of LDS buffer `buf` into the accumulators of order z
     auto compute = [&](int buf, int z) {
       const int bbuf = DIR == 0 ? buf : 0;
       const int bz = DIR == 0 ? 0 : z;
-      float ax[2][8], bx[8];
-      u16x8 ap[2][3], bp[3];
+      u32x4 af[2][3], bf[NB][3];
 #pragma unroll
       for (int r2 = 0; r2 < 2; ++r2) {
         const int row = wr * 32 + r2 * 16 + li;
         if constexpr (F32) {
-          const f32x4* p = reinterpret_cast<const f32x4*>(&sAf[buf][row * kLdF + 8 * kg]);
-          const f32x4 u = p[0], v = p[1];
 #pragma unroll
-          for (int s = 0; s < 4; ++s) {
-            ax[r2][s] = u[s];
-            ax[r2][4 + s] = v[s];
-          }
+          for (int h = 0; h < 2; ++h)
+            af[r2][h] = *reinterpret_cast<const u32x4*>(
+                &sA[buf][row * ROWB + swz<true>(row, kg + 4 * h) * 16]);
         } else {
 #pragma unroll
           for (int pc = 0; pc < NP; ++pc)
-            ap[r2][pc] =
-                *reinterpret_cast<const u16x8*>(&sAh[buf][(pc * kBM + row) * kLdH + 8 * kg]);
+            af[r2][pc] = *reinterpret_cast<const u32x4*>(
+                &sA[buf][pc * C::A_TILE + row * ROWB + swz<false>(row, kg) * 16]);
         }
       }
-      const int col = wc * 16 + li;
-      if constexpr (F32) {
-        const f32x4* p =
-            reinterpret_cast<const f32x4*>(&sBf[bbuf][(bz * kBN + col) * kLdF + 8 * kg]);
-        const f32x4 u = p[0], v = p[1];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          bx[s] = u[s];
-          bx[4 + s] = v[s];
+      for (int nb = 0; nb < NB; ++nb) {
+        const int col = wc * (BN / 2) + nb * 16 + li;
+        const unsigned char* base = &sB[bbuf][(bz * BN + col) * ROWB];
+        if constexpr (F32) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+            bf[nb][h] = *reinterpret_cast<const u32x4*>(base + swz<true>(col, kg + 4 * h) * 16);
+        } else {
+#pragma unroll
+          for (int pc = 0; pc < NP; ++pc)
+            bf[nb][pc] =
+                *reinterpret_cast<const u32x4*>(base + pc * C::B_TILE + swz<false>(col, kg) * 16);
         }
-      } else {
-#pragma unroll
-        for (int pc = 0; pc < NP; ++pc)
-          bp[pc] = *reinterpret_cast<const u16x8*>(
-              &sBh[bbuf][((pc * TR::BZ + bz) * kBN + col) * kLdH + 8 * kg]);
       }
 #pragma unroll
       for (int zz = 0; zz < kMT; ++zz) {
         if (zz != z) continue;
 #pragma unroll
         for (int r2 = 0; r2 < 2; ++r2)
-          mfma_tile<MODE>(acc[zz][r2], acc2[zz][r2], ax[r2], ap[r2], bx, bp);
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb) {
+            if constexpr (ACC2) mfma_step<MODE>(acc[zz][r2][nb], acc2[zz][r2][nb], af[r2], bf[nb]);
+            else mfma_step<MODE>(acc[zz][r2][nb], acc[zz][r2][nb], af[r2], bf[nb]);
+          }
       }
     };
 
     // ---- prologue: steps 0 and 1 in flight, step 0 in LDS buffer 0 ----
-    issue(ra[0], rb[0], 0);
-    issue(ra[1], rb[1], 1);
-    put_a(ra[0], 0);
-    if (DIR == 0 || steps[step0].b_klen > 0) put_b(rb[0], 0, steps[step0].cmap);
+    // cur, nxt: steps j and j+1 in scalar registers; dsc[p]: the fetched
+    // descriptor of the next step issued from register stage p.
+    GemmDesc dsc[2];
+    GemmStep cur, nxt;
+    {
+      const GemmDesc d0 = fetch_desc(0), d1 = fetch_desc(1);
+      dsc[0] = fetch_desc(2);
+      cur = d0.scalar();
+      issue(ra[0], rb[0], cur);
+      nxt = d1.scalar();
+      issue(ra[1], rb[1], nxt);
+      put_a(ra[0], 0);
+      if (DIR == 0 || cur.b_klen > 0) put_b(rb[0], 0, cur.cmap);
+    }
     __syncthreads();
 
     // ---- main loop, two steps per trip (register stage = step parity) ----
     auto body = [&](auto S, int j) {
       constexpr int s = decltype(S)::value;
-      issue(ra[s], rb[s], j + 2);
-      compute(s, steps[step0 + j].z);
+      dsc[s ^ 1] = fetch_desc(j + 3);
+      const GemmStep d2 = dsc[s].scalar();  // step j+2, fetched one step ago
+      issue(ra[s], rb[s], d2);
+      compute(s, cur.z);
       if (j + 1 < nsteps) {
-        const GemmStep nx = steps[step0 + j + 1];
         if constexpr (DIR == 0) {
-          put_b(rb[s ^ 1], s ^ 1, nx.cmap);
+          put_b(rb[s ^ 1], s ^ 1, nxt.cmap);
         } else {
-          if (nx.b_klen > 0) {
+          if (nxt.b_klen > 0) {
             __syncthreads();  // every wavefront is done with the resident B tile
-            put_b(rb[s ^ 1], 0, nx.cmap);
+            put_b(rb[s ^ 1], 0, nxt.cmap);
           }
         }
         put_a(ra[s ^ 1], s ^ 1);
       }
       __syncthreads();
+      cur = nxt;
+      nxt = d2;
     };
     for (int j = 0; j < nsteps; j += 2) {
       body(std::integral_constant<int, 0>{}, j);
@@ -360,65 +433,82 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
 
     // ---- epilogue ----
 #pragma unroll
-    for (int r2 = 0; r2 < 2; ++r2) {
-      const int col = wc * 16 + li;
-      const int row = wr * 32 + r2 * 16 + 4 * kg;
-      if (col >= ncol) continue;
-      if constexpr (DIR == 0) {
-        // rows row..row+3 x orders 0..3 = 16 contiguous floats of the waist
-        if (row >= P.rows[0]) continue;
-        SHT_GLOBAL float* cp = reinterpret_cast<SHT_GLOBAL float*>(P.c[0]) +
-                               (long long)(c0 + col) * P.ccs + (long long)row * kMT;
+    for (int r2 = 0; r2 < 2; ++r2)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          f32x4 o;
+      for (int nb = 0; nb < NB; ++nb) {
+        const int col = wc * (BN / 2) + nb * 16 + li;
+        const int row = wr * 32 + r2 * 16 + 4 * kg;
+        if (col >= ncol) continue;
+        if constexpr (DIR == 0) {
+          // rows row..row+3 x orders 0..3 = 16 contiguous floats of the waist
+          if (row >= P.rows[0]) continue;
+          SHT_GLOBAL float* cp = reinterpret_cast<SHT_GLOBAL float*>(P.c[0]) +
+                                 (long long)(c0 + col) * P.ccs + (long long)row * kMT;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            f32x4 o;
+#pragma unroll
+            for (int zz = 0; zz < kMT; ++zz) {
+              f32x4 v = acc[zz][r2][nb];
+              if constexpr (!F32 && ACC2) v = v + acc2[zz][r2][nb];
+              o[zz] = v[q];
+            }
+            *reinterpret_cast<SHT_GLOBAL f32x4*>(cp + q * kMT) = o;
+          }
+        } else {
 #pragma unroll
           for (int zz = 0; zz < kMT; ++zz) {
-            f32x4 v = acc[zz][r2];
-            if constexpr (!F32) v = v + acc2[zz][r2];
-            o[zz] = v[q];
+            if (row >= P.rows[zz]) continue;
+            f32x4 v = acc[zz][r2][nb];
+            if constexpr (!F32 && ACC2) v = v + acc2[zz][r2][nb];
+            *reinterpret_cast<SHT_GLOBAL f32x4*>(reinterpret_cast<SHT_GLOBAL float*>(P.c[zz]) +
+                                                 (long long)(c0 + col) * P.ccs + row) = v;
           }
-          *reinterpret_cast<SHT_GLOBAL f32x4*>(cp + q * kMT) = o;
-        }
-      } else {
-#pragma unroll
-        for (int zz = 0; zz < kMT; ++zz) {
-          if (row >= P.rows[zz]) continue;
-          f32x4 v = acc[zz][r2];
-          if constexpr (!F32) v = v + acc2[zz][r2];
-          *reinterpret_cast<SHT_GLOBAL f32x4*>(reinterpret_cast<SHT_GLOBAL float*>(P.c[zz]) +
-                                               (long long)(c0 + col) * P.ccs + row) = v;
         }
       }
-    }
   }
 }
-
-#undef SHT_MFMA_BF
 
 }  // namespace sht
 
-extern "C" hipError_t sht_launch_legendre_gemm(int mode, int dir, const sht::GemmPanel* panels,
-                                               const sht::GemmStep* steps, const int2* tiles,
-                                               int ntiles, int grid, hipStream_t s) {
-  if (ntiles <= 0) return hipSuccess;
-  sht::GemmWork w{panels, steps, tiles, ntiles};
-  dim3 g(grid), b(256);
-#define SHT_LAUNCH(M, D) hipLaunchKernelGGL((sht::legendre_gemm_kernel<M, D>), g, b, 0, s, w)
-  if (dir == 0) {
-    if (mode == sht::kFP32) SHT_LAUNCH(sht::kFP32, 0);
-    else if (mode == sht::kBF16X3) SHT_LAUNCH(sht::kBF16X3, 0);
-    else if (mode == sht::kBF16X6) SHT_LAUNCH(sht::kBF16X6, 0);
-    else return hipErrorInvalidValue;
+namespace {
+
+template <int MODE, int DIR>
+hipError_t launch_mode(const sht::GemmWork& w, int bn, int acc2, dim3 g, hipStream_t s) {
+  const dim3 b(256);
+  constexpr bool kSplit = MODE != sht::kFP32;
+  if (bn == 64) {
+    if (kSplit && acc2)
+      hipLaunchKernelGGL((sht::legendre_gemm_kernel<MODE, DIR, 64, kSplit>), g, b, 0, s, w);
+    else
+      hipLaunchKernelGGL((sht::legendre_gemm_kernel<MODE, DIR, 64, false>), g, b, 0, s, w);
   } else {
-    if (mode == sht::kFP32) SHT_LAUNCH(sht::kFP32, 1);
-    else if (mode == sht::kBF16X3) SHT_LAUNCH(sht::kBF16X3, 1);
-    else if (mode == sht::kBF16X6) SHT_LAUNCH(sht::kBF16X6, 1);
-    else return hipErrorInvalidValue;
+    if (kSplit && acc2)
+      hipLaunchKernelGGL((sht::legendre_gemm_kernel<MODE, DIR, 32, kSplit>), g, b, 0, s, w);
+    else
+      hipLaunchKernelGGL((sht::legendre_gemm_kernel<MODE, DIR, 32, false>), g, b, 0, s, w);
   }
-#undef SHT_LAUNCH
   return hipGetLastError();
 }
 
-extern "C" int sht_gemm_tile_rows(int dir) { (void)dir; return sht::kBM; }
-extern "C" int sht_gemm_tile_cols(int dir) { (void)dir; return sht::kBN; }
+template <int DIR>
+hipError_t launch_dir(int mode, const sht::GemmWork& w, int bn, int acc2, dim3 g,
+                      hipStream_t s) {
+  switch (mode) {
+    case sht::kFP32: return launch_mode<sht::kFP32, DIR>(w, bn, acc2, g, s);
+    case sht::kBF16X3: return launch_mode<sht::kBF16X3, DIR>(w, bn, acc2, g, s);
+    case sht::kBF16X6: return launch_mode<sht::kBF16X6, DIR>(w, bn, acc2, g, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+extern "C" hipError_t sht_launch_legendre_gemm(const sht::GemmLaunch* L, hipStream_t s) {
+  if (L->ntiles <= 0) return hipSuccess;
+  if (L->bn != 32 && L->bn != 64) return hipErrorInvalidValue;
+  const sht::GemmWork w{L->panels, L->steps, L->tiles, L->ntiles};
+  const dim3 g(L->grid);
+  return L->dir == 0 ? launch_dir<0>(L->mode, w, L->bn, L->acc2, g, s)
+                     : launch_dir<1>(L->mode, w, L->bn, L->acc2, g, s);
+}

@@ -10,6 +10,19 @@ constexpr int kMT = 4;           // m-tile of the waist layout W[c][m/4][p][i][m
 constexpr int kFftThreads = 256;
 constexpr int kMaxStages = 16;
 
+// Split a float into `np` bf16 pieces (2 or 3), each the round-to-nearest-even
+// bf16 of what the previous pieces leave (the residuals are exact in fp32),
+// and store them `pstride` elements apart.
+__device__ inline void store_bf16_pieces(unsigned short* dst, long long pstride, int np,
+                                         float x) {
+  const __bf16 h0 = (__bf16)x;
+  const float r = x - (float)h0;
+  const __bf16 h1 = (__bf16)r;
+  dst[0] = __builtin_bit_cast(unsigned short, h0);
+  dst[pstride] = __builtin_bit_cast(unsigned short, h1);
+  if (np > 2) dst[2 * pstride] = __builtin_bit_cast(unsigned short, (__bf16)(r - (float)h1));
+}
+
 // ---- tables (sht_tables.hip) ----
 // The block of order m and parity p starts at koff[2m+p] * JHP elements, holds
 // JHP folded latitudes i x KP = kp(m,p) degrees (n - m)/2, both multiples of
@@ -37,6 +50,7 @@ struct TableOut {
 // ---- FFT (sht_fft.hip) ----
 struct FftPlan {
   int N;                      // nlon
+  int wave_L;                 // N / 64 when the wave-resident path runs, else 0
   int nstages;
   int radix[kMaxStages];
   const float2* tw;           // tw[k] = exp(-2 pi i k / N), k < N
@@ -45,7 +59,10 @@ struct FftPlan {
 struct FoldJob {
   const float* grid0;         // field rows at grid0 + f*nlat*nlon (u for vector jobs)
   const float* grid1;         // v rows for vector jobs, else NULL
-  float* waist0;              // scalar: waist; vector: [iV, iU] per level
+  float* waist0;              // scalar: waist; vector: [iV, iU] per level; fp32 or
+                              // `split` bf16 pieces (unsigned short) of plane stride `pstride`
+  long long pstride;          // elements between bf16 piece planes
+  int split;                  // 0: fp32; 2 or 3: bf16 pieces
   const float* rowscale;      // per folded row: 1/nlon x quadrature weight (x 1/a)
   int nfields;                // scalar fields, or u/v level pairs
   int col0;                   // first data column of this job in the waist
@@ -100,9 +117,12 @@ struct GemmStep {
                               // group of 4 columns and bit 1+c negates column c of a group
 };
 
+constexpr int kGemmBM = 64;      // GEMM block tile rows
+
 struct GemmPanel {
   unsigned long long c[kMT];  // SYN: c[0] = waist block + r0 rows; ANA: packed block of order z + r0
   long long piece_bytes;      // bf16 piece stride of the tables
+  long long bpiece_bytes;     // bf16 piece stride of the data (B)
   int step0, nsteps;          // steps[step0 .. step0+nsteps), followed by two null steps
   int N;                      // data columns
   int bcs;                    // B column stride, floats
@@ -112,13 +132,26 @@ struct GemmPanel {
 };
 static_assert(sizeof(GemmStep) == 32, "GemmStep is one 8-dword scalar load");
 
+// One grouped-GEMM launch: precision mode, direction (0 SYN, 1 ANA), column
+// tile (32 or 64), separate accumulators for the small split products.
+struct GemmLaunch {
+  int mode, dir, bn, acc2;
+  const GemmPanel* panels;
+  const GemmStep* steps;
+  const int2* tiles;
+  int ntiles, grid;
+};
+
 // ---- spectral data movement (sht_pack.hip) ----
 enum PackMode : int { kPlain = 0, kIM = 1, kICoef = 2, kVector = 3 };
 
 struct PackJob {
   const float2* in0;          // [f][n][m]; vorticity for VECTOR
   const float2* in1;          // divergence for VECTOR
-  float* out0;                // packed ([-psi, chi] per level for VECTOR)
+  float* out0;                // packed ([-psi, chi] per level for VECTOR); fp32 or
+                              // `split` bf16 pieces (unsigned short) of plane stride `pstride`
+  long long pstride;          // elements between bf16 piece planes
+  int split;                  // 0: fp32; 2 or 3: bf16 pieces
   int nfields;                // fields, or levels for VECTOR
   int col0;
   int ldp;                    // floats per packed column
@@ -160,12 +193,10 @@ hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s);
 hipError_t sht_launch_irfft_unfold(const sht::UnfoldArgs* a, hipStream_t s);
 size_t sht_fold_lds_bytes(int N, int T, int vector, int rp);
 size_t sht_unfold_lds_bytes(int N, int T, int rp);
-hipError_t sht_fft_allow_lds(size_t bytes);
-hipError_t sht_launch_legendre_gemm(int mode, int dir, const sht::GemmPanel* panels,
-                                    const sht::GemmStep* steps, const int2* tiles, int ntiles,
-                                    int grid, hipStream_t s);
-int sht_gemm_tile_rows(int dir);
-int sht_gemm_tile_cols(int dir);
+hipError_t sht_fft_allow_lds(size_t bytes, size_t wave_bytes);
+int sht_fft_wave_L(int N);                       // 0: no wave-resident path for N
+size_t sht_fft_wave_lds_bytes(int N, int vector);
+hipError_t sht_launch_legendre_gemm(const sht::GemmLaunch* launch, hipStream_t s);
 hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s);
 hipError_t sht_launch_unpack(const sht::UnpackArgs* a, hipStream_t s);
 hipError_t sht_launch_spectral_elementwise(const float2* in, float2* out, long long total, int T,

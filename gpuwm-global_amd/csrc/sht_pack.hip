@@ -6,6 +6,9 @@
 // degrees (pack) or 32 degrees x 32 orders (unpack) of one field, so global
 // reads and writes are 64-256 byte runs on both sides.
 //
+// For the split-precision plans the packed values are written as bf16 pieces
+// (piece planes `pstride` elements apart), the form the GEMM reads.
+//
 // Fused transforms (pack), per field or per vorticity/divergence level:
 //   PLAIN   c
 //   IM      i*m*c                (reference transform.py:1752-1760)
@@ -91,7 +94,12 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
     }
     for (int cl = 0; cl < ncol; ++cl) {
       const size_t col = (size_t)a.job.col0 + (size_t)f * ncol + cl;
-      a.job.out0[col * a.job.ldp + kbeg + nidx] = o[cl];
+      const size_t idx = col * a.job.ldp + kbeg + nidx;
+      if (a.job.split == 0)
+        a.job.out0[idx] = o[cl];
+      else
+        store_bf16_pieces(reinterpret_cast<unsigned short*>(a.job.out0) + idx, a.job.pstride,
+                          a.job.split, o[cl]);
     }
   }
 }

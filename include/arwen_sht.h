@@ -156,9 +156,24 @@ int arwen_sht_batch_launch(arwen_sht_batch* batch, void* stream);
  * the last event.  Measurement hook; no reference counterpart (the
  * reference's StepProfiler times whole sections, profile.py:87-330). */
 int arwen_sht_batch_profile(arwen_sht_batch* batch, void* stream, int reps, double* ms);
-/* GEMM schedule knobs: tiles per launch grid (0 = auto).  Used by the
- * autotuner; takes effect at the next run (graphs are re-captured). */
-int arwen_sht_batch_set_grid(arwen_sht_batch* batch, int gemm_blocks);
+/* Grouped-GEMM schedule of a batch, per direction (0 synthesis, 1 analysis):
+ * column tile (32 or 64 data columns per workgroup) and persistent grid
+ * (workgroups, 0 = 8 per CU); for the split precisions, whether the small
+ * products keep their own accumulators (1, default) or share the main one
+ * (0).  Changing it rebuilds the work lists and drops the captured graph.
+ * No reference counterpart (the reference has no kernel schedule). */
+typedef struct arwen_sht_gemm_config {
+  int cols[2];
+  int grid[2];
+  int split_acc;
+} arwen_sht_gemm_config;
+int arwen_sht_batch_get_config(const arwen_sht_batch* batch, arwen_sht_gemm_config* out);
+int arwen_sht_batch_set_config(arwen_sht_batch* batch, const arwen_sht_gemm_config* config);
+/* Autotuner: times each direction's GEMM launch `reps` times on `stream` for
+ * every column tile and grid, keeps the fastest and returns it in `out`
+ * (optional).  Synchronizes; overwrites the batch's intermediate buffers. */
+int arwen_sht_batch_autotune(arwen_sht_batch* batch, void* stream, int reps,
+                             arwen_sht_gemm_config* out);
 /* Algorithmic work of the batch: Legendre FLOPs (parity folded, triangular)
  * and FFT bytes (grid + waist once each) per run. */
 int arwen_sht_batch_work(const arwen_sht_batch* batch, double* legendre_flops, double* fft_bytes);
