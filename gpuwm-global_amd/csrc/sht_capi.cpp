@@ -405,6 +405,10 @@ void build_lists(int dir, const std::vector<HostGroup>& groups, int bn, GemmList
         steps.resize(pn.step0);
         continue;
       }
+      if (pn.nsteps + 2 > sht::kGemmMaxSteps)
+        fail(ARWEN_SHT_EINVAL, "truncation too large for the GEMM step list (" +
+                                   std::to_string(pn.nsteps) + " steps in one panel, at most " +
+                                   std::to_string(sht::kGemmMaxSteps - 2) + ")");
       // two null steps: the kernel's loads run two steps ahead
       sht::GemmStep null{};
       null.a = steps[pn.step0].a;
@@ -585,7 +589,6 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       pa.koff = P->d_koff.as<int>();
       pa.T = T;
       pa.mtiles = (T + 1 + 31) / 32;
-      pa.ntiles = (P->kp(0, 0) + 31) / 32;
       pa.job.in0 = static_cast<const float2*>(J.in0);
       pa.job.in1 = static_cast<const float2*>(J.in1);
       pa.job.out0 = js->packed_a.as<float>();

@@ -617,22 +617,29 @@ __device__ __forceinline__ void wave_fft64(float2 (&v)[F][L], float2* buf, const
 constexpr int kWaveBlock = 4;  // wavefronts per block of the wave-resident kernels
 
 // Analysis: one wavefront per (field, folded row); F = 2 transforms u and v
-// of a vector level together.
+// of a vector level together.  The block's four wavefronts hold four
+// consecutive rows of one field: each folds its spectrum into registers and
+// parks the values in its LDS span, then the block writes every (column,
+// m-tile, parity) run of 4 rows x 4 orders as one contiguous piece (64 bytes
+// fp32, 32 bytes per bf16 piece).
 template <int L, int F>
 __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave_kernel(FoldArgs a) {
   extern __shared__ float2 lds[];
   constexpr int N = 64 * L, SP = wave_span<L>();
+  constexpr int NV = 4 * F;  // folded values per order: columns x parities
+  static_assert(kMT == 4 && kWaveBlock == 4, "4 rows x 4 orders per run");
+  typedef __attribute__((ext_vector_type(4))) float f4;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float2* tw = lds;
   float2* buf = lds + N + (size_t)w * F * SP;
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
+  const int item0 = blockIdx.x * kWaveBlock;  // JHP is a multiple of 4: one field per block
+  if (item0 >= a.job.nfields * a.JHP) return;  // block-uniform
   __syncthreads();
-  const int item = blockIdx.x * kWaveBlock + w;
-  if (item >= a.job.nfields * a.JHP) return;  // no block barrier below
-  const int fld = item / a.JHP, i = item % a.JHP;
+  const int fld = item0 / a.JHP, i0 = item0 % a.JHP, i = i0 + w;
   const bool live = i < a.JH;
   const int T1 = a.T + 1;
-  const int Tpad = ((T1 + kMT - 1) / kMT) * kMT;
+  const int MT = (T1 + kMT - 1) / kMT;  // m-tiles
   const size_t plane = (size_t)a.nlat * N;
   if (live) {
     float2 v[F][L];
@@ -648,110 +655,167 @@ __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave_kernel(FoldArg
     });
     wave_fft64<L, F, -1>(v, buf, tw, lane);
   }
-  // Fold and write one m-tile (4 orders) per lane and round: per column and
-  // parity 4 contiguous values (16 bytes fp32, 8 bytes per bf16 piece).
+  // fold into registers: val[r][e][z] for m-tile mt = lane + 64 r
   const float s = live ? a.job.rowscale[i] : 0.f;
-  typedef __attribute__((ext_vector_type(4))) float f4;
-  typedef __attribute__((ext_vector_type(4))) unsigned short h4;
-  auto put4 = [&](int c, int p, int mt, f4 val) {
-    const size_t idx = waist_index(c, a.job.ldw, a.JHP, mt * kMT, p, i);
-    if (a.job.split == 0) {
-      *reinterpret_cast<f4*>(a.job.waist0 + idx) = val;
-    } else {
-      unsigned short* h = reinterpret_cast<unsigned short*>(a.job.waist0) + idx;
-      h4 p0, p1, p2;
+  constexpr int RT = (N / 8 + 64) / 64;  // rounds of 64 m-tiles: MT <= N/8 + 1
+  float val[RT][NV][4];
 #pragma unroll
-      for (int z = 0; z < 4; ++z) {
-        const __bf16 b0 = (__bf16)val[z];
-        const float r = val[z] - (float)b0;
-        const __bf16 b1 = (__bf16)r;
-        p0[z] = __builtin_bit_cast(unsigned short, b0);
-        p1[z] = __builtin_bit_cast(unsigned short, b1);
-        p2[z] = __builtin_bit_cast(unsigned short, (__bf16)(r - (float)b1));
-      }
-      *reinterpret_cast<h4*>(h) = p0;
-      *reinterpret_cast<h4*>(h + a.job.pstride) = p1;
-      if (a.job.split > 2) *reinterpret_cast<h4*>(h + 2 * a.job.pstride) = p2;
-    }
-  };
-  static_assert(kMT == 4, "m-tile of four orders");
-  for (int mt = lane; mt * kMT < Tpad; mt += 64) {
-    // e[f][z], o[f][z]: folded values of order 4 mt + z
-    float2 e[2][4], o[2][4];
+  for (int r = 0; r < RT; ++r) {
+    const int mt = lane + 64 * r;
 #pragma unroll
     for (int z = 0; z < 4; ++z) {
       const int m = mt * kMT + z;
+      float2 e[2], o[2];
 #pragma unroll
       for (int f = 0; f < 2; ++f) {
-        e[f][z] = make_float2(0.f, 0.f);
-        o[f][z] = make_float2(0.f, 0.f);
-        if (f < F && live && m < T1) {
+        e[f] = make_float2(0.f, 0.f);
+        o[f] = make_float2(0.f, 0.f);
+        if (f < F && live && mt < MT && m < T1) {
           const float2 zm = buf[f * SP + m];
           const float2 zc = cconj(buf[f * SP + (N - m) % N]);
           const float2 xn = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
           const float2 d = csub(zm, zc);
           const float2 xs = make_float2(0.5f * d.y, -0.5f * d.x);
-          e[f][z] = make_float2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
-          o[f][z] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
+          e[f] = make_float2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
+          o[f] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
+        }
+      }
+      if constexpr (F == 1) {
+        // value e = 2 * column + parity
+        val[r][0][z] = e[0].x;
+        val[r][1][z] = o[0].x;
+        val[r][2][z] = e[0].y;
+        val[r][3][z] = o[0].y;
+      } else {
+        // level columns [iV, iU] = [-V.im, V.re, -U.im, U.re]; e = 2 * column + parity
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const float2 U = p == 0 ? e[0] : o[0];
+          const float2 V = p == 0 ? e[1] : o[1];
+          val[r][0 + p][z] = -V.y;
+          val[r][2 + p][z] = V.x;
+          val[r][4 + p][z] = -U.y;
+          val[r][6 + p][z] = U.x;
         }
       }
     }
-    if constexpr (F == 1) {
-      const int c = a.job.col0 + 2 * fld;
-      put4(c, 0, mt, f4{e[0][0].x, e[0][1].x, e[0][2].x, e[0][3].x});
-      put4(c + 1, 0, mt, f4{e[0][0].y, e[0][1].y, e[0][2].y, e[0][3].y});
-      put4(c, 1, mt, f4{o[0][0].x, o[0][1].x, o[0][2].x, o[0][3].x});
-      put4(c + 1, 1, mt, f4{o[0][0].y, o[0][1].y, o[0][2].y, o[0][3].y});
-    } else {
-      // level columns [iV, iU] = [-V.im, V.re, -U.im, U.re] (fold p)
-      const int c = a.job.col0 + 4 * fld;
+  }
+  wave_sync();  // the spectrum is read; the span now holds stage[e][mt][z]
+  float* stage = reinterpret_cast<float*>(buf);
 #pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const float2(&U)[4] = p == 0 ? e[0] : o[0];
-        const float2(&V)[4] = p == 0 ? e[1] : o[1];
-        put4(c, p, mt, f4{-V[0].y, -V[1].y, -V[2].y, -V[3].y});
-        put4(c + 1, p, mt, f4{V[0].x, V[1].x, V[2].x, V[3].x});
-        put4(c + 2, p, mt, f4{-U[0].y, -U[1].y, -U[2].y, -U[3].y});
-        put4(c + 3, p, mt, f4{U[0].x, U[1].x, U[2].x, U[3].x});
+  for (int r = 0; r < RT; ++r) {
+    const int mt = lane + 64 * r;
+    if (mt < MT) {
+#pragma unroll
+      for (int e = 0; e < NV; ++e)
+        *reinterpret_cast<f4*>(stage + ((size_t)e * MT + mt) * 4) =
+            f4{val[r][e][0], val[r][e][1], val[r][e][2], val[r][e][3]};
+    }
+  }
+  __syncthreads();
+  // block write: run (e, mt) = rows i0..i0+3 x orders 4 mt .. 4 mt + 3
+  const int ncolf = 2 * F;  // columns per field or level
+  for (int u = threadIdx.x; u < NV * MT; u += blockDim.x) {
+    const int e = u / MT, mt = u % MT;
+    const int c = a.job.col0 + ncolf * fld + (e >> 1), p = e & 1;
+    f4 run[4];
+#pragma unroll
+    for (int ww = 0; ww < 4; ++ww)
+      run[ww] = *reinterpret_cast<const f4*>(
+          reinterpret_cast<const float*>(lds + N + (size_t)ww * F * SP) + ((size_t)e * MT + mt) * 4);
+    const size_t idx = waist_index(c, a.job.ldw, a.JHP, mt * kMT, p, i0);  // 16 contiguous
+    if (a.job.split == 0) {
+      f4* dst = reinterpret_cast<f4*>(a.job.waist0 + idx);
+#pragma unroll
+      for (int ww = 0; ww < 4; ++ww) dst[ww] = run[ww];
+    } else {
+      typedef __attribute__((ext_vector_type(8))) unsigned short h8;
+      h8 pc[3][2];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const float x = run[q >> 2][q & 3];
+        const __bf16 b0 = (__bf16)x;
+        const float r1 = x - (float)b0;
+        const __bf16 b1 = (__bf16)r1;
+        pc[0][q >> 3][q & 7] = __builtin_bit_cast(unsigned short, b0);
+        pc[1][q >> 3][q & 7] = __builtin_bit_cast(unsigned short, b1);
+        pc[2][q >> 3][q & 7] = __builtin_bit_cast(unsigned short, (__bf16)(r1 - (float)b1));
+      }
+      unsigned short* h = reinterpret_cast<unsigned short*>(a.job.waist0) + idx;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (k >= a.job.split) break;
+        h8* dst = reinterpret_cast<h8*>(h + k * a.job.pstride);
+        dst[0] = pc[k][0];
+        dst[1] = pc[k][1];
       }
     }
   }
 }
 
-// Synthesis: one wavefront per (field, folded row).
+// Synthesis: one wavefront per (field, folded row).  The block's four
+// wavefronts hold four consecutive rows: the block reads every (column,
+// m-tile, parity) run of 4 rows x 4 orders of the waist as one contiguous
+// 64-byte piece and hands each row its values through LDS.
 template <int L>
 __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(UnfoldArgs a) {
   extern __shared__ float2 lds[];
   constexpr int N = 64 * L, SP = wave_span<L>();
+  static_assert(kMT == 4 && kWaveBlock == 4, "4 rows x 4 orders per run");
+  typedef __attribute__((ext_vector_type(4))) float f4;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float2* tw = lds;
   float2* buf = lds + N + (size_t)w * SP;
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
-  __syncthreads();
-  const int item = blockIdx.x * kWaveBlock + w;
-  if (item >= a.job.nfields * a.JH) return;  // no block barrier below
-  const int fld = item / a.JH, i = item % a.JH;
+  const int JH4 = (a.JH + 3) / 4;  // row groups of four
+  const int grp = blockIdx.x;
+  if (grp >= a.job.nfields * JH4) return;  // block-uniform
+  const int fld = grp / JH4, i0 = (grp % JH4) * 4, i = i0 + w;
+  const bool live = i < a.JH;
   const int T1 = a.T + 1;
+  const int MT = (T1 + kMT - 1) / kMT;
   const int c = a.job.col0 + a.job.fstride * fld;
   const float* W = a.job.waist;
+  // block read into each row's span: stage[e][mt][z], e = 2 * column + parity
+  for (int u = threadIdx.x; u < 4 * MT; u += blockDim.x) {
+    const int e = u / MT, mt = u % MT;
+    const f4* src = reinterpret_cast<const f4*>(
+        W + waist_index(c + (e >> 1), a.job.ldw, a.JHP, mt * kMT, e & 1, i0));
+#pragma unroll
+    for (int ww = 0; ww < 4; ++ww)
+      *reinterpret_cast<f4*>(reinterpret_cast<float*>(lds + N + (size_t)ww * SP) +
+                             ((size_t)e * MT + mt) * 4) = src[ww];
+  }
+  __syncthreads();
+  constexpr int RT = (N / 8 + 64) / 64;  // rounds of 64 m-tiles: MT <= N/8 + 1
+  f4 er[RT], ei[RT], orr[RT], oi[RT];
+  const float* stage = reinterpret_cast<const float*>(buf);
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    const int mt = lane + 64 * r;
+    if (mt < MT) {
+      er[r] = *reinterpret_cast<const f4*>(stage + ((size_t)0 * MT + mt) * 4);
+      orr[r] = *reinterpret_cast<const f4*>(stage + ((size_t)1 * MT + mt) * 4);
+      ei[r] = *reinterpret_cast<const f4*>(stage + ((size_t)2 * MT + mt) * 4);
+      oi[r] = *reinterpret_cast<const f4*>(stage + ((size_t)3 * MT + mt) * 4);
+    }
+  }
+  if (!live) return;  // no block barrier below
+  wave_sync();  // the staged values are in registers; the span becomes the spectrum
   const float s = a.job.rowscale[i];
   // Z[k] = A + iB at k = m, conj(A) + i conj(B) at k = N - m; zero above T
   for (int k = lane; k < N; k += 64) buf[k] = make_float2(0.f, 0.f);
   wave_sync();
-  // one m-tile (4 orders) per lane and round: 16-byte waist reads
-  typedef __attribute__((ext_vector_type(4))) float f4;
-  static_assert(kMT == 4, "m-tile of four orders");
-  for (int mt = lane; mt * kMT < T1; mt += 64) {
-    const f4 er = *reinterpret_cast<const f4*>(W + waist_index(c, a.job.ldw, a.JHP, mt * kMT, 0, i));
-    const f4 ei = *reinterpret_cast<const f4*>(W + waist_index(c + 1, a.job.ldw, a.JHP, mt * kMT, 0, i));
-    const f4 orr = *reinterpret_cast<const f4*>(W + waist_index(c, a.job.ldw, a.JHP, mt * kMT, 1, i));
-    const f4 oi = *reinterpret_cast<const f4*>(W + waist_index(c + 1, a.job.ldw, a.JHP, mt * kMT, 1, i));
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    const int mt = lane + 64 * r;
+    if (mt >= MT) continue;
 #pragma unroll
     for (int z = 0; z < 4; ++z) {
       const int m = mt * kMT + z;
       if (m >= T1) break;
-      float2 A = make_float2((er[z] + orr[z]) * s, (ei[z] + oi[z]) * s);
-      float2 B = make_float2((er[z] - orr[z]) * s, (ei[z] - oi[z]) * s);
+      float2 A = make_float2((er[r][z] + orr[r][z]) * s, (ei[r][z] + oi[r][z]) * s);
+      float2 B = make_float2((er[r][z] - orr[r][z]) * s, (ei[r][z] - oi[r][z]) * s);
       if (a.job.imul) {
         A = make_float2(-A.y, A.x);
         B = make_float2(-B.y, B.x);
@@ -865,9 +929,9 @@ extern "C" hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s
 
 extern "C" hipError_t sht_launch_irfft_unfold(const sht::UnfoldArgs* a, hipStream_t s) {
   if (const int L = a->fft.wave_L) {
-    const int items = a->job.nfields * a->JH;
-    if (items == 0) return hipSuccess;
-    const dim3 g((items + sht::kWaveBlock - 1) / sht::kWaveBlock), b(64 * sht::kWaveBlock);
+    const int groups = a->job.nfields * ((a->JH + 3) / 4);
+    if (groups == 0) return hipSuccess;
+    const dim3 g(groups), b(64 * sht::kWaveBlock);
     const size_t lds = sht::unfold_wave_lds_bytes(a->fft.N);
 #define SHT_UNFOLD(LL) hipLaunchKernelGGL((sht::irfft_unfold_wave_kernel<LL>), g, b, lds, s, *a)
     SHT_L_DISPATCH(L, SHT_UNFOLD)

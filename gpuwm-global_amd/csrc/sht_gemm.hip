@@ -19,11 +19,12 @@
 // is such a map of the first, so only one copy is stored.
 //
 // Pipeline: two register stages and two LDS buffers.  While the wavefronts
-// run step j's MFMAs out of LDS, the loads of steps j+1 and j+2 and the
-// descriptor of step j+3 are in flight; step j+1 is then written to the other
-// LDS buffer and one barrier ends the step.  No scalar-memory load is issued
-// inside the loop, so LDS waits never wait on one.  Each panel's list ends in
-// two null steps (extent 0), so every load is issued unconditionally.
+// run step j's MFMAs out of LDS, the loads of steps j+1 and j+2 are in
+// flight; step j+1 is then written to the other LDS buffer and one barrier
+// ends the step.  The panel's step descriptors sit in LDS for the whole tile,
+// so no memory load other than the tiles is waited on inside the loop.  Each
+// panel's list ends in two null steps (extent 0), so every load is issued
+// unconditionally.
 //
 // LDS tiles are unpadded rows of 32 k (128 bytes fp32, 64 bytes per bf16
 // piece) whose 16-byte chunks are XOR-swizzled by row, so that the MFMA
@@ -172,6 +173,7 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
   constexpr int NP = C::NP, ESZ = C::ESZ, ROWB = C::ROWB, CHS = C::CHS, NB = C::NB;
   __shared__ __attribute__((aligned(16))) unsigned char sA[2][NP * C::A_TILE];
   __shared__ __attribute__((aligned(16))) unsigned char sB[C::B_BUFS][NP * C::B_TILE];
+  __shared__ u32x4 sDesc[2 * kGemmMaxSteps];  // the current panel's step list
 
   const SHT_CONST GemmPanel* panels = (const SHT_CONST GemmPanel*)w.panels;
   const SHT_CONST int2* tiles = (const SHT_CONST int2*)w.tiles;
@@ -192,15 +194,19 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
 
     // ---- register stages and their loads ----
     u32x4 ra[2][C::A_REGS], rb[2][C::B_REGS];
-    // step descriptors are read with vector loads (the vmcnt queue, in order
-    // with the tile loads) one step before they are needed, and moved to
-    // scalar registers with readfirstlane
-    const __amdgpu_buffer_rsrc_t srs =
-        make_rsrc(reinterpret_cast<unsigned long long>(w.steps) + (unsigned long long)step0 * 32,
-                  (unsigned)(nsteps + 2) * 32u);
+    // The panel's step list (with its two null steps) is copied to LDS once
+    // per tile; a step's descriptor is then an LDS broadcast read, off the
+    // memory queue that the tile loads wait in.
+    {
+      const __amdgpu_buffer_rsrc_t srs = make_rsrc(
+          reinterpret_cast<unsigned long long>(w.steps) + (unsigned long long)step0 * 32,
+          (unsigned)(nsteps + 2) * 32u);
+      for (int e = tid; e < 2 * (nsteps + 2); e += 256) sDesc[e] = load16(srs, (unsigned)e * 16u);
+      __syncthreads();
+    }
     auto fetch_desc = [&](int j) {
-      const unsigned off = (unsigned)min(j, nsteps + 1) * 32u;
-      return GemmDesc{load16(srs, off), load16(srs, off + 16u)};
+      const int e = 2 * min(j, nsteps + 1);
+      return GemmDesc{sDesc[e], sDesc[e + 1]};
     };
     auto issue = [&](u32x4(&A)[C::A_REGS], u32x4(&B)[C::B_REGS], const GemmStep& s) {
       // A: one contiguous tile per piece

@@ -118,6 +118,7 @@ struct GemmStep {
 };
 
 constexpr int kGemmBM = 64;      // GEMM block tile rows
+constexpr int kGemmMaxSteps = 160;  // steps per panel incl. the two null steps (LDS-resident)
 
 struct GemmPanel {
   unsigned long long c[kMT];  // SYN: c[0] = waist block + r0 rows; ANA: packed block of order z + r0
@@ -162,7 +163,7 @@ struct PackJob {
 struct PackArgs {
   const int* koff;            // koff[2m+p]
   int T;
-  int mtiles, ntiles;         // ceil((T+1)/32), ceil(KP(0,0)/32)
+  int mtiles;                 // ceil((T+1)/32): 32-order tiles
   PackJob job;
 };
 

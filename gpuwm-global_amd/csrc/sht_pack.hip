@@ -2,9 +2,9 @@
 // indexed [n, m] <-> packed per-order blocks C[c][m][p][n_idx] read and written
 // by the grouped GEMM, plus the elementwise spectral operators.
 //
-// Both movers are tiled through LDS: a block owns 32 orders x 32 packed
-// degrees (pack) or 32 degrees x 32 orders (unpack) of one field, so global
-// reads and writes are 64-256 byte runs on both sides.
+// Both movers are tiled through LDS: a block owns 32 orders x 64 degrees of
+// one field (or level), so each value is read once and global reads and
+// writes are 128-256 byte runs on both sides.
 //
 // For the split-precision plans the packed values are written as bf16 pieces
 // (piece planes `pstride` elements apart), the form the GEMM reads.
@@ -28,52 +28,60 @@ namespace sht {
 
 
 
-constexpr int kTile = 32;
-constexpr int kRows = 3 * kTile - 1;  // input degrees spanned by one pack tile
+constexpr int kTile = 32;   // orders per block
+constexpr int kChunk = 64;  // degrees per pass of a block
 
+// Degree of order m, parity p, in the chunk starting at n0, for lane j: the
+// j-th degree of that parity in [n0, n0 + 64).
+__device__ __forceinline__ int chunk_degree(int n0, int m, int p, int j) {
+  return n0 + 2 * j + ((m + p - n0) & 1);
+}
+
+// One block per (field or level, 32 orders, 64 degrees from the tile's first
+// order): every input value is read once and every packed (m, p) run is
+// written as 32 consecutive degrees.
 __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
-  __shared__ float2 tile[2][kRows][kTile + 1];
+  __shared__ float2 tile[2][kChunk][kTile + 1];  // [input][n - n0][m - m0]
   const int T = a.T, T1 = T + 1;
-  int b = blockIdx.x;
-  const int nt = b % a.ntiles;
-  b /= a.ntiles;
-  const int mt = b % a.mtiles;
-  const int f = b / a.mtiles;
+  const int nchunks = (T1 + kChunk - 1) / kChunk;
+  const int c = blockIdx.x % nchunks;
+  const int mt = (blockIdx.x / nchunks) % a.mtiles;
+  const int f = blockIdx.x / nchunks / a.mtiles;
   if (f >= a.job.nfields) return;
-  const int m0 = mt * kTile, t0 = nt * kTile;
-  const int nlo = m0 + 2 * t0;
+  const int m0 = mt * kTile;
+  const int n0 = m0 + kChunk * c;
+  if (n0 > T) return;  // block-uniform
   const bool vec = a.job.mode == kVector;
   const int nin = vec ? 2 : 1;
-  const size_t plane = (size_t)T1 * T1;
-  for (int s = 0; s < nin; ++s) {
-    const float2* in = (s == 0 ? a.job.in0 : a.job.in1) + (size_t)f * plane;
-    for (int e = threadIdx.x; e < kRows * kTile; e += blockDim.x) {
-      const int r = e / kTile, mi = e % kTile;
-      const int n = nlo + r, m = m0 + mi;
-      float2 v = make_float2(0.f, 0.f);
-      if (n <= T && m <= T && m <= n) v = in[(size_t)n * T1 + m];
-      tile[s][r][mi] = v;
-    }
-  }
-  __syncthreads();
-  const double a2 = (double)a.job.radius * (double)a.job.radius;
   const int ncol = vec ? 4 : 2;
-  // outputs: (mi, p, tj) with tj fastest
-  for (int e = threadIdx.x; e < kTile * 2 * kTile; e += blockDim.x) {
-    const int tj = e % kTile;
-    const int p = (e / kTile) % 2;
-    const int mi = e / (2 * kTile);
-    const int m = m0 + mi;
-    if (m > T) continue;
-    const int nidx = t0 + tj;
-    const int kbeg = a.koff[2 * m + p];
-    const int kp = a.koff[2 * m + p + 1] - kbeg;
-    if (nidx >= kp) continue;
-    const int n = m + p + 2 * nidx;
-    const int r = mi + p + 2 * tj;  // row in the tile
-    float o[4] = {0.f, 0.f, 0.f, 0.f};
-    if (n <= T) {
-      float2 c = tile[0][r][mi];
+  const size_t plane = (size_t)T1 * T1;
+  const double a2 = (double)a.job.radius * (double)a.job.radius;
+  {
+    for (int s = 0; s < nin; ++s) {
+      const float2* in = (s == 0 ? a.job.in0 : a.job.in1) + (size_t)f * plane;
+      for (int e = threadIdx.x; e < kChunk * kTile; e += blockDim.x) {
+        const int r = e / kTile, mi = e % kTile;
+        const int n = n0 + r, m = m0 + mi;
+        float2 v = make_float2(0.f, 0.f);
+        if (n <= T && m <= T && m <= n) v = in[(size_t)n * T1 + m];
+        tile[s][r][mi] = v;
+      }
+    }
+    __syncthreads();
+    // outputs (mi, p, j), j fastest: 32 consecutive packed degrees per run
+    for (int e = threadIdx.x; e < kTile * 2 * 32; e += blockDim.x) {
+      const int j = e % 32;
+      const int p = (e / 32) % 2;
+      const int mi = e / 64;
+      const int m = m0 + mi;
+      if (m > T) continue;
+      const int n = chunk_degree(n0, m, p, j);
+      if (n > T || n < m + p) continue;
+      const int nidx = (n - m - p) >> 1;
+      const int kbeg = a.koff[2 * m + p];
+      const int r = n - n0;
+      float o[4] = {0.f, 0.f, 0.f, 0.f};
+      const float2 c = tile[0][r][mi];
       if (a.job.mode == kPlain) {
         o[0] = c.x; o[1] = c.y;
       } else if (a.job.mode == kIM) {
@@ -81,7 +89,7 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
       } else if (a.job.mode == kICoef) {
         o[0] = -c.y; o[1] = c.x;
       } else {
-        float2 d = tile[1][r][mi];
+        const float2 d = tile[1][r][mi];
         float2 psi = make_float2(0.f, 0.f), chi = make_float2(0.f, 0.f);
         if (n >= 1) {
           const float eig = (float)(-((double)n * ((double)n + 1.0)) / a2);
@@ -91,67 +99,66 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
         }
         o[0] = -psi.x; o[1] = -psi.y; o[2] = chi.x; o[3] = chi.y;
       }
-    }
-    for (int cl = 0; cl < ncol; ++cl) {
-      const size_t col = (size_t)a.job.col0 + (size_t)f * ncol + cl;
-      const size_t idx = col * a.job.ldp + kbeg + nidx;
-      if (a.job.split == 0)
-        a.job.out0[idx] = o[cl];
-      else
-        store_bf16_pieces(reinterpret_cast<unsigned short*>(a.job.out0) + idx, a.job.pstride,
-                          a.job.split, o[cl]);
+      for (int cl = 0; cl < ncol; ++cl) {
+        const size_t col = (size_t)a.job.col0 + (size_t)f * ncol + cl;
+        const size_t idx = col * a.job.ldp + kbeg + nidx;
+        if (a.job.split == 0)
+          a.job.out0[idx] = o[cl];
+        else
+          store_bf16_pieces(reinterpret_cast<unsigned short*>(a.job.out0) + idx, a.job.pstride,
+                            a.job.split, o[cl]);
+      }
     }
   }
 }
 
-
-
-
-
+// One block per (field or level, 32 orders, 64 rows).  Entries above the
+// triangle (m > n) are written as zeros; chunks that lie wholly above it need
+// no reads.
 __global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs a) {
-  __shared__ float tile[4][kTile][kTile + 1];  // [col][n - n0][m - m0]
+  __shared__ float tile[4][kChunk][kTile + 1];  // [col][n - n0][m - m0]
   const int T = a.T, T1 = T + 1;
-  int b = blockIdx.x;
-  const int mt = b % a.tiles;
-  b /= a.tiles;
-  const int ntl = b % a.tiles;
-  const int f = b / a.tiles;
+  const int nchunks = (T1 + kChunk - 1) / kChunk;
+  const int c = blockIdx.x % nchunks;
+  const int mt = (blockIdx.x / nchunks) % a.tiles;
+  const int f = blockIdx.x / nchunks / a.tiles;
   if (f >= a.job.nfields) return;
-  const int n0 = ntl * kTile, m0 = mt * kTile;
+  const int m0 = mt * kTile;
+  const int n0 = kChunk * c;
   const int ncol = a.job.vector ? 4 : 2;
-  // Zero the tile, then gather each (m, p) run of packed degrees.
-  for (int e = threadIdx.x; e < 4 * kTile * (kTile + 1); e += blockDim.x)
-    (&tile[0][0][0])[e] = 0.f;
-  __syncthreads();
-  for (int e = threadIdx.x; e < ncol * kTile * 2 * 16; e += blockDim.x) {
-    const int j = e % 16;
-    const int p = (e / 16) % 2;
-    const int mi = (e / 32) % kTile;
-    const int cl = e / (32 * kTile);
-    const int m = m0 + mi;
-    if (m > T) continue;
-    // first packed index whose degree is >= n0
-    int base = n0 - m - p;
-    int jlo = base <= 0 ? 0 : (base + 1) / 2;
-    const int nidx = jlo + j;
-    const int n = m + p + 2 * nidx;
-    if (n < n0 || n >= n0 + kTile || n > T) continue;
-    const int kbeg = a.koff[2 * m + p];
-    const size_t col = (size_t)a.job.col0 + (size_t)f * ncol + cl;
-    float v = a.job.in[col * a.job.ldp + kbeg + nidx];
-    if (m == 0 && (cl & 1)) v = 0.f;  // m = 0 realified
-    tile[cl][n - n0][mi] = v;
-  }
-  __syncthreads();
-  const size_t plane = (size_t)T1 * T1;
   const int nout = a.job.vector ? 2 : 1;
-  for (int o = 0; o < nout; ++o) {
-    float2* out = (o == 0 ? a.job.out0 : a.job.out1) + (size_t)f * plane;
-    for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
-      const int mi = e % kTile, ni = e / kTile;
-      const int n = n0 + ni, m = m0 + mi;
-      if (n > T || m > T) continue;
-      out[(size_t)n * T1 + m] = make_float2(tile[2 * o][ni][mi], tile[2 * o + 1][ni][mi]);
+  const size_t plane = (size_t)T1 * T1;
+  {
+    const bool any = n0 + kChunk - 1 >= m0;  // some m <= n in this chunk
+    if (any) {
+      // gather each (col, m, p) run of packed degrees that falls in the chunk
+      for (int e = threadIdx.x; e < ncol * kTile * 2 * 32; e += blockDim.x) {
+        const int j = e % 32;
+        const int p = (e / 32) % 2;
+        const int mi = (e / 64) % kTile;
+        const int cl = e / (64 * kTile);
+        const int m = m0 + mi;
+        if (m > T) continue;
+        const int n = chunk_degree(n0, m, p, j);
+        if (n > T || n < m + p) continue;
+        const int nidx = (n - m - p) >> 1;
+        const size_t col = (size_t)a.job.col0 + (size_t)f * ncol + cl;
+        float v = a.job.in[col * a.job.ldp + a.koff[2 * m + p] + nidx];
+        if (m == 0 && (cl & 1)) v = 0.f;  // m = 0 realified
+        tile[cl][n - n0][mi] = v;
+      }
+      __syncthreads();
+    }
+    for (int o = 0; o < nout; ++o) {
+      float2* out = (o == 0 ? a.job.out0 : a.job.out1) + (size_t)f * plane;
+      for (int e = threadIdx.x; e < kChunk * kTile; e += blockDim.x) {
+        const int mi = e % kTile, r = e / kTile;
+        const int n = n0 + r, m = m0 + mi;
+        if (n > T || m > T) continue;
+        const float2 v = m <= n ? make_float2(tile[2 * o][r][mi], tile[2 * o + 1][r][mi])
+                                : make_float2(0.f, 0.f);
+        out[(size_t)n * T1 + m] = v;
+      }
     }
   }
 }
@@ -188,14 +195,14 @@ __global__ void spectral_elementwise_kernel(const float2* __restrict__ in,
 }  // namespace sht
 
 extern "C" hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s) {
-  int blocks = a->job.nfields * a->mtiles * a->ntiles;
+  int blocks = a->job.nfields * a->mtiles * ((a->T + 1 + 63) / 64);
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(sht::pack_kernel, dim3(blocks), dim3(256), 0, s, *a);
   return hipGetLastError();
 }
 
 extern "C" hipError_t sht_launch_unpack(const sht::UnpackArgs* a, hipStream_t s) {
-  int blocks = a->job.nfields * a->tiles * a->tiles;
+  int blocks = a->job.nfields * a->tiles * ((a->T + 1 + 63) / 64);
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(sht::unpack_kernel, dim3(blocks), dim3(256), 0, s, *a);
   return hipGetLastError();
