@@ -1,0 +1,163 @@
+"""Grouped batches through the C ABI: schedule configs, the autotuner, graph
+replays and mixed truncations (needs an MI355X).
+
+A batch is the engine's own API (no reference counterpart); these tests pin
+it to the single-operation path and to the FP64 oracle:
+  * every column tile and grid gives bit-identical results (each output keeps
+    its accumulation order), and so does the autotuner's choice;
+  * sharing one accumulator in the split modes stays within the mode's
+    tolerance of the oracle;
+  * graph replays, with eager runs between them, match the eager result;
+  * a batch mixing T21 and T63 jobs matches the single operations bit for bit.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle.sht import OracleTransform, red_spectrum
+
+from _helpers import hip_transform, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"float32": 2e-6, "bf16x6": 2e-6, "bf16x3": 6e-5}
+
+
+@pytest.fixture(scope="module")
+def oracles():
+    return {
+        21: OracleTransform.create(21),                       # 33 x 66: Stockham FFT path
+        63: OracleTransform.create(63, dealias_factor=1.0),   # 64 x 128: wave-resident FFT path
+    }
+
+
+def _round_trip(oracle, precision, levels=2, seed=0):
+    """Device buffers and jobs of a round trip: wind + inverse, then vordiv + forward."""
+    import torch
+    from arwen_spectral import TransformBatch
+
+    tr = hip_transform(oracle, precision)
+    T = oracle.grid.truncation
+    dev = torch.device("cuda", torch.cuda.current_device())
+    c64 = lambda a: torch.from_numpy(np.ascontiguousarray(a).astype(np.complex64)).to(dev)
+    zeta = c64(oracle.project(red_spectrum(T, (levels,), seed=seed + 1, scale=1e-5)))
+    div = c64(oracle.project(red_spectrum(T, (levels,), seed=seed + 2, scale=1e-5)))
+    scal = c64(oracle.project(red_spectrum(T, (levels + 1,), seed=seed + 3, scale=1.0, mean=0.5)))
+    g = lambda n: torch.empty((n, *tr.grid.shape), dtype=torch.float32, device=dev)
+    s = lambda n: torch.empty((n, *tr.spectral_shape), dtype=torch.complex64, device=dev)
+    bufs = dict(zeta=zeta, div=div, scal=scal, u=g(levels), v=g(levels), grid=g(levels + 1),
+                z2=s(levels), d2=s(levels), s2=s(levels + 1))
+    jobs = [("wind", tr, (zeta, div), (bufs["u"], bufs["v"])),
+            ("inverse", tr, scal, bufs["grid"]),
+            ("vordiv", tr, (bufs["u"], bufs["v"]), (bufs["z2"], bufs["d2"])),
+            ("forward", tr, bufs["grid"], bufs["s2"])]
+    return tr, TransformBatch(jobs), bufs
+
+
+def _outputs(bufs):
+    return {k: bufs[k].clone() for k in ("u", "v", "grid", "z2", "d2", "s2")}
+
+
+def _run(batch):
+    import torch
+
+    batch.run()
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("precision", ["float32", "bf16x6"])
+@pytest.mark.parametrize("T", [21, 63])
+def test_schedule_configs_and_autotune_are_bit_identical(oracles, T, precision):
+    import torch
+
+    o = oracles[T]
+    _, batch, bufs = _round_trip(o, precision)
+    assert batch.config() == {"cols": [32, 32], "grid": [0, 0], "split_acc": 1}
+    _run(batch)
+    ref = _outputs(bufs)
+    # the round trip itself against the oracle
+    zeta = bufs["zeta"].cpu().numpy()
+    ou, ov = o.wind_from_vordiv(zeta, bufs["div"].cpu().numpy())
+    assert max(rel_err(ref["u"], ou), rel_err(ref["v"], ov)) < TOL[precision]
+    assert rel_err(ref["grid"], o.inverse(bufs["scal"].cpu().numpy())) < TOL[precision]
+    for cols in ((64, 64), (64, 32), (32, 64)):
+        for grid in ((0, 0), (64, 1024)):
+            batch.set_config(cols=cols, grid=grid)
+            _run(batch)
+            for k, v in _outputs(bufs).items():
+                assert torch.equal(v, ref[k]), (cols, grid, k)
+    tuned = batch.autotune(reps=1)
+    assert tuned["cols"][0] in (32, 64) and tuned["cols"][1] in (32, 64)
+    _run(batch)
+    for k, v in _outputs(bufs).items():
+        assert torch.equal(v, ref[k]), ("autotuned", tuned, k)
+
+
+@pytest.mark.parametrize("T", [21, 63])
+def test_shared_accumulator_stays_within_tolerance(oracles, T):
+    o = oracles[T]
+    _, batch, bufs = _round_trip(o, "bf16x6")
+    batch.set_config(split_acc=0)
+    _run(batch)
+    zeta, div = bufs["zeta"].cpu().numpy(), bufs["div"].cpu().numpy()
+    ou, ov = o.wind_from_vordiv(zeta, div)
+    assert max(rel_err(bufs["u"], ou), rel_err(bufs["v"], ov)) < TOL["bf16x6"]
+    assert rel_err(bufs["grid"], o.inverse(bufs["scal"].cpu().numpy())) < TOL["bf16x6"]
+    oz, od = o.vordiv_from_wind(bufs["u"].cpu().numpy().astype(np.float64),
+                                bufs["v"].cpu().numpy().astype(np.float64))
+    assert max(rel_err(bufs["z2"], oz), rel_err(bufs["d2"], od)) < TOL["bf16x6"]
+
+
+def test_config_rejects_bad_values(oracles):
+    _, batch, _ = _round_trip(oracles[21], "float32", levels=1)
+    for kw in (dict(cols=(48, 32)), dict(grid=(-1, 0)), dict(split_acc=2)):
+        with pytest.raises(ValueError):
+            batch.set_config(**kw)
+    assert batch.config() == {"cols": [32, 32], "grid": [0, 0], "split_acc": 1}
+
+
+@pytest.mark.parametrize("precision", ["float32", "bf16x6"])
+def test_graph_replays_match_eager_runs(oracles, precision):
+    import torch
+
+    _, batch, bufs = _round_trip(oracles[63], precision)
+    _run(batch)
+    ref = _outputs(bufs)
+    batch.capture()
+    for rep in range(4):
+        for k in ref:
+            bufs[k].fill_(float("nan") if bufs[k].is_floating_point() else 0)
+        torch.cuda.synchronize()
+        batch.launch()
+        torch.cuda.synchronize()
+        for k, v in _outputs(bufs).items():
+            assert torch.equal(v, ref[k]), (rep, k)
+        _run(batch)  # eager work between replays
+        for k, v in _outputs(bufs).items():
+            assert torch.equal(v, ref[k]), ("eager", rep, k)
+
+
+def test_mixed_truncation_batch_matches_single_operations(oracles):
+    import torch
+    from arwen_spectral import TransformBatch
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    jobs, singles = [], []
+    for T in (21, 63):
+        o = oracles[T]
+        tr = hip_transform(o, "bf16x6")
+        coeff = torch.from_numpy(o.project(red_spectrum(T, (3,), seed=T, scale=1.0))
+                                 .astype(np.complex64)).to(dev)
+        grid = torch.empty((3, *tr.grid.shape), dtype=torch.float32, device=dev)
+        back = torch.empty((3, *tr.spectral_shape), dtype=torch.complex64, device=dev)
+        jobs += [("inverse", tr, coeff, grid), ("forward", tr, grid, back)]
+        singles.append((tr, coeff, grid, back))
+    batch = TransformBatch(jobs)
+    _run(batch)
+    for tr, coeff, grid, back in singles:
+        g1 = tr.inverse(coeff)
+        b1 = tr.forward(g1)
+        torch.cuda.synchronize()
+        assert torch.equal(g1, grid)
+        assert torch.equal(b1, back)
