@@ -580,13 +580,18 @@ __device__ __forceinline__ void wave_fft64(float2 (&v)[F][L], float2* buf, const
     }
   });
   wave_sync();
+  // S2[k1][c][b ^ c]: the B writes (lanes over b) and the C reads (lanes
+  // over c) both hit distinct banks (measured)
   static_for<RND>([&](auto r_) {
     constexpr int r = decltype(r_)::value;
     const int u = lane + 64 * r;
     if (u < U) {
       const int f = u / (8 * L), k1 = (u >> 3) % L, b = u & 7;
-      float2* row = buf + f * SP + k1 * kRowSpan + b;
-      static_for<8>([&](auto c_) { row[8 * decltype(c_)::value] = w[r][c_]; });
+      float2* row = buf + f * SP + k1 * kRowSpan;
+      static_for<8>([&](auto c_) {
+        constexpr int c = decltype(c_)::value;
+        row[8 * c + (b ^ c)] = w[r][c];
+      });
     }
   });
   wave_sync();
@@ -597,7 +602,7 @@ __device__ __forceinline__ void wave_fft64(float2 (&v)[F][L], float2* buf, const
     if (u < U) {
       const int f = u / (8 * L), k1 = (u >> 3) % L, c = u & 7;
       const float2* row = buf + f * SP + k1 * kRowSpan + 8 * c;
-      static_for<8>([&](auto b_) { w[r][b_] = row[decltype(b_)::value]; });
+      static_for<8>([&](auto b_) { w[r][b_] = row[decltype(b_)::value ^ c]; });
       dft<8, S>(w[r]);
     }
   });
@@ -655,61 +660,57 @@ __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave_kernel(FoldArg
     });
     wave_fft64<L, F, -1>(v, buf, tw, lane);
   }
-  // fold into registers: val[r][e][z] for m-tile mt = lane + 64 r
+  // fold into registers, one order per lane and round (consecutive orders
+  // across lanes, so the spectrum reads are contiguous): val[r][e] for order
+  // m = lane + 64 r, e = 2 * column + parity
   const float s = live ? a.job.rowscale[i] : 0.f;
-  constexpr int RT = (N / 8 + 64) / 64;  // rounds of 64 m-tiles: MT <= N/8 + 1
-  float val[RT][NV][4];
+  const int MO = MT * kMT;               // orders incl. the last tile's padding
+  constexpr int RO = (N / 2 + 4 + 63) / 64;  // rounds: MO <= N/2 + 4
+  float val[RO][NV];
 #pragma unroll
-  for (int r = 0; r < RT; ++r) {
-    const int mt = lane + 64 * r;
+  for (int r = 0; r < RO; ++r) {
+    const int m = lane + 64 * r;
+    float2 e[2], o[2];
 #pragma unroll
-    for (int z = 0; z < 4; ++z) {
-      const int m = mt * kMT + z;
-      float2 e[2], o[2];
-#pragma unroll
-      for (int f = 0; f < 2; ++f) {
-        e[f] = make_float2(0.f, 0.f);
-        o[f] = make_float2(0.f, 0.f);
-        if (f < F && live && mt < MT && m < T1) {
-          const float2 zm = buf[f * SP + m];
-          const float2 zc = cconj(buf[f * SP + (N - m) % N]);
-          const float2 xn = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
-          const float2 d = csub(zm, zc);
-          const float2 xs = make_float2(0.5f * d.y, -0.5f * d.x);
-          e[f] = make_float2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
-          o[f] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
-        }
+    for (int f = 0; f < 2; ++f) {
+      e[f] = make_float2(0.f, 0.f);
+      o[f] = make_float2(0.f, 0.f);
+      if (f < F && live && m < T1) {
+        const float2 zm = buf[f * SP + m];
+        const float2 zc = cconj(buf[f * SP + (N - m) % N]);
+        const float2 xn = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
+        const float2 d = csub(zm, zc);
+        const float2 xs = make_float2(0.5f * d.y, -0.5f * d.x);
+        e[f] = make_float2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
+        o[f] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
       }
-      if constexpr (F == 1) {
-        // value e = 2 * column + parity
-        val[r][0][z] = e[0].x;
-        val[r][1][z] = o[0].x;
-        val[r][2][z] = e[0].y;
-        val[r][3][z] = o[0].y;
-      } else {
-        // level columns [iV, iU] = [-V.im, V.re, -U.im, U.re]; e = 2 * column + parity
+    }
+    if constexpr (F == 1) {
+      val[r][0] = e[0].x;
+      val[r][1] = o[0].x;
+      val[r][2] = e[0].y;
+      val[r][3] = o[0].y;
+    } else {
+      // level columns [iV, iU] = [-V.im, V.re, -U.im, U.re]
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-          const float2 U = p == 0 ? e[0] : o[0];
-          const float2 V = p == 0 ? e[1] : o[1];
-          val[r][0 + p][z] = -V.y;
-          val[r][2 + p][z] = V.x;
-          val[r][4 + p][z] = -U.y;
-          val[r][6 + p][z] = U.x;
-        }
+      for (int p = 0; p < 2; ++p) {
+        const float2 U = p == 0 ? e[0] : o[0];
+        const float2 V = p == 0 ? e[1] : o[1];
+        val[r][0 + p] = -V.y;
+        val[r][2 + p] = V.x;
+        val[r][4 + p] = -U.y;
+        val[r][6 + p] = U.x;
       }
     }
   }
-  wave_sync();  // the spectrum is read; the span now holds stage[e][mt][z]
+  wave_sync();  // the spectrum is read; the span now holds stage[e][m]
   float* stage = reinterpret_cast<float*>(buf);
 #pragma unroll
-  for (int r = 0; r < RT; ++r) {
-    const int mt = lane + 64 * r;
-    if (mt < MT) {
+  for (int r = 0; r < RO; ++r) {
+    const int m = lane + 64 * r;
+    if (m < MO) {
 #pragma unroll
-      for (int e = 0; e < NV; ++e)
-        *reinterpret_cast<f4*>(stage + ((size_t)e * MT + mt) * 4) =
-            f4{val[r][e][0], val[r][e][1], val[r][e][2], val[r][e][3]};
+      for (int e = 0; e < NV; ++e) stage[(size_t)e * MO + m] = val[r][e];
     }
   }
   __syncthreads();
@@ -787,18 +788,19 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
                              ((size_t)e * MT + mt) * 4) = src[ww];
   }
   __syncthreads();
-  constexpr int RT = (N / 8 + 64) / 64;  // rounds of 64 m-tiles: MT <= N/8 + 1
-  f4 er[RT], ei[RT], orr[RT], oi[RT];
+  // one order per lane and round: stage[e][m], e = 2 * column + parity
+  const int MO = MT * kMT;
+  constexpr int RO = (N / 2 + 4 + 63) / 64;  // rounds: MO <= N/2 + 4
+  float er[RO], ei[RO], orr[RO], oi[RO];
   const float* stage = reinterpret_cast<const float*>(buf);
 #pragma unroll
-  for (int r = 0; r < RT; ++r) {
-    const int mt = lane + 64 * r;
-    if (mt < MT) {
-      er[r] = *reinterpret_cast<const f4*>(stage + ((size_t)0 * MT + mt) * 4);
-      orr[r] = *reinterpret_cast<const f4*>(stage + ((size_t)1 * MT + mt) * 4);
-      ei[r] = *reinterpret_cast<const f4*>(stage + ((size_t)2 * MT + mt) * 4);
-      oi[r] = *reinterpret_cast<const f4*>(stage + ((size_t)3 * MT + mt) * 4);
-    }
+  for (int r = 0; r < RO; ++r) {
+    const int m = lane + 64 * r;
+    const bool in = m < MO;
+    er[r] = in ? stage[m] : 0.f;
+    orr[r] = in ? stage[(size_t)MO + m] : 0.f;
+    ei[r] = in ? stage[(size_t)2 * MO + m] : 0.f;
+    oi[r] = in ? stage[(size_t)3 * MO + m] : 0.f;
   }
   if (!live) return;  // no block barrier below
   wave_sync();  // the staged values are in registers; the span becomes the spectrum
@@ -807,26 +809,21 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
   for (int k = lane; k < N; k += 64) buf[k] = make_float2(0.f, 0.f);
   wave_sync();
 #pragma unroll
-  for (int r = 0; r < RT; ++r) {
-    const int mt = lane + 64 * r;
-    if (mt >= MT) continue;
-#pragma unroll
-    for (int z = 0; z < 4; ++z) {
-      const int m = mt * kMT + z;
-      if (m >= T1) break;
-      float2 A = make_float2((er[r][z] + orr[r][z]) * s, (ei[r][z] + oi[r][z]) * s);
-      float2 B = make_float2((er[r][z] - orr[r][z]) * s, (ei[r][z] - oi[r][z]) * s);
-      if (a.job.imul) {
-        A = make_float2(-A.y, A.x);
-        B = make_float2(-B.y, B.x);
-      }
-      if (m == 0) {
-        A.y = 0.f;
-        B.y = 0.f;
-      }
-      buf[m] = make_float2(A.x - B.y, A.y + B.x);
-      if (m > 0) buf[N - m] = make_float2(A.x + B.y, -A.y + B.x);
+  for (int r = 0; r < RO; ++r) {
+    const int m = lane + 64 * r;
+    if (m >= T1) continue;
+    float2 A = make_float2((er[r] + orr[r]) * s, (ei[r] + oi[r]) * s);
+    float2 B = make_float2((er[r] - orr[r]) * s, (ei[r] - oi[r]) * s);
+    if (a.job.imul) {
+      A = make_float2(-A.y, A.x);
+      B = make_float2(-B.y, B.x);
     }
+    if (m == 0) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    buf[m] = make_float2(A.x - B.y, A.y + B.x);
+    if (m > 0) buf[N - m] = make_float2(A.x + B.y, -A.y + B.x);
   }
   wave_sync();
   float2 v[1][L];
