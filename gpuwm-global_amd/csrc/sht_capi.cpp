@@ -376,32 +376,41 @@ void build_lists(int dir, const std::vector<HostGroup>& groups, int bn, GemmList
       for (int sg = 0; sg < g.nseg; ++sg) {
         int kmax = 0;
         for (int z = 0; z < g.mcount; ++z) kmax = std::max(kmax, g.K[sg][z]);
+        // every k-chunk is a trip of kMT steps, step z for order z (the
+        // kernel fixes each step's order at compile time); an order that
+        // does not reach this chunk or row tile gets a null step (extent 0:
+        // its loads return zeros).  ANA: the chunk's B tile, shared by the
+        // orders, comes with step 0.
         for (int k0 = 0; k0 < kmax; k0 += 32) {
-          bool first = true;
-          for (int z = 0; z < g.mcount; ++z) {
-            if (k0 >= g.K[sg][z]) continue;
-            if (dir == 1 && g.Arows[sg][z] <= r0) continue;  // order ends above this tile
+          for (int z = 0; z < sht::kMT; ++z) {
             sht::GemmStep st{};
-            const int chunk_rows = g.Arows[sg][z];
-            st.a = (unsigned long long)(g.A[sg][z] +
-                                        ((size_t)k0 * chunk_rows + (size_t)r0 * 32) * g.esz);
-            st.a_bytes = (unsigned)(std::min(TR, chunk_rows - r0) * 32 * g.esz);
             st.z = z;
             st.cmap = g.cmap[sg];
-            if (dir == 0) {
-              st.b = (unsigned long long)(g.B[sg][z] + (size_t)k0 * g.esz);
-              st.b_klen = g.K[sg][z] - k0;
-            } else if (first) {
-              st.b = (unsigned long long)(g.B[sg][0] + (size_t)k0 * sht::kMT * g.esz);
-              st.b_klen = g.K[sg][z] - k0;
+            const bool live = z < g.mcount && k0 < g.K[sg][z] &&
+                              (dir == 0 || g.Arows[sg][z] > r0);
+            if (live) {
+              const int chunk_rows = g.Arows[sg][z];
+              st.a = (unsigned long long)(g.A[sg][z] +
+                                          ((size_t)k0 * chunk_rows + (size_t)r0 * 32) * g.esz);
+              st.a_bytes = (unsigned)(std::min(TR, chunk_rows - r0) * 32 * g.esz);
+            } else {
+              st.a = (unsigned long long)g.A[sg][0];
             }
-            first = false;
+            if (dir == 0) {
+              st.b = (unsigned long long)(live ? g.B[sg][z] + (size_t)k0 * g.esz : g.B[sg][0]);
+              st.b_klen = live ? g.K[sg][z] - k0 : 0;
+            } else if (z == 0) {
+              st.b = (unsigned long long)(g.B[sg][0] + (size_t)k0 * sht::kMT * g.esz);
+              st.b_klen = g.K[sg][0] - k0;
+            }
             steps.push_back(st);
           }
         }
       }
       pn.nsteps = (int)steps.size() - pn.step0;
-      if (pn.nsteps == 0) {  // nothing reaches this row tile
+      bool any = false;
+      for (int e = pn.step0; e < (int)steps.size(); ++e) any = any || steps[e].a_bytes > 0;
+      if (!any) {  // nothing reaches this row tile
         steps.resize(pn.step0);
         continue;
       }

@@ -211,7 +211,10 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
       const int e = 2 * min(j, nsteps + 1);
       return GemmDesc{sDesc[e], sDesc[e + 1]};
     };
-    auto issue = [&](u32x4(&A)[C::A_REGS], u32x4(&B)[C::B_REGS], const GemmStep& s) {
+    // WITH_B: the step may carry a B tile (SYN: every step; ANA: the first
+    // step of each k-chunk, position 0 of a trip of kMT steps)
+    auto issue = [&](u32x4(&A)[C::A_REGS], u32x4(&B)[C::B_REGS], const GemmStep& s,
+                     auto WITH_B) {
       // A: one contiguous tile per piece
 #pragma unroll
       for (int pc = 0; pc < NP; ++pc) {
@@ -244,7 +247,7 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
             }
           }
         }
-      } else {
+      } else if constexpr (decltype(WITH_B)::value) {
         // k-chunk of the waist: per column 32 latitudes x 4 orders; a thread
         // takes 4 latitudes (a quad) of one column
         const unsigned bytes =
@@ -348,10 +351,11 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
           acc2[a][b][n] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
 
-    // one step out of LDS buffer `buf` into the accumulators of order z
-    auto compute = [&](int buf, int z) {
+    // one step out of LDS buffer `buf` into the accumulators of order Z
+    auto compute = [&](int buf, auto ZC) {
+      constexpr int Z = decltype(ZC)::value;
       const int bbuf = DIR == 0 ? buf : 0;
-      const int bz = DIR == 0 ? 0 : z;
+      constexpr int bz = DIR == 0 ? 0 : Z;
       u32x4 af[2][3], bf[NB][3];
 #pragma unroll
       for (int r2 = 0; r2 < 2; ++r2) {
@@ -384,16 +388,12 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
         }
       }
 #pragma unroll
-      for (int zz = 0; zz < kMT; ++zz) {
-        if (zz != z) continue;
+      for (int r2 = 0; r2 < 2; ++r2)
 #pragma unroll
-        for (int r2 = 0; r2 < 2; ++r2)
-#pragma unroll
-          for (int nb = 0; nb < NB; ++nb) {
-            if constexpr (ACC2) mfma_step<MODE>(acc[zz][r2][nb], acc2[zz][r2][nb], af[r2], bf[nb]);
-            else mfma_step<MODE>(acc[zz][r2][nb], acc[zz][r2][nb], af[r2], bf[nb]);
-          }
-      }
+        for (int nb = 0; nb < NB; ++nb) {
+          if constexpr (ACC2) mfma_step<MODE>(acc[Z][r2][nb], acc2[Z][r2][nb], af[r2], bf[nb]);
+          else mfma_step<MODE>(acc[Z][r2][nb], acc[Z][r2][nb], af[r2], bf[nb]);
+        }
     };
 
     // ---- prologue: steps 0 and 1 in flight, step 0 in LDS buffer 0 ----
@@ -405,29 +405,31 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
       const GemmDesc d0 = fetch_desc(0), d1 = fetch_desc(1);
       dsc[0] = fetch_desc(2);
       cur = d0.scalar();
-      issue(ra[0], rb[0], cur);
+      issue(ra[0], rb[0], cur, std::true_type{});
       nxt = d1.scalar();
-      issue(ra[1], rb[1], nxt);
+      issue(ra[1], rb[1], nxt, std::integral_constant<bool, DIR == 0>{});
       put_a(ra[0], 0);
-      if (DIR == 0 || cur.b_klen > 0) put_b(rb[0], 0, cur.cmap);
+      put_b(rb[0], 0, cur.cmap);
     }
     __syncthreads();
 
-    // ---- main loop, two steps per trip (register stage = step parity) ----
-    auto body = [&](auto S, int j) {
-      constexpr int s = decltype(S)::value;
+    // ---- main loop: trips of kMT steps, one per order Z of the panel (the
+    // host pads every k-chunk to kMT steps), so each step's accumulators and
+    // register stage (parity of Z) are fixed at compile time ----
+    auto body = [&](auto ZC, int j) {
+      constexpr int Z = decltype(ZC)::value;
+      constexpr int s = Z & 1;
       dsc[s ^ 1] = fetch_desc(j + 3);
       const GemmStep d2 = dsc[s].scalar();  // step j+2, fetched one step ago
-      issue(ra[s], rb[s], d2);
-      compute(s, cur.z);
+      // step j+2 is position (Z+2) % kMT of its trip
+      issue(ra[s], rb[s], d2, std::integral_constant<bool, DIR == 0 || Z == 2>{});
+      compute(s, ZC);
       if (j + 1 < nsteps) {
         if constexpr (DIR == 0) {
           put_b(rb[s ^ 1], s ^ 1, nxt.cmap);
-        } else {
-          if (nxt.b_klen > 0) {
-            __syncthreads();  // every wavefront is done with the resident B tile
-            put_b(rb[s ^ 1], 0, nxt.cmap);
-          }
+        } else if constexpr (Z == kMT - 1) {
+          __syncthreads();  // every wavefront is done with the resident B tile
+          put_b(rb[s ^ 1], 0, nxt.cmap);
         }
         put_a(ra[s ^ 1], s ^ 1);
       }
@@ -435,9 +437,12 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
       cur = nxt;
       nxt = d2;
     };
-    for (int j = 0; j < nsteps; j += 2) {
+    static_assert(kMT == 4, "trip of four steps");
+    for (int j = 0; j < nsteps; j += kMT) {
       body(std::integral_constant<int, 0>{}, j);
-      if (j + 1 < nsteps) body(std::integral_constant<int, 1>{}, j + 1);
+      body(std::integral_constant<int, 1>{}, j + 1);
+      body(std::integral_constant<int, 2>{}, j + 2);
+      body(std::integral_constant<int, 3>{}, j + 3);
     }
 
     // ---- epilogue ----
