@@ -291,8 +291,7 @@ struct HostGroup {
   int cmap[2] = {0, 0};             // B column map per segment (GemmStep::cmap)
   int bcs = 0, ccs = 0;             // column strides (floats)
   long long piece_bytes = 0;        // table piece stride (split modes)
-  long long bpiece_bytes = 0;       // data piece stride (split modes)
-  size_t esz = 4;                   // table and data element bytes (4 fp32, 2 bf16 pieces)
+  size_t esz = 4;                   // table element bytes (4 fp32, 2 bf16 pieces); data are fp32
 };
 
 // Device work lists of one GEMM direction, built for one column tile.
@@ -365,7 +364,6 @@ void build_lists(int dir, const std::vector<HostGroup>& groups, int bn, GemmList
       pn.bcs = g.bcs;
       pn.ccs = g.ccs;
       pn.piece_bytes = g.piece_bytes;
-      pn.bpiece_bytes = g.bpiece_bytes;
       for (int z = 0; z < sht::kMT; ++z) {
         pn.rows[z] = z < g.mcount ? std::max(0, std::min(TR, g.R[z] - r0)) : 0;
         if (dir == 0)
@@ -397,10 +395,11 @@ void build_lists(int dir, const std::vector<HostGroup>& groups, int bn, GemmList
               st.a = (unsigned long long)g.A[sg][0];
             }
             if (dir == 0) {
-              st.b = (unsigned long long)(live ? g.B[sg][z] + (size_t)k0 * g.esz : g.B[sg][0]);
+              st.b = (unsigned long long)(live ? g.B[sg][z] + (size_t)k0 * sizeof(float)
+                                               : g.B[sg][0]);
               st.b_klen = live ? g.K[sg][z] - k0 : 0;
             } else if (z == 0) {
-              st.b = (unsigned long long)(g.B[sg][0] + (size_t)k0 * sht::kMT * g.esz);
+              st.b = (unsigned long long)(g.B[sg][0] + (size_t)k0 * sht::kMT * sizeof(float));
               st.b_klen = g.K[sg][0] - k0;
             }
             steps.push_back(st);
@@ -478,12 +477,10 @@ size_t waist_offset(const arwen_sht_plan* P, int m, int p) {
 constexpr int kMapSynVec = 1 | (1 << 1) | (1 << 4);
 constexpr int kMapAnaVec = 1 | (1 << 2) | (1 << 3);
 
-// GEMM data operand: base address, element bytes (4 fp32, 2 bf16 pieces) and
-// piece plane stride in bytes.
+// GEMM data operand (fp32 in every precision mode; the split modes cut it
+// into bf16 pieces as it enters the GEMM's LDS tiles).
 struct DataRef {
   const char* base;
-  size_t esz;
-  long long piece_bytes;
 };
 
 // Synthesis group: segments (table kind, parity, packed data, column map).
@@ -503,7 +500,6 @@ void add_syn_group(std::vector<HostGroup>& b, const arwen_sht_plan* P, int mb, i
   g.ccs = P->ldw;
   g.bcs = P->ldp;
   g.C[0] = waist + waist_offset(P, mb * sht::kMT, q);
-  g.bpiece_bytes = segs[0].packed.piece_bytes;
   for (int sg = 0; sg < nseg; ++sg) g.cmap[sg] = segs[sg].cmap;
   for (int mm = 0; mm < g.mcount; ++mm) {
     const int m = mb * sht::kMT + mm;
@@ -512,7 +508,7 @@ void add_syn_group(std::vector<HostGroup>& b, const arwen_sht_plan* P, int mb, i
       const int p = q ^ segs[sg].pflip;
       g.A[sg][mm] = table_block(P, segs[sg].kind, true, m, p, &g.piece_bytes, &g.esz);
       g.Arows[sg][mm] = P->JHP;
-      g.B[sg][mm] = segs[sg].packed.base + (size_t)P->koff[2 * m + p] * segs[sg].packed.esz;
+      g.B[sg][mm] = segs[sg].packed.base + (size_t)P->koff[2 * m + p] * sizeof(float);
       g.K[sg][mm] = P->kp(m, p);
     }
   }
@@ -535,10 +531,9 @@ void add_ana_group(std::vector<HostGroup>& b, const arwen_sht_plan* P, int mb, i
   g.mcount = std::min(sht::kMT, P->T + 1 - mb * sht::kMT);
   g.ccs = P->ldp;
   g.bcs = P->ldw;
-  g.bpiece_bytes = segs[0].waist.piece_bytes;
   for (int sg = 0; sg < nseg; ++sg) {
     g.B[sg][0] = segs[sg].waist.base +
-                 waist_offset(P, mb * sht::kMT, q ^ segs[sg].pflip) * segs[sg].waist.esz;
+                 waist_offset(P, mb * sht::kMT, q ^ segs[sg].pflip) * sizeof(float);
     g.cmap[sg] = segs[sg].cmap;
   }
   for (int mm = 0; mm < g.mcount; ++mm) {
@@ -558,9 +553,6 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
   if (njobs <= 0 || !jobs) fail(ARWEN_SHT_EINVAL, "a batch needs at least one job");
   hip_check(hipGetDevice(&B->device), "hipGetDevice");
   B->precision = jobs[0].plan ? jobs[0].plan->precision : 0;
-  // split modes keep GEMM data operands as bf16 pieces in planes
-  const int np = B->precision == ARWEN_SHT_BF16X6 ? 3 : B->precision == ARWEN_SHT_BF16X3 ? 2 : 0;
-  const size_t desz = np ? 2 : 4;
   auto& syn = B->groups[0];
   auto& ana = B->groups[1];
   for (int j = 0; j < njobs; ++j) {
@@ -589,8 +581,7 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       // ---- pack ----
       const int ncol = vec ? 4 : 2;
       js->ncols = ncol * n;
-      const size_t pplane = (size_t)js->ncols * P->ldp;  // elements per piece plane
-      js->packed_a.alloc(pplane * desz * (np ? np : 1));
+      js->packed_a.alloc((size_t)js->ncols * P->ldp * 4);
       js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
       const bool grad = J.op == ARWEN_SHT_OP_GRADIENT;
       if (grad) js->waist_b.alloc((size_t)js->ncols * P->ldw * 4);
@@ -601,8 +592,6 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       pa.job.in0 = static_cast<const float2*>(J.in0);
       pa.job.in1 = static_cast<const float2*>(J.in1);
       pa.job.out0 = js->packed_a.as<float>();
-      pa.job.split = np;
-      pa.job.pstride = (long long)pplane;
       pa.job.nfields = n;
       pa.job.col0 = 0;
       pa.job.ldp = P->ldp;
@@ -612,7 +601,7 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
                                                           : sht::kPlain;
       B->packs.push_back(pa);
       // ---- grouped GEMM: one group per (m-tile, output parity) ----
-      const DataRef pk{js->packed_a.as<char>(), desz, (long long)(pplane * desz)};
+      const DataRef pk{js->packed_a.as<char>()};
       for (int mb = 0; mb < P->MB; ++mb)
         for (int q = 0; q < 2; ++q) {
           if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
@@ -677,8 +666,7 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       // ---- FFT + fold ----
       const int ncol = vec ? 4 : 2;
       js->ncols = ncol * n;
-      const size_t wplane = (size_t)js->ncols * P->ldw;  // elements per piece plane
-      js->waist_a.alloc(wplane * desz * (np ? np : 1));
+      js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
       js->packed_out.alloc((size_t)js->ncols * P->ldp * 4);
       sht::FoldArgs fa{};
       fa.fft = P->fft;
@@ -693,8 +681,6 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       fa.job.grid0 = static_cast<const float*>(J.in0);
       fa.job.grid1 = static_cast<const float*>(J.in1);
       fa.job.waist0 = js->waist_a.as<float>();
-      fa.job.split = np;
-      fa.job.pstride = (long long)wplane;
       fa.job.rowscale = vec ? P->scale_fold_vec.as<float>() : P->scale_fold.as<float>();
       fa.job.nfields = n;
       fa.job.col0 = 0;
@@ -702,7 +688,7 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       fa.job.vector = vec ? 1 : 0;
       B->folds.push_back(fa);
       B->fft_bytes += (vec ? 2.0 : 1.0) * n * fftb;
-      const DataRef wst{js->waist_a.as<char>(), desz, (long long)(wplane * desz)};
+      const DataRef wst{js->waist_a.as<char>()};
       for (int mb = 0; mb < P->MB; ++mb)
         for (int q = 0; q < 2; ++q) {
           if (vec) {

@@ -2,8 +2,8 @@
 // (synthesis).
 //
 // The waist is stored m-tiled:  W[c][m/4][p][i][m%4], per data column c
-// (2*field + re/im): fp32, or for the split-precision plans the analysis
-// waist as bf16 pieces in planes (the form the GEMM reads).
+// (2*field + re/im), fp32 in every precision mode (the split modes cut the
+// analysis waist into bf16 pieces as the GEMM loads it).
 //
 // A block holds `rp` wavefronts; wavefront w owns folded row i0 + w (the
 // mirror pair north = nlat-1-i, south = i) and runs its own FFT in its own
@@ -268,12 +268,7 @@ __global__ void __launch_bounds__(kFftThreads) rfft_fold_kernel(FoldArgs a) {
         }
       }
       auto put_w = [&](int c, int p, float v) {
-        const size_t idx = waist_index(c, a.job.ldw, a.JHP, m, p, i);
-        if (a.job.split == 0)
-          a.job.waist0[idx] = v;
-        else
-          store_bf16_pieces(reinterpret_cast<unsigned short*>(a.job.waist0) + idx,
-                            a.job.pstride, a.job.split, v);
+        a.job.waist0[waist_index(c, a.job.ldw, a.JHP, m, p, i)] = v;
       };
       if (!a.job.vector) {
         const int c = a.job.col0 + 2 * item;
@@ -625,8 +620,7 @@ constexpr int kWaveBlock = 4;  // wavefronts per block of the wave-resident kern
 // of a vector level together.  The block's four wavefronts hold four
 // consecutive rows of one field: each folds its spectrum into registers and
 // parks the values in its LDS span, then the block writes every (column,
-// m-tile, parity) run of 4 rows x 4 orders as one contiguous piece (64 bytes
-// fp32, 32 bytes per bf16 piece).
+// m-tile, parity) run of 4 rows x 4 orders as one contiguous 64-byte piece.
 template <int L, int F>
 __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave_kernel(FoldArgs a) {
   extern __shared__ float2 lds[];
@@ -725,32 +719,9 @@ __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave_kernel(FoldArg
       run[ww] = *reinterpret_cast<const f4*>(
           reinterpret_cast<const float*>(lds + N + (size_t)ww * F * SP) + ((size_t)e * MT + mt) * 4);
     const size_t idx = waist_index(c, a.job.ldw, a.JHP, mt * kMT, p, i0);  // 16 contiguous
-    if (a.job.split == 0) {
-      f4* dst = reinterpret_cast<f4*>(a.job.waist0 + idx);
+    f4* dst = reinterpret_cast<f4*>(a.job.waist0 + idx);
 #pragma unroll
-      for (int ww = 0; ww < 4; ++ww) dst[ww] = run[ww];
-    } else {
-      typedef __attribute__((ext_vector_type(8))) unsigned short h8;
-      h8 pc[3][2];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const float x = run[q >> 2][q & 3];
-        const __bf16 b0 = (__bf16)x;
-        const float r1 = x - (float)b0;
-        const __bf16 b1 = (__bf16)r1;
-        pc[0][q >> 3][q & 7] = __builtin_bit_cast(unsigned short, b0);
-        pc[1][q >> 3][q & 7] = __builtin_bit_cast(unsigned short, b1);
-        pc[2][q >> 3][q & 7] = __builtin_bit_cast(unsigned short, (__bf16)(r1 - (float)b1));
-      }
-      unsigned short* h = reinterpret_cast<unsigned short*>(a.job.waist0) + idx;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        if (k >= a.job.split) break;
-        h8* dst = reinterpret_cast<h8*>(h + k * a.job.pstride);
-        dst[0] = pc[k][0];
-        dst[1] = pc[k][1];
-      }
-    }
+    for (int ww = 0; ww < 4; ++ww) dst[ww] = run[ww];
   }
 }
 

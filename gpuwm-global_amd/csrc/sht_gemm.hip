@@ -46,9 +46,10 @@
 //   BF16X6 three pieces each: a0b0 + (a0b1 + a1b0) + (a0b2 + a1b1 + a2b0),
 //          every term down to 2^-16 relative.  With ACC2 the five small
 //          products accumulate apart from a0b0 and are added at the end.
-// Tables are split on the device from FP64 (sht_tables.hip) and data by the
-// kernels that write them (sht_pack.hip, sht_fft.hip), each piece the
-// round-to-nearest-even bf16 of what the previous pieces leave.  Lane l of a
+// Tables are split once on the device from FP64 (sht_tables.hip); the data
+// stay fp32 in memory and are split as their tile is written to LDS.  Each
+// piece is the round-to-nearest-even bf16 of what the previous pieces leave.
+// Lane l of a
 // 16x16 MFMA: (l & 15) is the row/column and (l >> 4) selects the k chunk;
 // A and B use the same k mapping.
 #include <type_traits>
@@ -106,9 +107,6 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(unsigned long long a
 __device__ __forceinline__ u32x4 load16(__amdgpu_buffer_rsrc_t r, unsigned off) {
   return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
 }
-__device__ __forceinline__ u32x2 load8(__amdgpu_buffer_rsrc_t r, unsigned off) {
-  return __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
-}
 
 // 16-byte chunk swizzle of an LDS row: fp32 rows have 8 chunks, bf16 rows 4.
 // Chosen by measurement on gfx950 (SQ_LDS_BANK_CONFLICT over the exact
@@ -130,17 +128,33 @@ struct GemmCfg {
   static constexpr int BZ = DIR == 0 ? 1 : kMT;
   static constexpr int NB = BN / 32;        // 16-column MFMA blocks per wavefront
   static constexpr int A_TILE = kGemmBM * ROWB;        // bytes per piece
-  static constexpr int B_TILE = BZ * BN * ROWB;        // bytes per piece
+  static constexpr int B_TILE = BZ * BN * ROWB;        // LDS bytes per piece
   static constexpr int B_BUFS = DIR == 0 ? 2 : 1;      // ANA keeps one k-chunk resident
-  // register stage: 16-byte registers per thread
+  // register stage: 16-byte registers per thread.  Tables come as stored
+  // pieces; data (B) come as fp32 and are split when written to LDS.
   static constexpr int A_PER = A_TILE / (256 * 16);    // per piece
   static constexpr int A_REGS = NP * A_PER;
-  static constexpr int B_BYTES = B_TILE / 256;         // per thread and piece
-  static constexpr bool B_HALF = B_BYTES == 8;         // one 8-byte load per piece
-  static constexpr int B_PER = B_HALF ? 1 : B_BYTES / 16;
-  static constexpr int B_REGS = NP * B_PER;
-  static_assert(A_PER >= 1 && B_BYTES >= 8, "tile shape");
+  static constexpr int B_REGS = BZ * BN * kBK * 4 / (256 * 16);
+  static_assert(A_PER >= 1 && B_REGS >= 1, "tile shape");
 };
+
+// Four fp32 values as `np` bf16 piece vectors (each the round-to-nearest-even
+// bf16 of what the previous pieces leave; the residuals are exact in fp32).
+__device__ __forceinline__ void split4(const float (&x)[4], int np, u32x2 (&out)[3]) {
+  float r[4] = {x[0], x[1], x[2], x[3]};
+#pragma unroll
+  for (int pc = 0; pc < 3; ++pc) {
+    if (pc >= np) break;
+    unsigned short h[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const __bf16 b = (__bf16)r[e];
+      h[e] = __builtin_bit_cast(unsigned short, b);
+      r[e] -= (float)b;
+    }
+    out[pc] = u32x2{h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16)};
+  }
+}
 
 template <int MODE>
 __device__ __forceinline__ void mfma_step(f32x4& acc, f32x4& acc2, const u32x4 (&a)[3],
@@ -192,7 +206,7 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
     const GemmPanel P = panels[tile.x];
     const int c0 = tile.y;
     const int ncol = min(BN, P.N - c0);
-    const unsigned long long coloff = (unsigned long long)c0 * P.bcs * ESZ;
+    const unsigned long long coloff = (unsigned long long)c0 * P.bcs * 4;  // fp32 data
     const int step0 = P.step0, nsteps = P.nsteps;
 
     // ---- register stages and their loads ----
@@ -226,47 +240,29 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
       }
       const int rev = (s.cmap & 1) ? 3 : 0;
       if constexpr (DIR == 0) {
-        // packed columns, k contiguous
+        // packed fp32 columns, k contiguous: a thread takes 4 k of one column
         const unsigned bytes =
-            s.b_klen > 0 ? (unsigned)(((ncol - 1) * P.bcs + s.b_klen) * ESZ) : 0u;
+            s.b_klen > 0 ? (unsigned)(((ncol - 1) * P.bcs + s.b_klen) * 4) : 0u;
+        const __amdgpu_buffer_rsrc_t r = make_rsrc(s.b + coloff, bytes);
 #pragma unroll
-        for (int pc = 0; pc < NP; ++pc) {
-          const __amdgpu_buffer_rsrc_t r =
-              make_rsrc(s.b + coloff + (unsigned long long)(pc * P.bpiece_bytes), bytes);
-          if constexpr (C::B_HALF) {
-            const int col = tid >> 3;
-            const u32x2 v = load8(r, (unsigned)((col ^ rev) * P.bcs * ESZ + (tid & 7) * 8));
-            B[pc] = u32x4{v[0], v[1], 0u, 0u};
-          } else {
-#pragma unroll
-            for (int u = 0; u < C::B_PER; ++u) {
-              const int q = tid + 256 * u;
-              const int col = q / CHS;
-              B[pc * C::B_PER + u] =
-                  load16(r, (unsigned)((col ^ rev) * P.bcs * ESZ + (q % CHS) * 16));
-            }
-          }
+        for (int u = 0; u < C::B_REGS; ++u) {
+          const int q = tid + 256 * u;
+          const int col = q >> 3;
+          B[u] = load16(r, (unsigned)((col ^ rev) * P.bcs * 4 + (q & 7) * 16));
         }
       } else if constexpr (decltype(WITH_B)::value) {
-        // k-chunk of the waist: per column 32 latitudes x 4 orders; a thread
-        // takes 4 latitudes (a quad) of one column
+        // k-chunk of the fp32 waist: per column 32 latitudes x 4 orders; a
+        // thread takes 4 latitudes (a quad, 64 bytes) of one column
         const unsigned bytes =
-            s.b_klen > 0 ? (unsigned)(((ncol - 1) * P.bcs + s.b_klen * kMT) * ESZ) : 0u;
-        constexpr int QB = 4 * kMT * ESZ;  // bytes of one quad: 64 fp32, 32 bf16
-        constexpr int PER = QB / 16;
+            s.b_klen > 0 ? (unsigned)(((ncol - 1) * P.bcs + s.b_klen * kMT) * 4) : 0u;
+        const __amdgpu_buffer_rsrc_t r = make_rsrc(s.b + coloff, bytes);
 #pragma unroll
-        for (int pc = 0; pc < NP; ++pc) {
-          const __amdgpu_buffer_rsrc_t r =
-              make_rsrc(s.b + coloff + (unsigned long long)(pc * P.bpiece_bytes), bytes);
+        for (int u = 0; u < BN / 32; ++u) {
+          const int q = tid + 256 * u;
+          const int col = q >> 3, quad = q & 7;
+          const unsigned off = (unsigned)((col ^ rev) * P.bcs * 4 + quad * 64);
 #pragma unroll
-          for (int u = 0; u < BN / 32; ++u) {
-            const int q = tid + 256 * u;
-            const int col = q >> 3, quad = q & 7;
-            const unsigned off = (unsigned)((col ^ rev) * P.bcs * ESZ + quad * QB);
-#pragma unroll
-            for (int h = 0; h < PER; ++h)
-              B[(pc * (BN / 32) + u) * PER + h] = load16(r, off + 16u * h);
-          }
+          for (int h = 0; h < 4; ++h) B[u * 4 + h] = load16(r, off + 16u * h);
         }
       }
     };
@@ -282,61 +278,56 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
               &sA[buf][pc * C::A_TILE + row * ROWB + swz<F32>(row, ch) * 16]) = A[pc * C::A_PER + u];
         }
     };
-    // B stage -> LDS; `cmap` bits 1..4 negate columns 0..3 of each group of four
+    // B stage -> LDS (split into pieces in the bf16 modes); `cmap` bits 1..4
+    // negate columns 0..3 of each group of four
     auto put_b = [&](const u32x4(&B)[C::B_REGS], int buf, int cmap) {
-      constexpr unsigned kSign = F32 ? 0x80000000u : 0x80008000u;
+      auto flip_of = [&](int col) { return ((cmap >> (1 + (col & 3))) & 1) ? 0x80000000u : 0u; };
       if constexpr (DIR == 0) {
-        if constexpr (C::B_HALF) {
-          const int col = tid >> 3, e = tid & 7;
-          const unsigned flip = ((cmap >> (1 + (col & 3))) & 1) ? kSign : 0u;
 #pragma unroll
-          for (int pc = 0; pc < NP; ++pc)
-            *reinterpret_cast<u32x2*>(&sB[buf][pc * C::B_TILE + col * ROWB +
-                                               swz<F32>(col, e >> 1) * 16 + (e & 1) * 8]) =
-                u32x2{B[pc][0] ^ flip, B[pc][1] ^ flip};
-        } else {
+        for (int u = 0; u < C::B_REGS; ++u) {
+          const int q = tid + 256 * u;
+          const int col = q >> 3, ch = q & 7;  // 4 k starting at 4 ch
+          const u32x4 v = B[u] ^ flip_of(col);
+          unsigned char* row = &sB[buf][col * ROWB];
+          if constexpr (F32) {
+            *reinterpret_cast<u32x4*>(row + swz<true>(col, ch) * 16) = v;
+          } else {
+            const float x[4] = {__uint_as_float(v[0]), __uint_as_float(v[1]),
+                                __uint_as_float(v[2]), __uint_as_float(v[3])};
+            u32x2 pcs[3];
+            split4(x, NP, pcs);
 #pragma unroll
-          for (int pc = 0; pc < NP; ++pc)
-#pragma unroll
-            for (int u = 0; u < C::B_PER; ++u) {
-              const int q = tid + 256 * u;
-              const int col = q / CHS, ch = q % CHS;
-              const unsigned flip = ((cmap >> (1 + (col & 3))) & 1) ? kSign : 0u;
-              *reinterpret_cast<u32x4*>(
-                  &sB[buf][pc * C::B_TILE + col * ROWB + swz<F32>(col, ch) * 16]) =
-                  B[pc * C::B_PER + u] ^ flip;
-            }
+            for (int pc = 0; pc < NP; ++pc)
+              *reinterpret_cast<u32x2*>(row + pc * C::B_TILE + swz<false>(col, ch >> 1) * 16 +
+                                        (ch & 1) * 8) = pcs[pc];
+          }
         }
       } else {
         (void)buf;
-        constexpr int PER = 4 * kMT * ESZ / 16;
 #pragma unroll
-        for (int pc = 0; pc < NP; ++pc)
+        for (int u = 0; u < BN / 32; ++u) {
+          const int q = tid + 256 * u;
+          const int col = q >> 3, quad = q & 7;
+          const unsigned flip = flip_of(col);
+          const u32x4* v = &B[u * 4];  // v[h] = latitude 4 quad + h, four orders
 #pragma unroll
-          for (int u = 0; u < BN / 32; ++u) {
-            const int q = tid + 256 * u;
-            const int col = q >> 3, quad = q & 7;
-            const unsigned flip = ((cmap >> (1 + (col & 3))) & 1) ? kSign : 0u;
-            const u32x4* v = &B[(pc * (BN / 32) + u) * PER];
+          for (int z = 0; z < kMT; ++z) {
+            unsigned char* row = &sB[0][(z * BN + col) * ROWB];
+            const u32x4 w = u32x4{v[0][z], v[1][z], v[2][z], v[3][z]} ^ flip;
+            if constexpr (F32) {
+              *reinterpret_cast<u32x4*>(row + swz<true>(col, quad) * 16) = w;
+            } else {
+              const float x[4] = {__uint_as_float(w[0]), __uint_as_float(w[1]),
+                                  __uint_as_float(w[2]), __uint_as_float(w[3])};
+              u32x2 pcs[3];
+              split4(x, NP, pcs);
 #pragma unroll
-            for (int z = 0; z < kMT; ++z) {
-              unsigned char* row = &sB[0][pc * C::B_TILE + (z * BN + col) * ROWB];
-              if constexpr (F32) {
-                // v[h] = latitude 4 quad + h, four orders: chunk `quad` of row (z, col)
-                *reinterpret_cast<u32x4*>(row + swz<true>(col, quad) * 16) =
-                    u32x4{v[0][z], v[1][z], v[2][z], v[3][z]} ^ flip;
-              } else {
-                // v[h] = latitudes 4 quad + 2h, 2h+1 (four orders each, bf16 pairs)
-                const int sh = (z & 1) * 16;
-                const unsigned w0 = (v[0][z >> 1] >> sh) & 0xFFFFu;
-                const unsigned w1 = (v[0][2 + (z >> 1)] >> sh) & 0xFFFFu;
-                const unsigned w2 = (v[1][z >> 1] >> sh) & 0xFFFFu;
-                const unsigned w3 = (v[1][2 + (z >> 1)] >> sh) & 0xFFFFu;
-                *reinterpret_cast<u32x2*>(row + swz<false>(col, quad >> 1) * 16 + (quad & 1) * 8) =
-                    u32x2{(w0 | (w1 << 16)) ^ flip, (w2 | (w3 << 16)) ^ flip};
-              }
+              for (int pc = 0; pc < NP; ++pc)
+                *reinterpret_cast<u32x2*>(row + pc * C::B_TILE + swz<false>(col, quad >> 1) * 16 +
+                                          (quad & 1) * 8) = pcs[pc];
             }
           }
+        }
       }
     };
 

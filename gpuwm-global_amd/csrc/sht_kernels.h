@@ -10,19 +10,6 @@ constexpr int kMT = 4;           // m-tile of the waist layout W[c][m/4][p][i][m
 constexpr int kFftThreads = 256;
 constexpr int kMaxStages = 16;
 
-// Split a float into `np` bf16 pieces (2 or 3), each the round-to-nearest-even
-// bf16 of what the previous pieces leave (the residuals are exact in fp32),
-// and store them `pstride` elements apart.
-__device__ inline void store_bf16_pieces(unsigned short* dst, long long pstride, int np,
-                                         float x) {
-  const __bf16 h0 = (__bf16)x;
-  const float r = x - (float)h0;
-  const __bf16 h1 = (__bf16)r;
-  dst[0] = __builtin_bit_cast(unsigned short, h0);
-  dst[pstride] = __builtin_bit_cast(unsigned short, h1);
-  if (np > 2) dst[2 * pstride] = __builtin_bit_cast(unsigned short, (__bf16)(r - (float)h1));
-}
-
 // ---- tables (sht_tables.hip) ----
 // The block of order m and parity p starts at koff[2m+p] * JHP elements, holds
 // JHP folded latitudes i x KP = kp(m,p) degrees (n - m)/2, both multiples of
@@ -59,10 +46,7 @@ struct FftPlan {
 struct FoldJob {
   const float* grid0;         // field rows at grid0 + f*nlat*nlon (u for vector jobs)
   const float* grid1;         // v rows for vector jobs, else NULL
-  float* waist0;              // scalar: waist; vector: [iV, iU] per level; fp32 or
-                              // `split` bf16 pieces (unsigned short) of plane stride `pstride`
-  long long pstride;          // elements between bf16 piece planes
-  int split;                  // 0: fp32; 2 or 3: bf16 pieces
+  float* waist0;              // scalar: waist; vector: [iV, iU] per level (fp32)
   const float* rowscale;      // per folded row: 1/nlon x quadrature weight (x 1/a)
   int nfields;                // scalar fields, or u/v level pairs
   int col0;                   // first data column of this job in the waist
@@ -122,8 +106,7 @@ constexpr int kGemmMaxSteps = 160;  // steps per panel incl. the two null steps 
 
 struct GemmPanel {
   unsigned long long c[kMT];  // SYN: c[0] = waist block + r0 rows; ANA: packed block of order z + r0
-  long long piece_bytes;      // bf16 piece stride of the tables
-  long long bpiece_bytes;     // bf16 piece stride of the data (B)
+  long long piece_bytes;      // bf16 piece stride of the tables (data B are fp32)
   int step0, nsteps;          // steps[step0 .. step0+nsteps), followed by two null steps
   int N;                      // data columns
   int bcs;                    // B column stride, floats
@@ -149,10 +132,7 @@ enum PackMode : int { kPlain = 0, kIM = 1, kICoef = 2, kVector = 3 };
 struct PackJob {
   const float2* in0;          // [f][n][m]; vorticity for VECTOR
   const float2* in1;          // divergence for VECTOR
-  float* out0;                // packed ([-psi, chi] per level for VECTOR); fp32 or
-                              // `split` bf16 pieces (unsigned short) of plane stride `pstride`
-  long long pstride;          // elements between bf16 piece planes
-  int split;                  // 0: fp32; 2 or 3: bf16 pieces
+  float* out0;                // packed ([-psi, chi] per level for VECTOR; fp32)
   int nfields;                // fields, or levels for VECTOR
   int col0;
   int ldp;                    // floats per packed column

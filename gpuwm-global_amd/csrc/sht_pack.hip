@@ -6,8 +6,8 @@
 // one field (or level), so each value is read once and global reads and
 // writes are 128-256 byte runs on both sides.
 //
-// For the split-precision plans the packed values are written as bf16 pieces
-// (piece planes `pstride` elements apart), the form the GEMM reads.
+// Packed values are fp32 in every precision mode; the split modes cut them
+// into bf16 pieces as the GEMM loads them.
 //
 // Fused transforms (pack), per field or per vorticity/divergence level:
 //   PLAIN   c
@@ -101,12 +101,7 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
       }
       for (int cl = 0; cl < ncol; ++cl) {
         const size_t col = (size_t)a.job.col0 + (size_t)f * ncol + cl;
-        const size_t idx = col * a.job.ldp + kbeg + nidx;
-        if (a.job.split == 0)
-          a.job.out0[idx] = o[cl];
-        else
-          store_bf16_pieces(reinterpret_cast<unsigned short*>(a.job.out0) + idx, a.job.pstride,
-                            a.job.split, o[cl]);
+        a.job.out0[col * a.job.ldp + kbeg + nidx] = o[cl];
       }
     }
   }
