@@ -631,17 +631,16 @@ __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave_kernel(FoldArg
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float2* tw = lds;
   float2* buf = lds + N + (size_t)w * F * SP;
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
   const int item0 = blockIdx.x * kWaveBlock;  // JHP is a multiple of 4: one field per block
   if (item0 >= a.job.nfields * a.JHP) return;  // block-uniform
-  __syncthreads();
   const int fld = item0 / a.JHP, i0 = item0 % a.JHP, i = i0 + w;
   const bool live = i < a.JH;
   const int T1 = a.T + 1;
   const int MT = (T1 + kMT - 1) / kMT;  // m-tiles
   const size_t plane = (size_t)a.nlat * N;
+  // the rows' loads go out before the twiddle table is staged
+  float2 v[F][L];
   if (live) {
-    float2 v[F][L];
     static_for<F>([&](auto f_) {
       constexpr int f = decltype(f_)::value;
       const float* g = (f == 0 ? a.job.grid0 : a.job.grid1) + (size_t)fld * plane;
@@ -652,8 +651,10 @@ __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave_kernel(FoldArg
         v[f][n1] = make_float2(rn[64 * n1 + lane], rs[64 * n1 + lane]);
       });
     });
-    wave_fft64<L, F, -1>(v, buf, tw, lane);
   }
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
+  __syncthreads();
+  if (live) wave_fft64<L, F, -1>(v, buf, tw, lane);
   // fold into registers, one order per lane and round (consecutive orders
   // across lanes, so the spectrum reads are contiguous): val[r][e] for order
   // m = lane + 64 r, e = 2 * column + parity
@@ -738,7 +739,6 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float2* tw = lds;
   float2* buf = lds + N + (size_t)w * SP;
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
   const int JH4 = (a.JH + 3) / 4;  // row groups of four
   const int grp = blockIdx.x;
   if (grp >= a.job.nfields * JH4) return;  // block-uniform
@@ -748,15 +748,32 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
   const int MT = (T1 + kMT - 1) / kMT;
   const int c = a.job.col0 + a.job.fstride * fld;
   const float* W = a.job.waist;
-  // block read into each row's span: stage[e][mt][z], e = 2 * column + parity
-  for (int u = threadIdx.x; u < 4 * MT; u += blockDim.x) {
-    const int e = u / MT, mt = u % MT;
-    const f4* src = reinterpret_cast<const f4*>(
-        W + waist_index(c + (e >> 1), a.job.ldw, a.JHP, mt * kMT, e & 1, i0));
+  // block read into each row's span: stage[e][mt][z], e = 2 * column + parity.
+  // All of a thread's loads go out first, then the twiddle table is staged.
+  constexpr int RU = (4 * (N / 8 + 1) + 64 * kWaveBlock - 1) / (64 * kWaveBlock);  // 4 MT <= N/2 + 4
+  f4 run[RU][4];
 #pragma unroll
-    for (int ww = 0; ww < 4; ++ww)
-      *reinterpret_cast<f4*>(reinterpret_cast<float*>(lds + N + (size_t)ww * SP) +
-                             ((size_t)e * MT + mt) * 4) = src[ww];
+  for (int r = 0; r < RU; ++r) {
+    const int u = threadIdx.x + 64 * kWaveBlock * r;
+    if (u < 4 * MT) {
+      const int e = u / MT, mt = u % MT;
+      const f4* src = reinterpret_cast<const f4*>(
+          W + waist_index(c + (e >> 1), a.job.ldw, a.JHP, mt * kMT, e & 1, i0));
+#pragma unroll
+      for (int ww = 0; ww < 4; ++ww) run[r][ww] = src[ww];
+    }
+  }
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
+#pragma unroll
+  for (int r = 0; r < RU; ++r) {
+    const int u = threadIdx.x + 64 * kWaveBlock * r;
+    if (u < 4 * MT) {
+      const int e = u / MT, mt = u % MT;
+#pragma unroll
+      for (int ww = 0; ww < 4; ++ww)
+        *reinterpret_cast<f4*>(reinterpret_cast<float*>(lds + N + (size_t)ww * SP) +
+                               ((size_t)e * MT + mt) * 4) = run[r][ww];
+    }
   }
   __syncthreads();
   // one order per lane and round: stage[e][m], e = 2 * column + parity
@@ -804,11 +821,12 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
   const size_t plane = (size_t)a.nlat * N;
   float* rn = a.job.grid + (size_t)fld * plane + (size_t)(a.nlat - 1 - i) * N;
   float* rs = a.job.grid + (size_t)fld * plane + (size_t)i * N;
-  for (int k = lane; k < N; k += 64) {
+  static_for<L>([&](auto q_) {
+    const int k = lane + 64 * decltype(q_)::value;
     const float2 x = buf[k];
     rn[k] = x.x;
     rs[k] = x.y;
-  }
+  });
 }
 
 // L of the wave-resident path for N, or 0 (generic path).

@@ -23,11 +23,6 @@
 
 namespace sht {
 
-
-
-
-
-
 constexpr int kTile = 32;   // orders per block
 constexpr int kChunk = 64;  // degrees per pass of a block
 
@@ -37,11 +32,23 @@ __device__ __forceinline__ int chunk_degree(int n0, int m, int p, int j) {
   return n0 + 2 * j + ((m + p - n0) & 1);
 }
 
+// The offsets koff[2m+p] of the tile's 32 orders x 2 parities, staged in LDS
+// so that no data access waits on a dependent load.
+__device__ __forceinline__ void stage_koff(int* skoff, const int* koff, int m0, int T) {
+  if (threadIdx.x < 2 * kTile) {
+    const int m = m0 + (int)(threadIdx.x >> 1);
+    skoff[threadIdx.x] = m <= T ? koff[2 * m0 + threadIdx.x] : 0;
+  }
+}
+
 // One block per (field or level, 32 orders, 64 degrees from the tile's first
 // order): every input value is read once and every packed (m, p) run is
-// written as 32 consecutive degrees.
+// written as 32 consecutive degrees.  Each thread issues all of its loads
+// before it touches LDS, so a block keeps 16-32 KB in flight.
+template <int NIN>
 __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
-  __shared__ float2 tile[2][kChunk][kTile + 1];  // [input][n - n0][m - m0]
+  __shared__ float2 tile[NIN][kChunk][kTile + 1];  // [input][n - n0][m - m0]
+  __shared__ int skoff[2 * kTile];
   const int T = a.T, T1 = T + 1;
   const int nchunks = (T1 + kChunk - 1) / kChunk;
   const int c = blockIdx.x % nchunks;
@@ -51,67 +58,78 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
   const int m0 = mt * kTile;
   const int n0 = m0 + kChunk * c;
   if (n0 > T) return;  // block-uniform
-  const bool vec = a.job.mode == kVector;
-  const int nin = vec ? 2 : 1;
-  const int ncol = vec ? 4 : 2;
+  constexpr int NCOL = 2 * NIN;
+  constexpr int IT = kChunk * kTile / 256;  // loads per thread and input
   const size_t plane = (size_t)T1 * T1;
-  const double a2 = (double)a.job.radius * (double)a.job.radius;
+  const int tid = threadIdx.x;
+  stage_koff(skoff, a.koff, m0, T);
   {
-    for (int s = 0; s < nin; ++s) {
+    float2 v[NIN][IT];
+#pragma unroll
+    for (int s = 0; s < NIN; ++s) {
       const float2* in = (s == 0 ? a.job.in0 : a.job.in1) + (size_t)f * plane;
-      for (int e = threadIdx.x; e < kChunk * kTile; e += blockDim.x) {
-        const int r = e / kTile, mi = e % kTile;
-        const int n = n0 + r, m = m0 + mi;
-        float2 v = make_float2(0.f, 0.f);
-        if (n <= T && m <= T && m <= n) v = in[(size_t)n * T1 + m];
-        tile[s][r][mi] = v;
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int e = tid + 256 * it;
+        const int n = n0 + e / kTile, m = m0 + e % kTile;
+        const bool ok = n <= T && m <= n;  // m <= n <= T
+        const float2 x = in[ok ? (size_t)n * T1 + m : 0];
+        v[s][it] = ok ? x : make_float2(0.f, 0.f);
       }
     }
-    __syncthreads();
-    // outputs (mi, p, j), j fastest: 32 consecutive packed degrees per run
-    for (int e = threadIdx.x; e < kTile * 2 * 32; e += blockDim.x) {
-      const int j = e % 32;
-      const int p = (e / 32) % 2;
-      const int mi = e / 64;
-      const int m = m0 + mi;
-      if (m > T) continue;
-      const int n = chunk_degree(n0, m, p, j);
-      if (n > T || n < m + p) continue;
-      const int nidx = (n - m - p) >> 1;
-      const int kbeg = a.koff[2 * m + p];
-      const int r = n - n0;
-      float o[4] = {0.f, 0.f, 0.f, 0.f};
-      const float2 c = tile[0][r][mi];
-      if (a.job.mode == kPlain) {
-        o[0] = c.x; o[1] = c.y;
-      } else if (a.job.mode == kIM) {
-        o[0] = -(float)m * c.y; o[1] = (float)m * c.x;
-      } else if (a.job.mode == kICoef) {
-        o[0] = -c.y; o[1] = c.x;
-      } else {
-        const float2 d = tile[1][r][mi];
-        float2 psi = make_float2(0.f, 0.f), chi = make_float2(0.f, 0.f);
-        if (n >= 1) {
-          const float eig = (float)(-((double)n * ((double)n + 1.0)) / a2);
-          psi = make_float2(c.x / eig, c.y / eig);
-          chi = make_float2(d.x / eig, d.y / eig);
-          if (m == 0) { psi.y = 0.f; chi.y = 0.f; }
-        }
-        o[0] = -psi.x; o[1] = -psi.y; o[2] = chi.x; o[3] = chi.y;
+#pragma unroll
+    for (int s = 0; s < NIN; ++s)
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int e = tid + 256 * it;
+        tile[s][e / kTile][e % kTile] = v[s][it];
       }
-      for (int cl = 0; cl < ncol; ++cl) {
-        const size_t col = (size_t)a.job.col0 + (size_t)f * ncol + cl;
-        a.job.out0[col * a.job.ldp + kbeg + nidx] = o[cl];
+  }
+  __syncthreads();
+  const double a2 = (double)a.job.radius * (double)a.job.radius;
+  // outputs (mi, p, j), j fastest: 32 consecutive packed degrees per run
+  const int j = tid % 32, p = (tid / 32) % 2;
+#pragma unroll
+  for (int it = 0; it < kTile * 2 * 32 / 256; ++it) {
+    const int mi = tid / 64 + 4 * it;
+    const int m = m0 + mi;
+    const int n = chunk_degree(n0, m, p, j);
+    if (m > T || n > T || n < m + p) continue;
+    const int nidx = (n - m - p) >> 1;
+    const int r = n - n0;
+    float o[4] = {0.f, 0.f, 0.f, 0.f};
+    const float2 cv = tile[0][r][mi];
+    if (a.job.mode == kPlain) {
+      o[0] = cv.x; o[1] = cv.y;
+    } else if (a.job.mode == kIM) {
+      o[0] = -(float)m * cv.y; o[1] = (float)m * cv.x;
+    } else if (a.job.mode == kICoef) {
+      o[0] = -cv.y; o[1] = cv.x;
+    } else if constexpr (NIN == 2) {
+      const float2 d = tile[NIN - 1][r][mi];
+      float2 psi = make_float2(0.f, 0.f), chi = make_float2(0.f, 0.f);
+      if (n >= 1) {
+        const float eig = (float)(-((double)n * ((double)n + 1.0)) / a2);
+        psi = make_float2(cv.x / eig, cv.y / eig);
+        chi = make_float2(d.x / eig, d.y / eig);
+        if (m == 0) { psi.y = 0.f; chi.y = 0.f; }
       }
+      o[0] = -psi.x; o[1] = -psi.y; o[2] = chi.x; o[3] = chi.y;
     }
+    float* dst = a.job.out0 + ((size_t)a.job.col0 + (size_t)f * NCOL) * a.job.ldp +
+                 skoff[2 * mi + p] + nidx;
+#pragma unroll
+    for (int cl = 0; cl < NCOL; ++cl) dst[(size_t)cl * a.job.ldp] = o[cl];
   }
 }
 
 // One block per (field or level, 32 orders, 64 rows).  Entries above the
 // triangle (m > n) are written as zeros; chunks that lie wholly above it need
-// no reads.
+// no reads.  As in pack, every thread issues all of its loads first.
+template <int NCOL>
 __global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs a) {
-  __shared__ float tile[4][kChunk][kTile + 1];  // [col][n - n0][m - m0]
+  __shared__ float tile[NCOL][kChunk][kTile + 1];  // [col][n - n0][m - m0]
+  __shared__ int skoff[2 * kTile];
   const int T = a.T, T1 = T + 1;
   const int nchunks = (T1 + kChunk - 1) / kChunk;
   const int c = blockIdx.x % nchunks;
@@ -120,40 +138,53 @@ __global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs a) {
   if (f >= a.job.nfields) return;
   const int m0 = mt * kTile;
   const int n0 = kChunk * c;
-  const int ncol = a.job.vector ? 4 : 2;
-  const int nout = a.job.vector ? 2 : 1;
+  constexpr int NOUT = NCOL / 2;
+  constexpr int IT = kTile * 2 * 32 / 256;  // loads per thread and column
   const size_t plane = (size_t)T1 * T1;
-  {
-    const bool any = n0 + kChunk - 1 >= m0;  // some m <= n in this chunk
-    if (any) {
-      // gather each (col, m, p) run of packed degrees that falls in the chunk
-      for (int e = threadIdx.x; e < ncol * kTile * 2 * 32; e += blockDim.x) {
-        const int j = e % 32;
-        const int p = (e / 32) % 2;
-        const int mi = (e / 64) % kTile;
-        const int cl = e / (64 * kTile);
-        const int m = m0 + mi;
-        if (m > T) continue;
-        const int n = chunk_degree(n0, m, p, j);
-        if (n > T || n < m + p) continue;
-        const int nidx = (n - m - p) >> 1;
-        const size_t col = (size_t)a.job.col0 + (size_t)f * ncol + cl;
-        float v = a.job.in[col * a.job.ldp + a.koff[2 * m + p] + nidx];
-        if (m == 0 && (cl & 1)) v = 0.f;  // m = 0 realified
-        tile[cl][n - n0][mi] = v;
+  const int tid = threadIdx.x;
+  if (n0 + kChunk - 1 >= m0) {  // some m <= n in this chunk (block-uniform)
+    stage_koff(skoff, a.koff, m0, T);
+    __syncthreads();
+    // gather each (col, m, p) run of packed degrees that falls in the chunk
+    const int j = tid % 32, p = (tid / 32) % 2;
+    const float* in = a.job.in + ((size_t)a.job.col0 + (size_t)f * NCOL) * a.job.ldp;
+    float v[NCOL][IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int mi = tid / 64 + 4 * it;
+      const int m = m0 + mi;
+      const int n = chunk_degree(n0, m, p, j);
+      const bool ok = m <= T && n <= T && n >= m + p;
+      const size_t idx = ok ? (size_t)skoff[2 * mi + p] + ((n - m - p) >> 1) : 0;
+#pragma unroll
+      for (int cl = 0; cl < NCOL; ++cl) {
+        const float x = in[(size_t)cl * a.job.ldp + idx];
+        v[cl][it] = (ok && !(m == 0 && (cl & 1))) ? x : 0.f;  // m = 0 realified
       }
-      __syncthreads();
     }
-    for (int o = 0; o < nout; ++o) {
-      float2* out = (o == 0 ? a.job.out0 : a.job.out1) + (size_t)f * plane;
-      for (int e = threadIdx.x; e < kChunk * kTile; e += blockDim.x) {
-        const int mi = e % kTile, r = e / kTile;
-        const int n = n0 + r, m = m0 + mi;
-        if (n > T || m > T) continue;
-        const float2 v = m <= n ? make_float2(tile[2 * o][r][mi], tile[2 * o + 1][r][mi])
-                                : make_float2(0.f, 0.f);
-        out[(size_t)n * T1 + m] = v;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int mi = tid / 64 + 4 * it;
+      const int n = chunk_degree(n0, m0 + mi, p, j);
+      if (n - n0 < kChunk) {
+#pragma unroll
+        for (int cl = 0; cl < NCOL; ++cl) tile[cl][n - n0][mi] = v[cl][it];
       }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) {
+    float2* out = (o == 0 ? a.job.out0 : a.job.out1) + (size_t)f * plane;
+#pragma unroll
+    for (int it = 0; it < kChunk * kTile / 256; ++it) {
+      const int e = tid + 256 * it;
+      const int mi = e % kTile, r = e / kTile;
+      const int n = n0 + r, m = m0 + mi;
+      if (n > T || m > T) continue;
+      const float2 v = m <= n ? make_float2(tile[2 * o][r][mi], tile[2 * o + 1][r][mi])
+                              : make_float2(0.f, 0.f);
+      out[(size_t)n * T1 + m] = v;
     }
   }
 }
@@ -192,14 +223,20 @@ __global__ void spectral_elementwise_kernel(const float2* __restrict__ in,
 extern "C" hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s) {
   int blocks = a->job.nfields * a->mtiles * ((a->T + 1 + 63) / 64);
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(sht::pack_kernel, dim3(blocks), dim3(256), 0, s, *a);
+  if (a->job.mode == sht::kVector)
+    hipLaunchKernelGGL(sht::pack_kernel<2>, dim3(blocks), dim3(256), 0, s, *a);
+  else
+    hipLaunchKernelGGL(sht::pack_kernel<1>, dim3(blocks), dim3(256), 0, s, *a);
   return hipGetLastError();
 }
 
 extern "C" hipError_t sht_launch_unpack(const sht::UnpackArgs* a, hipStream_t s) {
   int blocks = a->job.nfields * a->tiles * ((a->T + 1 + 63) / 64);
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(sht::unpack_kernel, dim3(blocks), dim3(256), 0, s, *a);
+  if (a->job.vector)
+    hipLaunchKernelGGL(sht::unpack_kernel<4>, dim3(blocks), dim3(256), 0, s, *a);
+  else
+    hipLaunchKernelGGL(sht::unpack_kernel<2>, dim3(blocks), dim3(256), 0, s, *a);
   return hipGetLastError();
 }
 
