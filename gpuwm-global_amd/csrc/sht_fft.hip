@@ -23,6 +23,7 @@
 // Stockham autosort, radix 4/2/3 butterflies and a generic-radix DFT for any
 // other prime factor, twiddles from a table of e^{-2 pi i k/N} computed in
 // FP64 and rounded once.
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 
@@ -539,6 +540,10 @@ __device__ __forceinline__ void wave_sync() {
 constexpr int kRowSpan = 72;  // LDS row of the transposes: 64 points + 8 pad
 template <int L>
 constexpr int wave_span() { return L * kRowSpan; }  // float2 per sequence (>= 64 L)
+// Between the wavefronts' spans: 4 float2 (8 banks) of padding, so that the
+// eight wavefronts' copies of one 16-byte piece (read or written by eight
+// consecutive lanes) fall in eight different bank groups.
+constexpr int kSpanPad = 4;
 
 // F sequences of N = 64 L points: v[f][n1] = x_f[64 n1 + lane] on entry; on
 // return buf[f * span + k] holds X_f[k], k < N.
@@ -614,219 +619,240 @@ __device__ __forceinline__ void wave_fft64(float2 (&v)[F][L], float2* buf, const
   wave_sync();
 }
 
-constexpr int kWaveBlock = 4;  // wavefronts per block of the wave-resident kernels
+// Wavefronts (rows) per block of the wave-resident kernels, RB: with 8 a
+// (column, m-tile, parity) run of 8 rows x 4 orders is one whole 128-byte
+// line, so no line of the waist is shared between blocks; 4 where 8 rows'
+// LDS does not fit (the longest rows).
 
 // Analysis: one wavefront per (field, folded row); F = 2 transforms u and v
-// of a vector level together.  The block's four wavefronts hold four
-// consecutive rows of one field: each folds its spectrum into registers and
-// parks the values in its LDS span, then the block writes every (column,
-// m-tile, parity) run of 4 rows x 4 orders as one contiguous 64-byte piece.
-template <int L, int F>
+// of a vector level together.  A group is kWaveBlock consecutive rows of one
+// field (JHP is a multiple of 32): each wavefront folds its row's spectrum
+// into registers and parks the values in its LDS span, then the block writes
+// every (column, m-tile, parity) run of RB rows x 4 orders as one piece.  Blocks are persistent: while a group is transformed, the rows of
+// the block's next group are already being loaded into registers.
+template <int L, int F, int kWaveBlock>
 __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave_kernel(FoldArgs a) {
   extern __shared__ float2 lds[];
   constexpr int N = 64 * L, SP = wave_span<L>();
   constexpr int NV = 4 * F;  // folded values per order: columns x parities
-  static_assert(kMT == 4 && kWaveBlock == 4, "4 rows x 4 orders per run");
+  static_assert(kMT == 4, "a row's four orders are one 16-byte vector");
   typedef __attribute__((ext_vector_type(4))) float f4;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int WS = F * SP + kSpanPad;  // span stride per wavefront
   float2* tw = lds;
-  float2* buf = lds + N + (size_t)w * F * SP;
-  const int item0 = blockIdx.x * kWaveBlock;  // JHP is a multiple of 4: one field per block
-  if (item0 >= a.job.nfields * a.JHP) return;  // block-uniform
-  const int fld = item0 / a.JHP, i0 = item0 % a.JHP, i = i0 + w;
-  const bool live = i < a.JH;
+  float2* buf = lds + N + (size_t)w * WS;
+  const int ngroups = a.job.nfields * (a.JHP / kWaveBlock);
   const int T1 = a.T + 1;
   const int MT = (T1 + kMT - 1) / kMT;  // m-tiles
+  const int MO = MT * kMT;               // orders incl. the last tile's padding
   const size_t plane = (size_t)a.nlat * N;
-  // the rows' loads go out before the twiddle table is staged
+  int g = blockIdx.x;
+  if (g >= ngroups) return;  // block-uniform
   float2 v[F][L];
-  if (live) {
+  auto load_rows = [&](int grp) {
+    const int item = grp * kWaveBlock + w;
+    const int fld = item / a.JHP, i = item % a.JHP;
+    if (i >= a.JH) return;
     static_for<F>([&](auto f_) {
       constexpr int f = decltype(f_)::value;
-      const float* g = (f == 0 ? a.job.grid0 : a.job.grid1) + (size_t)fld * plane;
-      const float* rn = g + (size_t)(a.nlat - 1 - i) * N;
-      const float* rs = g + (size_t)i * N;
+      const float* gp = (f == 0 ? a.job.grid0 : a.job.grid1) + (size_t)fld * plane;
+      const float* rn = gp + (size_t)(a.nlat - 1 - i) * N;
+      const float* rs = gp + (size_t)i * N;
       static_for<L>([&](auto n1_) {
         constexpr int n1 = decltype(n1_)::value;
         v[f][n1] = make_float2(rn[64 * n1 + lane], rs[64 * n1 + lane]);
       });
     });
-  }
+  };
+  load_rows(g);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
   __syncthreads();
-  if (live) wave_fft64<L, F, -1>(v, buf, tw, lane);
-  // fold into registers, one order per lane and round (consecutive orders
-  // across lanes, so the spectrum reads are contiguous): val[r][e] for order
-  // m = lane + 64 r, e = 2 * column + parity
-  const float s = live ? a.job.rowscale[i] : 0.f;
-  const int MO = MT * kMT;               // orders incl. the last tile's padding
-  constexpr int RO = (N / 2 + 4 + 63) / 64;  // rounds: MO <= N/2 + 4
-  float val[RO][NV];
+  for (; g < ngroups; g += gridDim.x) {
+    const int item0 = g * kWaveBlock;
+    const int fld = item0 / a.JHP, i0 = item0 % a.JHP, i = i0 + w;
+    const bool live = i < a.JH;
+    if (live) wave_fft64<L, F, -1>(v, buf, tw, lane);  // v is free from here on
+    if (g + (int)gridDim.x < ngroups) load_rows(g + gridDim.x);
+    // fold into registers, one order per lane and round (consecutive orders
+    // across lanes, so the spectrum reads are contiguous): val[r][e] for
+    // order m = lane + 64 r, e = 2 * column + parity
+    const float s = live ? a.job.rowscale[i] : 0.f;
+    constexpr int RO = (N / 2 + 4 + 63) / 64;  // rounds: MO <= N/2 + 4
+    float val[RO][NV];
 #pragma unroll
-  for (int r = 0; r < RO; ++r) {
-    const int m = lane + 64 * r;
-    float2 e[2], o[2];
+    for (int r = 0; r < RO; ++r) {
+      const int m = lane + 64 * r;
+      float2 e[2], o[2];
 #pragma unroll
-    for (int f = 0; f < 2; ++f) {
-      e[f] = make_float2(0.f, 0.f);
-      o[f] = make_float2(0.f, 0.f);
-      if (f < F && live && m < T1) {
-        const float2 zm = buf[f * SP + m];
-        const float2 zc = cconj(buf[f * SP + (N - m) % N]);
-        const float2 xn = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
-        const float2 d = csub(zm, zc);
-        const float2 xs = make_float2(0.5f * d.y, -0.5f * d.x);
-        e[f] = make_float2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
-        o[f] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
+      for (int f = 0; f < 2; ++f) {
+        e[f] = make_float2(0.f, 0.f);
+        o[f] = make_float2(0.f, 0.f);
+        if (f < F && live && m < T1) {
+          const float2 zm = buf[f * SP + m];
+          const float2 zc = cconj(buf[f * SP + (N - m) % N]);
+          const float2 xn = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
+          const float2 d = csub(zm, zc);
+          const float2 xs = make_float2(0.5f * d.y, -0.5f * d.x);
+          e[f] = make_float2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
+          o[f] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
+        }
+      }
+      if constexpr (F == 1) {
+        val[r][0] = e[0].x;
+        val[r][1] = o[0].x;
+        val[r][2] = e[0].y;
+        val[r][3] = o[0].y;
+      } else {
+        // level columns [iV, iU] = [-V.im, V.re, -U.im, U.re]
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const float2 U = p == 0 ? e[0] : o[0];
+          const float2 V = p == 0 ? e[1] : o[1];
+          val[r][0 + p] = -V.y;
+          val[r][2 + p] = V.x;
+          val[r][4 + p] = -U.y;
+          val[r][6 + p] = U.x;
+        }
       }
     }
-    if constexpr (F == 1) {
-      val[r][0] = e[0].x;
-      val[r][1] = o[0].x;
-      val[r][2] = e[0].y;
-      val[r][3] = o[0].y;
-    } else {
-      // level columns [iV, iU] = [-V.im, V.re, -U.im, U.re]
+    wave_sync();  // the spectrum is read; the span now holds stage[e][m]
+    float* stage = reinterpret_cast<float*>(buf);
 #pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const float2 U = p == 0 ? e[0] : o[0];
-        const float2 V = p == 0 ? e[1] : o[1];
-        val[r][0 + p] = -V.y;
-        val[r][2 + p] = V.x;
-        val[r][4 + p] = -U.y;
-        val[r][6 + p] = U.x;
+    for (int r = 0; r < RO; ++r) {
+      const int m = lane + 64 * r;
+      if (m < MO) {
+#pragma unroll
+        for (int e = 0; e < NV; ++e) stage[(size_t)e * MO + m] = val[r][e];
       }
     }
-  }
-  wave_sync();  // the spectrum is read; the span now holds stage[e][m]
-  float* stage = reinterpret_cast<float*>(buf);
-#pragma unroll
-  for (int r = 0; r < RO; ++r) {
-    const int m = lane + 64 * r;
-    if (m < MO) {
-#pragma unroll
-      for (int e = 0; e < NV; ++e) stage[(size_t)e * MO + m] = val[r][e];
+    __syncthreads();
+    // block write: run (e, mt) = rows i0 .. i0+RB-1 x orders 4 mt .. 4 mt + 3,
+    // one contiguous piece; consecutive lanes take its rows (16 bytes each)
+    const int ncolf = 2 * F;  // columns per field or level
+    for (int u = threadIdx.x; u < NV * MT * kWaveBlock; u += blockDim.x) {
+      const int ww = u % kWaveBlock, rn = u / kWaveBlock;
+      const int e = rn / MT, mt = rn % MT;
+      const int c = a.job.col0 + ncolf * fld + (e >> 1), p = e & 1;
+      *reinterpret_cast<f4*>(a.job.waist0 + waist_index(c, a.job.ldw, a.JHP, mt * kMT, p, i0 + ww)) =
+          *reinterpret_cast<const f4*>(reinterpret_cast<const float*>(lds + N + (size_t)ww * WS) +
+                                       ((size_t)e * MT + mt) * 4);
     }
-  }
-  __syncthreads();
-  // block write: run (e, mt) = rows i0..i0+3 x orders 4 mt .. 4 mt + 3
-  const int ncolf = 2 * F;  // columns per field or level
-  for (int u = threadIdx.x; u < NV * MT; u += blockDim.x) {
-    const int e = u / MT, mt = u % MT;
-    const int c = a.job.col0 + ncolf * fld + (e >> 1), p = e & 1;
-    f4 run[4];
-#pragma unroll
-    for (int ww = 0; ww < 4; ++ww)
-      run[ww] = *reinterpret_cast<const f4*>(
-          reinterpret_cast<const float*>(lds + N + (size_t)ww * F * SP) + ((size_t)e * MT + mt) * 4);
-    const size_t idx = waist_index(c, a.job.ldw, a.JHP, mt * kMT, p, i0);  // 16 contiguous
-    f4* dst = reinterpret_cast<f4*>(a.job.waist0 + idx);
-#pragma unroll
-    for (int ww = 0; ww < 4; ++ww) dst[ww] = run[ww];
+    __syncthreads();  // the spans are free for the next group
   }
 }
 
-// Synthesis: one wavefront per (field, folded row).  The block's four
-// wavefronts hold four consecutive rows: the block reads every (column,
-// m-tile, parity) run of 4 rows x 4 orders of the waist as one contiguous
-// 64-byte piece and hands each row its values through LDS.
-template <int L>
+// Synthesis: one wavefront per (field, folded row).  A group is kWaveBlock
+// consecutive rows of one field: the block reads every (column, m-tile,
+// parity) run of RB rows x 4 orders of the waist as one piece and hands each
+// row its values through LDS.  Blocks are persistent:
+// the next group's waist runs are loaded into registers while the current
+// group is transformed.
+template <int L, int kWaveBlock>
 __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(UnfoldArgs a) {
   extern __shared__ float2 lds[];
   constexpr int N = 64 * L, SP = wave_span<L>();
-  static_assert(kMT == 4 && kWaveBlock == 4, "4 rows x 4 orders per run");
+  static_assert(kMT == 4, "a row's four orders are one 16-byte vector");
   typedef __attribute__((ext_vector_type(4))) float f4;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int WS = SP + kSpanPad;  // span stride per wavefront
   float2* tw = lds;
-  float2* buf = lds + N + (size_t)w * SP;
-  const int JH4 = (a.JH + 3) / 4;  // row groups of four
-  const int grp = blockIdx.x;
-  if (grp >= a.job.nfields * JH4) return;  // block-uniform
-  const int fld = grp / JH4, i0 = (grp % JH4) * 4, i = i0 + w;
-  const bool live = i < a.JH;
+  float2* buf = lds + N + (size_t)w * WS;
+  const int JH4 = (a.JH + kWaveBlock - 1) / kWaveBlock;  // row groups
+  const int ngroups = a.job.nfields * JH4;
   const int T1 = a.T + 1;
   const int MT = (T1 + kMT - 1) / kMT;
-  const int c = a.job.col0 + a.job.fstride * fld;
-  const float* W = a.job.waist;
-  // block read into each row's span: stage[e][mt][z], e = 2 * column + parity.
-  // All of a thread's loads go out first, then the twiddle table is staged.
-  constexpr int RU = (4 * (N / 8 + 1) + 64 * kWaveBlock - 1) / (64 * kWaveBlock);  // 4 MT <= N/2 + 4
-  f4 run[RU][4];
-#pragma unroll
-  for (int r = 0; r < RU; ++r) {
-    const int u = threadIdx.x + 64 * kWaveBlock * r;
-    if (u < 4 * MT) {
-      const int e = u / MT, mt = u % MT;
-      const f4* src = reinterpret_cast<const f4*>(
-          W + waist_index(c + (e >> 1), a.job.ldw, a.JHP, mt * kMT, e & 1, i0));
-#pragma unroll
-      for (int ww = 0; ww < 4; ++ww) run[r][ww] = src[ww];
-    }
-  }
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
-#pragma unroll
-  for (int r = 0; r < RU; ++r) {
-    const int u = threadIdx.x + 64 * kWaveBlock * r;
-    if (u < 4 * MT) {
-      const int e = u / MT, mt = u % MT;
-#pragma unroll
-      for (int ww = 0; ww < 4; ++ww)
-        *reinterpret_cast<f4*>(reinterpret_cast<float*>(lds + N + (size_t)ww * SP) +
-                               ((size_t)e * MT + mt) * 4) = run[r][ww];
-    }
-  }
-  __syncthreads();
-  // one order per lane and round: stage[e][m], e = 2 * column + parity
   const int MO = MT * kMT;
-  constexpr int RO = (N / 2 + 4 + 63) / 64;  // rounds: MO <= N/2 + 4
-  float er[RO], ei[RO], orr[RO], oi[RO];
-  const float* stage = reinterpret_cast<const float*>(buf);
-#pragma unroll
-  for (int r = 0; r < RO; ++r) {
-    const int m = lane + 64 * r;
-    const bool in = m < MO;
-    er[r] = in ? stage[m] : 0.f;
-    orr[r] = in ? stage[(size_t)MO + m] : 0.f;
-    ei[r] = in ? stage[(size_t)2 * MO + m] : 0.f;
-    oi[r] = in ? stage[(size_t)3 * MO + m] : 0.f;
-  }
-  if (!live) return;  // no block barrier below
-  wave_sync();  // the staged values are in registers; the span becomes the spectrum
-  const float s = a.job.rowscale[i];
-  // Z[k] = A + iB at k = m, conj(A) + i conj(B) at k = N - m; zero above T
-  for (int k = lane; k < N; k += 64) buf[k] = make_float2(0.f, 0.f);
-  wave_sync();
-#pragma unroll
-  for (int r = 0; r < RO; ++r) {
-    const int m = lane + 64 * r;
-    if (m >= T1) continue;
-    float2 A = make_float2((er[r] + orr[r]) * s, (ei[r] + oi[r]) * s);
-    float2 B = make_float2((er[r] - orr[r]) * s, (ei[r] - oi[r]) * s);
-    if (a.job.imul) {
-      A = make_float2(-A.y, A.x);
-      B = make_float2(-B.y, B.x);
-    }
-    if (m == 0) {
-      A.y = 0.f;
-      B.y = 0.f;
-    }
-    buf[m] = make_float2(A.x - B.y, A.y + B.x);
-    if (m > 0) buf[N - m] = make_float2(A.x + B.y, -A.y + B.x);
-  }
-  wave_sync();
-  float2 v[1][L];
-  static_for<L>([&](auto n1_) { v[0][n1_] = buf[64 * decltype(n1_)::value + lane]; });
-  wave_sync();
-  wave_fft64<L, 1, +1>(v, buf, tw, lane);
+  const float* W = a.job.waist;
   const size_t plane = (size_t)a.nlat * N;
-  float* rn = a.job.grid + (size_t)fld * plane + (size_t)(a.nlat - 1 - i) * N;
-  float* rs = a.job.grid + (size_t)fld * plane + (size_t)i * N;
-  static_for<L>([&](auto q_) {
-    const int k = lane + 64 * decltype(q_)::value;
-    const float2 x = buf[k];
-    rn[k] = x.x;
-    rs[k] = x.y;
-  });
+  int g = blockIdx.x;
+  if (g >= ngroups) return;  // block-uniform
+  // a thread's share of one group's runs: stage[e][mt][z], e = 2 * column +
+  // parity; consecutive lanes take one run's rows (16 bytes each)
+  constexpr int RU = (4 * (N / 8 + 1) + 63) / 64;  // 4 MT <= N/2 + 4 runs
+  f4 run[RU];
+  auto load_runs = [&](int grp) {
+    const int fld = grp / JH4, i0 = (grp % JH4) * kWaveBlock;
+    const int c = a.job.col0 + a.job.fstride * fld;
+#pragma unroll
+    for (int r = 0; r < RU; ++r) {
+      const int u = threadIdx.x + 64 * kWaveBlock * r;
+      const int ww = u % kWaveBlock, rn = u / kWaveBlock;
+      if (rn < 4 * MT) {
+        const int e = rn / MT, mt = rn % MT;
+        run[r] = *reinterpret_cast<const f4*>(
+            W + waist_index(c + (e >> 1), a.job.ldw, a.JHP, mt * kMT, e & 1, i0 + ww));
+      }
+    }
+  };
+  load_runs(g);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
+  for (; g < ngroups; g += gridDim.x) {
+    const int fld = g / JH4, i0 = (g % JH4) * kWaveBlock, i = i0 + w;
+    const bool live = i < a.JH;
+#pragma unroll
+    for (int r = 0; r < RU; ++r) {
+      const int u = threadIdx.x + 64 * kWaveBlock * r;
+      const int ww = u % kWaveBlock, rn = u / kWaveBlock;
+      if (rn < 4 * MT) {
+        const int e = rn / MT, mt = rn % MT;
+        *reinterpret_cast<f4*>(reinterpret_cast<float*>(lds + N + (size_t)ww * WS) +
+                               ((size_t)e * MT + mt) * 4) = run[r];
+      }
+    }
+    __syncthreads();  // spans (and, the first time, the twiddles) are in LDS
+    if (g + (int)gridDim.x < ngroups) load_runs(g + gridDim.x);
+    if (live) {
+      // one order per lane and round: stage[e][m], e = 2 * column + parity
+      constexpr int RO = (N / 2 + 4 + 63) / 64;  // rounds: MO <= N/2 + 4
+      float er[RO], ei[RO], orr[RO], oi[RO];
+      const float* stage = reinterpret_cast<const float*>(buf);
+#pragma unroll
+      for (int r = 0; r < RO; ++r) {
+        const int m = lane + 64 * r;
+        const bool in = m < MO;
+        er[r] = in ? stage[m] : 0.f;
+        orr[r] = in ? stage[(size_t)MO + m] : 0.f;
+        ei[r] = in ? stage[(size_t)2 * MO + m] : 0.f;
+        oi[r] = in ? stage[(size_t)3 * MO + m] : 0.f;
+      }
+      wave_sync();  // the staged values are in registers; the span becomes the spectrum
+      const float s = a.job.rowscale[i];
+      // Z[k] = A + iB at k = m, conj(A) + i conj(B) at k = N - m; zero above T
+      for (int k = lane; k < N; k += 64) buf[k] = make_float2(0.f, 0.f);
+      wave_sync();
+#pragma unroll
+      for (int r = 0; r < RO; ++r) {
+        const int m = lane + 64 * r;
+        if (m >= T1) continue;
+        float2 A = make_float2((er[r] + orr[r]) * s, (ei[r] + oi[r]) * s);
+        float2 B = make_float2((er[r] - orr[r]) * s, (ei[r] - oi[r]) * s);
+        if (a.job.imul) {
+          A = make_float2(-A.y, A.x);
+          B = make_float2(-B.y, B.x);
+        }
+        if (m == 0) {
+          A.y = 0.f;
+          B.y = 0.f;
+        }
+        buf[m] = make_float2(A.x - B.y, A.y + B.x);
+        if (m > 0) buf[N - m] = make_float2(A.x + B.y, -A.y + B.x);
+      }
+      wave_sync();
+      float2 v[1][L];
+      static_for<L>([&](auto n1_) { v[0][n1_] = buf[64 * decltype(n1_)::value + lane]; });
+      wave_sync();
+      wave_fft64<L, 1, +1>(v, buf, tw, lane);
+      float* rn = a.job.grid + (size_t)fld * plane + (size_t)(a.nlat - 1 - i) * N;
+      float* rs = a.job.grid + (size_t)fld * plane + (size_t)i * N;
+      static_for<L>([&](auto q_) {
+        const int k = lane + 64 * decltype(q_)::value;
+        const float2 x = buf[k];
+        rn[k] = x.x;
+        rs[k] = x.y;
+      });
+    }
+    __syncthreads();  // every span is consumed before the next group's runs land
+  }
 }
 
 // L of the wave-resident path for N, or 0 (generic path).
@@ -837,14 +863,32 @@ int wave_L(int N) {
     if (L == l) return L;
   return 0;
 }
-size_t fold_wave_lds_bytes(int N, int vector) {
-  const int L = N / 64;
-  return ((size_t)N + (size_t)kWaveBlock * (vector ? 2 : 1) * L * kRowSpan) * sizeof(float2);
+// Grid of a persistent wave-resident launch: as many blocks as the device's
+// LDS holds at once (at most 8 per CU), capped by the number of groups.
+int persistent_blocks(int groups, size_t lds_bytes) {
+  int dev = 0, cus = 0, lds_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+          hipSuccess || cus <= 0 || lds_cu <= 0) {
+    cus = 256;
+    lds_cu = 160 * 1024;
+  }
+  const int per_cu = std::max(1, std::min(8, (int)((size_t)lds_cu / std::max<size_t>(lds_bytes, 1))));
+  return std::max(1, std::min(groups, cus * per_cu));
 }
-size_t unfold_wave_lds_bytes(int N) {
+
+size_t fold_wave_lds_bytes(int N, int vector, int rb) {
   const int L = N / 64;
-  return ((size_t)N + (size_t)kWaveBlock * L * kRowSpan) * sizeof(float2);
+  return ((size_t)N + (size_t)rb * ((vector ? 2 : 1) * L * kRowSpan + kSpanPad)) * sizeof(float2);
 }
+size_t unfold_wave_lds_bytes(int N, int rb) {
+  const int L = N / 64;
+  return ((size_t)N + (size_t)rb * (L * kRowSpan + kSpanPad)) * sizeof(float2);
+}
+constexpr size_t kWaveLdsMax = 160 * 1024;  // LDS of one gfx950 workgroup
+// rows per block: 8 when their LDS fits, else 4
+int wave_rb(size_t lds8) { return lds8 <= kWaveLdsMax ? 8 : 4; }
 
 size_t fold_lds_bytes(int N, int T, int vector, int rp) {
   (void)T;
@@ -892,15 +936,20 @@ int fft_pf(int N) {
 
 extern "C" hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s) {
   if (const int L = a->fft.wave_L) {
-    const int items = a->job.nfields * a->JHP;
-    if (items == 0) return hipSuccess;
-    const dim3 g((items + sht::kWaveBlock - 1) / sht::kWaveBlock), b(64 * sht::kWaveBlock);
-    const size_t lds = sht::fold_wave_lds_bytes(a->fft.N, a->job.vector);
+    const int rb = sht::wave_rb(sht::fold_wave_lds_bytes(a->fft.N, a->job.vector, 8));
+    const int groups = a->job.nfields * (a->JHP / rb);
+    if (groups == 0) return hipSuccess;
+    const size_t lds = sht::fold_wave_lds_bytes(a->fft.N, a->job.vector, rb);
+    const dim3 g(sht::persistent_blocks(groups, lds)), b(64 * rb);
 #define SHT_FOLD(LL)                                                                      \
-  if (a->job.vector)                                                                      \
-    hipLaunchKernelGGL((sht::rfft_fold_wave_kernel<LL, 2>), g, b, lds, s, *a);            \
+  if (a->job.vector && rb == 8)                                                           \
+    hipLaunchKernelGGL((sht::rfft_fold_wave_kernel<LL, 2, 8>), g, b, lds, s, *a);         \
+  else if (a->job.vector)                                                                 \
+    hipLaunchKernelGGL((sht::rfft_fold_wave_kernel<LL, 2, 4>), g, b, lds, s, *a);         \
+  else if (rb == 8)                                                                       \
+    hipLaunchKernelGGL((sht::rfft_fold_wave_kernel<LL, 1, 8>), g, b, lds, s, *a);         \
   else                                                                                    \
-    hipLaunchKernelGGL((sht::rfft_fold_wave_kernel<LL, 1>), g, b, lds, s, *a)
+    hipLaunchKernelGGL((sht::rfft_fold_wave_kernel<LL, 1, 4>), g, b, lds, s, *a)
     SHT_L_DISPATCH(L, SHT_FOLD)
 #undef SHT_FOLD
     return hipGetLastError();
@@ -915,11 +964,16 @@ extern "C" hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s
 
 extern "C" hipError_t sht_launch_irfft_unfold(const sht::UnfoldArgs* a, hipStream_t s) {
   if (const int L = a->fft.wave_L) {
-    const int groups = a->job.nfields * ((a->JH + 3) / 4);
+    const int rb = sht::wave_rb(sht::unfold_wave_lds_bytes(a->fft.N, 8));
+    const int groups = a->job.nfields * ((a->JH + rb - 1) / rb);
     if (groups == 0) return hipSuccess;
-    const dim3 g(groups), b(64 * sht::kWaveBlock);
-    const size_t lds = sht::unfold_wave_lds_bytes(a->fft.N);
-#define SHT_UNFOLD(LL) hipLaunchKernelGGL((sht::irfft_unfold_wave_kernel<LL>), g, b, lds, s, *a)
+    const size_t lds = sht::unfold_wave_lds_bytes(a->fft.N, rb);
+    const dim3 g(sht::persistent_blocks(groups, lds)), b(64 * rb);
+#define SHT_UNFOLD(LL)                                                                    \
+  if (rb == 8)                                                                            \
+    hipLaunchKernelGGL((sht::irfft_unfold_wave_kernel<LL, 8>), g, b, lds, s, *a);         \
+  else                                                                                    \
+    hipLaunchKernelGGL((sht::irfft_unfold_wave_kernel<LL, 4>), g, b, lds, s, *a)
     SHT_L_DISPATCH(L, SHT_UNFOLD)
 #undef SHT_UNFOLD
     return hipGetLastError();
@@ -943,9 +997,12 @@ static hipError_t allow_lds_one(size_t bytes) {
 
 template <int L>
 static hipError_t allow_lds_wave(size_t bytes) {
-  for (const void* k : {reinterpret_cast<const void*>(sht::rfft_fold_wave_kernel<L, 1>),
-                        reinterpret_cast<const void*>(sht::rfft_fold_wave_kernel<L, 2>),
-                        reinterpret_cast<const void*>(sht::irfft_unfold_wave_kernel<L>)}) {
+  for (const void* k : {reinterpret_cast<const void*>(sht::rfft_fold_wave_kernel<L, 1, 4>),
+                        reinterpret_cast<const void*>(sht::rfft_fold_wave_kernel<L, 2, 4>),
+                        reinterpret_cast<const void*>(sht::rfft_fold_wave_kernel<L, 1, 8>),
+                        reinterpret_cast<const void*>(sht::rfft_fold_wave_kernel<L, 2, 8>),
+                        reinterpret_cast<const void*>(sht::irfft_unfold_wave_kernel<L, 4>),
+                        reinterpret_cast<const void*>(sht::irfft_unfold_wave_kernel<L, 8>)}) {
     hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
   }
@@ -954,7 +1011,13 @@ static hipError_t allow_lds_wave(size_t bytes) {
 
 extern "C" int sht_fft_wave_L(int N) { return sht::wave_L(N); }
 extern "C" size_t sht_fft_wave_lds_bytes(int N, int vector) {
-  return vector >= 0 ? sht::fold_wave_lds_bytes(N, vector) : sht::unfold_wave_lds_bytes(N);
+  // the LDS a launch asks for: 8 rows per block where that fits, else 4
+  if (vector >= 0) {
+    const size_t b8 = sht::fold_wave_lds_bytes(N, vector, 8);
+    return sht::wave_rb(b8) == 8 ? b8 : sht::fold_wave_lds_bytes(N, vector, 4);
+  }
+  const size_t b8 = sht::unfold_wave_lds_bytes(N, 8);
+  return sht::wave_rb(b8) == 8 ? b8 : sht::unfold_wave_lds_bytes(N, 4);
 }
 
 extern "C" hipError_t sht_fft_allow_lds(size_t bytes, size_t wave_bytes) {
