@@ -139,10 +139,11 @@ def class_errors(rt):
     return out
 
 
-def cpu_baseline(part, sample_levels=4):
-    """FP64 oracle (reference numpy algorithm) on a bounded sample of the same
-    workload: the first `sample_levels` levels of each class plus ln ps, one
-    member.  Returns the timing entry and the oracle's outputs."""
+def cpu_baseline(part, sample_levels=LEVELS):
+    """FP64 oracle (reference numpy algorithm) on the same workload: by default
+    all 121 fields of one member (about 5 s per round trip on 16 cores), or the
+    first `sample_levels` levels of each class plus ln ps, scaled linearly.
+    Returns the timing entry and the oracle's outputs."""
     sys.path.insert(0, str(ROOT))
     from oracle.sht import OracleTransform
 
@@ -174,12 +175,13 @@ def cpu_baseline(part, sample_levels=4):
     steady = float(np.mean(times[1:]))
     fields_total = 3 * LEVELS + 1
     value = (fields_sample / fields_total) / steady  # 121-field round trips/s
+    what = (f"all {fields_total} fields of one member" if fields_sample == fields_total else
+             f"{fields_sample} of {fields_total} fields ({L} levels of vorticity, divergence, "
+             f"temperature + ln ps), scaled linearly to {fields_total} fields")
     entry = {"value": value, "unit": "round trips/s", "cores": cores, "kind": "port",
-             "sample": f"FP64 numpy oracle (reference per-order BLAS algorithm), T{T}, "
-                       f"{fields_sample} of {fields_total} fields ({L} levels of vorticity, "
-                       f"divergence, temperature + ln ps), steady mean of 2 after a first "
-                       f"call ({times[0]:.2f} s), scaled linearly to {fields_total} fields; "
-                       f"sample round trip {steady:.2f} s"}
+             "sample": f"FP64 numpy oracle (reference per-order BLAS algorithm), T{T}, {what}; "
+                       f"steady mean of 2 round trips ({steady:.2f} s each) after a first call "
+                       f"({times[0]:.2f} s)"}
     return entry, dict(u=u, v=v, grid=g, zeta=z2, div=d2, scal=s2, L=L)
 
 
@@ -198,7 +200,30 @@ def oracle_errors(part, ref):
     }
 
 
-def roofline(rt, phases, precision):
+MODE_ID = {"float32": 1, "bf16x3": 2, "bf16x6": 3}
+TRAFFIC_FILE = ROOT / "profiles" / "legendre_gemm_t383_traffic.json"
+
+
+def gemm_traffic(config, precision, workload):
+    """HBM bytes of the round trip's two GEMM launches (synthesis + analysis)
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, each counter
+    scaled by its copy-kernel calibration factor; None when the summary does
+    not cover this workload and kernel instantiation."""
+    if workload != "t383" or not TRAFFIC_FILE.exists():
+        return None
+    rec = json.loads(TRAFFIC_FILE.read_text())
+    total = 0.0
+    for d in (0, 1):
+        acc2 = "true" if (precision != "float32" and config["split_acc"]) else "false"
+        name = f"legendre_gemm_kernel<{MODE_ID[precision]}, {d}, {config['cols'][d]}, {acc2}>"
+        k = rec["kernels"].get(name)
+        if k is None:
+            return None
+        total += (k["fetch_kib"] * rec["fetch_factor"] + k["write_kib"] * rec["write_factor"]) * 1024
+    return total
+
+
+def roofline(rt, phases, precision, workload="t383"):
     """Dominant kernel of the round trip: Legendre GEMM (MFMA-bound) or FFT (HBM-bound)."""
     gemm_ms = phases["synthesis_gemm"] + phases["analysis_gemm"]
     fft_ms = phases["inverse_fft"] + phases["forward_fft"]
@@ -207,7 +232,9 @@ def roofline(rt, phases, precision):
         achieved = s * rt.flops / (gemm_ms * 1e-3) / 1e12
         return {"bound": "mfma", "kernel": "legendre_gemm (synthesis + analysis launches)",
                 "achieved": achieved, "peak": MFMA_PEAK[precision], "unit": "TFLOP/s",
-                "frac": achieved / MFMA_PEAK[precision], "traffic": None,
+                "frac": achieved / MFMA_PEAK[precision],
+                "traffic": gemm_traffic(rt.config, precision, workload),
+                "traffic_unit": "HBM bytes per round trip (both launches)",
                 "fp32_equivalent_tflops": rt.flops / (gemm_ms * 1e-3) / 1e12,
                 "mfma_products_per_fma": s}
     achieved = rt.fft_bytes / (fft_ms * 1e-3) / 1e9
@@ -237,6 +264,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--split-acc", type=int, default=None,
                     help="split modes: 1 = small products in their own accumulators")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB on the device three times (the known-size kernel "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the transform "
@@ -245,6 +275,15 @@ def main():
     import torch
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = 64 * 1024 * 1024
+        x = torch.ones(n, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "copies": 3}))
+        return
     specs = workload_specs(args.workload)
     for sp in specs:
         sp["fields"] = member_fields(sp["T"], sp["members"])
@@ -266,7 +305,8 @@ def main():
         ph32 = rt32.batch.profile(reps=3)
         fp32 = {"value": 1.0 / sec32, "ms_per_step": sec32 * 1e3, "phases_ms": ph32,
                 "gemm_config": rt32.config,
-                "roundtrip_error": class_errors(rt32), "roofline": roofline(rt32, ph32, "float32")}
+                "roundtrip_error": class_errors(rt32),
+                "roofline": roofline(rt32, ph32, "float32", args.workload)}
         print(f"[bench] float32: {sec32 * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
         del rt32
 
@@ -312,7 +352,7 @@ def main():
         "speedup_vs_fp32": (fp32["ms_per_step"] / (sec * 1e3)) if fp32 else None,
         "legendre_gflop_per_step": rt.flops / 1e9,
         "fft_mb_per_step": rt.fft_bytes / 1e6,
-        "roofline": roofline(rt, phases, args.precision),
+        "roofline": roofline(rt, phases, args.precision, args.workload),
         "cpu_baseline": cpu,
     }
     print(json.dumps(result))
