@@ -417,7 +417,7 @@ void build_lists(int dir, const std::vector<HostGroup>& groups, int bn, GemmList
         fail(ARWEN_SHT_EINVAL, "truncation too large for the GEMM step list (" +
                                    std::to_string(pn.nsteps) + " steps in one panel, at most " +
                                    std::to_string(sht::kGemmMaxSteps - 2) + ")");
-      // two null steps: the kernel's loads run two steps ahead
+      // two null steps: the kernel's loads run ahead of the list's end
       sht::GemmStep null{};
       null.a = steps[pn.step0].a;
       null.b = steps[pn.step0].b ? steps[pn.step0].b : steps[pn.step0].a;

@@ -18,13 +18,14 @@
 // negate some of them): the vector operators' second arrangement of the data
 // is such a map of the first, so only one copy is stored.
 //
-// Pipeline: two register stages and two LDS buffers.  While the wavefronts
-// run step j's MFMAs out of LDS, the loads of steps j+1 and j+2 are in
-// flight; step j+1 is then written to the other LDS buffer and one barrier
-// ends the step.  The panel's step descriptors sit in LDS for the whole tile,
-// so no memory load other than the tiles is waited on inside the loop.  Each
-// panel's list ends in two null steps (extent 0), so every load is issued
-// unconditionally.
+// Pipeline: four register stages and two LDS buffers.  While the wavefronts
+// run step j's MFMAs out of LDS, the loads of steps j+1, j+2 and j+3 are in
+// flight (a step waits on loads issued two steps before it, not one: the
+// loop is bound by memory latency otherwise); step j+1 is then written to the
+// other LDS buffer and one barrier ends the step.  The panel's step
+// descriptors sit in LDS for the whole tile, so no memory load other than the
+// tiles is waited on inside the loop.  Reads past the panel's list return its
+// last null step (extent 0), so every load is issued unconditionally.
 //
 // LDS tiles are unpadded rows of 32 k (128 bytes fp32, 64 bytes per bf16
 // piece) whose 16-byte chunks are XOR-swizzled by row, so that the MFMA
@@ -183,8 +184,9 @@ __device__ __forceinline__ void mfma_step(f32x4& acc, f32x4& acc2, const u32x4 (
 #undef SHT_MFMA_BF
 }
 
+// 32-column tiles fit three workgroups per CU (<= 168 VGPRs), 64-column ones two.
 template <int MODE, int DIR, int BN, bool ACC2>
-__global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
+__global__ void __launch_bounds__(256, BN == 32 ? 3 : 2) legendre_gemm_kernel(GemmWork w) {
   using C = GemmCfg<MODE, DIR, BN>;
   constexpr bool F32 = C::F32;
   constexpr int NP = C::NP, ESZ = C::ESZ, ROWB = C::ROWB, CHS = C::CHS, NB = C::NB;
@@ -209,8 +211,12 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
     const unsigned long long coloff = (unsigned long long)c0 * P.bcs * 4;  // fp32 data
     const int step0 = P.step0, nsteps = P.nsteps;
 
-    // ---- register stages and their loads ----
-    u32x4 ra[2][C::A_REGS], rb[2][C::B_REGS];
+    // ---- register stages and their loads: step j's tiles sit in stage
+    // j % 4 (its order within the trip), so loads run three steps ahead.
+    // ANA loads B once per trip (one stage). ----
+    constexpr int kStages = kMT;
+    constexpr int kBStages = DIR == 0 ? kStages : 1;
+    u32x4 ra[kStages][C::A_REGS], rb[kBStages][C::B_REGS];
     // The panel's step list (with its two null steps) is copied to LDS once
     // per tile; a step's descriptor is then an LDS broadcast read, off the
     // memory queue that the tile loads wait in.
@@ -387,46 +393,50 @@ __global__ void __launch_bounds__(256) legendre_gemm_kernel(GemmWork w) {
         }
     };
 
-    // ---- prologue: steps 0 and 1 in flight, step 0 in LDS buffer 0 ----
-    // cur, nxt: steps j and j+1 in scalar registers; dsc[p]: the fetched
-    // descriptor of the next step issued from register stage p.
-    GemmDesc dsc[2];
-    GemmStep cur, nxt;
+    // ---- prologue: steps 0, 1 and 2 in flight, step 0 in LDS buffer 0 ----
+    // q1, q2, q3: steps j+1, j+2, j+3 (scalar) at the start of step j; dn:
+    // the raw descriptor of step j+4, fetched one step ahead.
+    GemmStep q1, q2, q3;
+    GemmDesc dn;
     {
-      const GemmDesc d0 = fetch_desc(0), d1 = fetch_desc(1);
-      dsc[0] = fetch_desc(2);
-      cur = d0.scalar();
-      issue(ra[0], rb[0], cur, std::true_type{});
-      nxt = d1.scalar();
-      issue(ra[1], rb[1], nxt, std::integral_constant<bool, DIR == 0>{});
+      const GemmStep s0 = fetch_desc(0).scalar();
+      q1 = fetch_desc(1).scalar();
+      q2 = fetch_desc(2).scalar();
+      issue(ra[0], rb[0], s0, std::true_type{});
+      issue(ra[1], rb[DIR == 0 ? 1 : 0], q1, std::integral_constant<bool, DIR == 0>{});
+      issue(ra[2], rb[DIR == 0 ? 2 : 0], q2, std::integral_constant<bool, DIR == 0>{});
+      q3 = fetch_desc(3).scalar();
+      dn = fetch_desc(4);
       put_a(ra[0], 0);
-      put_b(rb[0], 0, cur.cmap);
+      put_b(rb[0], 0, s0.cmap);
     }
     __syncthreads();
 
     // ---- main loop: trips of kMT steps, one per order Z of the panel (the
-    // host pads every k-chunk to kMT steps), so each step's accumulators and
-    // register stage (parity of Z) are fixed at compile time ----
+    // host pads every k-chunk to kMT steps), so each step's accumulators,
+    // register stage and LDS buffer are fixed at compile time ----
     auto body = [&](auto ZC, int j) {
       constexpr int Z = decltype(ZC)::value;
-      constexpr int s = Z & 1;
-      dsc[s ^ 1] = fetch_desc(j + 3);
-      const GemmStep d2 = dsc[s].scalar();  // step j+2, fetched one step ago
-      // step j+2 is position (Z+2) % kMT of its trip
-      issue(ra[s], rb[s], d2, std::integral_constant<bool, DIR == 0 || Z == 2>{});
+      constexpr int s = Z & 1;               // LDS buffer of step j
+      constexpr int R3 = (Z + 3) % kStages;  // register stage (= trip position) of step j+3
+      constexpr int R1 = (Z + 1) % kStages;  // register stage of step j+1
+      issue(ra[R3], rb[DIR == 0 ? R3 : 0], q3, std::integral_constant<bool, DIR == 0 || R3 == 0>{});
+      const GemmStep q4 = dn.scalar();
+      dn = fetch_desc(j + 5);
       compute(s, ZC);
       if (j + 1 < nsteps) {
         if constexpr (DIR == 0) {
-          put_b(rb[s ^ 1], s ^ 1, nxt.cmap);
-        } else if constexpr (Z == kMT - 1) {
+          put_b(rb[R1], s ^ 1, q1.cmap);
+        } else if constexpr (R1 == 0) {
           __syncthreads();  // every wavefront is done with the resident B tile
-          put_b(rb[s ^ 1], 0, nxt.cmap);
+          put_b(rb[0], 0, q1.cmap);
         }
-        put_a(ra[s ^ 1], s ^ 1);
+        put_a(ra[R1], s ^ 1);
       }
       __syncthreads();
-      cur = nxt;
-      nxt = d2;
+      q1 = q2;
+      q2 = q3;
+      q3 = q4;
     };
     static_assert(kMT == 4, "trip of four steps");
     for (int j = 0; j < nsteps; j += kMT) {
