@@ -124,19 +124,18 @@ struct GemmCfg {
   static constexpr bool F32 = MODE == kFP32;
   static constexpr int NP = F32 ? 1 : (MODE == kBF16X6 ? 3 : 2);  // stored pieces
   static constexpr int ESZ = F32 ? 4 : 2;   // element bytes of tables and data
-  static constexpr int ROWB = kBK * ESZ;    // LDS row bytes
-  static constexpr int CHS = ROWB / 16;     // 16-byte chunks per row
+  static constexpr int ROWB = kBK * ESZ;    // bytes of one row of a 32-k tile
   static constexpr int BZ = DIR == 0 ? 1 : kMT;
-  static constexpr int NB = BN / 32;        // 16-column MFMA blocks per wavefront
-  static constexpr int A_TILE = kGemmBM * ROWB;        // bytes per piece
+  static constexpr int NB = BN / 16;        // 16-column MFMA blocks per wavefront
   static constexpr int B_TILE = BZ * BN * ROWB;        // LDS bytes per piece
   static constexpr int B_BUFS = DIR == 0 ? 2 : 1;      // ANA keeps one k-chunk resident
-  // register stage: 16-byte registers per thread.  Tables come as stored
-  // pieces; data (B) come as fp32 and are split when written to LDS.
-  static constexpr int A_PER = A_TILE / (256 * 16);    // per piece
-  static constexpr int A_REGS = NP * A_PER;
+  // register stage, in 16-byte registers per lane: the wavefront's own A
+  // fragments (16 rows x 32 k: one register per bf16 piece, two for fp32),
+  // loaded straight from the tiled table; the thread's share of the B tile,
+  // fp32 (split into pieces when written to LDS).
+  static constexpr int A_REGS = F32 ? 2 : NP;
   static constexpr int B_REGS = BZ * BN * kBK * 4 / (256 * 16);
-  static_assert(A_PER >= 1 && B_REGS >= 1, "tile shape");
+  static_assert(B_REGS >= 1, "tile shape");
 };
 
 // Four fp32 values as `np` bf16 piece vectors (each the round-to-nearest-even
@@ -189,8 +188,7 @@ template <int MODE, int DIR, int BN, bool ACC2>
 __global__ void __launch_bounds__(256, BN == 32 ? 3 : 2) legendre_gemm_kernel(GemmWork w) {
   using C = GemmCfg<MODE, DIR, BN>;
   constexpr bool F32 = C::F32;
-  constexpr int NP = C::NP, ESZ = C::ESZ, ROWB = C::ROWB, CHS = C::CHS, NB = C::NB;
-  __shared__ __attribute__((aligned(16))) unsigned char sA[2][NP * C::A_TILE];
+  constexpr int NP = C::NP, ROWB = C::ROWB, NB = C::NB;
   __shared__ __attribute__((aligned(16))) unsigned char sB[C::B_BUFS][NP * C::B_TILE];
   __shared__ u32x4 sDesc[2 * kGemmMaxSteps];  // the current panel's step list
 
@@ -200,7 +198,7 @@ __global__ void __launch_bounds__(256, BN == 32 ? 3 : 2) legendre_gemm_kernel(Ge
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, kg = lane >> 4;
-  const int wr = wave >> 1, wc = wave & 1;
+  const int arow = 16 * wave + li;  // the row of this lane's A fragment
 
   for (int t = blockIdx.x; t < w.ntiles; t += gridDim.x) {
     const int2 tile = tiles[t];
@@ -235,14 +233,18 @@ __global__ void __launch_bounds__(256, BN == 32 ? 3 : 2) legendre_gemm_kernel(Ge
     // step of each k-chunk, position 0 of a trip of kMT steps)
     auto issue = [&](u32x4(&A)[C::A_REGS], u32x4(&B)[C::B_REGS], const GemmStep& s,
                      auto WITH_B) {
-      // A: one contiguous tile per piece
+      // A: this lane's fragment of each piece, 16 bytes (bf16: k chunk kg of
+      // row arow; fp32: chunks kg and kg + 4); rows past a_bytes read 0
 #pragma unroll
       for (int pc = 0; pc < NP; ++pc) {
         const __amdgpu_buffer_rsrc_t r =
             make_rsrc(s.a + (unsigned long long)(pc * P.piece_bytes), s.a_bytes);
-#pragma unroll
-        for (int u = 0; u < C::A_PER; ++u)
-          A[pc * C::A_PER + u] = load16(r, (unsigned)(tid + 256 * u) * 16u);
+        if constexpr (F32) {
+          A[0] = load16(r, (unsigned)(arow * ROWB + kg * 16));
+          A[1] = load16(r, (unsigned)(arow * ROWB + (kg + 4) * 16));
+        } else {
+          A[pc] = load16(r, (unsigned)(arow * ROWB + kg * 16));
+        }
       }
       const int rev = (s.cmap & 1) ? 3 : 0;
       if constexpr (DIR == 0) {
@@ -271,18 +273,6 @@ __global__ void __launch_bounds__(256, BN == 32 ? 3 : 2) legendre_gemm_kernel(Ge
           for (int h = 0; h < 4; ++h) B[u * 4 + h] = load16(r, off + 16u * h);
         }
       }
-    };
-    // A stage -> LDS buffer (swizzled rows)
-    auto put_a = [&](const u32x4(&A)[C::A_REGS], int buf) {
-#pragma unroll
-      for (int pc = 0; pc < NP; ++pc)
-#pragma unroll
-        for (int u = 0; u < C::A_PER; ++u) {
-          const int q = tid + 256 * u;
-          const int row = q / CHS, ch = q % CHS;
-          *reinterpret_cast<u32x4*>(
-              &sA[buf][pc * C::A_TILE + row * ROWB + swz<F32>(row, ch) * 16]) = A[pc * C::A_PER + u];
-        }
     };
     // B stage -> LDS (split into pieces in the bf16 modes); `cmap` bits 1..4
     // negate columns 0..3 of each group of four
@@ -319,12 +309,12 @@ __global__ void __launch_bounds__(256, BN == 32 ? 3 : 2) legendre_gemm_kernel(Ge
 #pragma unroll
           for (int z = 0; z < kMT; ++z) {
             unsigned char* row = &sB[0][(z * BN + col) * ROWB];
-            const u32x4 w = u32x4{v[0][z], v[1][z], v[2][z], v[3][z]} ^ flip;
+            const u32x4 wv = u32x4{v[0][z], v[1][z], v[2][z], v[3][z]} ^ flip;
             if constexpr (F32) {
-              *reinterpret_cast<u32x4*>(row + swz<true>(col, quad) * 16) = w;
+              *reinterpret_cast<u32x4*>(row + swz<true>(col, quad) * 16) = wv;
             } else {
-              const float x[4] = {__uint_as_float(w[0]), __uint_as_float(w[1]),
-                                  __uint_as_float(w[2]), __uint_as_float(w[3])};
+              const float x[4] = {__uint_as_float(wv[0]), __uint_as_float(wv[1]),
+                                  __uint_as_float(wv[2]), __uint_as_float(wv[3])};
               u32x2 pcs[3];
               split4(x, NP, pcs);
 #pragma unroll
@@ -337,63 +327,45 @@ __global__ void __launch_bounds__(256, BN == 32 ? 3 : 2) legendre_gemm_kernel(Ge
       }
     };
 
-    f32x4 acc[kMT][2][NB], acc2[kMT][2][NB];
+    f32x4 acc[kMT][NB], acc2[kMT][NB];
 #pragma unroll
     for (int a = 0; a < kMT; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int n = 0; n < NB; ++n) {
-          acc[a][b][n] = f32x4{0.f, 0.f, 0.f, 0.f};
-          acc2[a][b][n] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
+      for (int n = 0; n < NB; ++n) {
+        acc[a][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc2[a][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
 
-    // one step out of LDS buffer `buf` into the accumulators of order Z
+    // one step into the accumulators of order Z: A fragments from register
+    // stage Z, B fragments from LDS buffer `buf`
     auto compute = [&](int buf, auto ZC) {
       constexpr int Z = decltype(ZC)::value;
       const int bbuf = DIR == 0 ? buf : 0;
       constexpr int bz = DIR == 0 ? 0 : Z;
-      u32x4 af[2][3], bf[NB][3];
+      u32x4 af[3];
 #pragma unroll
-      for (int r2 = 0; r2 < 2; ++r2) {
-        const int row = wr * 32 + r2 * 16 + li;
-        if constexpr (F32) {
-#pragma unroll
-          for (int h = 0; h < 2; ++h)
-            af[r2][h] = *reinterpret_cast<const u32x4*>(
-                &sA[buf][row * ROWB + swz<true>(row, kg + 4 * h) * 16]);
-        } else {
-#pragma unroll
-          for (int pc = 0; pc < NP; ++pc)
-            af[r2][pc] = *reinterpret_cast<const u32x4*>(
-                &sA[buf][pc * C::A_TILE + row * ROWB + swz<false>(row, kg) * 16]);
-        }
-      }
+      for (int r = 0; r < C::A_REGS; ++r) af[r] = ra[Z][r];
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb) {
-        const int col = wc * (BN / 2) + nb * 16 + li;
+        const int col = nb * 16 + li;
         const unsigned char* base = &sB[bbuf][(bz * BN + col) * ROWB];
+        u32x4 bf[3];
         if constexpr (F32) {
 #pragma unroll
           for (int h = 0; h < 2; ++h)
-            bf[nb][h] = *reinterpret_cast<const u32x4*>(base + swz<true>(col, kg + 4 * h) * 16);
+            bf[h] = *reinterpret_cast<const u32x4*>(base + swz<true>(col, kg + 4 * h) * 16);
         } else {
 #pragma unroll
           for (int pc = 0; pc < NP; ++pc)
-            bf[nb][pc] =
+            bf[pc] =
                 *reinterpret_cast<const u32x4*>(base + pc * C::B_TILE + swz<false>(col, kg) * 16);
         }
+        if constexpr (ACC2) mfma_step<MODE>(acc[Z][nb], acc2[Z][nb], af, bf);
+        else mfma_step<MODE>(acc[Z][nb], acc[Z][nb], af, bf);
       }
-#pragma unroll
-      for (int r2 = 0; r2 < 2; ++r2)
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-          if constexpr (ACC2) mfma_step<MODE>(acc[Z][r2][nb], acc2[Z][r2][nb], af[r2], bf[nb]);
-          else mfma_step<MODE>(acc[Z][r2][nb], acc[Z][r2][nb], af[r2], bf[nb]);
-        }
     };
 
-    // ---- prologue: steps 0, 1 and 2 in flight, step 0 in LDS buffer 0 ----
+    // ---- prologue: steps 0, 1 and 2 in flight, step 0's B in LDS buffer 0 ----
     // q1, q2, q3: steps j+1, j+2, j+3 (scalar) at the start of step j; dn:
     // the raw descriptor of step j+4, fetched one step ahead.
     GemmStep q1, q2, q3;
@@ -407,7 +379,6 @@ __global__ void __launch_bounds__(256, BN == 32 ? 3 : 2) legendre_gemm_kernel(Ge
       issue(ra[2], rb[DIR == 0 ? 2 : 0], q2, std::integral_constant<bool, DIR == 0>{});
       q3 = fetch_desc(3).scalar();
       dn = fetch_desc(4);
-      put_a(ra[0], 0);
       put_b(rb[0], 0, s0.cmap);
     }
     __syncthreads();
@@ -417,23 +388,24 @@ __global__ void __launch_bounds__(256, BN == 32 ? 3 : 2) legendre_gemm_kernel(Ge
     // register stage and LDS buffer are fixed at compile time ----
     auto body = [&](auto ZC, int j) {
       constexpr int Z = decltype(ZC)::value;
-      constexpr int s = Z & 1;               // LDS buffer of step j
+      constexpr int s = Z & 1;               // LDS buffer of step j's B (SYN)
       constexpr int R3 = (Z + 3) % kStages;  // register stage (= trip position) of step j+3
       constexpr int R1 = (Z + 1) % kStages;  // register stage of step j+1
       issue(ra[R3], rb[DIR == 0 ? R3 : 0], q3, std::integral_constant<bool, DIR == 0 || R3 == 0>{});
       const GemmStep q4 = dn.scalar();
       dn = fetch_desc(j + 5);
       compute(s, ZC);
-      if (j + 1 < nsteps) {
-        if constexpr (DIR == 0) {
-          put_b(rb[R1], s ^ 1, q1.cmap);
-        } else if constexpr (R1 == 0) {
-          __syncthreads();  // every wavefront is done with the resident B tile
-          put_b(rb[0], 0, q1.cmap);
-        }
-        put_a(ra[R1], s ^ 1);
+      if constexpr (DIR == 0) {
+        // step j+1's B goes to the other buffer; the barrier orders it before
+        // step j+1 reads it and after every read of this buffer two steps ago
+        if (j + 1 < nsteps) put_b(rb[R1], s ^ 1, q1.cmap);
+        __syncthreads();
+      } else if constexpr (R1 == 0) {
+        // ANA: the resident B tile changes with the next trip
+        __syncthreads();  // every wavefront is done with it
+        if (j + 1 < nsteps) put_b(rb[0], 0, q1.cmap);
+        __syncthreads();
       }
-      __syncthreads();
       q1 = q2;
       q2 = q3;
       q3 = q4;
@@ -446,41 +418,41 @@ __global__ void __launch_bounds__(256, BN == 32 ? 3 : 2) legendre_gemm_kernel(Ge
       body(std::integral_constant<int, 3>{}, j + 3);
     }
 
-    // ---- epilogue ----
+    // ---- epilogue: this lane's rows are 16 wave + 4 kg .. + 3 of column
+    // 16 nb + li ----
 #pragma unroll
-    for (int r2 = 0; r2 < 2; ++r2)
+    for (int nb = 0; nb < NB; ++nb) {
+      const int col = nb * 16 + li;
+      const int row = 16 * wave + 4 * kg;
+      if (col >= ncol) continue;
+      if constexpr (DIR == 0) {
+        // rows row..row+3 x orders 0..3 = 16 contiguous floats of the waist
+        if (row >= P.rows[0]) continue;
+        SHT_GLOBAL float* cp = reinterpret_cast<SHT_GLOBAL float*>(P.c[0]) +
+                               (long long)(c0 + col) * P.ccs + (long long)row * kMT;
 #pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const int col = wc * (BN / 2) + nb * 16 + li;
-        const int row = wr * 32 + r2 * 16 + 4 * kg;
-        if (col >= ncol) continue;
-        if constexpr (DIR == 0) {
-          // rows row..row+3 x orders 0..3 = 16 contiguous floats of the waist
-          if (row >= P.rows[0]) continue;
-          SHT_GLOBAL float* cp = reinterpret_cast<SHT_GLOBAL float*>(P.c[0]) +
-                                 (long long)(c0 + col) * P.ccs + (long long)row * kMT;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            f32x4 o;
-#pragma unroll
-            for (int zz = 0; zz < kMT; ++zz) {
-              f32x4 v = acc[zz][r2][nb];
-              if constexpr (!F32 && ACC2) v = v + acc2[zz][r2][nb];
-              o[zz] = v[q];
-            }
-            *reinterpret_cast<SHT_GLOBAL f32x4*>(cp + q * kMT) = o;
-          }
-        } else {
+        for (int q = 0; q < 4; ++q) {
+          f32x4 o;
 #pragma unroll
           for (int zz = 0; zz < kMT; ++zz) {
-            if (row >= P.rows[zz]) continue;
-            f32x4 v = acc[zz][r2][nb];
-            if constexpr (!F32 && ACC2) v = v + acc2[zz][r2][nb];
-            *reinterpret_cast<SHT_GLOBAL f32x4*>(reinterpret_cast<SHT_GLOBAL float*>(P.c[zz]) +
-                                                 (long long)(c0 + col) * P.ccs + row) = v;
+            f32x4 v = acc[zz][nb];
+            if constexpr (!F32 && ACC2) v = v + acc2[zz][nb];
+            o[zz] = v[q];
           }
+          *reinterpret_cast<SHT_GLOBAL f32x4*>(cp + q * kMT) = o;
+        }
+      } else {
+#pragma unroll
+        for (int zz = 0; zz < kMT; ++zz) {
+          if (row >= P.rows[zz]) continue;
+          f32x4 v = acc[zz][nb];
+          if constexpr (!F32 && ACC2) v = v + acc2[zz][nb];
+          *reinterpret_cast<SHT_GLOBAL f32x4*>(reinterpret_cast<SHT_GLOBAL float*>(P.c[zz]) +
+                                               (long long)(c0 + col) * P.ccs + row) = v;
         }
       }
+    }
+    __syncthreads();  // sB and sDesc are reused by the next tile
   }
 }
 
