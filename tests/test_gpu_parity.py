@@ -207,3 +207,33 @@ def test_t383_roundtrip_and_parity(precision):
     check = tr.transform_check(seed=1)
     _record(f"check383_{precision}", check)
     assert check["roundtrip_relative_linf"] < (1e-6 if precision != "bf16x3" else 1e-4)
+
+
+@pytest.mark.parametrize("precision", ["float32", "bf16x6"])
+def test_t767_roundtrips_on_the_longest_rows(precision):
+    """T767 on 768 x 1536 (device nodes): nlon = 64 x 24 is the longest
+    wave-resident FFT, and the u,v analysis there is the one launch whose
+    eight-row blocks do not fit LDS (it runs four rows per block).  No FP64
+    oracle at this size: round trips against the inputs instead, with the
+    bounds of the T383 round trips (scalars) and of the measured vector
+    amplification floor (about 3.5e-5 for D at T767)."""
+    from arwen_spectral import SphericalHarmonicTransform, VorticityDivergenceOperator
+
+    tr = SphericalHarmonicTransform.create(767, dealias_factor=1.0, backend="hip",
+                                           precision=precision)
+    assert tr.grid.shape == (768, 1536)
+    coeff = red_spectrum(767, (2,), seed=7, scale=1.0)
+    back = tr.forward(tr.inverse(coeff))
+    err = rel_err(back, coeff)
+    _record(f"roundtrip767_{precision}", err)
+    assert err < 2 * TOL[precision]
+    op = VorticityDivergenceOperator(tr)
+    zeta = red_spectrum(767, (2,), seed=8, scale=1e-5)
+    div = red_spectrum(767, (2,), seed=9, scale=1e-5)
+    zeta[..., 0, 0] = 0.0
+    div[..., 0, 0] = 0.0
+    u, v = op.wind_from_vordiv(zeta, div)
+    z2, d2 = op.vordiv_from_wind(u, v)
+    err = max(rel_err(z2, zeta), rel_err(d2, div))
+    _record(f"vector_roundtrip767_{precision}", err)
+    assert err < 1e-4
