@@ -7,7 +7,9 @@ Drop-in for the reference's ``arwen_global.spectral`` transform layer:
 (Gauss nodes and Legendre tables in FP64 on the device, parity-folded
 grouped Legendre GEMMs on MFMA in FP32 or split BF16, fused radix-2/3/4
 longitude FFTs).  ``TransformBatch`` runs many transforms as one grouped
-launch per direction, optionally as a hipGraph.
+launch per direction, optionally as a hipGraph.  ``FourierWaist`` is the
+(*lead, nlat, T+1) buffer between the FFT and the Legendre contraction that
+a latitude-band pipeline fills and drains.
 """
 from __future__ import annotations
 
@@ -15,10 +17,12 @@ from .backend import Backend, get_backend
 from .batch import TransformBatch
 from .constants import EARTH_RADIUS_M
 from .grid import GaussianGrid
-from .transform import DEFAULT_BAND, SphericalHarmonicTransform
+from .transform import DEFAULT_BAND, LegendreTable, SphericalHarmonicTransform
 from .vector import VorticityDivergenceOperator
+from .waist import FourierWaist, latitude_band_edges
 
 __all__ = [
     "Backend", "get_backend", "TransformBatch", "EARTH_RADIUS_M", "GaussianGrid",
     "DEFAULT_BAND", "SphericalHarmonicTransform", "VorticityDivergenceOperator",
+    "FourierWaist", "LegendreTable", "latitude_band_edges",
 ]

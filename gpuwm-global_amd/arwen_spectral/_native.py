@@ -21,6 +21,13 @@ OP_GRADIENT = 4
 OP_UV_FROM_VORDIV = 5
 OP_ANALYSIS = 6
 OP_VORDIV_FROM_UV = 7
+OP_SYNTHESIS_TO_WAIST = 8
+OP_ZONAL_TO_WAIST = 9
+OP_MERIDIONAL_TO_WAIST = 10
+OP_ANALYSIS_FROM_WAIST = 11
+OP_VORDIV_FROM_WAIST = 12
+# kind argument of arwen_sht_synthesis_to_waist
+WAIST_BASIS, WAIST_ZONAL, WAIST_DERIVATIVE = 0, 1, 2
 
 SPECTRAL_PROJECT, SPECTRAL_LAPLACIAN, SPECTRAL_INVERSE_LAPLACIAN = 0, 1, 2
 
@@ -30,7 +37,9 @@ EXPORTS = (
     "arwen_sht_gauss_nodes", "arwen_sht_plan_table_bytes", "arwen_sht_plan_table_f64",
     "arwen_sht_analysis", "arwen_sht_synthesis", "arwen_sht_zonal_derivative",
     "arwen_sht_meridional_derivative", "arwen_sht_gradient", "arwen_sht_uv_from_vordiv",
-    "arwen_sht_vordiv_from_uv", "arwen_sht_spectral_op", "arwen_sht_batch_create",
+    "arwen_sht_vordiv_from_uv", "arwen_sht_synthesis_to_waist", "arwen_sht_analysis_from_waist",
+    "arwen_sht_vordiv_from_waist", "arwen_sht_waist_from_rows", "arwen_sht_rows_from_waist",
+    "arwen_sht_spectral_op", "arwen_sht_batch_create",
     "arwen_sht_batch_destroy", "arwen_sht_batch_run", "arwen_sht_batch_capture",
     "arwen_sht_batch_launch", "arwen_sht_batch_get_config", "arwen_sht_batch_set_config",
     "arwen_sht_batch_autotune", "arwen_sht_batch_work", "arwen_sht_batch_profile",
@@ -114,6 +123,11 @@ def load():
             "arwen_sht_gradient": (I, [P, P, P, P, I, P]),
             "arwen_sht_uv_from_vordiv": (I, [P, P, P, P, P, I, P]),
             "arwen_sht_vordiv_from_uv": (I, [P, P, P, P, P, I, P]),
+            "arwen_sht_synthesis_to_waist": (I, [P, P, P, I, I, P]),
+            "arwen_sht_analysis_from_waist": (I, [P, P, P, I, P]),
+            "arwen_sht_vordiv_from_waist": (I, [P, P, P, P, P, I, P]),
+            "arwen_sht_waist_from_rows": (I, [P, P, P, I, I, I, P]),
+            "arwen_sht_rows_from_waist": (I, [P, P, P, I, I, I, P]),
             "arwen_sht_spectral_op": (I, [P, I, P, P, I, P]),
             "arwen_sht_batch_create": (I, [ctypes.POINTER(Job), I, ctypes.POINTER(P)]),
             "arwen_sht_batch_destroy": (I, [P]),

@@ -21,12 +21,23 @@ from . import _native
 from .backend import Backend, get_backend
 from .constants import EARTH_RADIUS_M
 from .grid import GaussianGrid
+from .waist import FourierWaist, latitude_band_edges
 
 DEFAULT_BAND = 32  # reference legendre.py:36; accepted for API compatibility
 
 
 def _vp(t) -> ctypes.c_void_p:
     return ctypes.c_void_p(t.data_ptr())
+
+
+@dataclass(frozen=True)
+class LegendreTable:
+    """Names one of a plan's device-resident Legendre tables: ``analysis``
+    (P w in the analysis layout), ``basis`` (P) or ``derivative`` (dP/dphi).
+    The reference hands its contractions table objects
+    (transform.py:122-216, 902-934); here the tables live inside the plan
+    and these names select them."""
+    name: str
 
 
 @dataclass
@@ -68,6 +79,11 @@ class SphericalHarmonicTransform:
         torch = self.backend.xp
         n = np.arange(g.truncation + 1, dtype=np.float64)
         self._eigen = -(n * (n + 1.0)) / (g.radius_m * g.radius_m)
+        self._analysis = LegendreTable("analysis")
+        self._basis = LegendreTable("basis")
+        self._derivative_basis = LegendreTable("derivative")
+        self._coslat = torch.tensor(np.asarray(g.cos_lat, dtype=np.float64)[:, None],
+                                    dtype=self.backend.float_dtype, device=self.backend.device())
 
     def __del__(self):
         plan = getattr(self, "_plan", None)
@@ -237,6 +253,132 @@ class SphericalHarmonicTransform:
 
     def inverse_laplacian(self, coeff):
         return self._spectral_op(_native.SPECTRAL_INVERSE_LAPLACIAN, coeff)
+
+    # ---- the Fourier waist (transform.py:1276-1472, 1629-1796) ----------
+    def _edges(self, bands):
+        return latitude_band_edges(self.grid.nlat,
+                                   self.latitude_bands if bands is None else bands)
+
+    def _waist_tensor(self, values):
+        v = self.backend.asarray(values, dtype=self.backend.complex_dtype).contiguous()
+        want = (self.grid.nlat, self.truncation + 1)
+        if tuple(v.shape[-2:]) != want:
+            raise ValueError(f"Fourier waist has trailing shape {tuple(v.shape[-2:])}, "
+                             f"expected {want}")
+        return v
+
+    def _empty_waist(self, lead):
+        torch = self.backend.xp
+        return torch.empty((*lead, self.grid.nlat, self.truncation + 1),
+                           dtype=self.backend.complex_dtype, device=self.backend.device())
+
+    def _waist_from_rows(self, rows, values, row0: int, nrows: int) -> None:
+        nlead = int(np.prod(rows.shape[:-2], dtype=np.int64))
+        _native.check(self._lib.arwen_sht_waist_from_rows(
+            self._plan, _vp(rows), _vp(values), nlead, int(nrows), int(row0),
+            self.backend.stream()))
+
+    def open_waist(self, lead, *, bands: int | None = None) -> FourierWaist:
+        """An empty ``(*lead, nlat, T+1)`` waist for a band pipeline to fill
+        (transform.py:1276-1292)."""
+        return FourierWaist.open(self, lead, self._edges(bands))
+
+    def fourier_waist(self, field, *, bands: int | None = None) -> FourierWaist:
+        """The head of an analysis: rfft of every latitude row / nlon, orders
+        m <= T (transform.py:1294-1404).  Each row is transformed alone, so
+        the band count changes no bit; the waist carries the schedule."""
+        f = self._grid_tensor(field)
+        edges = self._edges(bands)
+        values = self._empty_waist(f.shape[:-2])
+        self._waist_from_rows(f, values, 0, self.grid.nlat)
+        return FourierWaist(self, values, edges)
+
+    def _require(self, table, allowed, what):
+        if table is None:
+            return allowed[0]
+        if not isinstance(table, LegendreTable) or table not in allowed:
+            names = ", ".join(t.name for t in allowed)
+            raise ValueError(f"{what} contracts with the plan's {names} table; got {table!r}")
+        return table
+
+    def _analysis_from_waist(self, values):
+        out = self._empty_spectral(values.shape[:-2])
+        _native.check(self._lib.arwen_sht_analysis_from_waist(
+            self._plan, _vp(values), _vp(out), self._count(values), self.backend.stream()))
+        return out
+
+    def contract_waist(self, waist: FourierWaist, table=None):
+        """The tail of an analysis: contract a full waist to coefficients,
+        m = 0 realified; consumes the waist (transform.py:1435-1452)."""
+        self._require(table, (self._analysis,), "contract_waist")
+        waist.close()
+        return self._analysis_from_waist(self._waist_tensor(waist.take()))
+
+    def contract_values(self, values, table=None):
+        """:meth:`contract_waist` on a waist's buffer or a leading slice of it
+        (transform.py:1454-1472)."""
+        self._require(table, (self._analysis,), "contract_values")
+        return self._analysis_from_waist(self._waist_tensor(values))
+
+    def _to_waist(self, coeff, kind: int):
+        c = self._spectral_tensor(coeff)
+        values = self._empty_waist(c.shape[:-2])
+        _native.check(self._lib.arwen_sht_synthesis_to_waist(
+            self._plan, _vp(c), _vp(values), self._count(c), int(kind), self.backend.stream()))
+        return values
+
+    def contract_to_waist(self, coeff, table=None, *, bands: int | None = None) -> FourierWaist:
+        """The head of a synthesis: the Legendre sum of ``coeff`` with the
+        basis (or dP/dphi) into a waist that carries the drain schedule
+        (transform.py:1629-1652)."""
+        table = self._require(table, (self._basis, self._derivative_basis), "contract_to_waist")
+        kind = _native.WAIST_BASIS if table is self._basis else _native.WAIST_DERIVATIVE
+        return FourierWaist(self, self._to_waist(coeff, kind), self._edges(bands))
+
+    def _rows_to_grid(self, values, waist: FourierWaist, r0: int, r1: int):
+        r0, r1 = int(r0), int(r1)
+        if not (0 <= r0 < r1 <= self.grid.nlat):
+            raise ValueError(f"latitude band [{r0}, {r1}) is not a nonempty range of the "
+                             f"{self.grid.nlat} rows")
+        torch = self.backend.xp
+        out = torch.empty((*waist.lead, r1 - r0, self.grid.nlon),
+                          dtype=self.backend.float_dtype, device=self.backend.device())
+        nlead = int(np.prod(waist.lead, dtype=np.int64))
+        _native.check(self._lib.arwen_sht_rows_from_waist(
+            self._plan, _vp(values), _vp(out), nlead, r1 - r0, r0, self.backend.stream()))
+        return out
+
+    def waist_band_to_grid(self, waist: FourierWaist, r0: int, r1: int):
+        """One latitude band of a waist in grid space, the waist kept
+        (transform.py:1704-1711)."""
+        return self._rows_to_grid(waist.values, waist, r0, r1)
+
+    def waist_to_grid(self, waist: FourierWaist):
+        """The whole waist in grid space; consumes it (transform.py:1713-1735).
+        One call drains every band: each row's inverse FFT is its own, so
+        the result is the band-by-band drain's."""
+        values = waist.values
+        out = self._rows_to_grid(values, waist, 0, self.grid.nlat)
+        waist.take()
+        return out
+
+    def gradient_waists(self, coeff, *, bands: int | None = None):
+        """The two contractions of :meth:`gradient` before their drains:
+        (c i m against the basis, c against dP/dphi) (transform.py:1770-1787)."""
+        c = self._spectral_tensor(coeff)
+        edges = self._edges(bands)
+        return (FourierWaist(self, self._to_waist(c, _native.WAIST_ZONAL), edges),
+                FourierWaist(self, self._to_waist(c, _native.WAIST_DERIVATIVE), edges))
+
+    def gradient_band(self, waists, r0: int, r1: int):
+        """One latitude band of :meth:`gradient` from its waists
+        (transform.py:1789-1796)."""
+        east_waist, north_waist = waists
+        east = self.waist_band_to_grid(east_waist, r0, r1)
+        east /= self.grid.radius_m * self._coslat[int(r0):int(r1)]
+        north = self.waist_band_to_grid(north_waist, r0, r1)
+        north /= self.grid.radius_m
+        return east, north
 
     # ---- small helpers (transform.py:1813-1835) -------------------------
     def zeros(self, *leading: int):

@@ -13,6 +13,7 @@ from dataclasses import dataclass
 
 from . import _native
 from .transform import SphericalHarmonicTransform, _vp
+from .waist import FourierWaist
 
 
 @dataclass
@@ -46,6 +47,51 @@ class VorticityDivergenceOperator:
         _native.check(t._lib.arwen_sht_vordiv_from_uv(t._plan, _vp(uu), _vp(vv), _vp(zeta),
                                                       _vp(div), t._count(uu), t.backend.stream()))
         return zeta, div
+
+    # ---- through the Fourier waist (reference vector.py:75-165) ---------
+    def vordiv_from_wind_waist(self, waist: FourierWaist):
+        """:meth:`vordiv_from_wind` from the stacked ``(2, *lead, nlat, T+1)``
+        waist of (u, v) that a band pipeline filled; the waist is read whole
+        and kept (vector.py:75-82)."""
+        return self._vordiv_from_fourier(waist)
+
+    def _wind_fourier(self, pair, *, bands: int | None = None) -> FourierWaist:
+        """The waist of a stacked ``(2, ..., nlat, nlon)`` wind pair
+        (vector.py:84-111)."""
+        return self.transform.fourier_waist(pair, bands=bands)
+
+    def _pair_waists(self, fourier):
+        t = self.transform
+        if isinstance(fourier, FourierWaist):
+            fourier.close()
+            raw = fourier.values
+        else:
+            raw = t.backend.asarray(fourier, dtype=t.backend.complex_dtype)
+        if raw.dim() < 3 or raw.shape[0] != 2:
+            raise ValueError("a vector waist stacks (u, v) on the leading axis, got shape "
+                             f"{tuple(raw.shape)}")
+        if raw.shape[-1] < t.truncation + 1:
+            raise ValueError(f"zonal spectrum has {raw.shape[-1]} orders, fewer than the "
+                             f"{t.truncation + 1} the truncation keeps")
+        fm = t._waist_tensor(raw[..., : t.truncation + 1])
+        return fm[0], fm[1]
+
+    def _vordiv_from_fourier(self, fourier):
+        """Vorticity and divergence from the (u, v) waist, or a raw zonal
+        spectrum at least T+1 orders wide, read whole in latitude; both
+        projected (vector.py:113-134)."""
+        t = self.transform
+        uw, vw = self._pair_waists(fourier)
+        zeta = t._empty_spectral(uw.shape[:-2])
+        div = t._empty_spectral(uw.shape[:-2])
+        _native.check(t._lib.arwen_sht_vordiv_from_waist(t._plan, _vp(uw), _vp(vw), _vp(zeta),
+                                                         _vp(div), t._count(uw),
+                                                         t.backend.stream()))
+        return zeta, div
+
+    def divergence_from_fourier(self, fourier):
+        """The divergence half of :meth:`_vordiv_from_fourier` (vector.py:136-165)."""
+        return self._vordiv_from_fourier(fourier)[1]
 
     def flux_curl_divergence(self, eastward_flux, northward_flux):
         return self.vordiv_from_wind(eastward_flux, northward_flux)

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <list>
 #include <memory>
@@ -121,6 +122,7 @@ struct arwen_sht_plan {
   DevBuf d_mu, d_w, d_koff, d_tw;
   DevBuf tab_syn[3], tab_ana[3];
   DevBuf scale_fold, scale_fold_vec, scale_one, scale_inv_a;
+  DevBuf scale_waist, scale_waist_vec;  // 2 pi w (x 1/a): folds of a reference waist (already / nlon)
   sht::FftPlan fft{};
   int rp_fold = 4, rp_fold_vec = 4, rp_unfold = 4;  // folded rows (wavefronts) per FFT block
   int fft_gpb = 2;                                   // row groups walked by one FFT block
@@ -256,16 +258,21 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
 
   // Per folded-row scales.
   std::vector<float> sf(P->JHP, 0.f), sfv(P->JHP, 0.f), one(P->JHP, 1.f), inva(P->JHP, 0.f);
+  std::vector<float> sw(P->JHP, 0.f), swv(P->JHP, 0.f);
   for (int i = 0; i < P->JH; ++i) {
     const int jn = P->nlat - 1 - i;
     double wq = P->w[jn];
     if (jn == i) wq *= 0.5;  // the equator row is its own mirror
     sf[i] = (float)(2.0 * M_PI * wq / P->nlon);
     sfv[i] = (float)(2.0 * M_PI * wq / (P->radius * P->nlon));
+    sw[i] = (float)(2.0 * M_PI * wq);
+    swv[i] = (float)(2.0 * M_PI * wq / P->radius);
     inva[i] = (float)(1.0 / P->radius);
   }
   upload(P->scale_fold, sf);
   upload(P->scale_fold_vec, sfv);
+  upload(P->scale_waist, sw);
+  upload(P->scale_waist_vec, swv);
   upload(P->scale_one, one);
   upload(P->scale_inv_a, inva);
   hip_check(hipDeviceSynchronize(), "plan build");
@@ -317,6 +324,8 @@ struct arwen_sht_batch {
   std::vector<sht::UnfoldArgs> unfolds;
   std::vector<sht::FoldArgs> folds;
   std::vector<sht::UnpackArgs> unpacks;
+  std::vector<sht::WaistUnfoldArgs> waist_unfolds;  // synthesis into a reference waist
+  std::vector<sht::WaistFoldArgs> waist_folds;      // analysis from a reference waist
   std::vector<HostGroup> groups[2];  // GEMM work groups per direction (0 SYN, 1 ANA)
   GemmLists lists[2];
   arwen_sht_gemm_config config{{32, 32}, {0, 0}, 1};
@@ -335,7 +344,16 @@ arwen_sht_plan::~arwen_sht_plan() {
 
 namespace {
 
-bool is_synthesis(int op) { return op >= ARWEN_SHT_OP_SYNTHESIS && op <= ARWEN_SHT_OP_UV_FROM_VORDIV; }
+bool is_synthesis(int op) {
+  return (op >= ARWEN_SHT_OP_SYNTHESIS && op <= ARWEN_SHT_OP_UV_FROM_VORDIV) ||
+         (op >= ARWEN_SHT_OP_SYNTHESIS_TO_WAIST && op <= ARWEN_SHT_OP_MERIDIONAL_TO_WAIST);
+}
+bool to_waist(int op) {
+  return op >= ARWEN_SHT_OP_SYNTHESIS_TO_WAIST && op <= ARWEN_SHT_OP_MERIDIONAL_TO_WAIST;
+}
+bool from_waist(int op) {
+  return op == ARWEN_SHT_OP_ANALYSIS_FROM_WAIST || op == ARWEN_SHT_OP_VORDIV_FROM_WAIST;
+}
 
 constexpr int kXcds = 8;  // MI355X: 8 XCDs x 32 CUs
 
@@ -564,9 +582,10 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
     if (P->device != B->device)
       fail(ARWEN_SHT_EINVAL, "job " + std::to_string(j) + "'s plan belongs to another device");
     if (J.count < 0) fail(ARWEN_SHT_EINVAL, "job count must be >= 0");
-    const bool vec = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV || J.op == ARWEN_SHT_OP_VORDIV_FROM_UV;
+    const bool vec = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV || J.op == ARWEN_SHT_OP_VORDIV_FROM_UV ||
+                     J.op == ARWEN_SHT_OP_VORDIV_FROM_WAIST;
     const bool two_out = vec || J.op == ARWEN_SHT_OP_GRADIENT;
-    if (J.op < ARWEN_SHT_OP_SYNTHESIS || J.op > ARWEN_SHT_OP_VORDIV_FROM_UV)
+    if (J.op < ARWEN_SHT_OP_SYNTHESIS || J.op > ARWEN_SHT_OP_VORDIV_FROM_WAIST)
       fail(ARWEN_SHT_EINVAL, "unknown op " + std::to_string(J.op));
     if (J.count > 0 && (!J.in0 || !J.out0 || (vec && !J.in1) || (two_out && !J.out1)))
       fail(ARWEN_SHT_EINVAL, "job " + std::to_string(j) + " is missing a buffer");
@@ -596,9 +615,10 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       pa.job.col0 = 0;
       pa.job.ldp = P->ldp;
       pa.job.radius = (float)P->radius;
-      pa.job.mode = J.op == ARWEN_SHT_OP_ZONAL_DERIVATIVE ? sht::kIM
-                    : vec                                 ? sht::kVector
-                                                          : sht::kPlain;
+      pa.job.mode = (J.op == ARWEN_SHT_OP_ZONAL_DERIVATIVE || J.op == ARWEN_SHT_OP_ZONAL_TO_WAIST)
+                        ? sht::kIM
+                    : vec ? sht::kVector
+                          : sht::kPlain;
       B->packs.push_back(pa);
       // ---- grouped GEMM: one group per (m-tile, output parity) ----
       const DataRef pk{js->packed_a.as<char>()};
@@ -609,7 +629,8 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
             // dP/dphi flips the parity, so its segment reads the other parity.
             SynSeg sg[2] = {{kD, 1, pk, 0}, {kPm, 0, pk, kMapSynVec}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 2);
-          } else if (J.op == ARWEN_SHT_OP_MERIDIONAL_DERIVATIVE) {
+          } else if (J.op == ARWEN_SHT_OP_MERIDIONAL_DERIVATIVE ||
+                     J.op == ARWEN_SHT_OP_MERIDIONAL_TO_WAIST) {
             SynSeg sg[1] = {{kD, 1, pk, 0}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
           } else if (grad) {
@@ -617,7 +638,7 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), se, 1);
             SynSeg sn[1] = {{kD, 1, pk, 0}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_b.as<float>(), sn, 1);
-          } else {  // SYNTHESIS, ZONAL_DERIVATIVE
+          } else {  // SYNTHESIS, ZONAL_DERIVATIVE and their waist forms
             SynSeg sg[1] = {{kP, 0, pk, 0}};
             add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
           }
@@ -653,7 +674,21 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
         B->unfolds.push_back(ua);
         B->fft_bytes += n * fftb;
       };
-      if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
+      if (to_waist(J.op)) {
+        // the reference's waist (*lead, nlat, T+1): north = E + O, south = E - O
+        sht::WaistUnfoldArgs wu{};
+        wu.waist = js->waist_a.as<float>();
+        wu.out = static_cast<float2*>(J.out0);
+        wu.T = T;
+        wu.nlat = P->nlat;
+        wu.JH = P->JH;
+        wu.JHP = P->JHP;
+        wu.ldw = P->ldw;
+        wu.nfields = n;
+        wu.col0 = 0;
+        wu.fstride = 2;
+        B->waist_unfolds.push_back(wu);
+      } else if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
         unfold(js->waist_a, J.out0, 0, 4, P->scale_inv_a, 0);
         unfold(js->waist_a, J.out1, 2, 4, P->scale_inv_a, 0);
       } else if (grad) {
@@ -668,6 +703,23 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       js->ncols = ncol * n;
       js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
       js->packed_out.alloc((size_t)js->ncols * P->ldp * 4);
+      if (from_waist(J.op)) {
+        // a reference waist (*lead, nlat, T+1), already divided by nlon
+        sht::WaistFoldArgs wf{};
+        wf.w0 = static_cast<const float2*>(J.in0);
+        wf.w1 = static_cast<const float2*>(J.in1);
+        wf.waist = js->waist_a.as<float>();
+        wf.rowscale = vec ? P->scale_waist_vec.as<float>() : P->scale_waist.as<float>();
+        wf.T = T;
+        wf.nlat = P->nlat;
+        wf.JH = P->JH;
+        wf.JHP = P->JHP;
+        wf.ldw = P->ldw;
+        wf.nfields = n;
+        wf.col0 = 0;
+        wf.vector = vec ? 1 : 0;
+        B->waist_folds.push_back(wf);
+      }
       sht::FoldArgs fa{};
       fa.fft = P->fft;
       fa.T = T;
@@ -686,8 +738,10 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       fa.job.col0 = 0;
       fa.job.ldw = P->ldw;
       fa.job.vector = vec ? 1 : 0;
-      B->folds.push_back(fa);
-      B->fft_bytes += (vec ? 2.0 : 1.0) * n * fftb;
+      if (!from_waist(J.op)) {
+        B->folds.push_back(fa);
+        B->fft_bytes += (vec ? 2.0 : 1.0) * n * fftb;
+      }
       const DataRef wst{js->waist_a.as<char>()};
       for (int mb = 0; mb < P->MB; ++mb)
         for (int q = 0; q < 2; ++q) {
@@ -761,8 +815,10 @@ void run_batch(arwen_sht_batch* B, hipStream_t s, hipEvent_t* ev = nullptr) {
   hip_check(launch_gemm(B, 0, s), "synthesis GEMM launch");
   mark(2);
   for (auto& a : B->unfolds) hip_check(sht_launch_irfft_unfold(&a, s), "irfft launch");
+  for (auto& a : B->waist_unfolds) hip_check(sht_launch_waist_unfold(&a, s), "waist unfold launch");
   mark(3);
   for (auto& a : B->folds) hip_check(sht_launch_rfft_fold(&a, s), "rfft launch");
+  for (auto& a : B->waist_folds) hip_check(sht_launch_waist_fold(&a, s), "waist fold launch");
   mark(4);
   hip_check(launch_gemm(B, 1, s), "analysis GEMM launch");
   mark(5);
@@ -899,6 +955,75 @@ int arwen_sht_plan_table_f64(const arwen_sht_plan* P, int kind, double* out) {
           out[((size_t)m * T1 + n) * P->JH + i] = h[base + (size_t)i * P->kp(m, p) + idx];
       }
   });
+}
+
+int arwen_sht_synthesis_to_waist(arwen_sht_plan* P, const void* coeff, void* waist, int count,
+                                 int kind, void* s) {
+  if (kind < 0 || kind > 2)
+    return guarded([&] {
+      fail(ARWEN_SHT_EINVAL, "waist kind must be 0 (basis), 1 (basis after i*m) or 2 (dP/dphi), got " +
+                                 std::to_string(kind));
+    });
+  const int op = kind == 0   ? ARWEN_SHT_OP_SYNTHESIS_TO_WAIST
+                 : kind == 1 ? ARWEN_SHT_OP_ZONAL_TO_WAIST
+                             : ARWEN_SHT_OP_MERIDIONAL_TO_WAIST;
+  return single_op(P, op, coeff, nullptr, waist, nullptr, count, s);
+}
+
+int arwen_sht_analysis_from_waist(arwen_sht_plan* P, const void* waist, void* coeff, int count,
+                                  void* s) {
+  return single_op(P, ARWEN_SHT_OP_ANALYSIS_FROM_WAIST, waist, nullptr, coeff, nullptr, count, s);
+}
+
+int arwen_sht_vordiv_from_waist(arwen_sht_plan* P, const void* uw, const void* vw, void* vort,
+                                void* div, int count, void* s) {
+  return single_op(P, ARWEN_SHT_OP_VORDIV_FROM_WAIST, uw, vw, vort, div, count, s);
+}
+
+namespace {
+int waist_rows(arwen_sht_plan* P, const float* rows_in, float* rows_out, const void* waist_in,
+               void* waist_out, int nlead, int nrows, int row0, void* stream, bool to_rows) {
+  return guarded([&] {
+    if (!P) fail(ARWEN_SHT_EINVAL, "plan is NULL");
+    if (nlead < 0 || nrows < 0) fail(ARWEN_SHT_EINVAL, "nlead and nrows must be >= 0");
+    if (row0 < 0 || row0 + nrows > P->nlat)
+      fail(ARWEN_SHT_EINVAL, "rows [" + std::to_string(row0) + ", " + std::to_string(row0 + nrows) +
+                                 ") are outside the waist's " + std::to_string(P->nlat) +
+                                 " latitudes");
+    if ((long long)nlead * nrows == 0) return;
+    if ((long long)nlead * nrows > (long long)INT32_MAX / 64)
+      fail(ARWEN_SHT_EINVAL, "nlead x nrows is too large for one call");
+    if ((to_rows && (!waist_in || !rows_out)) || (!to_rows && (!rows_in || !waist_out)))
+      fail(ARWEN_SHT_EINVAL, "a buffer is NULL");
+    int dev;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != P->device) fail(ARWEN_SHT_EINVAL, "plan used on a device other than its own");
+    sht::WaistRowsArgs a{};
+    a.fft = P->fft;
+    a.T = P->T;
+    a.nlat = P->nlat;
+    a.rows_in = rows_in;
+    a.rows_out = rows_out;
+    a.waist_in = static_cast<const float2*>(waist_in);
+    a.waist_out = static_cast<float2*>(waist_out);
+    a.nlead = nlead;
+    a.nrows = nrows;
+    a.row0 = row0;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hip_check(to_rows ? sht_launch_rows_from_waist(&a, st) : sht_launch_waist_from_rows(&a, st),
+              to_rows ? "rows_from_waist launch" : "waist_from_rows launch");
+  });
+}
+}  // namespace
+
+int arwen_sht_waist_from_rows(arwen_sht_plan* P, const float* rows, void* waist, int nlead,
+                              int nrows, int row0, void* s) {
+  return waist_rows(P, rows, nullptr, nullptr, waist, nlead, nrows, row0, s, false);
+}
+
+int arwen_sht_rows_from_waist(arwen_sht_plan* P, const void* waist, float* rows, int nlead,
+                              int nrows, int row0, void* s) {
+  return waist_rows(P, nullptr, rows, waist, nullptr, nlead, nrows, row0, s, true);
 }
 
 int arwen_sht_analysis(arwen_sht_plan* P, const float* grid, void* coeff, int count, void* s) {

@@ -899,6 +899,188 @@ size_t unfold_lds_bytes(int N, int T, int rp) {
   return (size_t)(1 + rp * 2) * N * sizeof(float2);
 }
 
+// ---- the reference's Fourier waist (*lead, nlat, T+1) complex ----
+// (FourierWaist, reference transform.py:417-629; fourier_waist :1294-1404,
+// _rows_to_grid :1654-1702.)  One wavefront per row, the row alone in the
+// FFT (imaginary part zero): a row's spectrum depends on that row only, so a
+// band fill or drain writes the bits the whole-field call writes.
+
+constexpr int kRowWaves = 4;  // wavefronts (rows) per block
+
+__device__ __forceinline__ size_t waist_row(const WaistRowsArgs& a, int r) {
+  return ((size_t)(r / a.nrows) * a.nlat + a.row0 + r % a.nrows) * (size_t)(a.T + 1);
+}
+
+// rows -> waist: X[m] / nlon for m <= T.
+template <int L>
+__global__ void __launch_bounds__(64 * kRowWaves) waist_from_rows_kernel(WaistRowsArgs a) {
+  extern __shared__ float2 lds[];
+  constexpr int N = 64 * L, SP = wave_span<L>();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  float2* tw = lds;
+  float2* buf = lds + N + (size_t)w * SP;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
+  __syncthreads();
+  const int r = blockIdx.x * kRowWaves + w;
+  if (r >= a.nlead * a.nrows) return;  // no block barrier below
+  const float* x = a.rows_in + (size_t)r * N;
+  float2 v[1][L];
+  static_for<L>([&](auto n1_) {
+    constexpr int n1 = decltype(n1_)::value;
+    v[0][n1] = make_float2(x[64 * n1 + lane], 0.f);
+  });
+  wave_fft64<L, 1, -1>(v, buf, tw, lane);
+  const float inv = 1.0f / (float)N;
+  float2* out = a.waist_out + waist_row(a, r);
+  for (int m = lane; m <= a.T; m += 64) {
+    const float2 z = buf[m];
+    out[m] = make_float2(z.x * inv, z.y * inv);
+  }
+}
+
+// waist -> rows: x[k] = Re X[0] + 2 Re sum_{m=1..T} X[m] e^{2 pi i k m / N}
+// (irfft of nlon X with the orders above T zero).
+template <int L>
+__global__ void __launch_bounds__(64 * kRowWaves) rows_from_waist_kernel(WaistRowsArgs a) {
+  extern __shared__ float2 lds[];
+  constexpr int N = 64 * L, SP = wave_span<L>();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  float2* tw = lds;
+  float2* buf = lds + N + (size_t)w * SP;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
+  __syncthreads();
+  const int r = blockIdx.x * kRowWaves + w;
+  if (r >= a.nlead * a.nrows) return;  // no block barrier below
+  const float2* X = a.waist_in + waist_row(a, r);
+  for (int k = lane; k < N; k += 64) buf[k] = make_float2(0.f, 0.f);
+  wave_sync();
+  for (int m = lane; m <= a.T; m += 64) {
+    const float2 A = X[m];
+    if (m == 0) {
+      buf[0] = make_float2(A.x, 0.f);  // irfft reads the real part of the m = 0 bin
+    } else {
+      buf[m] = A;
+      buf[N - m] = cconj(A);
+    }
+  }
+  wave_sync();
+  float2 v[1][L];
+  static_for<L>([&](auto n1_) { v[0][n1_] = buf[64 * decltype(n1_)::value + lane]; });
+  wave_sync();
+  wave_fft64<L, 1, +1>(v, buf, tw, lane);
+  float* out = a.rows_out + (size_t)r * N;
+  for (int k = lane; k < N; k += 64) out[k] = buf[k].x;
+}
+
+// The same two maps for an nlon that is not 64 L: direct sums against the
+// twiddle table, one thread per output value.
+__global__ void waist_from_rows_dft_kernel(WaistRowsArgs a) {
+  const int N = a.fft.N, T1 = a.T + 1;
+  const long long total = (long long)a.nlead * a.nrows * T1;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(e / T1), m = (int)(e % T1);
+    const float* x = a.rows_in + (size_t)r * N;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int j = 0; j < N; ++j) {
+      const float2 t = a.fft.tw[(int)(((long long)j * m) % N)];  // e^{-2 pi i j m / N}
+      acc.x += x[j] * t.x;
+      acc.y += x[j] * t.y;
+    }
+    a.waist_out[waist_row(a, r) + m] = make_float2(acc.x / (float)N, acc.y / (float)N);
+  }
+}
+
+__global__ void rows_from_waist_dft_kernel(WaistRowsArgs a) {
+  const int N = a.fft.N, T1 = a.T + 1;
+  const long long total = (long long)a.nlead * a.nrows * N;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(e / N), k = (int)(e % N);
+    const float2* X = a.waist_in + waist_row(a, r);
+    float acc = X[0].x;
+    for (int m = 1; m < T1; ++m) {
+      const float2 t = a.fft.tw[(int)(((long long)k * m) % N)];  // conj: e^{+2 pi i k m / N}
+      acc += 2.f * (X[m].x * t.x + X[m].y * t.y);
+    }
+    a.rows_out[(size_t)r * N + k] = acc;
+  }
+}
+
+// reference waist -> folded analysis waist: one thread per (field, folded row
+// i < JHP, order m < 4 ceil((T+1)/4)); padding rows and orders are zeros.
+__global__ void waist_fold_kernel(WaistFoldArgs a) {
+  const int T1 = a.T + 1, MO = (T1 + kMT - 1) / kMT * kMT;
+  const long long total = (long long)a.nfields * a.JHP * MO;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int m = (int)(e % MO);
+    const int i = (int)((e / MO) % a.JHP);
+    const int f = (int)(e / ((long long)MO * a.JHP));
+    const bool live = i < a.JH && m < T1;
+    const float s = live ? a.rowscale[i] : 0.f;
+    const size_t rn = ((size_t)f * a.nlat + (a.nlat - 1 - (live ? i : 0))) * T1 + (live ? m : 0);
+    const size_t rs = ((size_t)f * a.nlat + (live ? i : 0)) * T1 + (live ? m : 0);
+    float2 e0[2], o0[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      e0[k] = make_float2(0.f, 0.f);
+      o0[k] = make_float2(0.f, 0.f);
+      const float2* W = k == 0 ? a.w0 : a.w1;
+      if (live && (k == 0 || a.vector)) {
+        const float2 xn = W[rn], xs = W[rs];
+        e0[k] = make_float2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
+        o0[k] = make_float2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
+      }
+    }
+    if (!a.vector) {
+      const int c = a.col0 + 2 * f;
+      a.waist[waist_index(c, a.ldw, a.JHP, m, 0, i)] = e0[0].x;
+      a.waist[waist_index(c, a.ldw, a.JHP, m, 1, i)] = o0[0].x;
+      a.waist[waist_index(c + 1, a.ldw, a.JHP, m, 0, i)] = e0[0].y;
+      a.waist[waist_index(c + 1, a.ldw, a.JHP, m, 1, i)] = o0[0].y;
+    } else {
+      // level columns [iV, iU] = [-V.im, V.re, -U.im, U.re] (w0 = u, w1 = v)
+      const int c = a.col0 + 4 * f;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const float2 U = p == 0 ? e0[0] : o0[0];
+        const float2 V = p == 0 ? e0[1] : o0[1];
+        a.waist[waist_index(c + 0, a.ldw, a.JHP, m, p, i)] = -V.y;
+        a.waist[waist_index(c + 1, a.ldw, a.JHP, m, p, i)] = V.x;
+        a.waist[waist_index(c + 2, a.ldw, a.JHP, m, p, i)] = -U.y;
+        a.waist[waist_index(c + 3, a.ldw, a.JHP, m, p, i)] = U.x;
+      }
+    }
+  }
+}
+
+// folded synthesis waist -> reference waist; one thread per (field, folded
+// row i < JH, order m <= T).
+__global__ void waist_unfold_kernel(WaistUnfoldArgs a) {
+  const int T1 = a.T + 1;
+  const long long total = (long long)a.nfields * a.JH * T1;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int m = (int)(e % T1);
+    const int i = (int)((e / T1) % a.JH);
+    const int f = (int)(e / ((long long)T1 * a.JH));
+    const int c = a.col0 + a.fstride * f;
+    const float2 E = make_float2(a.waist[waist_index(c, a.ldw, a.JHP, m, 0, i)],
+                                 a.waist[waist_index(c + 1, a.ldw, a.JHP, m, 0, i)]);
+    const float2 O = make_float2(a.waist[waist_index(c, a.ldw, a.JHP, m, 1, i)],
+                                 a.waist[waist_index(c + 1, a.ldw, a.JHP, m, 1, i)]);
+    const int north = a.nlat - 1 - i;
+    a.out[((size_t)f * a.nlat + north) * T1 + m] = make_float2(E.x + O.x, E.y + O.y);
+    if (north != i) a.out[((size_t)f * a.nlat + i) * T1 + m] = make_float2(E.x - O.x, E.y - O.y);
+  }
+}
+
+size_t rows_wave_lds_bytes(int N) {
+  const int L = N / 64;
+  return ((size_t)N + (size_t)kRowWaves * L * kRowSpan) * sizeof(float2);
+}
+
 // Register prefetch depth per lane: PF * 64 >= N, or 0 (no prefetch) above 1536.
 int fft_pf(int N) {
   const int need = (N + kWave - 1) / kWave;
@@ -1002,7 +1184,9 @@ static hipError_t allow_lds_wave(size_t bytes) {
                         reinterpret_cast<const void*>(sht::rfft_fold_wave_kernel<L, 1, 8>),
                         reinterpret_cast<const void*>(sht::rfft_fold_wave_kernel<L, 2, 8>),
                         reinterpret_cast<const void*>(sht::irfft_unfold_wave_kernel<L, 4>),
-                        reinterpret_cast<const void*>(sht::irfft_unfold_wave_kernel<L, 8>)}) {
+                        reinterpret_cast<const void*>(sht::irfft_unfold_wave_kernel<L, 8>),
+                        reinterpret_cast<const void*>(sht::waist_from_rows_kernel<L>),
+                        reinterpret_cast<const void*>(sht::rows_from_waist_kernel<L>)}) {
     hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
   }
@@ -1038,3 +1222,56 @@ extern "C" size_t sht_fold_lds_bytes(int N, int T, int vector, int rp) {
   return sht::fold_lds_bytes(N, T, vector, rp);
 }
 extern "C" size_t sht_unfold_lds_bytes(int N, int T, int rp) { return sht::unfold_lds_bytes(N, T, rp); }
+
+static unsigned grid_for(long long total) {
+  long long b = (total + 255) / 256;
+  if (b > 65536) b = 65536;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+extern "C" hipError_t sht_launch_waist_from_rows(const sht::WaistRowsArgs* a, hipStream_t s) {
+  const long long R = (long long)a->nlead * a->nrows;
+  if (R <= 0) return hipSuccess;
+  if (const int L = a->fft.wave_L) {
+    const dim3 g((unsigned)((R + sht::kRowWaves - 1) / sht::kRowWaves)), b(64 * sht::kRowWaves);
+    const size_t lds = sht::rows_wave_lds_bytes(a->fft.N);
+#define SHT_W2R(LL) hipLaunchKernelGGL((sht::waist_from_rows_kernel<LL>), g, b, lds, s, *a)
+    SHT_L_DISPATCH(L, SHT_W2R)
+#undef SHT_W2R
+  } else {
+    hipLaunchKernelGGL(sht::waist_from_rows_dft_kernel, dim3(grid_for(R * (a->T + 1))), dim3(256),
+                       0, s, *a);
+  }
+  return hipGetLastError();
+}
+
+extern "C" hipError_t sht_launch_rows_from_waist(const sht::WaistRowsArgs* a, hipStream_t s) {
+  const long long R = (long long)a->nlead * a->nrows;
+  if (R <= 0) return hipSuccess;
+  if (const int L = a->fft.wave_L) {
+    const dim3 g((unsigned)((R + sht::kRowWaves - 1) / sht::kRowWaves)), b(64 * sht::kRowWaves);
+    const size_t lds = sht::rows_wave_lds_bytes(a->fft.N);
+#define SHT_R2W(LL) hipLaunchKernelGGL((sht::rows_from_waist_kernel<LL>), g, b, lds, s, *a)
+    SHT_L_DISPATCH(L, SHT_R2W)
+#undef SHT_R2W
+  } else {
+    hipLaunchKernelGGL(sht::rows_from_waist_dft_kernel, dim3(grid_for(R * a->fft.N)), dim3(256),
+                       0, s, *a);
+  }
+  return hipGetLastError();
+}
+
+extern "C" hipError_t sht_launch_waist_fold(const sht::WaistFoldArgs* a, hipStream_t s) {
+  const int MO = (a->T + 1 + sht::kMT - 1) / sht::kMT * sht::kMT;
+  const long long total = (long long)a->nfields * a->JHP * MO;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sht::waist_fold_kernel, dim3(grid_for(total)), dim3(256), 0, s, *a);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t sht_launch_waist_unfold(const sht::WaistUnfoldArgs* a, hipStream_t s) {
+  const long long total = (long long)a->nfields * a->JH * (a->T + 1);
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sht::waist_unfold_kernel, dim3(grid_for(total)), dim3(256), 0, s, *a);
+  return hipGetLastError();
+}

@@ -85,6 +85,36 @@ struct UnfoldArgs {
   UnfoldJob job;
 };
 
+// ---- the reference's Fourier waist (*lead, nlat, T+1) complex (sht_fft.hip) ----
+// Rows <-> waist rows: the real FFT along longitude of `nrows` rows per lead
+// index, divided by nlon and cut to m <= T (rows -> waist), or its inverse
+// (waist -> rows: irfft of nlon x the retained orders).
+struct WaistRowsArgs {
+  FftPlan fft;
+  int T, nlat;
+  const float* rows_in;       // (nlead, nrows, nlon) rows -> waist
+  float* rows_out;            // (nlead, nrows, nlon) waist -> rows
+  const float2* waist_in;     // (nlead, nlat, T+1)
+  float2* waist_out;
+  int nlead, nrows, row0;     // waist rows row0 .. row0 + nrows of each lead index
+};
+// Reference waist -> the analysis GEMM's folded waist W[c][m/4][p][i][m%4]
+// (parity sums of the north/south row pair times the quadrature weight).
+struct WaistFoldArgs {
+  const float2* w0;           // (nfields, nlat, T+1): scalar fields, or u
+  const float2* w1;           // v (vector jobs)
+  float* waist;
+  const float* rowscale;      // per folded row: 2 pi w (x 1/a for vector jobs)
+  int T, nlat, JH, JHP, ldw, nfields, col0, vector;
+};
+// The synthesis GEMM's folded waist -> reference waist (north = E + O,
+// south = E - O).
+struct WaistUnfoldArgs {
+  const float* waist;
+  float2* out;                // (nfields, nlat, T+1)
+  int T, nlat, JH, JHP, ldw, nfields, col0, fstride;
+};
+
 // ---- grouped GEMM (sht_gemm.hip) ----
 // A panel is one row tile of one work group: up to kMT consecutive orders m
 // of one job and one output parity.  Its k-steps (one order, 32 k each) are
@@ -178,6 +208,10 @@ hipError_t sht_fft_allow_lds(size_t bytes, size_t wave_bytes);
 int sht_fft_wave_L(int N);                       // 0: no wave-resident path for N
 size_t sht_fft_wave_lds_bytes(int N, int vector);
 hipError_t sht_launch_legendre_gemm(const sht::GemmLaunch* launch, hipStream_t s);
+hipError_t sht_launch_waist_from_rows(const sht::WaistRowsArgs* a, hipStream_t s);
+hipError_t sht_launch_rows_from_waist(const sht::WaistRowsArgs* a, hipStream_t s);
+hipError_t sht_launch_waist_fold(const sht::WaistFoldArgs* a, hipStream_t s);
+hipError_t sht_launch_waist_unfold(const sht::WaistUnfoldArgs* a, hipStream_t s);
 hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s);
 hipError_t sht_launch_unpack(const sht::UnpackArgs* a, hipStream_t s);
 hipError_t sht_launch_spectral_elementwise(const float2* in, float2* out, long long total, int T,

@@ -47,6 +47,14 @@ extern "C" {
 #define ARWEN_SHT_OP_UV_FROM_VORDIV 5      /* in0 vort, in1 div -> out0 u, out1 v    */
 #define ARWEN_SHT_OP_ANALYSIS 6            /* in0 grid -> out0 coeff                 */
 #define ARWEN_SHT_OP_VORDIV_FROM_UV 7      /* in0 u, in1 v -> out0 vort, out1 div    */
+/* The reference's Fourier waist: (count, nlat, T+1) complex64, the zonal
+ * spectrum of each latitude row divided by nlon and cut to m <= T
+ * (FourierWaist, reference transform.py:417-629). */
+#define ARWEN_SHT_OP_SYNTHESIS_TO_WAIST 8  /* in0 coeff -> out0 waist (basis)        */
+#define ARWEN_SHT_OP_ZONAL_TO_WAIST 9      /* in0 coeff -> out0 waist of c*i*m       */
+#define ARWEN_SHT_OP_MERIDIONAL_TO_WAIST 10 /* in0 coeff -> out0 waist (dP/dphi)     */
+#define ARWEN_SHT_OP_ANALYSIS_FROM_WAIST 11 /* in0 waist -> out0 coeff               */
+#define ARWEN_SHT_OP_VORDIV_FROM_WAIST 12  /* in0 u waist, in1 v waist -> vort, div  */
 
 typedef struct arwen_sht_plan arwen_sht_plan;
 typedef struct arwen_sht_batch arwen_sht_batch;
@@ -116,6 +124,34 @@ int arwen_sht_uv_from_vordiv(arwen_sht_plan* plan, const void* vort, const void*
 /* VorticityDivergenceOperator.vordiv_from_wind (reference vector.py:54-73, 113-134) */
 int arwen_sht_vordiv_from_uv(arwen_sht_plan* plan, const float* u, const float* v, void* vort,
                              void* div, int count, void* stream);
+
+/* The Fourier waist (count, nlat, T+1) complex64 of the reference's banded
+ * pipeline.
+ * contract_to_waist (reference transform.py:1629-1652): the Legendre
+ * contraction of `coeff` with the basis (kind 0), with the basis after a
+ * multiplication by i*m (kind 1; gradient_waists' east half, :1770-1787) or
+ * with dP/dphi (kind 2; its north half). */
+int arwen_sht_synthesis_to_waist(arwen_sht_plan* plan, const void* coeff, void* waist, int count,
+                                 int kind, void* stream);
+/* contract_waist / contract_values (transform.py:1435-1472): analysis of a
+ * filled waist, m = 0 realified. */
+int arwen_sht_analysis_from_waist(arwen_sht_plan* plan, const void* waist, void* coeff, int count,
+                                  void* stream);
+/* VorticityDivergenceOperator.vordiv_from_wind_waist / _vordiv_from_fourier
+ * (reference vector.py:75-134): vorticity and divergence from the waists of
+ * u and v, count levels each. */
+int arwen_sht_vordiv_from_waist(arwen_sht_plan* plan, const void* u_waist, const void* v_waist,
+                                void* vort, void* div, int count, void* stream);
+/* FourierWaist.fill_band / fourier_waist (transform.py:483-525, 1294-1404):
+ * `rows` (nlead, nrows, nlon) float32 -> waist rows row0 .. row0+nrows of
+ * each lead index of `waist` (nlead, nlat, T+1): rfft / nlon, m <= T. */
+int arwen_sht_waist_from_rows(arwen_sht_plan* plan, const float* rows, void* waist, int nlead,
+                              int nrows, int row0, void* stream);
+/* _rows_to_grid / waist_band_to_grid (transform.py:1654-1711): waist rows
+ * row0 .. row0+nrows of each lead index -> `rows` (nlead, nrows, nlon)
+ * float32: irfft of nlon x the retained orders. */
+int arwen_sht_rows_from_waist(arwen_sht_plan* plan, const void* waist, float* rows, int nlead,
+                              int nrows, int row0, void* stream);
 
 /* Elementwise spectral operators on (count, T+1, T+1) complex64:
  * op 0 = project (transform.py:1124-1144), 1 = laplacian (:1798-1802),

@@ -300,11 +300,16 @@ class OracleTransform:
         return np.fft.irfft(spectrum, n=self.grid.nlon, axis=-1).astype(np.float64)
 
     # ---- public operations ---------------------------------------------
-    def forward(self, field):
-        """Analysis (transform.py:1435-1490): waist, contract, realify m=0."""
-        out = self._contract(self.fourier_waist(field), self._analysis, "jn")
+    def contract_waist(self, values):
+        """The tail of an analysis (transform.py:1435-1472): contract a
+        (*lead, nlat, T+1) waist with the analysis table, realify m = 0."""
+        out = self._contract(np.asarray(values, dtype=np.complex128), self._analysis, "jn")
         out[..., :, 0] = out[..., :, 0].real
         return out
+
+    def forward(self, field):
+        """Analysis (transform.py:1474-1490): waist, contract, realify m=0."""
+        return self.contract_waist(self.fourier_waist(field))
 
     def contract_to_waist(self, coeff, table="basis"):
         c = np.asarray(coeff, dtype=np.complex128)
@@ -329,6 +334,21 @@ class OracleTransform:
         east = self.inverse_zonal_derivative(coeff) / (
             self.grid.radius_m * self.grid.cos_lat[:, None])
         north = self.inverse_meridional_derivative(coeff) / self.grid.radius_m
+        return east, north
+
+    def gradient_waists(self, coeff):
+        """(contract_to_waist(c i m), contract_to_waist(c, dP/dphi))
+        (transform.py:1770-1787)."""
+        c = np.asarray(coeff, dtype=np.complex128)
+        self._check_spectral(c)
+        return self.contract_to_waist(c * self._zonal), self.contract_to_waist(c, "derivative")
+
+    def gradient_band(self, waists, r0, r1):
+        """Rows r0..r1 of gradient() drained from its waists (transform.py:1789-1796)."""
+        ew, nw = waists
+        east = self.waist_to_grid(ew[..., r0:r1, :]) / (
+            self.grid.radius_m * self.grid.cos_lat[r0:r1, None])
+        north = self.waist_to_grid(nw[..., r0:r1, :]) / self.grid.radius_m
         return east, north
 
     def project(self, coeff):
@@ -404,7 +424,12 @@ class OracleTransform:
         self._check_grid(u)
         if u.shape != v.shape:
             raise ValueError(f"u/v shape mismatch: {u.shape} != {v.shape}")
-        fm = self.fourier_waist(np.stack([u, v]))
+        return self.vordiv_from_waist(self.fourier_waist(np.stack([u, v])))
+
+    def vordiv_from_waist(self, fourier):
+        """vector.py:113-134: the contractions of a stacked (2, ..., nlat, >= T+1)
+        (u, v) waist or raw zonal spectrum."""
+        fm = np.asarray(fourier, dtype=np.complex128)[..., : self.truncation + 1]
         rot = self._contract(fm * self._rotation_weight, self._basis, "nj", transposed=True)
         sh = self._contract(fm * self._shear_weight, self.dbasis, "nj", transposed=True)
         div = rot[0] - sh[1]
