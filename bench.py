@@ -223,20 +223,53 @@ def gemm_traffic(config, precision, workload):
     return total
 
 
+TABLE_BYTES = {"float32": 4, "bf16x3": 4, "bf16x6": 6}  # bytes per table element the GEMM reads
+
+
+def gemm_algorithmic_bytes(rt, precision):
+    """Unique HBM bytes of the round trip's two GEMM launches: each Legendre
+    table the launch uses read once (P, dP/dphi, P m/cos: K_tri x nlat/2
+    folded elements, no padding), and every field's coefficients (K_tri
+    complex64) and waist (nlat x (T+1) complex64) read or written once per
+    direction."""
+    total = 0.0
+    for part in rt.parts:
+        T, nlat = part["spec"]["T"], part["tr"].grid.nlat
+        members = part["zeta"].shape[0]
+        k_tri = (T + 1) * (T + 2) // 2
+        tables = 3 * k_tri * (nlat / 2) * TABLE_BYTES[precision]
+        fields = members * (3 * LEVELS + 1)
+        data = fields * (8.0 * k_tri + 8.0 * nlat * (T + 1))
+        total += 2 * (tables + data)
+    return total
+
+
 def roofline(rt, phases, precision, workload="t383"):
-    """Dominant kernel of the round trip: Legendre GEMM (MFMA-bound) or FFT (HBM-bound)."""
+    """Roofline of the dominant kernel.  For the Legendre GEMM the bound is
+    chosen by arithmetic intensity against the ridge point: MFMA products per
+    algorithmic byte above peak_mfma / peak_hbm is MFMA-bound, below it
+    HBM-bound; the other roof is reported beside it."""
     gemm_ms = phases["synthesis_gemm"] + phases["analysis_gemm"]
     fft_ms = phases["inverse_fft"] + phases["forward_fft"]
     s = PRODUCTS[precision]
     if gemm_ms >= fft_ms:
-        achieved = s * rt.flops / (gemm_ms * 1e-3) / 1e12
-        return {"bound": "mfma", "kernel": "legendre_gemm (synthesis + analysis launches)",
-                "achieved": achieved, "peak": MFMA_PEAK[precision], "unit": "TFLOP/s",
-                "frac": achieved / MFMA_PEAK[precision],
+        t = gemm_ms * 1e-3
+        alg_bytes = gemm_algorithmic_bytes(rt, precision)
+        intensity = s * rt.flops / alg_bytes
+        ridge = MFMA_PEAK[precision] * 1e12 / (HBM_PEAK_GBS * 1e9)
+        mfma = {"bound": "mfma", "achieved": s * rt.flops / t / 1e12,
+                "peak": MFMA_PEAK[precision], "unit": "TFLOP/s"}
+        hbm = {"bound": "hbm", "achieved": alg_bytes / t / 1e9, "peak": HBM_PEAK_GBS,
+               "unit": "GB/s"}
+        for r in (mfma, hbm):
+            r["frac"] = r["achieved"] / r["peak"]
+        main, other = (mfma, hbm) if intensity >= ridge else (hbm, mfma)
+        return {**main, "kernel": "legendre_gemm (synthesis + analysis launches)",
                 "traffic": gemm_traffic(rt.config, precision, workload),
                 "traffic_unit": "HBM bytes per round trip (both launches)",
-                "fp32_equivalent_tflops": rt.flops / (gemm_ms * 1e-3) / 1e12,
-                "mfma_products_per_fma": s}
+                "algorithmic_bytes": alg_bytes, "intensity_flop_per_byte": intensity,
+                "ridge_flop_per_byte": ridge, "other_roof": other,
+                "fp32_equivalent_tflops": rt.flops / t / 1e12, "mfma_products_per_fma": s}
     achieved = rt.fft_bytes / (fft_ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": "rfft_fold + irfft_unfold", "achieved": achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}

@@ -18,19 +18,27 @@
 // negate some of them): the vector operators' second arrangement of the data
 // is such a map of the first, so only one copy is stored.
 //
-// Pipeline: four register stages and two LDS buffers.  While the wavefronts
-// run step j's MFMAs out of LDS, the loads of steps j+1, j+2 and j+3 are in
-// flight (a step waits on loads issued two steps before it, not one: the
-// loop is bound by memory latency otherwise); step j+1 is then written to the
-// other LDS buffer and one barrier ends the step.  The panel's step
-// descriptors sit in LDS for the whole tile, so no memory load other than the
-// tiles is waited on inside the loop.  Reads past the panel's list return its
-// last null step (extent 0), so every load is issued unconditionally.
+// Block tile 64 x BN, four wavefronts of 16 rows x BN (16x16 MFMA blocks).
+// Each wavefront loads its own table fragments (16 rows x 32 k per piece)
+// from the tiled table straight into registers: no table tile passes through
+// LDS.  Only the data tile B, which all four wavefronts read, is staged in
+// LDS (SYN: two buffers, one per step; ANA: one k-chunk, resident for its
+// trip of four steps).
 //
-// LDS tiles are unpadded rows of 32 k (128 bytes fp32, 64 bytes per bf16
-// piece) whose 16-byte chunks are XOR-swizzled by row, so that the MFMA
-// fragment reads (16 rows x one chunk) and the tile writes (whole rows) are
-// free of bank conflicts (measured).
+// Pipeline: four register stages (step j's tiles sit in stage j % 4, its
+// order within the trip).  While the wavefronts run step j's MFMAs, the
+// loads of steps j+1, j+2 and j+3 are in flight (the loop is bound by memory
+// latency otherwise).  SYN writes step j+1's B to the other LDS buffer and
+// one barrier ends the step; ANA needs barriers only where its resident B
+// tile changes.  The panel's step descriptors sit in LDS for the whole tile,
+// so no memory load other than the tiles is waited on inside the loop.
+// Reads past the panel's list return its last null step (extent 0), so every
+// load is issued unconditionally.
+//
+// LDS B tiles are unpadded rows of 32 k, one row per column (128 bytes fp32,
+// 64 bytes per bf16 piece), whose 16-byte chunks are XOR-swizzled by row, so
+// that the MFMA fragment reads (16 columns x one chunk) and the tile writes
+// are free of bank conflicts (measured).
 //
 // SYN (synthesis): rows = folded latitudes, k = degrees, C = waist
 //   W[c][m/4][p][i][m%4].  Each lane keeps four order accumulators; the tile
@@ -40,7 +48,6 @@
 //   BN columns x 4 orders) is loaded once per k-chunk and serves the steps
 //   of all four orders; steps flagged with a B extent start a new k-chunk.
 //
-// Block tile 64 x BN, four wavefronts of 32 x BN/2 (16x16 MFMA blocks).
 // Precision modes:
 //   FP32   v_mfma_f32_16x16x4_f32, fp32 tables and data.
 //   BF16X3 v_mfma_f32_16x16x32_bf16 on hi+lo pieces: a0b0 + a0b1 + a1b0.
