@@ -12,7 +12,7 @@ run as one TransformBatch (one grouped Legendre GEMM launch per direction)
 replayed from a hipGraph, inputs resident in HBM.  ``value`` is the split-BF16
 (bf16x6) round trips/s; the same workload in FP32 MFMA is timed beside it.
 
-    python bench.py [--steps K] [--warmup W] [--workload t383|t383x32|mixed]
+    python bench.py [--steps K] [--warmup W] [--workload t383|t383x32|mixed|sample]
                     [--precision bf16x6|bf16x3|float32] [--no-cpu-baseline]
 
 Prints one JSON line.  Only ``cpu_baseline()`` touches the FP64 oracle: it
@@ -275,6 +275,93 @@ def roofline(rt, phases, precision, workload="t383"):
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
 
 
+SAMPLE_T, SAMPLE_FIELDS, SAMPLE_POINTS = 383, 40, 30000
+FP64_PEAK_TFLOPS = 78.6
+
+
+def sampler_bench(args, torch):
+    """Point sampler (sample_scalar): T383, 40 fields (one level stack) at
+    30,000 scattered points, inputs resident on the device; timed with HIP
+    events on the launch stream, float64 (`value`) and float32 beside it.
+    Algorithmic work per call: per point and packed (n, m), 4 FLOP per field
+    (a complex coefficient times a real basis value) plus 4 FLOP of FP64
+    recurrence; the bound is the FP64 vector peak."""
+    import ctypes
+
+    from arwen_spectral import SphericalHarmonicTransform, _native
+
+    T, F, N = SAMPLE_T, SAMPLE_FIELDS, SAMPLE_POINTS
+    t = SphericalHarmonicTransform.create(T, dealias_factor=1.0, backend="hip")
+    dev = torch.device("cuda", torch.cuda.current_device())
+    coeff = red_spectrum(T, (F,), 31, CLASS_SCALE["temperature"])
+    c = torch.from_numpy(coeff.astype(np.complex64)).to(dev)
+    rng = np.random.Generator(np.random.PCG64(32))
+    lat = np.deg2rad(rng.uniform(-90.0, 90.0, N))
+    lon = np.deg2rad(rng.uniform(0.0, 360.0, N))
+    x = torch.from_numpy(np.sin(lat)).to(dev)
+    cl = torch.from_numpy(np.cos(lat)).to(dev)
+    lo = torch.from_numpy(lon).to(dev)
+    lib = _native.load()
+    nbytes = int(lib.arwen_sht_sample_workspace_bytes(t.plan_handle, F, 0))
+    work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    out = torch.empty((F, N), dtype=torch.float64, device=dev)
+    vp = lambda a: ctypes.c_void_p(a.data_ptr())  # noqa: E731
+
+    def call(prec):
+        _native.check(lib.arwen_sht_sample(t.plan_handle, vp(c), None, F, vp(x), vp(cl), vp(lo), N,
+                                           0, prec, vp(work), nbytes, vp(out), None,
+                                           t.backend.stream()))
+
+    k_tri = (T + 1) * (T + 2) // 2
+    flops = float(N) * k_tri * (4.0 * F + 4.0)
+    res = {}
+    for prec, name in ((0, "float64"), (1, "float32")):
+        for _ in range(args.warmup):
+            call(prec)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.steps):
+            call(prec)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        res[name] = {"ms_per_call": ms, "value": F * N / (ms * 1e-3),
+                     "sample_max": float(out.abs().max().item())}
+        print(f"[bench] sampler {name}: {ms:.3f} ms per call", file=sys.stderr, flush=True)
+    ms = res["float64"]["ms_per_call"]
+    achieved = flops / (ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT))
+        from oracle import sampling as osamp
+        from oracle.sht import OracleTransform
+
+        o = OracleTransform.create(T, dealias_factor=1.0)
+        sub = 200
+        lat_d, lon_d = np.rad2deg(lat[:sub]), np.rad2deg(lon[:sub])
+        t0 = time.perf_counter()
+        osamp.sample_scalar(o, coeff.astype(np.complex64).astype(np.complex128), lat_d, lon_d)
+        sec = time.perf_counter() - t0
+        cpu = {"value": F * sub / sec, "unit": "point samples/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 numpy oracle of the reference's host sampler "
+                         f"(sampling.py:382-426), T{T}, {F} fields at {sub} of the {N} points"}
+    print(json.dumps({
+        "metric": f"T{T} point samples/s ({F} fields at {N} scattered points, sample_scalar)",
+        "value": res["float64"]["value"], "unit": "point samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded red-spectrum coefficients, uniform random points",
+        "config": {"workload": f"sample_scalar T{T}, {F} fields, {N} points", "T": T,
+                   "fields": F, "points": N},
+        "float32": res["float32"],
+        "roofline": {"bound": "valu_fp64", "kernel": "sample_kernel<8, 0, false> + sample_pack",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "algorithmic_flop_per_call": flops},
+        "cpu_baseline": cpu}))
+
+
 def workload_specs(name):
     if name == "t383":
         return [dict(T=383, members=1)]
@@ -316,6 +403,9 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n * 4, "copies": 3}))
+        return
+    if args.workload == "sample":
+        sampler_bench(args, torch)
         return
     specs = workload_specs(args.workload)
     for sp in specs:

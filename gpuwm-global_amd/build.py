@@ -3,9 +3,9 @@
     python gpuwm-global_amd/build.py [--jobs N]
 
 Compiles every source in ``csrc/`` with hipcc and links
-``arwen_spectral/libarwen_sht.so``.  The Legendre table kernel is built with
-``-ffp-contract=off`` so its FP64 recurrence keeps the reference's operand
-order without fused multiply-adds.  All device code is built with packed
+``arwen_spectral/libarwen_sht.so``.  The Legendre table and point sampler
+kernels are built with ``-ffp-contract=off`` so their FP64 recurrences keep
+the reference's operand order without fused multiply-adds.  All device code is built with packed
 FP32 instructions disabled.
 """
 from __future__ import annotations
@@ -30,8 +30,9 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{CSRC}", f"-I{INCLUDE}",
 ]
-EXTRA = {"sht_tables.hip": ["-ffp-contract=off"]}
-SOURCES = ["sht_tables.hip", "sht_fft.hip", "sht_gemm.hip", "sht_pack.hip", "sht_capi.cpp"]
+EXTRA = {"sht_tables.hip": ["-ffp-contract=off"], "sht_sample.hip": ["-ffp-contract=off"]}
+SOURCES = ["sht_tables.hip", "sht_fft.hip", "sht_gemm.hip", "sht_pack.hip", "sht_sample.hip",
+           "sht_capi.cpp"]
 
 
 def hipcc() -> str:

@@ -123,6 +123,7 @@ struct arwen_sht_plan {
   DevBuf tab_syn[3], tab_ana[3];
   DevBuf scale_fold, scale_fold_vec, scale_one, scale_inv_a;
   DevBuf scale_waist, scale_waist_vec;  // 2 pi w (x 1/a): folds of a reference waist (already / nlon)
+  DevBuf sample_rec, sample_diag;       // point sampler recurrence tables, built on first use
   sht::FftPlan fft{};
   int rp_fold = 4, rp_fold_vec = 4, rp_unfold = 4;  // folded rows (wavefronts) per FFT block
   int fft_gpb = 2;                                   // row groups walked by one FFT block
@@ -1067,6 +1068,102 @@ int arwen_sht_spectral_op(arwen_sht_plan* P, int op, const void* in, void* out, 
                                               static_cast<float2*>(out), total, P->T, op,
                                               (float)P->radius, static_cast<hipStream_t>(s)),
               "spectral op launch");
+  });
+}
+
+namespace {
+constexpr int kSampleColPad = 8;  // sampler field columns are padded to the widest field tile
+
+int sample_columns(int count, int kind) {
+  const long long cols = (kind == 2 ? 2LL : 1LL) * count;
+  return (int)((cols + kSampleColPad - 1) / kSampleColPad * kSampleColPad);
+}
+
+// The point sampler's recurrence coefficients per packed (n, m) and per order,
+// with the reference's numpy expressions (sampling.py:83-112) in double.
+void build_sample_tables(arwen_sht_plan* P) {
+  if (P->sample_rec.p) return;
+  const int T = P->T;
+  const long long K = (long long)(T + 1) * (T + 2) / 2;
+  std::vector<double> rec(3 * K, 0.0), diag(2 * (T + 1), 0.0);
+  long long k = 0;
+  for (int m = 0; m <= T; ++m)
+    for (int n = m; n <= T; ++n, ++k) {
+      const double nn = n, mm2 = (double)((long long)m * m), nm2 = (double)((long long)n * n - (long long)m * m);
+      if (n >= m + 2) {
+        rec[k] = std::sqrt((4.0 * nn * nn - 1.0) / nm2);
+        rec[K + k] = std::sqrt(((nn - 1.0) * (nn - 1.0) - mm2) / (4.0 * (nn - 1.0) * (nn - 1.0) - 1.0));
+      }
+      if (n > m) rec[2 * K + k] = std::sqrt(nm2 * (2.0 * nn + 1.0) / (2.0 * nn - 1.0));
+    }
+  for (int m = 1; m <= T; ++m) diag[m] = -std::sqrt((2 * m + 1) / (2.0 * m));
+  for (int m = 0; m <= T; ++m) diag[T + 1 + m] = std::sqrt((double)(2 * m + 3));
+  upload(P->sample_rec, rec);
+  upload(P->sample_diag, diag);
+}
+}  // namespace
+
+size_t arwen_sht_sample_workspace_bytes(const arwen_sht_plan* P, int count, int kind) {
+  if (!P || count < 0 || kind < 0 || kind > 2) return 0;
+  const long long K = (long long)(P->T + 1) * (P->T + 2) / 2;
+  return (size_t)K * sample_columns(count, kind) * sizeof(double2);  // either precision fits
+}
+
+int arwen_sht_sample(arwen_sht_plan* P, const void* in0, const void* in1, int count,
+                     const double* sin_lat, const double* cos_lat, const double* lon_rad,
+                     int npoints, int kind, int precision, void* workspace,
+                     size_t workspace_bytes, double* out0, double* out1, void* s) {
+  return guarded([&] {
+    if (!P) fail(ARWEN_SHT_EINVAL, "plan is NULL");
+    if (kind < 0 || kind > 2)
+      fail(ARWEN_SHT_EINVAL, "sample kind must be 0 (values), 1 (gradient) or 2 (wind), got " +
+                                 std::to_string(kind));
+    if (precision != 0 && precision != 1)
+      fail(ARWEN_SHT_EINVAL, "sample precision must be 0 (float64) or 1 (float32), got " +
+                                 std::to_string(precision));
+    if (count < 0 || npoints < 0) fail(ARWEN_SHT_EINVAL, "count and npoints must be >= 0");
+    if ((long long)count * npoints == 0) return;
+    if (!in0 || (kind == 2 && !in1)) fail(ARWEN_SHT_EINVAL, "an input field is NULL");
+    if (!sin_lat || !lon_rad || (kind != 0 && !cos_lat))
+      fail(ARWEN_SHT_EINVAL, "a point coordinate array is NULL");
+    if (!out0 || (kind != 0 && !out1)) fail(ARWEN_SHT_EINVAL, "an output is NULL");
+    const size_t need = arwen_sht_sample_workspace_bytes(P, count, kind);
+    if (!workspace || workspace_bytes < need)
+      fail(ARWEN_SHT_EINVAL, "sample workspace must hold " + std::to_string(need) + " bytes");
+    int dev;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != P->device) fail(ARWEN_SHT_EINVAL, "plan used on a device other than its own");
+    build_sample_tables(P);
+    hipStream_t st = static_cast<hipStream_t>(s);
+    const int ldc = sample_columns(count, kind);
+    sht::SamplePackArgs pa{};
+    pa.in0 = static_cast<const float2*>(in0);
+    pa.in1 = static_cast<const float2*>(in1);
+    pa.out = workspace;
+    pa.f64 = precision == 0 ? 1 : 0;
+    pa.radius = (double)(float)P->radius;  // the spectral operator kernel's lambda_n
+    pa.T = P->T;
+    pa.count = count;
+    pa.ldc = ldc;
+    pa.wind = kind == 2 ? 1 : 0;
+    hip_check(sht_launch_sample_pack(&pa, st), "sample pack launch");
+    sht::SampleArgs a{};
+    a.coef = workspace;
+    a.rec = P->sample_rec.as<double>();
+    a.diag = P->sample_diag.as<double>();
+    a.sin_lat = sin_lat;
+    a.cos_lat = cos_lat ? cos_lat : sin_lat;  // unread for values
+    a.lon = lon_rad;
+    a.out0 = out0;
+    a.out1 = out1 ? out1 : out0;  // unwritten for values
+    a.radius = P->radius;
+    a.T = P->T;
+    a.npts = npoints;
+    a.ldc = ldc;
+    a.nout = count;
+    a.kind = kind;
+    a.f32 = precision;
+    hip_check(sht_launch_sample(&a, st), "sample launch");
   });
 }
 

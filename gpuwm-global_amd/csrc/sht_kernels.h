@@ -115,6 +115,27 @@ struct WaistUnfoldArgs {
   int T, nlat, JH, JHP, ldw, nfields, col0, fstride;
 };
 
+// ---- point sampler (sht_sample.hip) ----
+struct SamplePackArgs {
+  const float2* in0;  // (count, T+1, T+1): the fields, or vorticity for winds
+  const float2* in1;  // divergence for winds
+  void* out;          // (K, ldc) complex128 (f64) or complex64, K = (T+1)(T+2)/2 in triangle order
+  double radius;
+  int T, count, ldc, wind, f64;
+};
+struct SampleArgs {
+  const void* coef;       // (K, ldc) from the pack
+  const double* rec;      // [3][K]: recurrence a, b and derivative c per packed (n, m)
+  const double* diag;     // [2][T+1]: -sqrt((2m+1)/2m), sqrt(2m+3)
+  const double* sin_lat;  // (npts)
+  const double* cos_lat;  // (npts), for the east metric
+  const double* lon;      // (npts) radians
+  double* out0;           // (nout, npts)
+  double* out1;
+  double radius;
+  int T, npts, ldc, nout, kind, f32;
+};
+
 // ---- grouped GEMM (sht_gemm.hip) ----
 // A panel is one row tile of one work group: up to kMT consecutive orders m
 // of one job and one output parity.  Its k-steps (one order, 32 k each) are
@@ -212,6 +233,8 @@ hipError_t sht_launch_waist_from_rows(const sht::WaistRowsArgs* a, hipStream_t s
 hipError_t sht_launch_rows_from_waist(const sht::WaistRowsArgs* a, hipStream_t s);
 hipError_t sht_launch_waist_fold(const sht::WaistFoldArgs* a, hipStream_t s);
 hipError_t sht_launch_waist_unfold(const sht::WaistUnfoldArgs* a, hipStream_t s);
+hipError_t sht_launch_sample_pack(const sht::SamplePackArgs* a, hipStream_t s);
+hipError_t sht_launch_sample(const sht::SampleArgs* a, hipStream_t s);
 hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s);
 hipError_t sht_launch_unpack(const sht::UnpackArgs* a, hipStream_t s);
 hipError_t sht_launch_spectral_elementwise(const float2* in, float2* out, long long total, int T,

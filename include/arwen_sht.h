@@ -159,6 +159,24 @@ int arwen_sht_rows_from_waist(arwen_sht_plan* plan, const void* waist, float* ro
 int arwen_sht_spectral_op(arwen_sht_plan* plan, int op, const void* in, void* out, int count,
                           void* stream);
 
+/* Spectral fields at arbitrary points: sample_scalar, sample_gradient and
+ * sample_wind (reference spectral/sampling.py:382-515).
+ * kind 0: in0 (count, T+1, T+1) complex64 -> out0 (count, npoints) float64,
+ *         the fields at the points (poles admitted);
+ * kind 1: in0 -> out0 east, out1 north derivative per metre (points off the
+ *         poles; east divides by radius * cos_lat);
+ * kind 2: in0 vorticity, in1 divergence (count levels) -> out0 u, out1 v
+ *         through psi, chi = inverse_laplacian (vorticity, divergence).
+ * Points (device, float64, npoints each): sin_lat, cos_lat (kinds 1 and 2)
+ * and lon_rad.  precision 0: float64 basis and sums; 1: float32 terms,
+ * float64 sums over orders (the reference's "float32" sampler).
+ * `workspace` (device) holds at least arwen_sht_sample_workspace_bytes. */
+size_t arwen_sht_sample_workspace_bytes(const arwen_sht_plan* plan, int count, int kind);
+int arwen_sht_sample(arwen_sht_plan* plan, const void* in0, const void* in1, int count,
+                     const double* sin_lat, const double* cos_lat, const double* lon_rad,
+                     int npoints, int kind, int precision, void* workspace,
+                     size_t workspace_bytes, double* out0, double* out1, void* stream);
+
 /* A batch: an ordered list of jobs executed as
  *   pack -> ONE grouped synthesis GEMM -> inverse FFTs        (synthesis jobs)
  *   FFTs -> ONE grouped analysis GEMM -> unpack               (analysis jobs)

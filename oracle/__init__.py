@@ -22,4 +22,9 @@ docstring and in ``tests/test_oracle_*.py``:
 * the transform operations -- brute-force O(T^2 nlat nlon) sums at small T,
   solid-body rotation, degree-exact Laplacian, constant mode, vector round
   trip.
+
+``oracle.sampling`` restates the reference's host point sampler
+(``spectral/sampling.py``), pinned by live ``scipy.special.sph_harm_y`` at
+random points and by the oracle's FFT synthesis, gradient and wind at the
+Gaussian grid's points.
 """
