@@ -302,7 +302,7 @@ def sampler_bench(args, torch):
     cl = torch.from_numpy(np.cos(lat)).to(dev)
     lo = torch.from_numpy(lon).to(dev)
     lib = _native.load()
-    nbytes = int(lib.arwen_sht_sample_workspace_bytes(t.plan_handle, F, 0))
+    nbytes = int(lib.arwen_sht_sample_workspace_bytes(t.plan_handle, F, 0, N))
     work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     out = torch.empty((F, N), dtype=torch.float64, device=dev)
     vp = lambda a: ctypes.c_void_p(a.data_ptr())  # noqa: E731
@@ -355,7 +355,8 @@ def sampler_bench(args, torch):
         "config": {"workload": f"sample_scalar T{T}, {F} fields, {N} points", "T": T,
                    "fields": F, "points": N},
         "float32": res["float32"],
-        "roofline": {"bound": "valu_fp64", "kernel": "sample_kernel<8, 0, false> + sample_pack",
+        "roofline": {"bound": "valu_fp64",
+                     "kernel": "sample_pack + sample_kernel<8, 0, false> + sample_reduce",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "algorithmic_flop_per_call": flops},

@@ -130,7 +130,7 @@ def load():
             "arwen_sht_waist_from_rows": (I, [P, P, P, I, I, I, P]),
             "arwen_sht_rows_from_waist": (I, [P, P, P, I, I, I, P]),
             "arwen_sht_spectral_op": (I, [P, I, P, P, I, P]),
-            "arwen_sht_sample_workspace_bytes": (ctypes.c_size_t, [P, I, I]),
+            "arwen_sht_sample_workspace_bytes": (ctypes.c_size_t, [P, I, I, I]),
             "arwen_sht_sample": (I, [P, P, P, I, P, P, P, I, I, I, P, ctypes.c_size_t, P, P, P]),
             "arwen_sht_batch_create": (I, [ctypes.POINTER(Job), I, ctypes.POINTER(P)]),
             "arwen_sht_batch_destroy": (I, [P]),

@@ -103,7 +103,8 @@ def _sample(transform: SphericalHarmonicTransform, kind: str, first, second, lat
         lo = torch.from_numpy(lon).to(dev)
         code = _KIND[kind]
         lib = t._lib
-        nbytes = int(lib.arwen_sht_sample_workspace_bytes(t._plan, count, code))
+        nbytes = int(lib.arwen_sht_sample_workspace_bytes(t._plan, count, code,
+                                                          min(chunk, npts)))
         work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         vp = ctypes.c_void_p
         for start in range(0, npts, chunk):

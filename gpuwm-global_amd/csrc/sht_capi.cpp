@@ -1103,10 +1103,33 @@ void build_sample_tables(arwen_sht_plan* P) {
 }
 }  // namespace
 
-size_t arwen_sht_sample_workspace_bytes(const arwen_sht_plan* P, int count, int kind) {
-  if (!P || count < 0 || kind < 0 || kind > 2) return 0;
+// Order-range parts per (point block, field tile): a function of T only, so
+// a point's arithmetic never depends on how many points share a launch.
+sht::SampleCuts sample_cuts(int T) {
+  sht::SampleCuts c{};
+  c.parts = std::max(1, std::min(sht::kSampleMaxParts, (T + 1) / 16));
+  const long long K = (long long)(T + 1) * (T + 2) / 2;
+  c.m[0] = 0;
+  long long acc = 0;
+  int r = 1;
+  for (int m = 0; m <= T && r < c.parts; ++m) {
+    acc += T + 1 - m;  // entries of orders 0..m
+    if (acc * c.parts >= (long long)r * K) c.m[r++] = m + 1;
+  }
+  for (; r < c.parts; ++r) c.m[r] = c.m[r - 1] + 1;  // not reached for T >= 15
+  c.m[c.parts] = T + 1;
+  return c;
+}
+
+size_t arwen_sht_sample_workspace_bytes(const arwen_sht_plan* P, int count, int kind,
+                                        int npoints) {
+  if (!P || count < 0 || kind < 0 || kind > 2 || npoints < 0) return 0;
   const long long K = (long long)(P->T + 1) * (P->T + 2) / 2;
-  return (size_t)K * sample_columns(count, kind) * sizeof(double2);  // either precision fits
+  const int ldc = sample_columns(count, kind);
+  const int R = sample_cuts(P->T).parts;
+  size_t bytes = (size_t)K * ldc * sizeof(double2);  // packed coefficients, either precision
+  if (R > 1) bytes += (size_t)2 * R * ldc * (size_t)npoints * sizeof(double);
+  return bytes;
 }
 
 int arwen_sht_sample(arwen_sht_plan* P, const void* in0, const void* in1, int count,
@@ -1127,7 +1150,7 @@ int arwen_sht_sample(arwen_sht_plan* P, const void* in0, const void* in1, int co
     if (!sin_lat || !lon_rad || (kind != 0 && !cos_lat))
       fail(ARWEN_SHT_EINVAL, "a point coordinate array is NULL");
     if (!out0 || (kind != 0 && !out1)) fail(ARWEN_SHT_EINVAL, "an output is NULL");
-    const size_t need = arwen_sht_sample_workspace_bytes(P, count, kind);
+    const size_t need = arwen_sht_sample_workspace_bytes(P, count, kind, npoints);
     if (!workspace || workspace_bytes < need)
       fail(ARWEN_SHT_EINVAL, "sample workspace must hold " + std::to_string(need) + " bytes");
     int dev;
@@ -1156,6 +1179,12 @@ int arwen_sht_sample(arwen_sht_plan* P, const void* in0, const void* in1, int co
     a.lon = lon_rad;
     a.out0 = out0;
     a.out1 = out1 ? out1 : out0;  // unwritten for values
+    a.cuts = sample_cuts(P->T);
+    const long long K = (long long)(P->T + 1) * (P->T + 2) / 2;
+    a.partial = a.cuts.parts > 1
+                    ? reinterpret_cast<double*>(static_cast<char*>(workspace) +
+                                                (size_t)K * ldc * sizeof(double2))
+                    : nullptr;
     a.radius = P->radius;
     a.T = P->T;
     a.npts = npoints;

@@ -123,6 +123,11 @@ struct SamplePackArgs {
   double radius;
   int T, count, ldc, wind, f64;
 };
+constexpr int kSampleMaxParts = 8;
+struct SampleCuts {
+  int parts;                      // R: order-range parts per (point block, field tile)
+  int m[kSampleMaxParts + 1];     // part r holds orders m[r] .. m[r+1]-1
+};
 struct SampleArgs {
   const void* coef;       // (K, ldc) from the pack
   const double* rec;      // [3][K]: recurrence a, b and derivative c per packed (n, m)
@@ -132,6 +137,8 @@ struct SampleArgs {
   const double* lon;      // (npts) radians
   double* out0;           // (nout, npts)
   double* out1;
+  double* partial;        // R > 1: (2 R, ldc, npts) raw sums of the parts
+  SampleCuts cuts;
   double radius;
   int T, npts, ldc, nout, kind, f32;
 };

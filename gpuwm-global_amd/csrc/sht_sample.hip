@@ -71,133 +71,235 @@ struct SampleAcc {
   using C = typename std::conditional<F32, float2, double2>::type;
 };
 
-// KIND 0: the field; 1: east and north derivatives per metre; 2: u and v
-// from column pairs (psi, chi).
+// The outputs of one point from its two sums (s0: values, or zonal sums
+// d/dlambda; s1: meridional sums d/dphi).  KIND 0: the field; 1: east and
+// north derivatives per metre; 2: u and v from column pairs (psi, chi):
+// u = -north(psi) + east(chi), v = east(psi) + north(chi) (reference
+// sampling.py:481-515).
+template <int KIND>
+__device__ __forceinline__ void sample_store(int col, int p, int npts, int nout, double s0,
+                                             double s1, double s0b, double s1b, double radius,
+                                             double coslat, double* out0, double* out1) {
+  if constexpr (KIND == 0) {
+    if (col < nout) out0[(size_t)col * npts + p] = s0;
+  } else if constexpr (KIND == 1) {
+    if (col < nout) {
+      out0[(size_t)col * npts + p] = s0 / (radius * coslat);
+      out1[(size_t)col * npts + p] = s1 / radius;
+    }
+  } else {
+    const int level = col >> 1;
+    if (level < nout) {
+      const double east_scale = radius * coslat;
+      const double psi_e = s0 / east_scale, psi_n = s1 / radius;
+      const double chi_e = s0b / east_scale, chi_n = s1b / radius;
+      out0[(size_t)level * npts + p] = -psi_n + chi_e;
+      out1[(size_t)level * npts + p] = psi_e + chi_n;
+    }
+  }
+}
+
+// One block: 256 x PT points (lane l takes points l, l + 256, ...) x FT field
+// columns x one part of the order range (orders mcut[r] .. mcut[r+1]-1, cut
+// so the parts hold equal shares of the triangle).  With one part the block
+// stores the outputs; with several it stores its two raw sums per (column,
+// point) into part r of `partial` and sample_reduce_kernel adds the parts in
+// order.  The (m, n) walk is one loop over the packed index k.  The
+// coefficients arrive through scalar loads; those return out of order, so a
+// wave can wait only for all of them, and the next step's loads are the only
+// ones in flight while a step computes: each lane takes PT points so that
+// every load feeds PT x FT x 2 multiply-adds.
+constexpr int kSamplePts = 2;  // PT
+
 template <int FT, int KIND, bool F32>
 __global__ void __launch_bounds__(256) sample_kernel(
     const typename SampleAcc<F32>::C* __restrict__ coef, const double* __restrict__ rec,
     const double* __restrict__ diag, const double* __restrict__ sin_lat,
-    const double* __restrict__ cos_lat, const double* __restrict__ lon, int T, int npts,
-    int ldc, int nout, double radius, double* __restrict__ out0, double* __restrict__ out1) {
+    const double* __restrict__ cos_lat, const double* __restrict__ lon, SampleCuts cuts, int T,
+    int npts, int ldc, int nout, double radius, double* __restrict__ out0,
+    double* __restrict__ out1, double* __restrict__ partial) {
   using Acc = typename SampleAcc<F32>::T;
+  using Cx = typename SampleAcc<F32>::C;
   constexpr bool GRAD = KIND != 0;
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  const bool live = p < npts;
-  const int pp = live ? p : npts - 1;  // idle lanes evaluate a real point, store nothing
+  constexpr int PT = kSamplePts;
   const int c0 = blockIdx.y * FT;
+  const int r = blockIdx.z;
+  const int m0 = cuts.m[r], m1 = cuts.m[r + 1];
   const long long K = (long long)(T + 1) * (T + 2) / 2;
   const double* rec_a = rec;
   const double* rec_b = rec + K;
   const double* rec_c = rec + 2 * K;
   const double* diag_f = diag;
   const double* diag_g = diag + (T + 1);
-  const typename SampleAcc<F32>::C* cf = coef + c0;
-
-  const double x = sin_lat[pp];
-  const double lam = lon[pp];
-  const double root = sqrt(fmax(0.0, 1.0 - x * x));
-  const double denom = x * x - 1.0;
+  const Cx* cf = coef + c0;
   const double pi = 3.14159265358979323846;
-  double dg = 1.0 / sqrt(4.0 * pi);
 
-  double o0[FT], o1[FT];
+  int pt[PT];
+  double x[PT], lam[PT], root[PT], denom[PT], dg[PT], prev[PT], prev2[PT];
 #pragma unroll
-  for (int f = 0; f < FT; ++f) o0[f] = o1[f] = 0.0;
+  for (int q = 0; q < PT; ++q) {
+    pt[q] = blockIdx.x * (256 * PT) + q * 256 + threadIdx.x;
+    const int pp = pt[q] < npts ? pt[q] : npts - 1;  // idle lanes evaluate a real point
+    x[q] = sin_lat[pp];
+    lam[q] = lon[pp];
+    root[q] = sqrt(fmax(0.0, 1.0 - x[q] * x[q]));
+    denom[q] = x[q] * x[q] - 1.0;
+    dg[q] = 1.0 / sqrt(4.0 * pi);
+    for (int mm = 1; mm <= m0; ++mm) dg[q] = diag_f[mm] * root[q] * dg[q];  // P_{m0 m0}
+    prev[q] = prev2[q] = 0.0;
+  }
 
-  long long k = 0;
-  for (int m = 0; m <= T; ++m) {
-    if (m > 0) dg = diag_f[m] * root * dg;
-    Acc ar[FT], ai[FT], dr[FT], di[FT];
+  double o0[PT][FT], o1[PT][FT];
+  Acc ar[PT][FT], ai[PT][FT], dr[PT][FT], di[PT][FT];
 #pragma unroll
-    for (int f = 0; f < FT; ++f) ar[f] = ai[f] = dr[f] = di[f] = Acc(0);
-    auto add = [&](long long kk, double P, double dP) {
-      const auto* c = cf + kk * ldc;
+  for (int q = 0; q < PT; ++q)
+#pragma unroll
+    for (int f = 0; f < FT; ++f) {
+      o0[q][f] = o1[q][f] = 0.0;
+      ar[q][f] = ai[q][f] = dr[q][f] = di[q][f] = Acc(0);
+    }
+
+  const long long k0 = tri_index(T, m0, m0);         // (n, m) = (m0, m0)
+  const long long k_last = tri_index(T, T, m1 - 1);  // (T, m1 - 1)
+  Cx cur[FT], nxt[FT];
+#pragma unroll
+  for (int f = 0; f < FT; ++f) cur[f] = cf[k0 * ldc + f];
+  double ca = rec_a[k0], cb = rec_b[k0], cc = rec_c[k0];
+  int m = m0, n = m0;
+  for (long long k = k0; k <= k_last; ++k) {
+    const long long kn = k < k_last ? k + 1 : k;  // the next step (the last one reloads itself)
+#pragma unroll
+    for (int f = 0; f < FT; ++f) nxt[f] = cf[kn * ldc + f];
+    const double na = rec_a[kn], nb = rec_b[kn], nc = rec_c[kn];
+    const double g = n == m + 1 ? diag_g[m] : 0.0;
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+      double P, dP = 0.0;
+      if (n == m) {
+        P = dg[q];
+        if constexpr (GRAD) dP = (m >= 1) ? root[q] * ((double)m * x[q] * dg[q]) / denom[q] : 0.0;
+      } else if (n == m + 1) {
+        P = g * x[q] * dg[q];
+        if constexpr (GRAD) dP = root[q] * ((double)n * x[q] * P - cc * prev[q]) / denom[q];
+      } else {
+        P = ca * (x[q] * prev[q] - cb * prev2[q]);
+        if constexpr (GRAD) dP = root[q] * ((double)n * x[q] * P - cc * prev[q]) / denom[q];
+      }
 #pragma unroll
       for (int f = 0; f < FT; ++f) {
-        const auto v = c[f];
         if constexpr (F32) {
-          ar[f] = fmaf(v.x, (float)P, ar[f]);
-          ai[f] = fmaf(v.y, (float)P, ai[f]);
+          ar[q][f] = fmaf(cur[f].x, (float)P, ar[q][f]);
+          ai[q][f] = fmaf(cur[f].y, (float)P, ai[q][f]);
           if constexpr (GRAD) {
-            dr[f] = fmaf(v.x, (float)dP, dr[f]);
-            di[f] = fmaf(v.y, (float)dP, di[f]);
+            dr[q][f] = fmaf(cur[f].x, (float)dP, dr[q][f]);
+            di[q][f] = fmaf(cur[f].y, (float)dP, di[q][f]);
           }
         } else {
-          ar[f] = fma(v.x, P, ar[f]);
-          ai[f] = fma(v.y, P, ai[f]);
+          ar[q][f] = fma(cur[f].x, P, ar[q][f]);
+          ai[q][f] = fma(cur[f].y, P, ai[q][f]);
           if constexpr (GRAD) {
-            dr[f] = fma(v.x, dP, dr[f]);
-            di[f] = fma(v.y, dP, di[f]);
+            dr[q][f] = fma(cur[f].x, dP, dr[q][f]);
+            di[q][f] = fma(cur[f].y, dP, di[q][f]);
           }
         }
       }
-    };
-    // n = m
-    double prev = dg, prev2 = 0.0;
-    double dP = 0.0;
-    if constexpr (GRAD) dP = (m >= 1) ? root * ((double)m * x * dg) / denom : 0.0;
-    add(k, dg, dP);
-    ++k;
-    if (m < T) {
-      // n = m + 1
-      const double p1 = diag_g[m] * x * dg;
-      if constexpr (GRAD) dP = root * ((double)(m + 1) * x * p1 - rec_c[k] * prev) / denom;
-      add(k, p1, dP);
-      ++k;
-      prev2 = prev;
-      prev = p1;
-      for (int n = m + 2; n <= T; ++n) {
-        const double pn = rec_a[k] * (x * prev - rec_b[k] * prev2);
-        if constexpr (GRAD) dP = root * ((double)n * x * pn - rec_c[k] * prev) / denom;
-        add(k, pn, dP);
-        ++k;
-        prev2 = prev;
-        prev = pn;
-      }
+      prev2[q] = prev[q];
+      prev[q] = P;
     }
-    double sm, cm;
-    sincos((double)m * lam, &sm, &cm);
-    const double mult = m == 0 ? 1.0 : 2.0;
+    if (n == T) {
+      // the order is complete: turn its amplitudes by e^{i m lambda}
+      const double mult = m == 0 ? 1.0 : 2.0;
 #pragma unroll
-    for (int f = 0; f < FT; ++f) {
-      const double re = (double)ar[f], im = (double)ai[f];
-      if constexpr (KIND == 0) {
-        o0[f] += mult * (re * cm - im * sm);
-      } else {
-        // Re(i m A e^{i m lambda}) and Re(dA e^{i m lambda})
-        o0[f] += mult * (-(double)m * (re * sm + im * cm));
-        o1[f] += mult * ((double)dr[f] * cm - (double)di[f] * sm);
+      for (int q = 0; q < PT; ++q) {
+        double sm, cm;
+        sincos((double)m * lam[q], &sm, &cm);
+#pragma unroll
+        for (int f = 0; f < FT; ++f) {
+          const double re = (double)ar[q][f], im = (double)ai[q][f];
+          if constexpr (KIND == 0) {
+            o0[q][f] += mult * (re * cm - im * sm);
+          } else {
+            // Re(i m A e^{i m lambda}) and Re(dA e^{i m lambda})
+            o0[q][f] += mult * (-(double)m * (re * sm + im * cm));
+            o1[q][f] += mult * ((double)dr[q][f] * cm - (double)di[q][f] * sm);
+          }
+          ar[q][f] = ai[q][f] = dr[q][f] = di[q][f] = Acc(0);
+        }
       }
+      ++m;
+      n = m;
+      if (m <= T) {
+        const double fm = diag_f[m];
+#pragma unroll
+        for (int q = 0; q < PT; ++q) dg[q] = fm * root[q] * dg[q];
+      }
+    } else {
+      ++n;
+    }
+#pragma unroll
+    for (int f = 0; f < FT; ++f) cur[f] = nxt[f];
+    ca = na;
+    cb = nb;
+    cc = nc;
+  }
+#pragma unroll
+  for (int q = 0; q < PT; ++q) {
+    const int p = pt[q];
+    if (p >= npts) continue;
+    if (gridDim.z > 1) {
+      // raw sums of this part: partial[r][c][p], then [R + r] for the second sum
+      const size_t plane = (size_t)ldc * npts;
+#pragma unroll
+      for (int f = 0; f < FT; ++f) {
+        partial[(size_t)r * plane + (size_t)(c0 + f) * npts + p] = o0[q][f];
+        if constexpr (GRAD)
+          partial[(size_t)(gridDim.z + r) * plane + (size_t)(c0 + f) * npts + p] = o1[q][f];
+      }
+      continue;
+    }
+    const double coslat = KIND == 0 ? 0.0 : cos_lat[p];
+    if constexpr (KIND == 2) {
+      static_assert(FT % 2 == 0, "wind tiles hold whole (psi, chi) pairs");
+#pragma unroll
+      for (int f = 0; f < FT; f += 2)
+        sample_store<KIND>(c0 + f, p, npts, nout, o0[q][f], o1[q][f], o0[q][f + 1],
+                           o1[q][f + 1], radius, coslat, out0, out1);
+    } else {
+#pragma unroll
+      for (int f = 0; f < FT; ++f)
+        sample_store<KIND>(c0 + f, p, npts, nout, o0[q][f], o1[q][f], 0.0, 0.0, radius, coslat,
+                           out0, out1);
     }
   }
-  if (!live) return;
-  if constexpr (KIND == 0) {
-#pragma unroll
-    for (int f = 0; f < FT; ++f)
-      if (c0 + f < nout) out0[(size_t)(c0 + f) * npts + p] = o0[f];
-  } else if constexpr (KIND == 1) {
-    const double east_scale = radius * cos_lat[p];
-#pragma unroll
-    for (int f = 0; f < FT; ++f)
-      if (c0 + f < nout) {
-        out0[(size_t)(c0 + f) * npts + p] = o0[f] / east_scale;
-        out1[(size_t)(c0 + f) * npts + p] = o1[f] / radius;
-      }
-  } else {
-    // columns 2l, 2l + 1 = psi, chi of level l: u = -north(psi) + east(chi),
-    // v = east(psi) + north(chi) (reference sampling.py:481-515)
-    static_assert(FT % 2 == 0, "wind tiles hold whole (psi, chi) pairs");
-    const double east_scale = radius * cos_lat[p];
-#pragma unroll
-    for (int f = 0; f < FT; f += 2) {
-      const int level = (c0 + f) >> 1;
-      if (level < nout) {
-        const double psi_e = o0[f] / east_scale, psi_n = o1[f] / radius;
-        const double chi_e = o0[f + 1] / east_scale, chi_n = o1[f + 1] / radius;
-        out0[(size_t)level * npts + p] = -psi_n + chi_e;
-        out1[(size_t)level * npts + p] = psi_e + chi_n;
+}
+
+// Adds the R parts' sums of each (column, point) in part order and stores
+// the outputs; one thread per (column or column pair, point).
+template <int KIND>
+__global__ void sample_reduce_kernel(const double* __restrict__ partial, int nparts, int ldc,
+                                     int npts, int nout, double radius,
+                                     const double* __restrict__ cos_lat, double* out0,
+                                     double* out1) {
+  constexpr int STRIDE = KIND == 2 ? 2 : 1;
+  const long long total = (long long)(ldc / STRIDE) * npts;
+  const size_t plane = (size_t)ldc * npts;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int p = (int)(e % npts);
+    const int col = (int)(e / npts) * STRIDE;
+    if ((KIND == 2 ? col >> 1 : col) >= nout) continue;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};  // s0, s1 of col; s0, s1 of col + 1 (winds)
+    for (int r = 0; r < nparts; ++r) {
+      const size_t o = (size_t)r * plane + (size_t)col * npts + p;
+      s[0] += partial[o];
+      if (KIND != 0) s[1] += partial[(size_t)nparts * plane + o];
+      if (KIND == 2) {
+        s[2] += partial[o + npts];
+        s[3] += partial[(size_t)nparts * plane + o + npts];
       }
     }
+    sample_store<KIND>(col, p, npts, nout, s[0], s[1], s[2], s[3], radius,
+                       KIND == 0 ? 0.0 : cos_lat[p], out0, out1);
   }
 }
 
@@ -219,13 +321,18 @@ int sht_sample_field_tile(int kind, int f32) { return (kind == 0 || f32) ? 8 : 4
 extern "C" hipError_t sht_launch_sample(const sht::SampleArgs* a, hipStream_t s) {
   if (a->npts <= 0 || a->ldc <= 0) return hipSuccess;
   const int ft = sht_sample_field_tile(a->kind, a->f32);
-  if (a->ldc % ft) return hipErrorInvalidValue;
-  const dim3 g((unsigned)((a->npts + 255) / 256), (unsigned)(a->ldc / ft)), b(256);
+  const int R = a->cuts.parts;
+  if (a->ldc % ft || R < 1 || R > sht::kSampleMaxParts || (R > 1 && !a->partial))
+    return hipErrorInvalidValue;
+  constexpr int kPointsPerBlock = 256 * sht::kSamplePts;
+  const dim3 g((unsigned)((a->npts + kPointsPerBlock - 1) / kPointsPerBlock),
+               (unsigned)(a->ldc / ft), (unsigned)R),
+      b(256);
 #define SHT_SAMPLE(FT, KIND, F32)                                                             \
   hipLaunchKernelGGL((sht::sample_kernel<FT, KIND, F32>), g, b, 0, s,                          \
                      static_cast<const typename sht::SampleAcc<F32>::C*>(a->coef), a->rec,   \
-                     a->diag, a->sin_lat, a->cos_lat, a->lon, a->T, a->npts, a->ldc, a->nout, \
-                     a->radius, a->out0, a->out1)
+                     a->diag, a->sin_lat, a->cos_lat, a->lon, a->cuts, a->T, a->npts, a->ldc, \
+                     a->nout, a->radius, a->out0, a->out1, a->partial)
   if (a->f32) {
     if (a->kind == 0) SHT_SAMPLE(8, 0, true);
     else if (a->kind == 1) SHT_SAMPLE(8, 1, true);
@@ -236,5 +343,18 @@ extern "C" hipError_t sht_launch_sample(const sht::SampleArgs* a, hipStream_t s)
     else SHT_SAMPLE(4, 2, false);
   }
 #undef SHT_SAMPLE
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || R == 1) return e;
+  const int stride = a->kind == 2 ? 2 : 1;
+  long long blocks = ((long long)(a->ldc / stride) * a->npts + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+#define SHT_REDUCE(KIND)                                                                       \
+  hipLaunchKernelGGL(sht::sample_reduce_kernel<KIND>, dim3((unsigned)blocks), dim3(256), 0, s,  \
+                     a->partial, R, a->ldc, a->npts, a->nout, a->radius, a->cos_lat, a->out0, \
+                     a->out1)
+  if (a->kind == 0) SHT_REDUCE(0);
+  else if (a->kind == 1) SHT_REDUCE(1);
+  else SHT_REDUCE(2);
+#undef SHT_REDUCE
   return hipGetLastError();
 }

@@ -170,8 +170,10 @@ int arwen_sht_spectral_op(arwen_sht_plan* plan, int op, const void* in, void* ou
  * Points (device, float64, npoints each): sin_lat, cos_lat (kinds 1 and 2)
  * and lon_rad.  precision 0: float64 basis and sums; 1: float32 terms,
  * float64 sums over orders (the reference's "float32" sampler).
- * `workspace` (device) holds at least arwen_sht_sample_workspace_bytes. */
-size_t arwen_sht_sample_workspace_bytes(const arwen_sht_plan* plan, int count, int kind);
+ * `workspace` (device) holds at least arwen_sht_sample_workspace_bytes
+ * (the packed coefficients, and per-part sums of the order range). */
+size_t arwen_sht_sample_workspace_bytes(const arwen_sht_plan* plan, int count, int kind,
+                                        int npoints);
 int arwen_sht_sample(arwen_sht_plan* plan, const void* in0, const void* in1, int count,
                      const double* sin_lat, const double* cos_lat, const double* lon_rad,
                      int npoints, int kind, int precision, void* workspace,

@@ -133,8 +133,8 @@ def test_regular_grid_and_argument_errors(pairs):
     cc = torch.from_numpy(c).to(dev)
     pts = torch.zeros(4, dtype=torch.float64, device=dev)
     out = torch.empty((2, 4), dtype=torch.float64, device=dev)
-    need = int(lib.arwen_sht_sample_workspace_bytes(t.plan_handle, 2, 0))
-    assert need == (22 * 23 // 2) * 8 * 16
+    need = int(lib.arwen_sht_sample_workspace_bytes(t.plan_handle, 2, 0, 4))
+    assert need == (22 * 23 // 2) * 8 * 16  # T21: one order-range part, no partial sums
     work = torch.empty(need, dtype=torch.uint8, device=dev)
     vp = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
     for kind, prec, nbytes, msg in ((3, 0, need, b"kind"), (0, 2, need, b"precision"),
