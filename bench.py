@@ -12,7 +12,7 @@ run as one TransformBatch (one grouped Legendre GEMM launch per direction)
 replayed from a hipGraph, inputs resident in HBM.  ``value`` is the split-BF16
 (bf16x6) round trips/s; the same workload in FP32 MFMA is timed beside it.
 
-    python bench.py [--steps K] [--warmup W] [--workload t383|t383x32|mixed|sample]
+    python bench.py [--steps K] [--warmup W] [--workload t383|t383x32|mixed|sample|vertical]
                     [--precision bf16x6|bf16x3|float32] [--no-cpu-baseline]
 
 Prints one JSON line.  Only ``cpu_baseline()`` touches the FP64 oracle: it
@@ -363,6 +363,74 @@ def sampler_bench(args, torch):
         "cpu_baseline": cpu}))
 
 
+def vertical_bench(args, torch):
+    """Semi-implicit stage solve on spectral columns (solve_shifted): T383,
+    40 levels of divergence and theta plus ln ps, synthetic well-posed
+    vertical operator, inputs resident on the device; HIP events on the
+    launch stream.  Algorithmic bytes per call: the triangles of the 2L + 1
+    input planes read once (complex64), the 2L + 1 output planes written
+    once (whole (T+1)^2 squares: the outputs are fresh arrays, zero above
+    the triangle), the per-degree inverses read once."""
+    import ctypes  # noqa: F401
+
+    from arwen_spectral import SphericalHarmonicTransform
+    from arwen_spectral.semi_implicit import vertical_solve_shifted, wave_operator
+
+    sys.path.insert(0, str(ROOT))
+    from oracle import semi_implicit as osi
+
+    T, L = 383, LEVELS
+    t = SphericalHarmonicTransform.create(T, dealias_factor=1.0, backend="hip")
+    dev = torch.device("cuda", torch.cuda.current_device())
+    b, c, m, modal = osi.synthetic_operator(L, seed=7)
+    k2 = wave_operator(t)
+    tw = 900.0
+    inv = osi.helmholtz_inverse(modal, k2, tw, 1.0)
+    f32 = lambda a: torch.from_numpy(np.asarray(a, dtype=np.float32)).to(dev)  # noqa: E731
+    c64 = lambda a: torch.from_numpy(a.astype(np.complex64)).to(dev)  # noqa: E731
+    d = c64(red_spectrum(T, (L,), 41, CLASS_SCALE["divergence"]))
+    th = c64(red_spectrum(T, (L,), 42, CLASS_SCALE["temperature"]))
+    ps = c64(red_spectrum(T, (), 43, CLASS_SCALE["lnps"]))
+    bm, cm, im, km = f32(b), f32(c), f32(inv), f32(k2)
+    for _ in range(args.warmup):
+        vertical_solve_shifted(t, d, th, ps, bm, cm, im, km, tw)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.steps):
+        vertical_solve_shifted(t, d, th, ps, bm, cm, im, km, tw)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    k_tri = (T + 1) * (T + 2) // 2
+    planes = 2 * L + 1
+    alg = planes * (8.0 * k_tri + 8.0 * (T + 1) ** 2) + 4.0 * (T + 1) * L * L
+    achieved = alg / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle.sht import OracleTransform
+
+        o = OracleTransform.create(T, dealias_factor=1.0)
+        dh, thh, psh = (x.cpu().numpy() for x in (d, th, ps))
+        t0 = time.perf_counter()
+        osi.solve_shifted(o, dh, thh, psh, b, c, inv, k2, tw)
+        sec = time.perf_counter() - t0
+        cpu = {"value": 1.0 / sec, "unit": "solves/s", "cores": min(len(os.sched_getaffinity(0)), 16),
+               "kind": "port", "sample": f"FP64 numpy oracle of solve_shifted (semi_implicit.py:"
+                                        f"944-959), T{T}, {L} levels, one call"}
+    print(json.dumps({
+        "metric": f"T{T} L{L} semi-implicit stage solves/s (solve_shifted on spectral columns)",
+        "value": 1e3 / ms, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded red-spectrum fields, synthetic well-posed vertical operator",
+        "config": {"workload": f"solve_shifted T{T}, {L} levels", "T": T, "levels": L},
+        "roofline": {"bound": "hbm", "kernel": "vertical_solve_kernel", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": alg},
+        "cpu_baseline": cpu}))
+
+
 def workload_specs(name):
     if name == "t383":
         return [dict(T=383, members=1)]
@@ -407,6 +475,9 @@ def main():
         return
     if args.workload == "sample":
         sampler_bench(args, torch)
+        return
+    if args.workload == "vertical":
+        vertical_bench(args, torch)
         return
     specs = workload_specs(args.workload)
     for sp in specs:

@@ -1196,6 +1196,47 @@ int arwen_sht_sample(arwen_sht_plan* P, const void* in0, const void* in1, int co
   });
 }
 
+int arwen_sht_vertical_solve(arwen_sht_plan* P, int nlev, const void* div, const void* theta,
+                             const void* lnps, const float* b_matrix, const float* c_matrix,
+                             const float* m_matrix, const float* inverse, const float* k2,
+                             float s_bx, float s_md, float s_c, float s_cd, float s_cn,
+                             void* div_out, void* theta_out, void* lnps_out, void* s) {
+  return guarded([&] {
+    if (!P) fail(ARWEN_SHT_EINVAL, "plan is NULL");
+    if (nlev < 1 || nlev > sht::kMaxLevels)
+      fail(ARWEN_SHT_EINVAL, "nlev must be in 1.." + std::to_string(sht::kMaxLevels) + ", got " +
+                                 std::to_string(nlev));
+    if (!div || !theta || !lnps || !div_out || !theta_out || !lnps_out)
+      fail(ARWEN_SHT_EINVAL, "a spectral input or output is NULL");
+    if (!b_matrix || !c_matrix || !inverse || !k2)
+      fail(ARWEN_SHT_EINVAL, "b_matrix, c_matrix, inverse and k2 are required");
+    if (s_md != 0.f && !m_matrix) fail(ARWEN_SHT_EINVAL, "s_md != 0 needs m_matrix");
+    int dev;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != P->device) fail(ARWEN_SHT_EINVAL, "plan used on a device other than its own");
+    sht::VerticalArgs a{};
+    a.div = static_cast<const float2*>(div);
+    a.theta = static_cast<const float2*>(theta);
+    a.lnps = static_cast<const float2*>(lnps);
+    a.b_matrix = b_matrix;
+    a.c_matrix = c_matrix;
+    a.m_matrix = s_md != 0.f ? m_matrix : nullptr;
+    a.inverse = inverse;
+    a.k2 = k2;
+    a.d_out = static_cast<float2*>(div_out);
+    a.theta_out = static_cast<float2*>(theta_out);
+    a.lnps_out = static_cast<float2*>(lnps_out);
+    a.s_bx = s_bx;
+    a.s_md = s_md;
+    a.s_c = s_c;
+    a.s_cd = s_cd;
+    a.s_cn = s_cn;
+    a.T = P->T;
+    a.nlev = nlev;
+    hip_check(sht_launch_vertical_solve(&a, static_cast<hipStream_t>(s)), "vertical solve launch");
+  });
+}
+
 int arwen_sht_batch_create(const arwen_sht_job* jobs, int njobs, arwen_sht_batch** out) {
   return guarded([&] {
     if (!out) fail(ARWEN_SHT_EINVAL, "out is NULL");

@@ -143,6 +143,24 @@ struct SampleArgs {
   int T, npts, ldc, nout, kind, f32;
 };
 
+// ---- semi-implicit vertical-mode solves (sht_vertical.hip) ----
+constexpr int kMaxLevels = 64;
+struct VerticalArgs {
+  const float2* div;      // (L, T+1, T+1)
+  const float2* theta;    // (L, T+1, T+1)
+  const float2* lnps;     // (T+1, T+1)
+  const float* b_matrix;  // (L, L+1)
+  const float* c_matrix;  // (L+1, L)
+  const float* m_matrix;  // (L, L), NULL when s_md = 0
+  const float* inverse;   // (T+1, L, L)
+  const float* k2;        // (T+1)
+  float2* d_out;
+  float2* theta_out;
+  float2* lnps_out;
+  float s_bx, s_md, s_c, s_cd, s_cn;
+  int T, nlev;
+};
+
 // ---- grouped GEMM (sht_gemm.hip) ----
 // A panel is one row tile of one work group: up to kMT consecutive orders m
 // of one job and one output parity.  Its k-steps (one order, 32 k each) are
@@ -242,6 +260,7 @@ hipError_t sht_launch_waist_fold(const sht::WaistFoldArgs* a, hipStream_t s);
 hipError_t sht_launch_waist_unfold(const sht::WaistUnfoldArgs* a, hipStream_t s);
 hipError_t sht_launch_sample_pack(const sht::SamplePackArgs* a, hipStream_t s);
 hipError_t sht_launch_sample(const sht::SampleArgs* a, hipStream_t s);
+hipError_t sht_launch_vertical_solve(const sht::VerticalArgs* a, hipStream_t s);
 hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s);
 hipError_t sht_launch_unpack(const sht::UnpackArgs* a, hipStream_t s);
 hipError_t sht_launch_spectral_elementwise(const float2* in, float2* out, long long total, int T,

@@ -179,6 +179,26 @@ int arwen_sht_sample(arwen_sht_plan* plan, const void* in0, const void* in1, int
                      int npoints, int kind, int precision, void* workspace,
                      size_t workspace_bytes, double* out0, double* out1, void* stream);
 
+/* The semi-implicit vertical-mode solves on spectral columns (reference
+ * semi_implicit.py: VerticalModeSemiImplicit.solve_shifted :921-959 and
+ * apply :887-919, each followed by _finish :815-843).  Inputs (device):
+ * div, theta (nlev, T+1, T+1) and lnps (T+1, T+1) complex64; the vertical
+ * operator's float32 matrices b_matrix (nlev, nlev+1), c_matrix (nlev+1,
+ * nlev), m_matrix (nlev, nlev; read only when s_md != 0), the per-degree
+ * Helmholtz inverses (T+1, nlev, nlev) and k2 (T+1).  Per column, with x =
+ * [theta, lnps]:
+ *   forcing = D + k2_n (s_bx B x + s_md M D);  d = inverse_n forcing;
+ *   x' = x + s_c C (s_cd D + s_cn d);
+ * then d(0,0) = D(0,0), x'(0,0) = x(0,0), and both projected.
+ * solve_shifted: s_bx = s_c = tau w, s_md = s_cd = 0, s_cn = 1;
+ * apply: s_bx = s_c = dt, s_md = -alpha (1 - alpha) dt^2, s_cd = 1 - alpha,
+ * s_cn = alpha.  nlev <= 64.  Outputs must not alias inputs. */
+int arwen_sht_vertical_solve(arwen_sht_plan* plan, int nlev, const void* div, const void* theta,
+                             const void* lnps, const float* b_matrix, const float* c_matrix,
+                             const float* m_matrix, const float* inverse, const float* k2,
+                             float s_bx, float s_md, float s_c, float s_cd, float s_cn,
+                             void* div_out, void* theta_out, void* lnps_out, void* stream);
+
 /* A batch: an ordered list of jobs executed as
  *   pack -> ONE grouped synthesis GEMM -> inverse FFTs        (synthesis jobs)
  *   FFTs -> ONE grouped analysis GEMM -> unpack               (analysis jobs)
