@@ -40,7 +40,7 @@ EXPORTS = (
     "arwen_sht_vordiv_from_uv", "arwen_sht_synthesis_to_waist", "arwen_sht_analysis_from_waist",
     "arwen_sht_vordiv_from_waist", "arwen_sht_waist_from_rows", "arwen_sht_rows_from_waist",
     "arwen_sht_spectral_op", "arwen_sht_sample_workspace_bytes", "arwen_sht_sample",
-    "arwen_sht_vertical_solve",
+    "arwen_sht_vertical_solve", "arwen_sht_vertical_tendencies", "arwen_sht_vertical_half_apply",
     "arwen_sht_batch_create",
     "arwen_sht_batch_destroy", "arwen_sht_batch_run", "arwen_sht_batch_capture",
     "arwen_sht_batch_launch", "arwen_sht_batch_get_config", "arwen_sht_batch_set_config",
@@ -136,6 +136,9 @@ def load():
             "arwen_sht_sample": (I, [P, P, P, I, P, P, P, I, I, I, P, ctypes.c_size_t, P, P, P]),
             "arwen_sht_vertical_solve": (I, [P, I, P, P, P, P, P, P, P, P, F, F, F, F, F,
                                              P, P, P, P]),
+            "arwen_sht_vertical_tendencies": (I, [P, I, P, P, P, P, P, P, F, P, P, P, P]),
+            "arwen_sht_vertical_half_apply": (I, [P, I, P, P, P, P, P, P, P, P, P, P, P, P, P,
+                                                  P]),
             "arwen_sht_batch_create": (I, [ctypes.POINTER(Job), I, ctypes.POINTER(P)]),
             "arwen_sht_batch_destroy": (I, [P]),
             "arwen_sht_batch_run": (I, [P, P]),

@@ -33,8 +33,8 @@ def _matrix(t, a, shape, name):
     return m
 
 
-def _solve(t: SphericalHarmonicTransform, divergence, theta, log_surface_pressure, b_matrix,
-           c_matrix, m_matrix, inverse, k2, s_bx, s_md, s_c, s_cd, s_cn):
+def _columns(t: SphericalHarmonicTransform, divergence, theta, log_surface_pressure, b_matrix,
+             c_matrix):
     d = t._spectral_tensor(divergence)
     th = t._spectral_tensor(theta)
     ps = t._spectral_tensor(log_surface_pressure)
@@ -42,13 +42,21 @@ def _solve(t: SphericalHarmonicTransform, divergence, theta, log_surface_pressur
         raise ValueError(
             "divergence and theta must be (nlev, T+1, T+1) and ln ps (T+1, T+1), got "
             f"{tuple(d.shape)}, {tuple(th.shape)}, {tuple(ps.shape)}")
-    L, T1 = int(d.shape[0]), t.truncation + 1
+    L = int(d.shape[0])
     b = _matrix(t, b_matrix, (L, L + 1), "b_matrix")
     c = _matrix(t, c_matrix, (L + 1, L), "c_matrix")
+    outs = (t._empty_spectral((L,)), t._empty_spectral((L,)), t._empty_spectral(()))
+    return d, th, ps, L, b, c, outs
+
+
+def _solve(t: SphericalHarmonicTransform, divergence, theta, log_surface_pressure, b_matrix,
+           c_matrix, m_matrix, inverse, k2, s_bx, s_md, s_c, s_cd, s_cn):
+    d, th, ps, L, b, c, (d_out, th_out, ps_out) = _columns(
+        t, divergence, theta, log_surface_pressure, b_matrix, c_matrix)
+    T1 = t.truncation + 1
     m = _matrix(t, m_matrix, (L, L), "structure") if m_matrix is not None else None
     h = _matrix(t, inverse, (T1, L, L), "inverse")
     k = _matrix(t, k2, (T1,), "k2")
-    d_out, th_out, ps_out = t._empty_spectral((L,)), t._empty_spectral((L,)), t._empty_spectral(())
     _native.check(t._lib.arwen_sht_vertical_solve(
         t._plan, L, _vp(d), _vp(th), _vp(ps), _vp(b), _vp(c),
         _vp(m) if m is not None else ctypes.c_void_p(0), _vp(h), _vp(k), float(s_bx), float(s_md),
@@ -81,4 +89,41 @@ def vertical_apply(transform: SphericalHarmonicTransform, divergence, theta, log
                   structure, inverse, k2, dt_w, -a * (1.0 - a) * dt_w * dt_w, dt_w, 1.0 - a, a)
 
 
-__all__ = ["vertical_apply", "vertical_solve_shifted", "wave_operator"]
+def vertical_linear_tendencies(transform: SphericalHarmonicTransform, divergence, theta,
+                               log_surface_pressure, b_matrix, c_matrix, k2, weight: float):
+    """``w L y``: the divergence, theta and ln ps tendencies of the linear
+    operator, each projected (reference semi_implicit.py:796-813)."""
+    t = transform
+    d, th, ps, L, b, c, (d_out, th_out, ps_out) = _columns(
+        t, divergence, theta, log_surface_pressure, b_matrix, c_matrix)
+    k = _matrix(t, k2, (t.truncation + 1,), "k2")
+    _native.check(t._lib.arwen_sht_vertical_tendencies(
+        t._plan, L, _vp(d), _vp(th), _vp(ps), _vp(b), _vp(c), _vp(k), float(weight), _vp(d_out),
+        _vp(th_out), _vp(ps_out), t.backend.stream()))
+    return d_out, th_out, ps_out
+
+
+def vertical_half_apply(transform: SphericalHarmonicTransform, divergence, theta,
+                        log_surface_pressure, b_matrix, c_matrix, p_d, q_d, p_x, q_x, k2):
+    """The square root H = P + Q L of the off-centred Crank-Nicolson map
+    (reference semi_implicit.py:845-878): d = P_D D + Q_D (k^2 B x),
+    x' = P_x x + Q_x (C D) with the operator's ``symmetric_half_maps``;
+    degree zero pinned and projected.  Applied twice it is
+    :func:`vertical_apply`."""
+    t = transform
+    d, th, ps, L, b, c, (d_out, th_out, ps_out) = _columns(
+        t, divergence, theta, log_surface_pressure, b_matrix, c_matrix)
+    T1 = t.truncation + 1
+    pd = _matrix(t, p_d, (T1, L, L), "p_d")
+    qd = _matrix(t, q_d, (T1, L, L), "q_d")
+    px = _matrix(t, p_x, (T1, L + 1, L + 1), "p_x")
+    qx = _matrix(t, q_x, (T1, L + 1, L + 1), "q_x")
+    k = _matrix(t, k2, (T1,), "k2")
+    _native.check(t._lib.arwen_sht_vertical_half_apply(
+        t._plan, L, _vp(d), _vp(th), _vp(ps), _vp(b), _vp(c), _vp(pd), _vp(qd), _vp(px), _vp(qx),
+        _vp(k), _vp(d_out), _vp(th_out), _vp(ps_out), t.backend.stream()))
+    return d_out, th_out, ps_out
+
+
+__all__ = ["vertical_apply", "vertical_half_apply", "vertical_linear_tendencies",
+           "vertical_solve_shifted", "wave_operator"]

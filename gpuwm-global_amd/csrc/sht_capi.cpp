@@ -1233,7 +1233,71 @@ int arwen_sht_vertical_solve(arwen_sht_plan* P, int nlev, const void* div, const
     a.s_cn = s_cn;
     a.T = P->T;
     a.nlev = nlev;
+    a.mode = 0;
     hip_check(sht_launch_vertical_solve(&a, static_cast<hipStream_t>(s)), "vertical solve launch");
+  });
+}
+
+namespace {
+void vertical_common(arwen_sht_plan* P, int nlev, const void* div, const void* theta,
+                     const void* lnps, const float* b, const float* c, const float* k2,
+                     void* div_out, void* theta_out, void* lnps_out, sht::VerticalArgs& a) {
+  if (!P) fail(ARWEN_SHT_EINVAL, "plan is NULL");
+  if (nlev < 1 || nlev > sht::kMaxLevels)
+    fail(ARWEN_SHT_EINVAL, "nlev must be in 1.." + std::to_string(sht::kMaxLevels) + ", got " +
+                               std::to_string(nlev));
+  if (!div || !theta || !lnps || !div_out || !theta_out || !lnps_out)
+    fail(ARWEN_SHT_EINVAL, "a spectral input or output is NULL");
+  if (!b || !c || !k2) fail(ARWEN_SHT_EINVAL, "b_matrix, c_matrix and k2 are required");
+  int dev;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev != P->device) fail(ARWEN_SHT_EINVAL, "plan used on a device other than its own");
+  a.div = static_cast<const float2*>(div);
+  a.theta = static_cast<const float2*>(theta);
+  a.lnps = static_cast<const float2*>(lnps);
+  a.b_matrix = b;
+  a.c_matrix = c;
+  a.k2 = k2;
+  a.d_out = static_cast<float2*>(div_out);
+  a.theta_out = static_cast<float2*>(theta_out);
+  a.lnps_out = static_cast<float2*>(lnps_out);
+  a.T = P->T;
+  a.nlev = nlev;
+}
+}  // namespace
+
+int arwen_sht_vertical_tendencies(arwen_sht_plan* P, int nlev, const void* div, const void* theta,
+                                  const void* lnps, const float* b_matrix, const float* c_matrix,
+                                  const float* k2, float weight, void* div_out, void* theta_out,
+                                  void* lnps_out, void* s) {
+  return guarded([&] {
+    sht::VerticalArgs a{};
+    vertical_common(P, nlev, div, theta, lnps, b_matrix, c_matrix, k2, div_out, theta_out,
+                    lnps_out, a);
+    a.s_bx = weight;
+    a.s_c = weight;
+    a.mode = 1;
+    hip_check(sht_launch_vertical_solve(&a, static_cast<hipStream_t>(s)),
+              "vertical tendencies launch");
+  });
+}
+
+int arwen_sht_vertical_half_apply(arwen_sht_plan* P, int nlev, const void* div, const void* theta,
+                                  const void* lnps, const float* b_matrix, const float* c_matrix,
+                                  const float* p_d, const float* q_d, const float* p_x,
+                                  const float* q_x, const float* k2, void* div_out,
+                                  void* theta_out, void* lnps_out, void* s) {
+  return guarded([&] {
+    sht::VerticalArgs a{};
+    vertical_common(P, nlev, div, theta, lnps, b_matrix, c_matrix, k2, div_out, theta_out,
+                    lnps_out, a);
+    if (!p_d || !q_d || !p_x || !q_x) fail(ARWEN_SHT_EINVAL, "the four half maps are required");
+    a.p_d = p_d;
+    a.q_d = q_d;
+    a.p_x = p_x;
+    a.q_x = q_x;
+    a.mode = 2;
+    hip_check(sht_launch_vertical_solve(&a, static_cast<hipStream_t>(s)), "half map launch");
   });
 }
 

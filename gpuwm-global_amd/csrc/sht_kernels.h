@@ -152,13 +152,17 @@ struct VerticalArgs {
   const float* b_matrix;  // (L, L+1)
   const float* c_matrix;  // (L+1, L)
   const float* m_matrix;  // (L, L), NULL when s_md = 0
-  const float* inverse;   // (T+1, L, L)
+  const float* inverse;   // (T+1, L, L), mode 0
+  const float* p_d;       // mode 2: (T+1, L, L) P_D, Q_D and (T+1, L+1, L+1) P_x, Q_x
+  const float* q_d;
+  const float* p_x;
+  const float* q_x;
   const float* k2;        // (T+1)
   float2* d_out;
   float2* theta_out;
   float2* lnps_out;
   float s_bx, s_md, s_c, s_cd, s_cn;
-  int T, nlev;
+  int T, nlev, mode;      // 0 solve/apply, 1 linear tendencies, 2 half map
 };
 
 // ---- grouped GEMM (sht_gemm.hip) ----

@@ -199,6 +199,23 @@ int arwen_sht_vertical_solve(arwen_sht_plan* plan, int nlev, const void* div, co
                              float s_bx, float s_md, float s_c, float s_cd, float s_cn,
                              void* div_out, void* theta_out, void* lnps_out, void* stream);
 
+/* linear_tendencies (semi_implicit.py:796-813): div_out = weight k2 B x,
+ * (theta_out, lnps_out) = weight C D, each projected. */
+int arwen_sht_vertical_tendencies(arwen_sht_plan* plan, int nlev, const void* div,
+                                  const void* theta, const void* lnps, const float* b_matrix,
+                                  const float* c_matrix, const float* k2, float weight,
+                                  void* div_out, void* theta_out, void* lnps_out, void* stream);
+/* half_apply (semi_implicit.py:845-878): the square root of the off-centred
+ * Crank-Nicolson map from the operator's symmetric_half_maps, per degree
+ * p_d, q_d (T+1, nlev, nlev) and p_x, q_x (T+1, nlev+1, nlev+1):
+ *   d = P_D D + Q_D (k2 B x);  x' = P_x x + Q_x (C D);
+ * degree zero pinned and projected as in _finish. */
+int arwen_sht_vertical_half_apply(arwen_sht_plan* plan, int nlev, const void* div,
+                                  const void* theta, const void* lnps, const float* b_matrix,
+                                  const float* c_matrix, const float* p_d, const float* q_d,
+                                  const float* p_x, const float* q_x, const float* k2,
+                                  void* div_out, void* theta_out, void* lnps_out, void* stream);
+
 /* A batch: an ordered list of jobs executed as
  *   pack -> ONE grouped synthesis GEMM -> inverse FFTs        (synthesis jobs)
  *   FFTs -> ONE grouped analysis GEMM -> unpack               (analysis jobs)

@@ -65,3 +65,60 @@ def apply(o, divergence, theta, lnps, b, c, m, inverse, k2, dt, alpha):
     d_new = np.einsum("nij,jnm->inm", inverse, forcing)
     x_new = x + dt * np.einsum("ij,jnm->inm", c, (1.0 - alpha) * d + alpha * d_new)
     return _finish(o, d_new, x_new, d, x)
+
+
+def x_modes(b, c):
+    """The (theta, ln ps) block C B's modes (semi_implicit.py:517-537): real
+    eigenvalues, values at 1e-9 of the largest set to zero."""
+    values, vectors = np.linalg.eig(c @ b)
+    values = values.real
+    values[np.abs(values) <= 1.0e-9 * np.max(np.abs(values))] = 0.0
+    return values, vectors.real
+
+
+def _half_map_scalars(frequencies, dt, alpha):
+    """semi_implicit.py:610-627: sqrt(g) = p + i q omega for the off-centred
+    Crank-Nicolson factor g of every mode frequency; omega = 0 gives p = 1,
+    q = dt / 2."""
+    y = frequencies * float(dt)
+    g = (1.0 + 1j * (1.0 - alpha) * y) / (1.0 - 1j * alpha * y)
+    root = np.sqrt(g)
+    p = root.real
+    q = np.where(frequencies > 0.0,
+                 root.imag / np.where(frequencies > 0.0, frequencies, 1.0), 0.5 * float(dt))
+    return p, q
+
+
+def symmetric_half_maps(modal, xmodal, k2, dt, alpha):
+    """semi_implicit.py:629-671: (P_D, Q_D, P_x, Q_x) per total degree."""
+    v, c2 = modal
+    xv, xw = xmodal
+    k = np.sqrt(np.asarray(k2, dtype=np.float64))
+    p_d, q_d = _half_map_scalars(k[:, None] * np.sqrt(c2)[None, :], dt, alpha)
+    p_x, q_x = _half_map_scalars(k[:, None] * np.sqrt(-xv)[None, :], dt, alpha)
+    vi, wi = np.linalg.inv(v), np.linalg.inv(xw)
+    return (np.einsum("ij,nj,jk->nik", v, p_d, vi), np.einsum("ij,nj,jk->nik", v, q_d, vi),
+            np.einsum("ij,nj,jk->nik", xw, p_x, wi), np.einsum("ij,nj,jk->nik", xw, q_x, wi))
+
+
+def half_apply(o, divergence, theta, lnps, b, c, maps, k2):
+    """semi_implicit.py:863-878 in float64."""
+    p_d, q_d, p_x, q_x = maps
+    d = np.asarray(divergence, dtype=np.complex128)
+    x = np.concatenate([np.asarray(theta, dtype=np.complex128),
+                        np.asarray(lnps, dtype=np.complex128)[None]], axis=0)
+    l_d = np.asarray(k2)[None, :, None] * np.einsum("ij,jnm->inm", b, x)
+    l_x = np.einsum("ij,jnm->inm", c, d)
+    d_new = np.einsum("nij,jnm->inm", p_d, d) + np.einsum("nij,jnm->inm", q_d, l_d)
+    x_new = np.einsum("nij,jnm->inm", p_x, x) + np.einsum("nij,jnm->inm", q_x, l_x)
+    return _finish(o, d_new, x_new, d, x)
+
+
+def linear_tendencies(o, divergence, theta, lnps, b, c, k2, w):
+    """semi_implicit.py:796-813 in float64: w L y, projected."""
+    d = np.asarray(divergence, dtype=np.complex128)
+    x = np.concatenate([np.asarray(theta, dtype=np.complex128),
+                        np.asarray(lnps, dtype=np.complex128)[None]], axis=0)
+    dt_ = w * np.asarray(k2)[None, :, None] * np.einsum("ij,jnm->inm", b, x)
+    xt = w * np.einsum("ij,jnm->inm", c, d)
+    return o.project(dt_), o.project(xt[:-1]), o.project(xt[-1])

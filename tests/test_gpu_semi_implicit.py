@@ -81,3 +81,34 @@ def test_vertical_solve_argument_errors():
     with pytest.raises(ValueError, match="nlev must be in 1..64"):
         vertical_solve_shifted(t, zb, zb, z[0], np.zeros((big, big + 1)), np.zeros((big + 1, big)),
                                np.zeros((22, big, big)), wave_operator(t), 100.0)
+
+
+@pytest.mark.parametrize("trunc,nlev,dealias", [(21, 5, 1.5), (63, 40, 1.0), (63, 64, 1.0)])
+def test_half_map_and_tendencies_match_the_oracle(trunc, nlev, dealias):
+    from arwen_spectral.semi_implicit import (vertical_half_apply, vertical_linear_tendencies,
+                                              wave_operator)
+
+    o = OracleTransform.create(trunc, dealias_factor=dealias)
+    t = hip_transform(o, "float32")
+    b, c, m, modal = osi.synthetic_operator(nlev, seed=nlev + 1)
+    k2 = wave_operator(t)
+    f32 = lambda a: np.asarray(a, dtype=np.float32)  # noqa: E731
+    c64 = lambda a: np.asarray(a).astype(np.complex64)  # noqa: E731
+    d = c64(o.project(red_spectrum(trunc, (nlev,), seed=4, scale=1e-5)))
+    th = c64(o.project(red_spectrum(trunc, (nlev,), seed=5, scale=10.0)))
+    ps = c64(o.project(red_spectrum(trunc, (), seed=6, scale=1e-2)))
+    maps = [f32(x) for x in osi.symmetric_half_maps(modal, osi.x_modes(b, c), k2, 1500.0, 0.6)]
+    want = osi.half_apply(o, d, th, ps, f32(b), f32(c), maps, f32(k2))
+    got = vertical_half_apply(t, d, th, ps, f32(b), f32(c), *maps, f32(k2))
+    errs = [_rel(g, w) for g, w in zip(got, want)]
+    print(f"T{trunc} L{nlev} half_apply rel err D {errs[0]:.2e} theta {errs[1]:.2e} "
+          f"lnps {errs[2]:.2e}")
+    assert max(errs) < TOL
+    assert np.array_equal(got[0][:, 0, 0].cpu().numpy(), d[:, 0, 0].real.astype(np.complex64))
+    w = 0.7
+    want = osi.linear_tendencies(o, d, th, ps, f32(b), f32(c), f32(k2), np.float32(w))
+    got = vertical_linear_tendencies(t, d, th, ps, f32(b), f32(c), f32(k2), w)
+    errs = [_rel(g, w_) for g, w_ in zip(got, want)]
+    print(f"T{trunc} L{nlev} linear_tendencies rel err D {errs[0]:.2e} theta {errs[1]:.2e} "
+          f"lnps {errs[2]:.2e}")
+    assert max(errs) < TOL

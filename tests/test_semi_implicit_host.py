@@ -72,3 +72,33 @@ def test_apply_is_the_off_centred_crank_nicolson_step():
     cd = lambda v: np.einsum("ij,jnm->inm", c, v)  # noqa: E731
     assert _residual(d1 - alpha * dt * bx(x1), d + (1 - alpha) * dt * bx(x)) < 1e-12
     assert _residual(x1 - alpha * dt * cd(d1), x + (1 - alpha) * dt * cd(d)) < 1e-12
+
+
+def test_half_map_applied_twice_is_the_crank_nicolson_step():
+    """H^2 = CN (semi_implicit.py:845-855): the symmetric split composes to
+    the off-centred Crank-Nicolson step exactly."""
+    o, b, c, m, modal, k2, d, th, ps = _case(seed=7)
+    dt, alpha = 1500.0, 0.6
+    maps = osi.symmetric_half_maps(modal, osi.x_modes(b, c), k2, dt, alpha)
+    once = osi.half_apply(o, d, th, ps, b, c, maps, k2)
+    twice = osi.half_apply(o, *once, b, c, maps, k2)
+    full = osi.apply(o, d, th, ps, b, c, m, osi.helmholtz_inverse(modal, k2, dt, alpha), k2, dt,
+                     alpha)
+    for got, want in zip(twice, full):
+        assert np.max(np.abs(got - want)) < 1e-11 * np.max(np.abs(want))
+
+
+def test_linear_tendencies_are_the_first_order_term_of_the_step():
+    """(apply_dt(y) - y) / dt -> L y as dt -> 0, with error O(dt)."""
+    o, b, c, m, modal, k2, d, th, ps = _case(seed=8)
+    lt = osi.linear_tendencies(o, d, th, ps, b, c, k2, 1.0)
+    errs = []
+    for dt in (1e-2, 1e-3):
+        step = osi.apply(o, d, th, ps, b, c, m, osi.helmholtz_inverse(modal, k2, dt, 0.5), k2,
+                         dt, 0.5)
+        diffs = [(s - y) / dt for s, y in zip(step, (o.project(d), o.project(th), o.project(ps)))]
+        tri = np.tri(11, dtype=bool)
+        tri[0, 0] = False
+        errs.append(max(float(np.max(np.abs((g - w)[..., tri])) / np.max(np.abs(w[..., tri])))
+                        for g, w in zip(diffs, lt)))
+    assert errs[0] < 1e-3 and errs[1] < errs[0] / 5
