@@ -235,23 +235,26 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   }
   upload(P->d_tw, tw);
   P->fft.tw = P->d_tw.as<float2>();
-  // Wavefronts (folded rows) per FFT block: as many as fit the LDS budget,
-  // at most 80 KiB so two blocks share a CU.
+  // Wavefronts (folded rows) per FFT block: as many as fit 80 KiB, so two
+  // blocks share a CU; one row per block up to the device's whole LDS where
+  // a single row needs more (nlon = 3072: 96 KiB for a u,v row pair).
   {
     int lds_max = 65536;
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, P->device);
     const size_t budget = std::min<size_t>((size_t)lds_max, 81920);
+    size_t need = budget;
     auto pick = [&](auto bytes) {
       int rp = 4;
       while (rp > 1 && bytes(rp) > budget) rp /= 2;
-      if (bytes(rp) > budget)
+      if (bytes(rp) > (size_t)lds_max)
         fail(ARWEN_SHT_EINVAL, "nlon=" + std::to_string(P->nlon) + " needs more LDS than the device has");
+      need = std::max(need, bytes(rp));
       return rp;
     };
     P->rp_fold = pick([&](int rp) { return sht_fold_lds_bytes(P->nlon, T, 0, rp); });
     P->rp_fold_vec = pick([&](int rp) { return sht_fold_lds_bytes(P->nlon, T, 1, rp); });
     P->rp_unfold = pick([&](int rp) { return sht_unfold_lds_bytes(P->nlon, T, rp); });
-    hip_check(sht_fft_allow_lds(budget, (size_t)lds_max), "FFT LDS attribute");
+    hip_check(sht_fft_allow_lds(need, (size_t)lds_max), "FFT LDS attribute");
     // wave-resident FFTs (N = 64 L) when their LDS fits the device
     const int L = sht_fft_wave_L(P->nlon);
     P->fft.wave_L = (L && sht_fft_wave_lds_bytes(P->nlon, 1) <= (size_t)lds_max) ? L : 0;

@@ -182,7 +182,8 @@ struct GemmStep {
 };
 
 constexpr int kGemmBM = 64;      // GEMM block tile rows
-constexpr int kGemmMaxSteps = 160;  // steps per panel incl. the two null steps (LDS-resident)
+constexpr int kGemmMaxSteps = 264;  // steps per panel incl. the two null steps (LDS-resident):
+                                     // two segments of JHP/32 chunks x 4 orders, JHP <= 992
 
 struct GemmPanel {
   unsigned long long c[kMT];  // SYN: c[0] = waist block + r0 rows; ANA: packed block of order z + r0

@@ -237,3 +237,34 @@ def test_t767_roundtrips_on_the_longest_rows(precision):
     err = max(rel_err(z2, zeta), rel_err(d2, div))
     _record(f"vector_roundtrip767_{precision}", err)
     assert err < 1e-4
+
+
+def test_t767_on_the_native_1536_grid():
+    """The reference's verification configuration: T767 on the 1536 x 3072
+    native grid (reference tests/test_global_spectral_transform.py:312-339):
+    cos(5 lon) cos(3 lat) analyses to order 5 only.  nlon = 3072 takes the
+    block Stockham FFT, and the u,v analysis's panels carry the longest step
+    lists (two segments of 24 k-chunks).  Round trips against the inputs, as
+    in the T767 test above."""
+    from arwen_spectral import GaussianGrid, SphericalHarmonicTransform, VorticityDivergenceOperator
+
+    native = GaussianGrid.for_shape(1536, 3072)
+    tr = SphericalHarmonicTransform.create(767, grid=native, backend="hip", precision="bf16x6")
+    assert tr.grid.shape == (1536, 3072)
+    g = tr.grid
+    field = np.cos(3.0 * g.lat_rad)[:, None] * np.cos(5.0 * g.lon_rad)[None, :]
+    energy = np.abs(tr.forward(field).cpu().numpy()) ** 2
+    assert energy[:, 5].sum() > 0.999 * energy.sum()
+    coeff = red_spectrum(767, (2,), seed=17, scale=1.0)
+    err = rel_err(tr.forward(tr.inverse(coeff)), coeff)
+    _record("roundtrip767_native_bf16x6", err)
+    assert err < 2 * TOL["bf16x6"]
+    op = VorticityDivergenceOperator(tr)
+    zeta = red_spectrum(767, (2,), seed=18, scale=1e-5)
+    div = red_spectrum(767, (2,), seed=19, scale=1e-5)
+    zeta[..., 0, 0] = 0.0
+    div[..., 0, 0] = 0.0
+    z2, d2 = op.vordiv_from_wind(*op.wind_from_vordiv(zeta, div))
+    err = max(rel_err(z2, zeta), rel_err(d2, div))
+    _record("vector_roundtrip767_native_bf16x6", err)
+    assert err < 1e-4
