@@ -64,13 +64,15 @@ __global__ void __launch_bounds__(256) vertical_kernel(VerticalArgs a) {
   float* sB = smem;           // L x L1
   float* sC = sB + L * L1;    // L1 x L
   float* sE = sC + L1 * L;    // mode-dependent matrices
-  float* sM = sE;             // L x L
-  float* sH = sM + L * L;     // L x L
+  float* sM = sE;             // L x L (mode 0 with M only)
+  float* sH = a.m_matrix ? sM + L * L : sM;  // L x L
   float* sPD = sE;            // L x L
   float* sQD = sPD + L * L;   // L x L
   float* sPX = sQD + L * L;   // L1 x L1
   float* sQX = sPX + L1 * L1; // L1 x L1
-  const int mat = a.mode == 0 ? 2 * L * L : a.mode == 2 ? 2 * L * L + 2 * L1 * L1 : 0;
+  const int mat = a.mode == 0   ? (a.m_matrix ? 2 : 1) * L * L
+                  : a.mode == 2 ? 2 * L * L + 2 * L1 * L1
+                                : 0;
   float2* sD = reinterpret_cast<float2*>(sE + mat);  // [level][col]
   float2* sX = sD + L * COLS;                         // [row][col]
   float2* sF = sX + L1 * COLS;                        // mode 0: forcing, then d; mode 2: l_D
@@ -211,9 +213,9 @@ __global__ void __launch_bounds__(256) vertical_kernel(VerticalArgs a) {
 // matrices would not leave LDS for 32 columns at 64 levels.
 static int vertical_cols(int mode) { return mode == 2 ? 16 : 32; }
 
-static size_t vertical_lds_bytes(int nlev, int mode) {
+static size_t vertical_lds_bytes(int nlev, int mode, bool with_m) {
   const size_t L = nlev, L1 = nlev + 1, cols = vertical_cols(mode);
-  const size_t mat = mode == 0 ? 2 * L * L : mode == 2 ? 2 * L * L + 2 * L1 * L1 : 0;
+  const size_t mat = mode == 0 ? (with_m ? 2 : 1) * L * L : mode == 2 ? 2 * L * L + 2 * L1 * L1 : 0;
   const size_t colsets = mode == 0 ? L + L1 + L : mode == 2 ? L + L1 + L + L1 : L + L1;
   return (2 * L * L1 + mat) * sizeof(float) + colsets * cols * sizeof(float2);
 }
@@ -222,7 +224,7 @@ extern "C" hipError_t sht_launch_vertical_solve(const sht::VerticalArgs* a, hipS
   if (a->nlev < 1 || a->nlev > sht::kMaxLevels || a->mode < 0 || a->mode > 2)
     return hipErrorInvalidValue;
   const int cols = vertical_cols(a->mode);
-  const size_t lds = vertical_lds_bytes(a->nlev, a->mode);
+  const size_t lds = vertical_lds_bytes(a->nlev, a->mode, a->m_matrix != nullptr);
   const void* k = cols == 16 ? reinterpret_cast<const void*>(sht::vertical_kernel<16>)
                              : reinterpret_cast<const void*>(sht::vertical_kernel<32>);
   hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
