@@ -329,6 +329,31 @@ def sampler_bench(args, torch):
         res[name] = {"ms_per_call": ms, "value": F * N / (ms * 1e-3),
                      "sample_max": float(out.abs().max().item())}
         print(f"[bench] sampler {name}: {ms:.3f} ms per call", file=sys.stderr, flush=True)
+    # the other kinds at float64: gradients of the same 40 fields, winds of
+    # 20 vorticity/divergence levels (40 psi/chi columns)
+    out1 = torch.empty((F, N), dtype=torch.float64, device=dev)
+    nb = int(lib.arwen_sht_sample_workspace_bytes(t.plan_handle, F, 2, N))
+    if nb > nbytes:
+        work = torch.empty(nb, dtype=torch.uint8, device=dev)
+        nbytes = nb
+    for kind, name, count, second in ((1, "gradient", F, None), (2, "wind", F // 2, c[F // 2:])):
+        def call_kind():
+            _native.check(lib.arwen_sht_sample(
+                t.plan_handle, vp(c), vp(second) if second is not None else None, count, vp(x),
+                vp(cl), vp(lo), N, kind, 0, vp(work), nbytes, vp(out), vp(out1),
+                t.backend.stream()))
+        call_kind()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.steps):
+            call_kind()
+        e1.record()
+        torch.cuda.synchronize()
+        kms = e0.elapsed_time(e1) / args.steps
+        res[name] = {"ms_per_call": kms, "value": count * N / (kms * 1e-3), "outputs": count}
+        print(f"[bench] sampler {name} (float64): {kms:.3f} ms per call", file=sys.stderr,
+              flush=True)
     ms = res["float64"]["ms_per_call"]
     achieved = flops / (ms * 1e-3) / 1e12
     cpu = None
@@ -354,7 +379,7 @@ def sampler_bench(args, torch):
         "data": "synthetic: seeded red-spectrum coefficients, uniform random points",
         "config": {"workload": f"sample_scalar T{T}, {F} fields, {N} points", "T": T,
                    "fields": F, "points": N},
-        "float32": res["float32"],
+        "float32": res["float32"], "gradient": res["gradient"], "wind": res["wind"],
         "roofline": {"bound": "valu_fp64",
                      "kernel": "sample_pack + sample_kernel<8, 0, false> + sample_reduce",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
