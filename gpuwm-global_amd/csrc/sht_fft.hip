@@ -24,6 +24,8 @@
 // other prime factor, twiddles from a table of e^{-2 pi i k/N} computed in
 // FP64 and rounded once.
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <type_traits>
 #include <utility>
 
@@ -619,6 +621,15 @@ __device__ __forceinline__ void wave_fft64(float2 (&v)[F][L], float2* buf, const
   wave_sync();
 }
 
+// Blocks per CU that a wave-resident launch's LDS allows (2 when a block
+// takes at most half of the 160 KB).  The kernels are compiled for that many
+// (the second launch bound is waves per SIMD: blocks x RB / 4), so that
+// registers do not cap the residency below what LDS allows.
+template <int L, int F, int RB>
+constexpr int wave_blocks_per_cu() {
+  return (64 * L + RB * (F * L * kRowSpan + kSpanPad)) * 8 * 2 <= 160 * 1024 ? 2 : 1;
+}
+
 // Wavefronts (rows) per block of the wave-resident kernels, RB: with 8 a
 // (column, m-tile, parity) run of 8 rows x 4 orders is one whole 128-byte
 // line, so no line of the waist is shared between blocks; 4 where 8 rows'
@@ -631,7 +642,8 @@ __device__ __forceinline__ void wave_fft64(float2 (&v)[F][L], float2* buf, const
 // every (column, m-tile, parity) run of RB rows x 4 orders as one piece.  Blocks are persistent: while a group is transformed, the rows of
 // the block's next group are already being loaded into registers.
 template <int L, int F, int kWaveBlock>
-__global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave_kernel(FoldArgs a) {
+__global__ void __launch_bounds__(64 * kWaveBlock, (F == 1 ? wave_blocks_per_cu<L, 1, kWaveBlock>() : 1) *
+                                        kWaveBlock / 4) rfft_fold_wave_kernel(FoldArgs a) {
   extern __shared__ float2 lds[];
   constexpr int N = 64 * L, SP = wave_span<L>();
   constexpr int NV = 4 * F;  // folded values per order: columns x parities
@@ -727,11 +739,16 @@ __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave_kernel(FoldArg
     }
     __syncthreads();
     // block write: run (e, mt) = rows i0 .. i0+RB-1 x orders 4 mt .. 4 mt + 3,
-    // one contiguous piece; consecutive lanes take its rows (16 bytes each)
+    // one contiguous piece; consecutive lanes take its rows (16 bytes each).
+    // e = rn / MT is floor((rn + 1/2) * (1/MT)) in fp32, exact for rn < 2^12
+    // and MT <= 256: the product is within 2^-11 of (rn + 1/2) / MT, which is
+    // at least 1 / (2 MT) >= 2^-9 from the nearest integer.
     const int ncolf = 2 * F;  // columns per field or level
+    const float inv_mt = 1.0f / (float)MT;
+    static_assert(NV * (N / 8 + 1) < 4096 && N / 8 + 1 <= 256, "range of the fp32 quotient");
     for (int u = threadIdx.x; u < NV * MT * kWaveBlock; u += blockDim.x) {
       const int ww = u % kWaveBlock, rn = u / kWaveBlock;
-      const int e = rn / MT, mt = rn % MT;
+      const int e = (int)(((float)rn + 0.5f) * inv_mt), mt = rn - e * MT;
       const int c = a.job.col0 + ncolf * fld + (e >> 1), p = e & 1;
       *reinterpret_cast<f4*>(a.job.waist0 + waist_index(c, a.job.ldw, a.JHP, mt * kMT, p, i0 + ww)) =
           *reinterpret_cast<const f4*>(reinterpret_cast<const float*>(lds + N + (size_t)ww * WS) +
@@ -767,22 +784,31 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
   int g = blockIdx.x;
   if (g >= ngroups) return;  // block-uniform
   // a thread's share of one group's runs: stage[e][mt][z], e = 2 * column +
-  // parity; consecutive lanes take one run's rows (16 bytes each)
+  // parity; consecutive lanes take one run's rows (16 bytes each).  Where a
+  // run sits relative to the group's first row, in the waist and in LDS,
+  // is the same for every group, so it is worked out once:
+  //   waist  W + c ldw + 4 i0 + roff[r],  roff = (e >> 1) ldw + 4 ((2 mt + (e & 1)) JHP + ww)
+  //   LDS    span ww, float 4 (e MT + mt)
+  // (the waist layout W[c][m/4][p][i][m%4]); rvalid bit r marks r < 4 MT.
   constexpr int RU = (4 * (N / 8 + 1) + 63) / 64;  // 4 MT <= N/2 + 4 runs
   f4 run[RU];
+  unsigned roff[RU], loff[RU], rvalid = 0;
+#pragma unroll
+  for (int r = 0; r < RU; ++r) {
+    const int u = threadIdx.x + 64 * kWaveBlock * r;
+    const int ww = u % kWaveBlock, rn = u / kWaveBlock;
+    const bool ok = rn < 4 * MT;
+    const unsigned e = ok ? (unsigned)(rn / MT) : 0u, mt = ok ? (unsigned)(rn % MT) : 0u;
+    rvalid |= (ok ? 1u : 0u) << r;
+    roff[r] = (e >> 1) * (unsigned)a.job.ldw + 4u * ((2u * mt + (e & 1u)) * (unsigned)a.JHP + ww);
+    loff[r] = (unsigned)(2 * N + 2 * ww * WS) + 4u * (e * (unsigned)MT + mt);
+  }
   auto load_runs = [&](int grp) {
     const int fld = grp / JH4, i0 = (grp % JH4) * kWaveBlock;
-    const int c = a.job.col0 + a.job.fstride * fld;
+    const float* base = W + (size_t)(a.job.col0 + a.job.fstride * fld) * a.job.ldw + 4 * i0;
 #pragma unroll
-    for (int r = 0; r < RU; ++r) {
-      const int u = threadIdx.x + 64 * kWaveBlock * r;
-      const int ww = u % kWaveBlock, rn = u / kWaveBlock;
-      if (rn < 4 * MT) {
-        const int e = rn / MT, mt = rn % MT;
-        run[r] = *reinterpret_cast<const f4*>(
-            W + waist_index(c + (e >> 1), a.job.ldw, a.JHP, mt * kMT, e & 1, i0 + ww));
-      }
-    }
+    for (int r = 0; r < RU; ++r)
+      if (rvalid & (1u << r)) run[r] = *reinterpret_cast<const f4*>(base + roff[r]);
   };
   load_runs(g);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
@@ -790,15 +816,8 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
     const int fld = g / JH4, i0 = (g % JH4) * kWaveBlock, i = i0 + w;
     const bool live = i < a.JH;
 #pragma unroll
-    for (int r = 0; r < RU; ++r) {
-      const int u = threadIdx.x + 64 * kWaveBlock * r;
-      const int ww = u % kWaveBlock, rn = u / kWaveBlock;
-      if (rn < 4 * MT) {
-        const int e = rn / MT, mt = rn % MT;
-        *reinterpret_cast<f4*>(reinterpret_cast<float*>(lds + N + (size_t)ww * WS) +
-                               ((size_t)e * MT + mt) * 4) = run[r];
-      }
-    }
+    for (int r = 0; r < RU; ++r)
+      if (rvalid & (1u << r)) *reinterpret_cast<f4*>(reinterpret_cast<float*>(lds) + loff[r]) = run[r];
     __syncthreads();  // spans (and, the first time, the twiddles) are in LDS
     if (g + (int)gridDim.x < ngroups) load_runs(g + gridDim.x);
     if (live) {
@@ -863,19 +882,48 @@ int wave_L(int N) {
     if (L == l) return L;
   return 0;
 }
-// Grid of a persistent wave-resident launch: as many blocks as the device's
-// LDS holds at once (at most 8 per CU), capped by the number of groups.
-int persistent_blocks(int groups, size_t lds_bytes) {
-  int dev = 0, cus = 0, lds_cu = 0;
+// Grid of a persistent wave-resident launch: as many blocks as fit on the
+// device at once (registers and LDS, from the runtime's occupancy query, at
+// most 8 per CU), capped by the number of groups.  A grid larger than what is
+// resident runs its surplus blocks after the first ones finish, which leaves
+// the CUs half idle at the end.  The query is cached per (kernel, LDS size).
+int persistent_blocks(const void* kernel, int threads, int groups, size_t lds_bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, size_t>, int> cache;
+  int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
-          hipSuccess || cus <= 0 || lds_cu <= 0) {
+      cus <= 0)
     cus = 256;
-    lds_cu = 160 * 1024;
+  int per_cu = 0;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_pair(kernel, lds_bytes);
+    const auto it = cache.find(key);
+    if (it != cache.end()) {
+      per_cu = it->second;
+    } else {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds_bytes) !=
+              hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = std::max(1, (int)((size_t)(160 * 1024) / std::max<size_t>(lds_bytes, 1)));
+      }
+      per_cu = std::min(per_cu, 8);
+      cache.emplace(key, per_cu);
+    }
   }
-  const int per_cu = std::max(1, std::min(8, (int)((size_t)lds_cu / std::max<size_t>(lds_bytes, 1))));
   return std::max(1, std::min(groups, cus * per_cu));
+}
+
+// One persistent launch of a wave-resident kernel (rb rows per block).
+template <class Args>
+hipError_t launch_persistent(void (*kernel)(Args), int groups, int rb, size_t lds, hipStream_t s,
+                             const Args& a) {
+  const void* k = reinterpret_cast<const void*>(kernel);
+  const dim3 g(persistent_blocks(k, 64 * rb, groups, lds)), b(64 * rb);
+  Args copy = a;
+  void* args[] = {&copy};
+  return hipLaunchKernel(k, g, b, args, lds, s);
 }
 
 size_t fold_wave_lds_bytes(int N, int vector, int rb) {
@@ -1122,19 +1170,16 @@ extern "C" hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s
     const int groups = a->job.nfields * (a->JHP / rb);
     if (groups == 0) return hipSuccess;
     const size_t lds = sht::fold_wave_lds_bytes(a->fft.N, a->job.vector, rb);
-    const dim3 g(sht::persistent_blocks(groups, lds)), b(64 * rb);
-#define SHT_FOLD(LL)                                                                      \
-  if (a->job.vector && rb == 8)                                                           \
-    hipLaunchKernelGGL((sht::rfft_fold_wave_kernel<LL, 2, 8>), g, b, lds, s, *a);         \
-  else if (a->job.vector)                                                                 \
-    hipLaunchKernelGGL((sht::rfft_fold_wave_kernel<LL, 2, 4>), g, b, lds, s, *a);         \
-  else if (rb == 8)                                                                       \
-    hipLaunchKernelGGL((sht::rfft_fold_wave_kernel<LL, 1, 8>), g, b, lds, s, *a);         \
-  else                                                                                    \
-    hipLaunchKernelGGL((sht::rfft_fold_wave_kernel<LL, 1, 4>), g, b, lds, s, *a)
+    hipError_t e = hipSuccess;
+#define SHT_FOLD(LL)                                                                          \
+  e = sht::launch_persistent(a->job.vector ? (rb == 8 ? sht::rfft_fold_wave_kernel<LL, 2, 8>   \
+                                                      : sht::rfft_fold_wave_kernel<LL, 2, 4>)  \
+                                           : (rb == 8 ? sht::rfft_fold_wave_kernel<LL, 1, 8>   \
+                                                      : sht::rfft_fold_wave_kernel<LL, 1, 4>), \
+                             groups, rb, lds, s, *a)
     SHT_L_DISPATCH(L, SHT_FOLD)
 #undef SHT_FOLD
-    return hipGetLastError();
+    return e;
   }
   size_t lds = sht::fold_lds_bytes(a->fft.N, a->T, a->job.vector, a->rp);
   int blocks = a->nchunks * a->job.nfields;
@@ -1150,15 +1195,14 @@ extern "C" hipError_t sht_launch_irfft_unfold(const sht::UnfoldArgs* a, hipStrea
     const int groups = a->job.nfields * ((a->JH + rb - 1) / rb);
     if (groups == 0) return hipSuccess;
     const size_t lds = sht::unfold_wave_lds_bytes(a->fft.N, rb);
-    const dim3 g(sht::persistent_blocks(groups, lds)), b(64 * rb);
-#define SHT_UNFOLD(LL)                                                                    \
-  if (rb == 8)                                                                            \
-    hipLaunchKernelGGL((sht::irfft_unfold_wave_kernel<LL, 8>), g, b, lds, s, *a);         \
-  else                                                                                    \
-    hipLaunchKernelGGL((sht::irfft_unfold_wave_kernel<LL, 4>), g, b, lds, s, *a)
+    hipError_t e = hipSuccess;
+#define SHT_UNFOLD(LL)                                                                       \
+  e = sht::launch_persistent(rb == 8 ? sht::irfft_unfold_wave_kernel<LL, 8>                 \
+                                     : sht::irfft_unfold_wave_kernel<LL, 4>,                \
+                             groups, rb, lds, s, *a)
     SHT_L_DISPATCH(L, SHT_UNFOLD)
 #undef SHT_UNFOLD
-    return hipGetLastError();
+    return e;
   }
   size_t lds = sht::unfold_lds_bytes(a->fft.N, a->T, a->rp);
   int blocks = a->nchunks * a->job.nfields;
