@@ -476,8 +476,10 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--split-acc", type=int, default=None,
-                    help="split modes: 1 = small products in their own accumulators")
+    ap.add_argument("--split-acc", type=int, default=0,
+                    help="split modes: 1 = small products in their own accumulators; 0 (the "
+                         "default) = one accumulator, which admits the 96-column GEMM tile and "
+                         "keeps the FP32 path's error class")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB on the device three times (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")

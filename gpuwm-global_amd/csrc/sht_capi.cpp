@@ -843,7 +843,12 @@ void drop_graph(arwen_sht_batch* B) {
 
 void apply_config(arwen_sht_batch* B, const arwen_sht_gemm_config& c) {
   for (int d = 0; d < 2; ++d) {
-    if (c.cols[d] != 32 && c.cols[d] != 64) fail(ARWEN_SHT_EINVAL, "cols must be 32 or 64");
+    if (c.cols[d] != 32 && c.cols[d] != 64 && c.cols[d] != 96)
+      fail(ARWEN_SHT_EINVAL, "cols must be 32, 64 or 96");
+    // the 96-column tile has no split accumulators; in the split modes it
+    // would change results, so it is offered only where they are off
+    if (c.cols[d] == 96 && c.split_acc && B->precision != ARWEN_SHT_FP32)
+      fail(ARWEN_SHT_EINVAL, "cols 96 needs split_acc = 0 in the split-BF16 modes");
     if (c.grid[d] < 0) fail(ARWEN_SHT_EINVAL, "grid must be >= 0");
   }
   if (c.split_acc != 0 && c.split_acc != 1) fail(ARWEN_SHT_EINVAL, "split_acc must be 0 or 1");
@@ -1419,8 +1424,10 @@ int arwen_sht_batch_autotune(arwen_sht_batch* B, void* stream, int reps,
       // Each direction is timed on its own, over every column tile and grid;
       // the operands are the batch's own work buffers (their contents do not
       // change the timing).  The split-accumulator choice is a precision
-      // setting and is kept as configured.
-      const int cols[2] = {32, 64};
+      // setting and is kept as configured; the 96-column tile, which has no
+      // split accumulators, is a candidate only where they are off.
+      const bool wide = B->precision == ARWEN_SHT_FP32 || !B->config.split_acc;
+      const std::vector<int> cols = wide ? std::vector<int>{32, 64, 96} : std::vector<int>{32, 64};
       const int grids[4] = {256 * 4, 256 * 8, 256 * 16, 256 * 32};
       for (int d = 0; d < 2; ++d) {
         if (B->lists[d].ntiles == 0) continue;

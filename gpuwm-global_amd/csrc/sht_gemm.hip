@@ -3,8 +3,8 @@
 //
 // Every Legendre contraction of a batch is a list of work items: a panel (a
 // group of up to kMT = 4 consecutive orders m of one job and one output
-// parity, times one 64-row tile) and a first column (BN = 32 or 64 columns per
-// item).  Per order m the group holds
+// parity, times one 64-row tile) and a first column (BN = 32, 64 or 96
+// columns per item; 96 has no split accumulators).  Per order m the group holds
 //     C_m[r][c] = sum_seg sum_k A_seg,m[r][k] * B_seg,m[k][c]
 // with A a Legendre table block and B a data block; two segments express the
 // vector operators (P*m/cos and dP/dphi against differently arranged data) as
@@ -190,7 +190,9 @@ __device__ __forceinline__ void mfma_step(f32x4& acc, f32x4& acc2, const u32x4 (
 #undef SHT_MFMA_BF
 }
 
-// 32-column tiles fit three workgroups per CU (<= 168 VGPRs), 64-column ones two.
+// 32-column tiles fit three workgroups per CU (<= 168 VGPRs), 64- and
+// 96-column ones two (the 96-column analysis tile's LDS, B tile plus step
+// list, is exactly half of the CU's 160 KB in bf16x6).
 template <int MODE, int DIR, int BN, bool ACC2>
 __global__ void __launch_bounds__(256, BN == 32 ? 3 : 2) legendre_gemm_kernel(GemmWork w) {
   using C = GemmCfg<MODE, DIR, BN>;
@@ -471,7 +473,10 @@ template <int MODE, int DIR>
 hipError_t launch_mode(const sht::GemmWork& w, int bn, int acc2, dim3 g, hipStream_t s) {
   const dim3 b(256);
   constexpr bool kSplit = MODE != sht::kFP32;
-  if (bn == 64) {
+  if (bn == 96) {
+    // no split accumulators: 96 columns leave no registers for them
+    hipLaunchKernelGGL((sht::legendre_gemm_kernel<MODE, DIR, 96, false>), g, b, 0, s, w);
+  } else if (bn == 64) {
     if (kSplit && acc2)
       hipLaunchKernelGGL((sht::legendre_gemm_kernel<MODE, DIR, 64, kSplit>), g, b, 0, s, w);
     else
@@ -500,7 +505,8 @@ hipError_t launch_dir(int mode, const sht::GemmWork& w, int bn, int acc2, dim3 g
 
 extern "C" hipError_t sht_launch_legendre_gemm(const sht::GemmLaunch* L, hipStream_t s) {
   if (L->ntiles <= 0) return hipSuccess;
-  if (L->bn != 32 && L->bn != 64) return hipErrorInvalidValue;
+  if (L->bn != 32 && L->bn != 64 && L->bn != 96) return hipErrorInvalidValue;
+  if (L->bn == 96 && L->acc2 && L->mode != sht::kFP32) return hipErrorInvalidValue;
   const sht::GemmWork w{L->panels, L->steps, L->tiles, L->ntiles};
   const dim3 g(L->grid);
   return L->dir == 0 ? launch_dir<0>(L->mode, w, L->bn, L->acc2, g, s)

@@ -182,8 +182,10 @@ struct GemmStep {
 };
 
 constexpr int kGemmBM = 64;      // GEMM block tile rows
-constexpr int kGemmMaxSteps = 264;  // steps per panel incl. the two null steps (LDS-resident):
-                                     // two segments of JHP/32 chunks x 4 orders, JHP <= 992
+constexpr int kGemmMaxSteps = 256;  // steps per panel incl. the two null steps (LDS-resident):
+                                     // two segments of JHP/32 chunks x 4 orders, JHP <= 992;
+                                     // 8 KB, which with the 72 KB B tile of the 96-column
+                                     // bf16x6 analysis leaves room for two blocks per CU
 
 struct GemmPanel {
   unsigned long long c[kMT];  // SYN: c[0] = waist block + r0 rows; ANA: packed block of order z + r0

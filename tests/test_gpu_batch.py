@@ -6,7 +6,8 @@ it to the single-operation path and to the FP64 oracle:
   * every column tile and grid gives bit-identical results (each output keeps
     its accumulation order), and so does the autotuner's choice;
   * sharing one accumulator in the split modes stays within the mode's
-    tolerance of the oracle;
+    tolerance of the oracle, and there the 96-column tile (which has no split
+    accumulators) is bit-identical to the 32- and 64-column ones;
   * graph replays, with eager runs between them, match the eager result;
   * a batch mixing T21 and T63 jobs matches the single operations bit for bit.
 """
@@ -81,14 +82,16 @@ def test_schedule_configs_and_autotune_are_bit_identical(oracles, T, precision):
     ou, ov = o.wind_from_vordiv(zeta, bufs["div"].cpu().numpy())
     assert max(rel_err(ref["u"], ou), rel_err(ref["v"], ov)) < TOL[precision]
     assert rel_err(ref["grid"], o.inverse(bufs["scal"].cpu().numpy())) < TOL[precision]
-    for cols in ((64, 64), (64, 32), (32, 64)):
+    wide = ((96, 96), (96, 32)) if precision == "float32" else ()
+    for cols in ((64, 64), (64, 32), (32, 64)) + wide:
         for grid in ((0, 0), (64, 1024)):
             batch.set_config(cols=cols, grid=grid)
             _run(batch)
             for k, v in _outputs(bufs).items():
                 assert torch.equal(v, ref[k]), (cols, grid, k)
     tuned = batch.autotune(reps=1)
-    assert tuned["cols"][0] in (32, 64) and tuned["cols"][1] in (32, 64)
+    allowed = (32, 64, 96) if precision == "float32" else (32, 64)
+    assert tuned["cols"][0] in allowed and tuned["cols"][1] in allowed
     _run(batch)
     for k, v in _outputs(bufs).items():
         assert torch.equal(v, ref[k]), ("autotuned", tuned, k)
@@ -96,6 +99,8 @@ def test_schedule_configs_and_autotune_are_bit_identical(oracles, T, precision):
 
 @pytest.mark.parametrize("T", [21, 63])
 def test_shared_accumulator_stays_within_tolerance(oracles, T):
+    import torch
+
     o = oracles[T]
     _, batch, bufs = _round_trip(o, "bf16x6")
     batch.set_config(split_acc=0)
@@ -107,6 +112,18 @@ def test_shared_accumulator_stays_within_tolerance(oracles, T):
     oz, od = o.vordiv_from_wind(bufs["u"].cpu().numpy().astype(np.float64),
                                 bufs["v"].cpu().numpy().astype(np.float64))
     assert max(rel_err(bufs["z2"], oz), rel_err(bufs["d2"], od)) < TOL["bf16x6"]
+    # with one accumulator the 96-column tile joins the others, bit for bit
+    ref = _outputs(bufs)
+    for cols in ((96, 96), (96, 32), (64, 96)):
+        batch.set_config(cols=cols)
+        _run(batch)
+        for k, v in _outputs(bufs).items():
+            assert torch.equal(v, ref[k]), (cols, k)
+    tuned = batch.autotune(reps=1)
+    assert tuned["split_acc"] == 0 and set(tuned["cols"]) <= {32, 64, 96}
+    _run(batch)
+    for k, v in _outputs(bufs).items():
+        assert torch.equal(v, ref[k]), ("autotuned", tuned, k)
 
 
 def test_config_rejects_bad_values(oracles):
@@ -115,6 +132,14 @@ def test_config_rejects_bad_values(oracles):
         with pytest.raises(ValueError):
             batch.set_config(**kw)
     assert batch.config() == {"cols": [32, 32], "grid": [0, 0], "split_acc": 1}
+    # the 96-column tile has no split accumulators: with them on (the default)
+    # a split-mode batch refuses it
+    _, split, _ = _round_trip(oracles[21], "bf16x6", levels=1)
+    with pytest.raises(ValueError, match="split_acc = 0"):
+        split.set_config(cols=(96, 32))
+    split.set_config(split_acc=0)
+    split.set_config(cols=(96, 96))
+    assert split.config()["cols"] == [96, 96]
 
 
 @pytest.mark.parametrize("precision", ["float32", "bf16x6"])
