@@ -26,6 +26,17 @@ namespace sht {
 constexpr int kTile = 32;   // orders per block
 constexpr int kChunk = 64;  // degrees per pass of a block
 
+// Block index of an XCD-aware launch (grid a multiple of 8): workgroups are
+// dispatched round-robin over the 8 XCDs, so launch position b runs on XCD
+// b % 8; consecutive logical blocks (neighbouring degree chunks of one field
+// and order tile, which share the 128-byte lines where a packed run crosses a
+// chunk edge) are given to one XCD, where they run side by side and the
+// shared lines are fetched or written back once through that XCD's L2.
+__device__ __forceinline__ int xcd_block() {
+  const int q = (int)gridDim.x / 8;
+  return (int)(blockIdx.x % 8) * q + (int)(blockIdx.x / 8);
+}
+
 // Degree of order m, parity p, in the chunk starting at n0, for lane j: the
 // j-th degree of that parity in [n0, n0 + 64).
 __device__ __forceinline__ int chunk_degree(int n0, int m, int p, int j) {
@@ -51,9 +62,10 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
   __shared__ int skoff[2 * kTile];
   const int T = a.T, T1 = T + 1;
   const int nchunks = (T1 + kChunk - 1) / kChunk;
-  const int c = blockIdx.x % nchunks;
-  const int mt = (blockIdx.x / nchunks) % a.mtiles;
-  const int f = blockIdx.x / nchunks / a.mtiles;
+  const int b = xcd_block();
+  const int c = b % nchunks;
+  const int mt = (b / nchunks) % a.mtiles;
+  const int f = b / nchunks / a.mtiles;
   if (f >= a.job.nfields) return;
   const int m0 = mt * kTile;
   const int n0 = m0 + kChunk * c;
@@ -132,9 +144,10 @@ __global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs a) {
   __shared__ int skoff[2 * kTile];
   const int T = a.T, T1 = T + 1;
   const int nchunks = (T1 + kChunk - 1) / kChunk;
-  const int c = blockIdx.x % nchunks;
-  const int mt = (blockIdx.x / nchunks) % a.tiles;
-  const int f = blockIdx.x / nchunks / a.tiles;
+  const int b = xcd_block();
+  const int c = b % nchunks;
+  const int mt = (b / nchunks) % a.tiles;
+  const int f = b / nchunks / a.tiles;
   if (f >= a.job.nfields) return;
   const int m0 = mt * kTile;
   const int n0 = kChunk * c;
@@ -223,6 +236,7 @@ __global__ void spectral_elementwise_kernel(const float2* __restrict__ in,
 extern "C" hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s) {
   int blocks = a->job.nfields * a->mtiles * ((a->T + 1 + 63) / 64);
   if (blocks == 0) return hipSuccess;
+  blocks = (blocks + 7) / 8 * 8;  // XCD-aware order (xcd_block)
   if (a->job.mode == sht::kVector)
     hipLaunchKernelGGL(sht::pack_kernel<2>, dim3(blocks), dim3(256), 0, s, *a);
   else
@@ -233,6 +247,7 @@ extern "C" hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s) {
 extern "C" hipError_t sht_launch_unpack(const sht::UnpackArgs* a, hipStream_t s) {
   int blocks = a->job.nfields * a->tiles * ((a->T + 1 + 63) / 64);
   if (blocks == 0) return hipSuccess;
+  blocks = (blocks + 7) / 8 * 8;  // XCD-aware order (xcd_block)
   if (a->job.vector)
     hipLaunchKernelGGL(sht::unpack_kernel<4>, dim3(blocks), dim3(256), 0, s, *a);
   else
