@@ -147,19 +147,25 @@ struct GemmCfg {
 
 // Four fp32 values as `np` bf16 piece vectors (each the round-to-nearest-even
 // bf16 of what the previous pieces leave; the residuals are exact in fp32).
+// Pairs are converted together (v_cvt_pk_bf16_f32); a packed pair's values
+// return to fp32 as its low half shifted up and its high half masked.
 __device__ __forceinline__ void split4(const float (&x)[4], int np, u32x2 (&out)[3]) {
-  float r[4] = {x[0], x[1], x[2], x[3]};
+  typedef __attribute__((ext_vector_type(2))) float f2;
+  typedef __attribute__((ext_vector_type(2))) __bf16 b2;
+  f2 r[2] = {f2{x[0], x[1]}, f2{x[2], x[3]}};
 #pragma unroll
   for (int pc = 0; pc < 3; ++pc) {
     if (pc >= np) break;
-    unsigned short h[4];
+    unsigned w[2];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const __bf16 b = (__bf16)r[e];
-      h[e] = __builtin_bit_cast(unsigned short, b);
-      r[e] -= (float)b;
+    for (int h = 0; h < 2; ++h) {
+      w[h] = __builtin_bit_cast(unsigned, __builtin_convertvector(r[h], b2));
+      if (pc + 1 < np) {
+        r[h][0] -= __uint_as_float(w[h] << 16);
+        r[h][1] -= __uint_as_float(w[h] & 0xFFFF0000u);
+      }
     }
-    out[pc] = u32x2{h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16)};
+    out[pc] = u32x2{w[0], w[1]};
   }
 }
 
