@@ -250,11 +250,12 @@ int arwen_sht_batch_launch(arwen_sht_batch* batch, void* stream);
  * reference's StepProfiler times whole sections, profile.py:87-330). */
 int arwen_sht_batch_profile(arwen_sht_batch* batch, void* stream, int reps, double* ms);
 /* Grouped-GEMM schedule of a batch, per direction (0 synthesis, 1 analysis):
- * column tile (32 or 64 data columns per workgroup) and persistent grid
- * (workgroups, 0 = 8 per CU); for the split precisions, whether the small
- * products keep their own accumulators (1, default) or share the main one
- * (0).  Changing it rebuilds the work lists and drops the captured graph.
- * No reference counterpart (the reference has no kernel schedule). */
+ * column tile (32, 64 or 96 data columns per workgroup; 96 only with
+ * split_acc = 0 in the split precisions) and persistent grid (workgroups,
+ * 0 = 8 per CU); for the split precisions, whether the small products keep
+ * their own accumulators (1, default) or share the main one (0).  Changing
+ * it rebuilds the work lists and drops the captured graph.  No reference
+ * counterpart (the reference has no kernel schedule). */
 typedef struct arwen_sht_gemm_config {
   int cols[2];
   int grid[2];
