@@ -486,7 +486,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the transform "
-              "shards by members or fields with no exchange; see DESIGN.md)", file=sys.stderr)
+              "would shard by members or fields, with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
     import torch
 
