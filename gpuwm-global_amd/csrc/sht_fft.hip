@@ -622,12 +622,17 @@ __device__ __forceinline__ void wave_fft64(float2 (&v)[F][L], float2* buf, const
 }
 
 // Blocks per CU that a wave-resident launch's LDS allows (2 when a block
-// takes at most half of the 160 KB).  The kernels are compiled for that many
-// (the second launch bound is waves per SIMD: blocks x RB / 4), so that
-// registers do not cap the residency below what LDS allows.
+// takes at most half of the 160 KB).  The scalar fold is compiled for that
+// many (its second launch bound, waves per SIMD, is blocks x RB / 4), so that
+// registers do not cap its residency below what LDS allows; the unfold would
+// spill at that budget and the vector fold's LDS allows one block.
 template <int L, int F, int RB>
 constexpr int wave_blocks_per_cu() {
   return (64 * L + RB * (F * L * kRowSpan + kSpanPad)) * 8 * 2 <= 160 * 1024 ? 2 : 1;
+}
+template <int L, int F, int RB>
+constexpr int fold_waves_per_simd() {
+  return (F == 1 ? wave_blocks_per_cu<L, 1, RB>() : 1) * RB / 4;
 }
 
 // Wavefronts (rows) per block of the wave-resident kernels, RB: with 8 a
@@ -642,8 +647,8 @@ constexpr int wave_blocks_per_cu() {
 // every (column, m-tile, parity) run of RB rows x 4 orders as one piece.  Blocks are persistent: while a group is transformed, the rows of
 // the block's next group are already being loaded into registers.
 template <int L, int F, int kWaveBlock>
-__global__ void __launch_bounds__(64 * kWaveBlock, (F == 1 ? wave_blocks_per_cu<L, 1, kWaveBlock>() : 1) *
-                                        kWaveBlock / 4) rfft_fold_wave_kernel(FoldArgs a) {
+__global__ void __launch_bounds__(64 * kWaveBlock, (fold_waves_per_simd<L, F, kWaveBlock>()))
+    rfft_fold_wave_kernel(FoldArgs a) {
   extern __shared__ float2 lds[];
   constexpr int N = 64 * L, SP = wave_span<L>();
   constexpr int NV = 4 * F;  // folded values per order: columns x parities
