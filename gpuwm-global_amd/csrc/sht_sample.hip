@@ -166,32 +166,36 @@ __global__ void __launch_bounds__(256) sample_kernel(
       ar[q][f] = ai[q][f] = dr[q][f] = di[q][f] = Acc(0);
     }
 
+  // The packed triangle is order-major, so the steps (m, n) of the part are
+  // consecutive entries k0 .. k_last: the coefficient and recurrence pointers
+  // advance by one entry per step, and each order runs as its diagonal step,
+  // its first off-diagonal step and a branch-free loop over n = m+2 .. T.
+  // Every step loads the next entry's values while it computes (the last
+  // step reloads its own, so no load passes k_last).
   const long long k0 = tri_index(T, m0, m0);         // (n, m) = (m0, m0)
   const long long k_last = tri_index(T, T, m1 - 1);  // (T, m1 - 1)
+  const Cx* cp = cf + k0 * ldc;
+  const double *pa = rec_a + k0, *pb = rec_b + k0, *pc = rec_c + k0;
+  long long left = k_last - k0;  // steps after the current one
   Cx cur[FT], nxt[FT];
 #pragma unroll
-  for (int f = 0; f < FT; ++f) cur[f] = cf[k0 * ldc + f];
-  double ca = rec_a[k0], cb = rec_b[k0], cc = rec_c[k0];
-  int m = m0, n = m0;
-  for (long long k = k0; k <= k_last; ++k) {
-    const long long kn = k < k_last ? k + 1 : k;  // the next step (the last one reloads itself)
+  for (int f = 0; f < FT; ++f) cur[f] = cp[f];
+  double ca = pa[0], cb = pb[0], cc = pc[0];
+  // one step: P (and dP) of every point from `rule`, into the amplitudes
+  auto step = [&](auto rule) {
+    const long long adv = left > 0 ? 1 : 0;
+    cp += adv * ldc;
+    pa += adv;
+    pb += adv;
+    pc += adv;
+    --left;
 #pragma unroll
-    for (int f = 0; f < FT; ++f) nxt[f] = cf[kn * ldc + f];
-    const double na = rec_a[kn], nb = rec_b[kn], nc = rec_c[kn];
-    const double g = n == m + 1 ? diag_g[m] : 0.0;
+    for (int f = 0; f < FT; ++f) nxt[f] = cp[f];
+    const double na = pa[0], nb = pb[0], nc = pc[0];
 #pragma unroll
     for (int q = 0; q < PT; ++q) {
       double P, dP = 0.0;
-      if (n == m) {
-        P = dg[q];
-        if constexpr (GRAD) dP = (m >= 1) ? rd[q] * ((double)m * x[q] * dg[q]) : 0.0;
-      } else if (n == m + 1) {
-        P = g * x[q] * dg[q];
-        if constexpr (GRAD) dP = rd[q] * ((double)n * x[q] * P - cc * prev[q]);
-      } else {
-        P = ca * (x[q] * prev[q] - cb * prev2[q]);
-        if constexpr (GRAD) dP = rd[q] * ((double)n * x[q] * P - cc * prev[q]);
-      }
+      rule(q, P, dP);
 #pragma unroll
       for (int f = 0; f < FT; ++f) {
         if constexpr (F32) {
@@ -213,41 +217,57 @@ __global__ void __launch_bounds__(256) sample_kernel(
       prev2[q] = prev[q];
       prev[q] = P;
     }
-    if (n == T) {
-      // the order is complete: turn its amplitudes by e^{i m lambda}
-      const double mult = m == 0 ? 1.0 : 2.0;
-#pragma unroll
-      for (int q = 0; q < PT; ++q) {
-        double sm, cm;
-        sincos((double)m * lam[q], &sm, &cm);
-#pragma unroll
-        for (int f = 0; f < FT; ++f) {
-          const double re = (double)ar[q][f], im = (double)ai[q][f];
-          if constexpr (KIND == 0) {
-            o0[q][f] += mult * (re * cm - im * sm);
-          } else {
-            // Re(i m A e^{i m lambda}) and Re(dA e^{i m lambda})
-            o0[q][f] += mult * (-(double)m * (re * sm + im * cm));
-            o1[q][f] += mult * ((double)dr[q][f] * cm - (double)di[q][f] * sm);
-          }
-          ar[q][f] = ai[q][f] = dr[q][f] = di[q][f] = Acc(0);
-        }
-      }
-      ++m;
-      n = m;
-      if (m <= T) {
-        const double fm = diag_f[m];
-#pragma unroll
-        for (int q = 0; q < PT; ++q) dg[q] = fm * root[q] * dg[q];
-      }
-    } else {
-      ++n;
-    }
 #pragma unroll
     for (int f = 0; f < FT; ++f) cur[f] = nxt[f];
     ca = na;
     cb = nb;
     cc = nc;
+  };
+  for (int m = m0; m < m1; ++m) {
+    // n = m
+    step([&](int q, double& P, double& dP) {
+      P = dg[q];
+      if constexpr (GRAD) dP = (m >= 1) ? rd[q] * ((double)m * x[q] * dg[q]) : 0.0;
+    });
+    if (m < T) {
+      // n = m + 1
+      const double g = diag_g[m];
+      const int n = m + 1;
+      step([&](int q, double& P, double& dP) {
+        P = g * x[q] * dg[q];
+        if constexpr (GRAD) dP = rd[q] * ((double)n * x[q] * P - cc * prev[q]);
+      });
+      for (int n2 = m + 2; n2 <= T; ++n2) {
+        step([&](int q, double& P, double& dP) {
+          P = ca * (x[q] * prev[q] - cb * prev2[q]);
+          if constexpr (GRAD) dP = rd[q] * ((double)n2 * x[q] * P - cc * prev[q]);
+        });
+      }
+    }
+    // the order is complete: turn its amplitudes by e^{i m lambda}
+    const double mult = m == 0 ? 1.0 : 2.0;
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+      double sm, cm;
+      sincos((double)m * lam[q], &sm, &cm);
+#pragma unroll
+      for (int f = 0; f < FT; ++f) {
+        const double re = (double)ar[q][f], im = (double)ai[q][f];
+        if constexpr (KIND == 0) {
+          o0[q][f] += mult * (re * cm - im * sm);
+        } else {
+          // Re(i m A e^{i m lambda}) and Re(dA e^{i m lambda})
+          o0[q][f] += mult * (-(double)m * (re * sm + im * cm));
+          o1[q][f] += mult * ((double)dr[q][f] * cm - (double)di[q][f] * sm);
+        }
+        ar[q][f] = ai[q][f] = dr[q][f] = di[q][f] = Acc(0);
+      }
+    }
+    if (m + 1 <= T) {
+      const double fm = diag_f[m + 1];
+#pragma unroll
+      for (int q = 0; q < PT; ++q) dg[q] = fm * root[q] * dg[q];
+    }
   }
 #pragma unroll
   for (int q = 0; q < PT; ++q) {
