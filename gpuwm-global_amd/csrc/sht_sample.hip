@@ -170,28 +170,19 @@ __global__ void __launch_bounds__(256) sample_kernel(
   // consecutive entries k0 .. k_last: the coefficient and recurrence pointers
   // advance by one entry per step, and each order runs as its diagonal step,
   // its first off-diagonal step and a branch-free loop over n = m+2 .. T.
-  // Every step loads the next entry's values while it computes (the last
-  // step reloads its own, so no load passes k_last).
+  // Every step loads the next entry's values once it has used its own (the
+  // last step reloads its own, so no load passes k_last).
   const long long k0 = tri_index(T, m0, m0);         // (n, m) = (m0, m0)
   const long long k_last = tri_index(T, T, m1 - 1);  // (T, m1 - 1)
   const Cx* cp = cf + k0 * ldc;
   const double *pa = rec_a + k0, *pb = rec_b + k0, *pc = rec_c + k0;
   long long left = k_last - k0;  // steps after the current one
-  Cx cur[FT], nxt[FT];
+  Cx cur[FT];
 #pragma unroll
   for (int f = 0; f < FT; ++f) cur[f] = cp[f];
   double ca = pa[0], cb = pb[0], cc = pc[0];
   // one step: P (and dP) of every point from `rule`, into the amplitudes
   auto step = [&](auto rule) {
-    const long long adv = left > 0 ? 1 : 0;
-    cp += adv * ldc;
-    pa += adv;
-    pb += adv;
-    pc += adv;
-    --left;
-#pragma unroll
-    for (int f = 0; f < FT; ++f) nxt[f] = cp[f];
-    const double na = pa[0], nb = pb[0], nc = pc[0];
 #pragma unroll
     for (int q = 0; q < PT; ++q) {
       double P, dP = 0.0;
@@ -217,11 +208,21 @@ __global__ void __launch_bounds__(256) sample_kernel(
       prev2[q] = prev[q];
       prev[q] = P;
     }
+    // the next entry's values go into the same registers once this step has
+    // used them: scalar loads return out of order, so a wave waits for all of
+    // its outstanding ones at once and a second register set would buy no
+    // overlap, only scalar registers (and their spills)
+    const long long adv = left > 0 ? 1 : 0;
+    cp += adv * ldc;
+    pa += adv;
+    pb += adv;
+    pc += adv;
+    --left;
 #pragma unroll
-    for (int f = 0; f < FT; ++f) cur[f] = nxt[f];
-    ca = na;
-    cb = nb;
-    cc = nc;
+    for (int f = 0; f < FT; ++f) cur[f] = cp[f];
+    ca = pa[0];
+    cb = pb[0];
+    cc = pc[0];
   };
   for (int m = m0; m < m1; ++m) {
     // n = m
