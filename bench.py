@@ -78,14 +78,15 @@ def member_fields(truncation, members, seed0=0):
 class RoundTrip:
     """The device buffers and the batch of one round-trip workload."""
 
-    def __init__(self, specs, precision, torch, tune=True, split_acc=None):
+    def __init__(self, specs, precision, torch, tune=True, split_acc=None, fp64_orders=None):
         from arwen_spectral import SphericalHarmonicTransform, TransformBatch
 
         self.parts = []
         jobs = []
         dev = torch.device("cuda", torch.cuda.current_device())
         for spec in specs:
-            tr = SphericalHarmonicTransform.create(spec["T"], dealias_factor=1.0, precision=precision)
+            tr = SphericalHarmonicTransform.create(spec["T"], dealias_factor=1.0, precision=precision,
+                                                   fp64_orders=fp64_orders)
             f = spec["fields"]
             c64 = lambda a: torch.from_numpy(a.astype(np.complex64)).to(dev)
             zeta = c64(f["vorticity"]).contiguous()
@@ -480,6 +481,9 @@ def main():
                     help="split modes: 1 = small products in their own accumulators; 0 (the "
                          "default) = one accumulator, which admits the 96-column GEMM tile and "
                          "keeps the FP32 path's error class")
+    ap.add_argument("--fp64-orders", type=int, default=None,
+                    help="vector operators: zonal orders below this run in FP64 end to end "
+                         "(default: 0 for float32, 4 ceil((T+1)/96) for the split modes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB on the device three times (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -513,7 +517,8 @@ def main():
     print(f"[bench] workload {args.workload}: {nfields} fields, precision {args.precision}",
           file=sys.stderr, flush=True)
 
-    rt = RoundTrip(specs, args.precision, torch, not args.no_autotune, args.split_acc)
+    rt = RoundTrip(specs, args.precision, torch, not args.no_autotune, args.split_acc,
+                   args.fp64_orders)
     sec = time_steps(rt, args.steps, args.warmup, torch)
     phases = rt.batch.profile(reps=3)
     errors = class_errors(rt)

@@ -34,7 +34,8 @@ SPECTRAL_PROJECT, SPECTRAL_LAPLACIAN, SPECTRAL_INVERSE_LAPLACIAN = 0, 1, 2
 #: Every symbol the header declares; the CPU suite checks the library exports them.
 EXPORTS = (
     "arwen_sht_plan_create", "arwen_sht_plan_destroy", "arwen_sht_plan_grid",
-    "arwen_sht_gauss_nodes", "arwen_sht_plan_table_bytes", "arwen_sht_plan_table_f64",
+    "arwen_sht_gauss_nodes", "arwen_sht_plan_table_bytes", "arwen_sht_plan_fp64_orders",
+    "arwen_sht_plan_table_f64",
     "arwen_sht_analysis", "arwen_sht_synthesis", "arwen_sht_zonal_derivative",
     "arwen_sht_meridional_derivative", "arwen_sht_gradient", "arwen_sht_uv_from_vordiv",
     "arwen_sht_vordiv_from_uv", "arwen_sht_synthesis_to_waist", "arwen_sht_analysis_from_waist",
@@ -58,6 +59,7 @@ class PlanDesc(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("sin_lat", ctypes.POINTER(ctypes.c_double)),
         ("weights", ctypes.POINTER(ctypes.c_double)),
+        ("fp64_orders", ctypes.c_int),
     ]
 
 
@@ -118,6 +120,7 @@ def load():
             "arwen_sht_plan_grid": (I, [P, D, D]),
             "arwen_sht_gauss_nodes": (I, [I, D, D]),
             "arwen_sht_plan_table_bytes": (ctypes.c_size_t, [P]),
+            "arwen_sht_plan_fp64_orders": (I, [P]),
             "arwen_sht_plan_table_f64": (I, [P, I, D]),
             "arwen_sht_analysis": (I, [P, P, P, I, P]),
             "arwen_sht_synthesis": (I, [P, P, P, I, P]),

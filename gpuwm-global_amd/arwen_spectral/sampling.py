@@ -22,7 +22,10 @@ from .transform import SphericalHarmonicTransform
 #: The reference's host-path workspace bound (sampling.py:22-26); kept for
 #: callers that read it.  This engine has no basis workspace to bound.
 DEFAULT_SAMPLE_WORKSPACE_BYTES = 64 * 1024**2
-#: The reference's device-path workspace bound (sampling.py:28-33), likewise.
+#: The reference's device-path workspace bound (sampling.py:28-33).  Here the
+#: workspace holds the packed coefficients and per-part sums per point; with
+#: ``chunk_points=None`` the point chunk is the largest that keeps it within
+#: this bound (the cuts depend on T only, so chunking never changes a bit).
 DEFAULT_DEVICE_SAMPLE_WORKSPACE_BYTES = 512 * 1024**2
 #: ``"float64"``: basis and sums in FP64.  ``"float32"``: each basis value
 #: rounded to fp32 and one order's terms summed in fp32, the orders summed in
@@ -64,9 +67,12 @@ def _points(latitude_deg, longitude_deg):
     return np.deg2rad(lat.ravel()), np.deg2rad(lon.ravel()), lat.shape
 
 
-def _chunk(requested, npts: int) -> int:
+def _chunk(requested, npts: int, fixed_bytes: int = 0, per_point_bytes: int = 0) -> int:
     if requested is None:
-        return max(1, npts)
+        if per_point_bytes <= 0:
+            return max(1, npts)
+        room = max(DEFAULT_DEVICE_SAMPLE_WORKSPACE_BYTES - fixed_bytes, per_point_bytes)
+        return max(1, min(npts, room // per_point_bytes))
     if isinstance(requested, bool) or not isinstance(requested, (int, np.integer)) \
             or int(requested) < 1:
         raise ValueError("chunk_points must be a positive integer")
@@ -90,9 +96,13 @@ def _sample(transform: SphericalHarmonicTransform, kind: str, first, second, lat
     coslat = np.cos(lat)
     if kind != "scalar" and np.any(np.abs(coslat) < 1.0e-12):
         raise ValueError("vector/gradient sampling is undefined at the exact poles")
-    chunk = _chunk(chunk_points, lat.size)
     lead = tuple(c0.shape[:-2])
     count = t._count(c0)
+    lib = t._lib
+    code = _KIND[kind]
+    fixed = int(lib.arwen_sht_sample_workspace_bytes(t._plan, count, code, 0))
+    per_point = int(lib.arwen_sht_sample_workspace_bytes(t._plan, count, code, 1)) - fixed
+    chunk = _chunk(chunk_points, lat.size, fixed, per_point)
     npts = int(lat.size)
     f64 = torch.float64
     outs = [torch.empty((count, npts), dtype=f64, device=dev)
@@ -101,8 +111,6 @@ def _sample(transform: SphericalHarmonicTransform, kind: str, first, second, lat
         x = torch.from_numpy(np.sin(lat)).to(dev)
         cl = torch.from_numpy(coslat).to(dev)
         lo = torch.from_numpy(lon).to(dev)
-        code = _KIND[kind]
-        lib = t._lib
         nbytes = int(lib.arwen_sht_sample_workspace_bytes(t._plan, count, code,
                                                           min(chunk, npts)))
         work = torch.empty(nbytes, dtype=torch.uint8, device=dev)

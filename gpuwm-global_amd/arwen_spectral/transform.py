@@ -49,6 +49,10 @@ class SphericalHarmonicTransform:
     streaming: bool = False
     latitude_bands: int = 1
     order_partition: tuple = ()
+    #: the vector operators' zonal orders m < fp64_orders run in FP64 end to
+    #: end (arwen_sht_plan_desc.fp64_orders): None = the default (0 for
+    #: float32, 4 ceil((T+1)/192) for the split-BF16 modes), 0 = off
+    fp64_orders: int | None = None
     _plan: ctypes.c_void_p = field(default=None, init=False, repr=False)
 
     def __post_init__(self) -> None:
@@ -70,12 +74,15 @@ class SphericalHarmonicTransform:
         g = self.grid
         mu = np.ascontiguousarray(g.sin_lat, dtype=np.float64)
         w = np.ascontiguousarray(g.quadrature_weights, dtype=np.float64)
+        fp64 = -1 if self.fp64_orders is None else int(self.fp64_orders)
         desc = _native.PlanDesc(int(g.truncation), int(g.nlat), int(g.nlon), float(g.radius_m),
-                                self.backend.precision_code, _native.dptr(mu), _native.dptr(w))
+                                self.backend.precision_code, _native.dptr(mu), _native.dptr(w),
+                                fp64)
         plan = ctypes.c_void_p()
         _native.check(lib.arwen_sht_plan_create(ctypes.byref(desc), ctypes.byref(plan)))
         self._plan = plan
         self._lib = lib
+        self._fp64_orders = int(lib.arwen_sht_plan_fp64_orders(plan))
         torch = self.backend.xp
         n = np.arange(g.truncation + 1, dtype=np.float64)
         self._eigen = -(n * (n + 1.0)) / (g.radius_m * g.radius_m)
@@ -100,7 +107,8 @@ class SphericalHarmonicTransform:
                radius_m=None, grid: GaussianGrid | None = None, backend: str = "hip",
                precision: str = "float32", tensor_core_contractions: bool = False,
                legendre_band: int = DEFAULT_BAND, streaming: bool = False,
-               latitude_bands: int = 1, order_partition: tuple = ()):
+               latitude_bands: int = 1, order_partition: tuple = (),
+               fp64_orders: int | None = None):
         if grid is not None:
             if nlat is not None or nlon is not None or radius_m is not None \
                     or dealias_factor != 1.5:
@@ -115,7 +123,8 @@ class SphericalHarmonicTransform:
         return cls(grid, get_backend(backend, precision),
                    tensor_core_contractions=tensor_core_contractions,
                    legendre_band=legendre_band, streaming=streaming,
-                   latitude_bands=latitude_bands, order_partition=tuple(order_partition or ()))
+                   latitude_bands=latitude_bands, order_partition=tuple(order_partition or ()),
+                   fp64_orders=fp64_orders)
 
     # ---- shapes and identity (transform.py:1044-1104) -------------------
     @property
@@ -155,6 +164,8 @@ class SphericalHarmonicTransform:
                    "float_dtype": "float32"}
         if self.backend.precision != "float32":
             payload["legendre_precision"] = self.backend.precision
+        if self._fp64_orders:
+            payload["vector_fp64_orders"] = self._fp64_orders
         if int(self.legendre_band) != DEFAULT_BAND:
             payload["legendre_band"] = int(self.legendre_band)
         return payload
@@ -165,11 +176,19 @@ class SphericalHarmonicTransform:
                                          separators=(",", ":")).encode()).hexdigest()
 
     @property
+    def vector_fp64_orders(self) -> int:
+        """The zonal orders below which the vector operators run in FP64 (0: none),
+        as the plan resolved ``fp64_orders``."""
+        return self._fp64_orders
+
+    @property
     def legendre_table_nbytes(self) -> dict:
         total = int(self._lib.arwen_sht_plan_table_bytes(self._plan))
-        sixth = total // 6  # P, dP/dphi, P*m/cos in the synthesis and analysis layouts
+        jhp = -(-((self.grid.nlat + 1) // 2) // 32) * 32
+        low = 4 * self._fp64_orders * (self.truncation + 1) * jhp * 8
+        sixth = (total - low) // 6  # P, dP/dphi, P*m/cos in the synthesis and analysis layouts
         return {"basis": sixth, "analysis": sixth, "derivative": 2 * sixth,
-                "zonal": 2 * sixth, "scratch": 0}
+                "zonal": 2 * sixth, "vector_fp64": low, "scratch": 0}
 
     # ---- validation -----------------------------------------------------
     def _grid_tensor(self, field):

@@ -124,6 +124,11 @@ struct arwen_sht_plan {
   DevBuf scale_fold, scale_fold_vec, scale_one, scale_inv_a;
   DevBuf scale_waist, scale_waist_vec;  // 2 pi w (x 1/a): folds of a reference waist (already / nlon)
   DevBuf sample_rec, sample_diag;       // point sampler recurrence tables, built on first use
+  // vector operators' FP64 low orders m < m0: FP64 twiddles, per-row FP64
+  // scales and the FP64 tables of those orders (kinds dP/dphi and P m / cos)
+  int m0 = 0;
+  DevBuf d_tw64, scale_fold_vec64, scale_inv_a64;
+  DevBuf low_syn[3], low_ana[3];
   sht::FftPlan fft{};
   int rp_fold = 4, rp_fold_vec = 4, rp_unfold = 4;  // folded rows (wavefronts) per FFT block
   int fft_gpb = 2;                                   // row groups walked by one FFT block
@@ -136,7 +141,9 @@ struct arwen_sht_plan {
   int kp(int m, int p) const { return koff[2 * m + p + 1] - koff[2 * m + p]; }
   size_t table_bytes() const {
     size_t per = precision == ARWEN_SHT_FP32 ? 4 : 6;
-    return 6 * (size_t)tab_elems * per;
+    size_t low = 0;
+    for (int k = 0; k < 3; ++k) low += low_syn[k].bytes + low_ana[k].bytes;
+    return 6 * (size_t)tab_elems * per + low;
   }
   const void* table(int kind, bool syn, long long* piece) const {
     *piece = tab_elems;
@@ -145,6 +152,8 @@ struct arwen_sht_plan {
 };
 
 namespace {
+
+void build_sample_tables(arwen_sht_plan* P);
 
 void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   if (!d) fail(ARWEN_SHT_EINVAL, "plan descriptor is NULL");
@@ -202,6 +211,19 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   }
   upload(P->d_koff, P->koff);
 
+  // FP64 vector orders (resolved below once the FFT path is known).
+  if (d->fp64_orders < -1)
+    fail(ARWEN_SHT_EINVAL, "fp64_orders must be -1 (default), 0 or positive, got " +
+                               std::to_string(d->fp64_orders));
+  {
+    const int L = sht_fft_wave_L(P->nlon);
+    const bool wave = L && sht_fold64_lds_bytes(P->nlon) <= 160 * 1024;
+    int m0 = d->fp64_orders;
+    if (m0 < 0) m0 = P->precision == ARWEN_SHT_FP32 ? 0 : sht::kMT * ((T + 1 + 191) / 192);
+    m0 = SHT_ROUND_UP(m0, sht::kMT);
+    P->m0 = wave ? std::min(m0, P->MB * sht::kMT) : 0;
+  }
+
   // Legendre tables.
   sht::TableOut out{};
   out.elems = P->tab_elems;
@@ -217,6 +239,16 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
       out.bf_syn[k] = P->tab_syn[k].as<unsigned short>();
       out.bf_ana[k] = P->tab_ana[k].as<unsigned short>();
     }
+  }
+  if (P->m0 > 0) {
+    const size_t low_bytes = (size_t)P->m0 * (T + 1) * P->JHP * sizeof(double);
+    for (int k = 1; k < 3; ++k) {  // dP/dphi and P m / cos
+      P->low_syn[k].alloc(low_bytes, true);
+      P->low_ana[k].alloc(low_bytes, true);
+      out.f64_low_syn[k] = P->low_syn[k].as<double>();
+      out.f64_low_ana[k] = P->low_ana[k].as<double>();
+    }
+    out.low_m0 = P->m0;
   }
   hip_check(sht_launch_legendre_tables(T, P->nlat, P->JH, P->JHP, P->d_mu.as<double>(),
                                        P->d_koff.as<int>(), &out, 0),
@@ -235,6 +267,15 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   }
   upload(P->d_tw, tw);
   P->fft.tw = P->d_tw.as<float2>();
+  if (P->m0 > 0) {
+    std::vector<double2> tw64(P->nlon);
+    for (int k = 0; k < P->nlon; ++k) {
+      const double ang = -2.0 * M_PI * (double)k / (double)P->nlon;
+      tw64[k] = make_double2(std::cos(ang), std::sin(ang));
+    }
+    upload(P->d_tw64, tw64);
+    P->fft.tw64 = P->d_tw64.as<double2>();
+  }
   // Wavefronts (folded rows) per FFT block: as many as fit 80 KiB, so two
   // blocks share a CU; one row per block up to the device's whole LDS where
   // a single row needs more (nlon = 3072: 96 KiB for a u,v row pair).
@@ -255,6 +296,7 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
     P->rp_fold_vec = pick([&](int rp) { return sht_fold_lds_bytes(P->nlon, T, 1, rp); });
     P->rp_unfold = pick([&](int rp) { return sht_unfold_lds_bytes(P->nlon, T, rp); });
     hip_check(sht_fft_allow_lds(need, (size_t)lds_max), "FFT LDS attribute");
+    if (P->m0 > 0) hip_check(sht_low_allow_lds((size_t)lds_max), "low-order LDS attribute");
     // wave-resident FFTs (N = 64 L) when their LDS fits the device
     const int L = sht_fft_wave_L(P->nlon);
     P->fft.wave_L = (L && sht_fft_wave_lds_bytes(P->nlon, 1) <= (size_t)lds_max) ? L : 0;
@@ -279,6 +321,20 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   upload(P->scale_waist_vec, swv);
   upload(P->scale_one, one);
   upload(P->scale_inv_a, inva);
+  if (P->m0 > 0) {
+    std::vector<double> sfv64(P->JHP, 0.0), inva64(P->JHP, 0.0);
+    for (int i = 0; i < P->JH; ++i) {
+      const int jn = P->nlat - 1 - i;
+      double wq = P->w[jn];
+      if (jn == i) wq *= 0.5;
+      sfv64[i] = 2.0 * M_PI * wq / (P->radius * P->nlon);
+      inva64[i] = 1.0 / P->radius;
+    }
+    upload(P->scale_fold_vec64, sfv64);
+    upload(P->scale_inv_a64, inva64);
+  }
+  // the point sampler's recurrence coefficients, so that sampling allocates nothing
+  build_sample_tables(P);
   hip_check(hipDeviceSynchronize(), "plan build");
 }
 
@@ -317,6 +373,8 @@ struct JobState {
   DevBuf packed_a;            // synthesis input (packed coefficients)
   DevBuf waist_a, waist_b;    // synthesis output (b: gradient north) / analysis input
   DevBuf packed_out;          // analysis output
+  DevBuf low;                 // vector jobs: FP64 low waist [level][u|v][m < m0][p][i]
+  DevBuf low_pc;              // uv_from_vordiv: FP64 psi, chi of the orders m < m0
   int ncols = 0;
 };
 
@@ -330,6 +388,8 @@ struct arwen_sht_batch {
   std::vector<sht::UnpackArgs> unpacks;
   std::vector<sht::WaistUnfoldArgs> waist_unfolds;  // synthesis into a reference waist
   std::vector<sht::WaistFoldArgs> waist_folds;      // analysis from a reference waist
+  std::vector<sht::LowSynArgs> low_syns;            // vector jobs' FP64 low orders
+  std::vector<sht::LowAnaArgs> low_anas;
   std::vector<HostGroup> groups[2];  // GEMM work groups per direction (0 SYN, 1 ANA)
   GemmLists lists[2];
   arwen_sht_gemm_config config{{32, 32}, {0, 0}, 1};
@@ -624,9 +684,29 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
                     : vec ? sht::kVector
                           : sht::kPlain;
       B->packs.push_back(pa);
+      // ---- FP64 low orders of u, v (orders m < m0) ----
+      const bool low = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV && P->m0 > 0;
+      if (low) {
+        js->low.alloc((size_t)n * 2 * P->m0 * 2 * P->JHP * sizeof(double2));
+        js->low_pc.alloc((size_t)n * P->m0 * (T + 1) * sizeof(double4));
+        sht::LowSynArgs ls{};
+        ls.pc = js->low_pc.as<double4>();
+        ls.vort = static_cast<const float2*>(J.in0);
+        ls.div = static_cast<const float2*>(J.in1);
+        ls.dp = P->low_syn[kD].as<double>();
+        ls.pm = P->low_syn[kPm].as<double>();
+        ls.low = js->low.as<double2>();
+        ls.radius = P->radius;
+        ls.T = T;
+        ls.JH = P->JH;
+        ls.JHP = P->JHP;
+        ls.m0 = P->m0;
+        ls.nlev = n;
+        B->low_syns.push_back(ls);
+      }
       // ---- grouped GEMM: one group per (m-tile, output parity) ----
       const DataRef pk{js->packed_a.as<char>()};
-      for (int mb = 0; mb < P->MB; ++mb)
+      for (int mb = low ? P->m0 / sht::kMT : 0; mb < P->MB; ++mb)
         for (int q = 0; q < 2; ++q) {
           if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
             // U,V = -dP/dphi (psi) + i P m/cos (chi), i P m/cos (psi) + dP/dphi (chi):
@@ -675,6 +755,14 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
         ua.job.fstride = fstride;
         ua.job.ldw = P->ldw;
         ua.job.imul = imul;
+        if (low) {  // FP64 unfold; orders m < m0 from the low waist of u (col0 0) or v
+          const int f = col0 == 0 ? 0 : 1;
+          ua.job.f64 = 1;
+          ua.job.low = js->low.as<double2>() + (size_t)f * P->m0 * 2 * P->JHP;
+          ua.job.low_stride = 2 * P->m0 * 2 * P->JHP;
+          ua.job.rowscale64 = P->scale_inv_a64.as<double>();
+          ua.job.m0 = P->m0;
+        }
         B->unfolds.push_back(ua);
         B->fft_bytes += n * fftb;
       };
@@ -742,12 +830,43 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       fa.job.col0 = 0;
       fa.job.ldw = P->ldw;
       fa.job.vector = vec ? 1 : 0;
-      if (!from_waist(J.op)) {
+      const bool low = J.op == ARWEN_SHT_OP_VORDIV_FROM_UV && P->m0 > 0;
+      if (low) {
+        // FP64 folds of u and v, one field each: i U to level columns 2, 3 and
+        // i V to 0, 1 (the [iV, iU] arrangement), orders m < m0 also in FP64
+        js->low.alloc((size_t)n * 2 * P->m0 * 2 * P->JHP * sizeof(double2));
+        for (int f = 0; f < 2; ++f) {
+          sht::FoldArgs fh = fa;
+          fh.job.grid0 = static_cast<const float*>(f == 0 ? J.in0 : J.in1);
+          fh.job.grid1 = nullptr;
+          fh.job.vector = 0;
+          fh.job.col0 = f == 0 ? 2 : 0;
+          fh.job.fstride = 4;
+          fh.job.low = js->low.as<double2>() + (size_t)f * P->m0 * 2 * P->JHP;
+          fh.job.low_stride = 2 * P->m0 * 2 * P->JHP;
+          fh.job.rowscale64 = P->scale_fold_vec64.as<double>();
+          fh.job.m0 = P->m0;
+          B->folds.push_back(fh);
+        }
+        B->fft_bytes += 2.0 * n * fftb;
+        sht::LowAnaArgs la{};
+        la.low = js->low.as<double2>();
+        la.dp = P->low_ana[kD].as<double>();
+        la.pm = P->low_ana[kPm].as<double>();
+        la.vort = static_cast<float2*>(J.out0);
+        la.div = static_cast<float2*>(J.out1);
+        la.T = T;
+        la.JH = P->JH;
+        la.JHP = P->JHP;
+        la.m0 = P->m0;
+        la.nlev = n;
+        B->low_anas.push_back(la);
+      } else if (!from_waist(J.op)) {
         B->folds.push_back(fa);
         B->fft_bytes += (vec ? 2.0 : 1.0) * n * fftb;
       }
       const DataRef wst{js->waist_a.as<char>()};
-      for (int mb = 0; mb < P->MB; ++mb)
+      for (int mb = low ? P->m0 / sht::kMT : 0; mb < P->MB; ++mb)
         for (int q = 0; q < 2; ++q) {
           if (vec) {
             // zeta, D = Pm^T [iV, iU]_q + dP^T [U, -V]_{1-q}
@@ -770,6 +889,7 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
       ua.job.col0 = 0;
       ua.job.ldp = P->ldp;
       ua.job.vector = vec ? 1 : 0;
+      ua.job.m0 = low ? P->m0 : 0;
       B->unpacks.push_back(ua);
     }
     B->jobs.push_back(std::move(js));
@@ -817,6 +937,7 @@ void run_batch(arwen_sht_batch* B, hipStream_t s, hipEvent_t* ev = nullptr) {
   for (auto& a : B->packs) hip_check(sht_launch_pack(&a, s), "pack launch");
   mark(1);
   hip_check(launch_gemm(B, 0, s), "synthesis GEMM launch");
+  for (auto& a : B->low_syns) hip_check(sht_launch_low_synthesis(&a, s), "low-order synthesis launch");
   mark(2);
   for (auto& a : B->unfolds) hip_check(sht_launch_irfft_unfold(&a, s), "irfft launch");
   for (auto& a : B->waist_unfolds) hip_check(sht_launch_waist_unfold(&a, s), "waist unfold launch");
@@ -825,6 +946,7 @@ void run_batch(arwen_sht_batch* B, hipStream_t s, hipEvent_t* ev = nullptr) {
   for (auto& a : B->waist_folds) hip_check(sht_launch_waist_fold(&a, s), "waist fold launch");
   mark(4);
   hip_check(launch_gemm(B, 1, s), "analysis GEMM launch");
+  for (auto& a : B->low_anas) hip_check(sht_launch_low_analysis(&a, s), "low-order analysis launch");
   mark(5);
   for (auto& a : B->unpacks) hip_check(sht_launch_unpack(&a, s), "unpack launch");
   mark(6);
@@ -928,6 +1050,8 @@ int arwen_sht_plan_grid(const arwen_sht_plan* P, double* sin_lat, double* weight
 }
 
 size_t arwen_sht_plan_table_bytes(const arwen_sht_plan* P) { return P ? P->table_bytes() : 0; }
+
+int arwen_sht_plan_fp64_orders(const arwen_sht_plan* P) { return P ? P->m0 : 0; }
 
 int arwen_sht_gauss_nodes(int nlat, double* sin_lat, double* weights) {
   return guarded([&] {
@@ -1090,7 +1214,6 @@ int sample_columns(int count, int kind) {
 // The point sampler's recurrence coefficients per packed (n, m) and per order,
 // with the reference's numpy expressions (sampling.py:83-112) in double.
 void build_sample_tables(arwen_sht_plan* P) {
-  if (P->sample_rec.p) return;
   const int T = P->T;
   const long long K = (long long)(T + 1) * (T + 2) / 2;
   std::vector<double> rec(3 * K, 0.0), diag(2 * (T + 1), 0.0);
@@ -1106,8 +1229,14 @@ void build_sample_tables(arwen_sht_plan* P) {
     }
   for (int m = 1; m <= T; ++m) diag[m] = -std::sqrt((2 * m + 1) / (2.0 * m));
   for (int m = 0; m <= T; ++m) diag[T + 1 + m] = std::sqrt((double)(2 * m + 3));
-  upload(P->sample_rec, rec);
-  upload(P->sample_diag, diag);
+  // both uploads succeed before the plan holds either (the sampler checks both)
+  DevBuf r, dg;
+  upload(r, rec);
+  upload(dg, diag);
+  std::swap(P->sample_rec.p, r.p);
+  std::swap(P->sample_rec.bytes, r.bytes);
+  std::swap(P->sample_diag.p, dg.p);
+  std::swap(P->sample_diag.bytes, dg.bytes);
 }
 }  // namespace
 
@@ -1136,7 +1265,8 @@ size_t arwen_sht_sample_workspace_bytes(const arwen_sht_plan* P, int count, int 
   const int ldc = sample_columns(count, kind);
   const int R = sample_cuts(P->T).parts;
   size_t bytes = (size_t)K * ldc * sizeof(double2);  // packed coefficients, either precision
-  if (R > 1) bytes += (size_t)2 * R * ldc * (size_t)npoints * sizeof(double);
+  // per-part raw sums: one plane per part for values, two for derivatives
+  if (R > 1) bytes += (size_t)(kind == 0 ? 1 : 2) * R * ldc * (size_t)npoints * sizeof(double);
   return bytes;
 }
 
@@ -1164,7 +1294,8 @@ int arwen_sht_sample(arwen_sht_plan* P, const void* in0, const void* in1, int co
     int dev;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     if (dev != P->device) fail(ARWEN_SHT_EINVAL, "plan used on a device other than its own");
-    build_sample_tables(P);
+    if (!P->sample_rec.p || !P->sample_diag.p)
+      fail(ARWEN_SHT_EHIP, "the plan's sampler tables are missing");
     hipStream_t st = static_cast<hipStream_t>(s);
     const int ldc = sample_columns(count, kind);
     sht::SamplePackArgs pa{};

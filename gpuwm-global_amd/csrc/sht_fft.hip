@@ -41,10 +41,20 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// The same operations in FP64 (the vector operators' FP64 FFTs).
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ float2 cmk(float x, float y) { return make_float2(x, y); }
+__device__ __forceinline__ double2 cmk(double x, double y) { return make_double2(x, y); }
 
 // Twiddle e^{sign * 2 pi i k / N} from the forward table.
-__device__ __forceinline__ float2 twiddle(const float2* tw, int k, int sign) {
-  float2 t = tw[k];
+template <class C>
+__device__ __forceinline__ C twiddle(const C* tw, int k, int sign) {
+  C t = tw[k];
   return sign < 0 ? t : cconj(t);
 }
 
@@ -435,22 +445,26 @@ constexpr double taylor_sin(double x) {
   }
   return s;
 }
-// cos and sin of 2 pi j / 48, rounded once to fp32 (exact zeros snapped)
+// cos and sin of 2 pi j / 48, rounded once to fp32 (exact zeros snapped);
+// Ring48<double> keeps them in FP64 for the FP64 FFTs
+template <class R>
 struct Ring48 {
-  float c[48];
-  float s[48];
+  R c[48];
+  R s[48];
 };
-constexpr Ring48 make_ring48() {
-  Ring48 r{};
+template <class R>
+constexpr Ring48<R> make_ring48() {
+  Ring48<R> r{};
   for (int j = 0; j < 48; ++j) {
     const double x = kTwoPi * j / 48.0;
     const double c = taylor_cos(x), sn = taylor_sin(x);
-    r.c[j] = (float)((c < 1e-12 && c > -1e-12) ? 0.0 : c);
-    r.s[j] = (float)((sn < 1e-12 && sn > -1e-12) ? 0.0 : sn);
+    r.c[j] = (R)((c < 1e-12 && c > -1e-12) ? 0.0 : c);
+    r.s[j] = (R)((sn < 1e-12 && sn > -1e-12) ? 0.0 : sn);
   }
   return r;
 }
-constexpr Ring48 kRing48 = make_ring48();
+constexpr Ring48<float> kRing48 = make_ring48<float>();
+constexpr Ring48<double> kRing48d = make_ring48<double>();
 
 template <class Fn, int... I>
 __device__ __forceinline__ void static_for_impl(Fn&& fn, std::integer_sequence<int, I...>) {
@@ -462,44 +476,55 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
   static_for_impl(fn, std::make_integer_sequence<int, N>{});
 }
 
-// x * e^(S 2 pi i E / R), R | 48
-template <int E, int R, int S>
-__device__ __forceinline__ float2 rot(float2 x) {
+// x * e^(S 2 pi i E / R), R | 48; C is float2 or double2
+template <int E, int R, int S, class C>
+__device__ __forceinline__ C rot(C x) {
   constexpr int e = ((E % R) + R) % R;
   if constexpr (e == 0) {
     return x;
   } else if constexpr (2 * e == R) {
-    return make_float2(-x.x, -x.y);
+    return cmk(-x.x, -x.y);
   } else if constexpr (4 * e == R) {  // S i
-    return S > 0 ? make_float2(-x.y, x.x) : make_float2(x.y, -x.x);
+    return S > 0 ? cmk(-x.y, x.x) : cmk(x.y, -x.x);
   } else if constexpr (4 * e == 3 * R) {  // -S i
-    return S > 0 ? make_float2(x.y, -x.x) : make_float2(-x.y, x.x);
-  } else {
+    return S > 0 ? cmk(x.y, -x.x) : cmk(-x.y, x.x);
+  } else if constexpr (std::is_same<C, float2>::value) {
     constexpr int j = e * (48 / R);
     constexpr float c = kRing48.c[j], sn = S * kRing48.s[j];
     return make_float2(x.x * c - x.y * sn, x.x * sn + x.y * c);
+  } else {
+    constexpr int j = e * (48 / R);
+    constexpr double c = kRing48d.c[j], sn = S * kRing48d.s[j];
+    return make_double2(x.x * c - x.y * sn, x.x * sn + x.y * c);
   }
 }
 
 // In-register DFT of R points, natural order in and out, sign S.
-template <int R, int S>
-__device__ __forceinline__ void dft(float2 (&v)[R]) {
+template <int R, int S, class C>
+__device__ __forceinline__ void dft(C (&v)[R]) {
   if constexpr (R == 1) {
   } else if constexpr (R == 2) {
-    const float2 a = v[0], b = v[1];
+    const C a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
   } else if constexpr (R == 3) {
-    constexpr float h = 0.86602540378443864676f * S;  // sin(S 2 pi / 3)
-    const float2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-    const float2 m1 = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
-    const float2 m2 = make_float2(-h * t2.y, h * t2.x);
+    const C t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    C m1, m2;
+    if constexpr (std::is_same<C, float2>::value) {
+      constexpr float h = 0.86602540378443864676f * S;  // sin(S 2 pi / 3)
+      m1 = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+      m2 = make_float2(-h * t2.y, h * t2.x);
+    } else {
+      constexpr double h = 0.86602540378443864676 * S;
+      m1 = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+      m2 = make_double2(-h * t2.y, h * t2.x);
+    }
     v[0] = cadd(v[0], t1);
     v[1] = cadd(m1, m2);
     v[2] = csub(m1, m2);
   } else if constexpr (R == 4) {
-    const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-    const float2 a2 = cadd(v[1], v[3]), a3 = rot<1, 4, S>(csub(v[1], v[3]));
+    const C a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    const C a2 = cadd(v[1], v[3]), a3 = rot<1, 4, S>(csub(v[1], v[3]));
     v[0] = cadd(a0, a2);
     v[2] = csub(a0, a2);
     v[1] = cadd(a1, a3);
@@ -509,10 +534,10 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
     // X[k1 + P k2] = sum_n2 w_Q^(n2 k2) w_R^(n2 k1) sum_n1 x[Q n1 + n2] w_P^(n1 k1)
     constexpr int P = (R % 4 == 0) ? 4 : ((R % 3 == 0) ? 3 : 2);
     constexpr int Q = R / P;
-    float2 y[R];
+    C y[R];
     static_for<Q>([&](auto n2_) {
       constexpr int n2 = decltype(n2_)::value;
-      float2 t[P];
+      C t[P];
       static_for<P>([&](auto n1_) { t[n1_] = v[Q * decltype(n1_)::value + n2]; });
       dft<P, S>(t);
       static_for<P>([&](auto k1_) {
@@ -522,7 +547,7 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
     });
     static_for<P>([&](auto k1_) {
       constexpr int k1 = decltype(k1_)::value;
-      float2 t[Q];
+      C t[Q];
       static_for<Q>([&](auto n2_) { t[n2_] = y[decltype(n2_)::value * P + k1]; });
       dft<Q, S>(t);
       static_for<Q>([&](auto k2_) { v[k1 + P * decltype(k2_)::value] = t[k2_]; });
@@ -549,30 +574,29 @@ constexpr int kSpanPad = 4;
 
 // F sequences of N = 64 L points: v[f][n1] = x_f[64 n1 + lane] on entry; on
 // return buf[f * span + k] holds X_f[k], k < N.
-template <int L, int F, int S>
-__device__ __forceinline__ void wave_fft64(float2 (&v)[F][L], float2* buf, const float2* tw,
-                                           int lane) {
+template <int L, int F, int S, class C>
+__device__ __forceinline__ void wave_fft64(C (&v)[F][L], C* buf, const C* tw, int lane) {
   constexpr int SP = wave_span<L>();
   static_for<F>([&](auto f_) {
     constexpr int f = decltype(f_)::value;
     dft<L, S>(v[f]);
     static_for<L>([&](auto k1_) {
       constexpr int k1 = decltype(k1_)::value;
-      float2 x = v[f][k1];
+      C x = v[f][k1];
       if constexpr (k1 > 0) x = cmul(x, twiddle(tw, lane * k1, S));
       buf[f * SP + k1 * kRowSpan + lane] = x;
     });
   });
   wave_sync();
   constexpr int U = F * L * 8, RND = (U + 63) / 64;
-  float2 w[RND][8];
+  C w[RND][8];
   // B: units (f, k1, b)
   static_for<RND>([&](auto r_) {
     constexpr int r = decltype(r_)::value;
     const int u = lane + 64 * r;
     if (u < U) {
       const int f = u / (8 * L), k1 = (u >> 3) % L, b = u & 7;
-      const float2* row = buf + f * SP + k1 * kRowSpan + b;
+      const C* row = buf + f * SP + k1 * kRowSpan + b;
       static_for<8>([&](auto a_) { w[r][a_] = row[8 * decltype(a_)::value]; });
       dft<8, S>(w[r]);
       static_for<8>([&](auto c_) {
@@ -589,7 +613,7 @@ __device__ __forceinline__ void wave_fft64(float2 (&v)[F][L], float2* buf, const
     const int u = lane + 64 * r;
     if (u < U) {
       const int f = u / (8 * L), k1 = (u >> 3) % L, b = u & 7;
-      float2* row = buf + f * SP + k1 * kRowSpan;
+      C* row = buf + f * SP + k1 * kRowSpan;
       static_for<8>([&](auto c_) {
         constexpr int c = decltype(c_)::value;
         row[8 * c + (b ^ c)] = w[r][c];
@@ -603,7 +627,7 @@ __device__ __forceinline__ void wave_fft64(float2 (&v)[F][L], float2* buf, const
     const int u = lane + 64 * r;
     if (u < U) {
       const int f = u / (8 * L), k1 = (u >> 3) % L, c = u & 7;
-      const float2* row = buf + f * SP + k1 * kRowSpan + 8 * c;
+      const C* row = buf + f * SP + k1 * kRowSpan + 8 * c;
       static_for<8>([&](auto b_) { w[r][b_] = row[decltype(b_)::value ^ c]; });
       dft<8, S>(w[r]);
     }
@@ -614,7 +638,7 @@ __device__ __forceinline__ void wave_fft64(float2 (&v)[F][L], float2* buf, const
     const int u = lane + 64 * r;
     if (u < U) {
       const int f = u / (8 * L), k1 = (u >> 3) % L, c = u & 7;
-      float2* out = buf + f * SP + k1 + L * c;
+      C* out = buf + f * SP + k1 + L * c;
       static_for<8>([&](auto d_) { out[8 * L * decltype(d_)::value] = w[r][d_]; });
     }
   });
@@ -879,6 +903,227 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 wave-resident FFTs of the vector operators (u, v) for their FP64
+// correction.  The large low orders of a wind field leak fp32 rounding into
+// every order of an fp32 FFT; relative to the small high-degree vorticity and
+// divergence those leaks exceed 1e-6.  Here one wavefront transforms one
+// field's mirror row pair in FP64 (grid values fp32 as stored), and the
+// folded values go to the fp32 waist (orders the split-BF16 GEMM contracts)
+// and, for m < m0, in FP64 to the low waist (orders the FP64 low-order kernels
+// contract).  Same row grouping, persistence and block writes as the fp32
+// kernels above; LDS spans hold double2.
+// ---------------------------------------------------------------------------
+template <int L, int kWaveBlock>
+__global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave64_kernel(FoldArgs a) {
+  extern __shared__ double2 lds_d[];
+  constexpr int N = 64 * L, SP = wave_span<L>();
+  constexpr int NV = 4;  // folded values per order: 2 columns x 2 parities
+  typedef __attribute__((ext_vector_type(4))) float f4;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int WS = SP + kSpanPad;
+  double2* tw = lds_d;
+  double2* buf = lds_d + N + (size_t)w * WS;
+  const int ngroups = a.job.nfields * (a.JHP / kWaveBlock);
+  const int T1 = a.T + 1;
+  const int MT = (T1 + kMT - 1) / kMT;
+  const int MO = MT * kMT;
+  const size_t plane = (size_t)a.nlat * N;
+  int g = blockIdx.x;
+  if (g >= ngroups) return;  // block-uniform
+  float2 pre[L];             // the next mirror pair as stored (north, south)
+  auto load_rows = [&](int grp) {
+    const int item = grp * kWaveBlock + w;
+    const int fld = item / a.JHP, i = item % a.JHP;
+    if (i >= a.JH) return;
+    const float* gp = a.job.grid0 + (size_t)fld * plane;
+    const float* rn = gp + (size_t)(a.nlat - 1 - i) * N;
+    const float* rs = gp + (size_t)i * N;
+    static_for<L>([&](auto n1_) {
+      constexpr int n1 = decltype(n1_)::value;
+      pre[n1] = make_float2(rn[64 * n1 + lane], rs[64 * n1 + lane]);
+    });
+  };
+  load_rows(g);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw64[k];
+  __syncthreads();
+  for (; g < ngroups; g += gridDim.x) {
+    const int item0 = g * kWaveBlock;
+    const int fld = item0 / a.JHP, i0 = item0 % a.JHP, i = i0 + w;
+    const bool live = i < a.JH;
+    if (live) {
+      double2 v[1][L];
+      static_for<L>([&](auto n1_) {
+        constexpr int n1 = decltype(n1_)::value;
+        v[0][n1] = make_double2((double)pre[n1].x, (double)pre[n1].y);
+      });
+      wave_fft64<L, 1, -1>(v, buf, tw, lane);
+    }
+    if (g + (int)gridDim.x < ngroups) load_rows(g + gridDim.x);
+    const double s = live ? a.job.rowscale64[i] : 0.0;
+    double2* low = a.job.low + (size_t)fld * a.job.low_stride;
+    constexpr int RO = (N / 2 + 4 + 63) / 64;  // rounds: MO <= N/2 + 4
+    float val[RO][NV];
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+      const int m = lane + 64 * r;
+      double2 e = make_double2(0.0, 0.0), o = make_double2(0.0, 0.0);
+      if (live && m < T1) {
+        const double2 zm = buf[m];
+        const double2 zc = cconj(buf[(N - m) % N]);
+        const double2 xn = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+        const double2 d = csub(zm, zc);
+        const double2 xs = make_double2(0.5 * d.y, -0.5 * d.x);
+        e = make_double2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
+        o = make_double2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
+        if (m < a.job.m0) {
+          low[((size_t)m * 2 + 0) * a.JHP + i] = e;
+          low[((size_t)m * 2 + 1) * a.JHP + i] = o;
+        }
+      }
+      // i X: columns [-X.im, X.re], e = 2 * column + parity
+      val[r][0] = (float)-e.y;
+      val[r][1] = (float)-o.y;
+      val[r][2] = (float)e.x;
+      val[r][3] = (float)o.x;
+    }
+    wave_sync();  // the spectrum is read; the span now holds stage[e][m]
+    float* stage = reinterpret_cast<float*>(buf);
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+      const int m = lane + 64 * r;
+      if (m < MO) {
+#pragma unroll
+        for (int e = 0; e < NV; ++e) stage[(size_t)e * MO + m] = val[r][e];
+      }
+    }
+    __syncthreads();
+    // block write as in rfft_fold_wave_kernel (e = rn / MT from the fp32
+    // reciprocal, exact in this range)
+    const float inv_mt = 1.0f / (float)MT;
+    static_assert(NV * (N / 8 + 1) < 4096 && N / 8 + 1 <= 256, "range of the fp32 quotient");
+    for (int u = threadIdx.x; u < NV * MT * kWaveBlock; u += blockDim.x) {
+      const int ww = u % kWaveBlock, rn = u / kWaveBlock;
+      const int e = (int)(((float)rn + 0.5f) * inv_mt), mt = rn - e * MT;
+      const int c = a.job.col0 + a.job.fstride * fld + (e >> 1), p = e & 1;
+      *reinterpret_cast<f4*>(a.job.waist0 + waist_index(c, a.job.ldw, a.JHP, mt * kMT, p, i0 + ww)) =
+          *reinterpret_cast<const f4*>(reinterpret_cast<const float*>(lds_d + N + (size_t)ww * WS) +
+                                       ((size_t)e * MT + mt) * 4);
+    }
+    __syncthreads();  // the spans are free for the next group
+  }
+}
+
+template <int L, int kWaveBlock>
+__global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave64_kernel(UnfoldArgs a) {
+  extern __shared__ double2 lds_d[];
+  constexpr int N = 64 * L, SP = wave_span<L>();
+  static_assert(kMT == 4, "a row's four orders are one 16-byte vector");
+  typedef __attribute__((ext_vector_type(4))) float f4;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int WS = SP + kSpanPad;  // span stride per wavefront, double2
+  double2* tw = lds_d;
+  double2* buf = lds_d + N + (size_t)w * WS;
+  const int JH4 = (a.JH + kWaveBlock - 1) / kWaveBlock;
+  const int ngroups = a.job.nfields * JH4;
+  const int T1 = a.T + 1;
+  const int MT = (T1 + kMT - 1) / kMT;
+  const int MO = MT * kMT;
+  const float* W = a.job.waist;
+  const size_t plane = (size_t)a.nlat * N;
+  int g = blockIdx.x;
+  if (g >= ngroups) return;  // block-uniform
+  // the fp32 waist runs, staged as in irfft_unfold_wave_kernel (LDS offsets
+  // in floats: the twiddles take 4 N, a span 4 WS)
+  constexpr int RU = (4 * (N / 8 + 1) + 63) / 64;
+  f4 run[RU];
+  unsigned roff[RU], loff[RU], rvalid = 0;
+#pragma unroll
+  for (int r = 0; r < RU; ++r) {
+    const int u = threadIdx.x + 64 * kWaveBlock * r;
+    const int ww = u % kWaveBlock, rn = u / kWaveBlock;
+    const bool ok = rn < 4 * MT;
+    const unsigned e = ok ? (unsigned)(rn / MT) : 0u, mt = ok ? (unsigned)(rn % MT) : 0u;
+    rvalid |= (ok ? 1u : 0u) << r;
+    roff[r] = (e >> 1) * (unsigned)a.job.ldw + 4u * ((2u * mt + (e & 1u)) * (unsigned)a.JHP + ww);
+    loff[r] = (unsigned)(4 * N + 4 * ww * WS) + 4u * (e * (unsigned)MT + mt);
+  }
+  auto load_runs = [&](int grp) {
+    const int fld = grp / JH4, i0 = (grp % JH4) * kWaveBlock;
+    const float* base = W + (size_t)(a.job.col0 + a.job.fstride * fld) * a.job.ldw + 4 * i0;
+#pragma unroll
+    for (int r = 0; r < RU; ++r)
+      if (rvalid & (1u << r)) run[r] = *reinterpret_cast<const f4*>(base + roff[r]);
+  };
+  load_runs(g);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw64[k];
+  for (; g < ngroups; g += gridDim.x) {
+    const int fld = g / JH4, i0 = (g % JH4) * kWaveBlock, i = i0 + w;
+    const bool live = i < a.JH;
+#pragma unroll
+    for (int r = 0; r < RU; ++r)
+      if (rvalid & (1u << r)) *reinterpret_cast<f4*>(reinterpret_cast<float*>(lds_d) + loff[r]) = run[r];
+    __syncthreads();  // spans (and, the first time, the twiddles) are in LDS
+    if (g + (int)gridDim.x < ngroups) load_runs(g + gridDim.x);
+    if (live) {
+      constexpr int RO = (N / 2 + 4 + 63) / 64;
+      float er[RO], ei[RO], orr[RO], oi[RO];
+      const float* stage = reinterpret_cast<const float*>(buf);
+#pragma unroll
+      for (int r = 0; r < RO; ++r) {
+        const int m = lane + 64 * r;
+        const bool in = m < MO;
+        er[r] = in ? stage[m] : 0.f;
+        orr[r] = in ? stage[(size_t)MO + m] : 0.f;
+        ei[r] = in ? stage[(size_t)2 * MO + m] : 0.f;
+        oi[r] = in ? stage[(size_t)3 * MO + m] : 0.f;
+      }
+      wave_sync();  // the staged values are in registers; the span becomes the spectrum
+      const double s = a.job.rowscale64[i];
+      const double2* low = a.job.low + (size_t)fld * a.job.low_stride;
+      for (int k = lane; k < N; k += 64) buf[k] = make_double2(0.0, 0.0);
+      wave_sync();
+#pragma unroll
+      for (int r = 0; r < RO; ++r) {
+        const int m = lane + 64 * r;
+        if (m >= T1) continue;
+        double2 E = make_double2((double)er[r], (double)ei[r]);
+        double2 O = make_double2((double)orr[r], (double)oi[r]);
+        if (m < a.job.m0) {
+          E = low[((size_t)m * 2 + 0) * a.JHP + i];
+          O = low[((size_t)m * 2 + 1) * a.JHP + i];
+        }
+        double2 A = make_double2((E.x + O.x) * s, (E.y + O.y) * s);
+        double2 B = make_double2((E.x - O.x) * s, (E.y - O.y) * s);
+        if (a.job.imul) {
+          A = make_double2(-A.y, A.x);
+          B = make_double2(-B.y, B.x);
+        }
+        if (m == 0) {
+          A.y = 0.0;
+          B.y = 0.0;
+        }
+        buf[m] = make_double2(A.x - B.y, A.y + B.x);
+        if (m > 0) buf[N - m] = make_double2(A.x + B.y, -A.y + B.x);
+      }
+      wave_sync();
+      double2 v[1][L];
+      static_for<L>([&](auto n1_) { v[0][n1_] = buf[64 * decltype(n1_)::value + lane]; });
+      wave_sync();
+      wave_fft64<L, 1, +1>(v, buf, tw, lane);
+      float* rn = a.job.grid + (size_t)fld * plane + (size_t)(a.nlat - 1 - i) * N;
+      float* rs = a.job.grid + (size_t)fld * plane + (size_t)i * N;
+      static_for<L>([&](auto q_) {
+        const int k = lane + 64 * decltype(q_)::value;
+        const double2 x = buf[k];
+        rn[k] = (float)x.x;
+        rs[k] = (float)x.y;
+      });
+    }
+    __syncthreads();  // every span is consumed before the next group's runs land
+  }
+}
+
 // L of the wave-resident path for N, or 0 (generic path).
 int wave_L(int N) {
   if (N % 64) return 0;
@@ -931,6 +1176,10 @@ hipError_t launch_persistent(void (*kernel)(Args), int groups, int rb, size_t ld
   return hipLaunchKernel(k, g, b, args, lds, s);
 }
 
+size_t wave64_lds_bytes(int N, int rb) {
+  const int L = N / 64;
+  return ((size_t)N + (size_t)rb * (L * kRowSpan + kSpanPad)) * sizeof(double2);
+}
 size_t fold_wave_lds_bytes(int N, int vector, int rb) {
   const int L = N / 64;
   return ((size_t)N + (size_t)rb * ((vector ? 2 : 1) * L * kRowSpan + kSpanPad)) * sizeof(float2);
@@ -1170,6 +1419,22 @@ int fft_pf(int N) {
   }
 
 extern "C" hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s) {
+  if (a->job.low) {  // FP64 fold of one field of a vector pair (wave-resident N only)
+    const int L = a->fft.wave_L;
+    if (!L || !a->fft.tw64) return hipErrorInvalidValue;
+    const int rb = sht::wave_rb(sht::wave64_lds_bytes(a->fft.N, 8));
+    const int groups = a->job.nfields * (a->JHP / rb);
+    if (groups == 0) return hipSuccess;
+    const size_t lds = sht::wave64_lds_bytes(a->fft.N, rb);
+    hipError_t e = hipSuccess;
+#define SHT_FOLD64(LL)                                                                   \
+  e = sht::launch_persistent(rb == 8 ? sht::rfft_fold_wave64_kernel<LL, 8>               \
+                                     : sht::rfft_fold_wave64_kernel<LL, 4>,              \
+                             groups, rb, lds, s, *a)
+    SHT_L_DISPATCH(L, SHT_FOLD64)
+#undef SHT_FOLD64
+    return e;
+  }
   if (const int L = a->fft.wave_L) {
     const int rb = sht::wave_rb(sht::fold_wave_lds_bytes(a->fft.N, a->job.vector, 8));
     const int groups = a->job.nfields * (a->JHP / rb);
@@ -1195,6 +1460,22 @@ extern "C" hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s
 }
 
 extern "C" hipError_t sht_launch_irfft_unfold(const sht::UnfoldArgs* a, hipStream_t s) {
+  if (a->job.f64) {  // FP64 unfold (wave-resident N only)
+    const int L = a->fft.wave_L;
+    if (!L || !a->fft.tw64 || !a->job.low || !a->job.rowscale64) return hipErrorInvalidValue;
+    const int rb = sht::wave_rb(sht::wave64_lds_bytes(a->fft.N, 8));
+    const int groups = a->job.nfields * ((a->JH + rb - 1) / rb);
+    if (groups == 0) return hipSuccess;
+    const size_t lds = sht::wave64_lds_bytes(a->fft.N, rb);
+    hipError_t e = hipSuccess;
+#define SHT_UNFOLD64(LL)                                                                 \
+  e = sht::launch_persistent(rb == 8 ? sht::irfft_unfold_wave64_kernel<LL, 8>            \
+                                     : sht::irfft_unfold_wave64_kernel<LL, 4>,           \
+                             groups, rb, lds, s, *a)
+    SHT_L_DISPATCH(L, SHT_UNFOLD64)
+#undef SHT_UNFOLD64
+    return e;
+  }
   if (const int L = a->fft.wave_L) {
     const int rb = sht::wave_rb(sht::unfold_wave_lds_bytes(a->fft.N, 8));
     const int groups = a->job.nfields * ((a->JH + rb - 1) / rb);
@@ -1235,7 +1516,11 @@ static hipError_t allow_lds_wave(size_t bytes) {
                         reinterpret_cast<const void*>(sht::irfft_unfold_wave_kernel<L, 4>),
                         reinterpret_cast<const void*>(sht::irfft_unfold_wave_kernel<L, 8>),
                         reinterpret_cast<const void*>(sht::waist_from_rows_kernel<L>),
-                        reinterpret_cast<const void*>(sht::rows_from_waist_kernel<L>)}) {
+                        reinterpret_cast<const void*>(sht::rows_from_waist_kernel<L>),
+                        reinterpret_cast<const void*>(sht::rfft_fold_wave64_kernel<L, 4>),
+                        reinterpret_cast<const void*>(sht::rfft_fold_wave64_kernel<L, 8>),
+                        reinterpret_cast<const void*>(sht::irfft_unfold_wave64_kernel<L, 4>),
+                        reinterpret_cast<const void*>(sht::irfft_unfold_wave64_kernel<L, 8>)}) {
     hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
   }
@@ -1243,6 +1528,12 @@ static hipError_t allow_lds_wave(size_t bytes) {
 }
 
 extern "C" int sht_fft_wave_L(int N) { return sht::wave_L(N); }
+// LDS of the FP64 fold and unfold launches (8 rows per block where they fit, else 4)
+extern "C" size_t sht_fold64_lds_bytes(int N) {
+  const size_t b8 = sht::wave64_lds_bytes(N, 8);
+  return sht::wave_rb(b8) == 8 ? b8 : sht::wave64_lds_bytes(N, 4);
+}
+extern "C" size_t sht_unfold64_lds_bytes(int N) { return sht_fold64_lds_bytes(N); }
 extern "C" size_t sht_fft_wave_lds_bytes(int N, int vector) {
   // the LDS a launch asks for: 8 rows per block where that fits, else 4
   if (vector >= 0) {

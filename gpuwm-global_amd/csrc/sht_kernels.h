@@ -32,6 +32,12 @@ struct TableOut {
   unsigned short* bf_ana[3];
   double* f64_syn[3];         // optional FP64 copy (tests)
   long long elems;            // elements of one table in one layout
+  // FP64 tables of the orders m < low_m0 for the vector operators' FP64
+  // correction, over r = n - m (T+1 slots) and folded rows i (JHP):
+  //   low_syn [m][r][i]   low_ana [m][i][r]
+  double* f64_low_syn[3];
+  double* f64_low_ana[3];
+  int low_m0;
 };
 
 // ---- FFT (sht_fft.hip) ----
@@ -41,6 +47,7 @@ struct FftPlan {
   int nstages;
   int radix[kMaxStages];
   const float2* tw;           // tw[k] = exp(-2 pi i k / N), k < N
+  const double2* tw64;        // the same in FP64 (the FP64 FFTs)
 };
 
 struct FoldJob {
@@ -52,6 +59,15 @@ struct FoldJob {
   int col0;                   // first data column of this job in the waist
   int ldw;                    // floats per data column
   int vector;                 // 0 scalar, 1 vector pair
+  // FP64 fold of one field of a vector pair (rfft_fold_wave64_kernel):
+  // writes i X to columns col0 + fstride f + {0, 1} of the fp32 waist and,
+  // for orders m < m0, the folded (E, O) of X in FP64 to low + f low_stride,
+  // laid out [m][p][i]
+  double2* low;
+  const double* rowscale64;   // per folded row, FP64
+  int fstride;
+  int m0;
+  int low_stride;             // double2 per field
 };
 
 struct FoldArgs {
@@ -73,6 +89,14 @@ struct UnfoldJob {
   int fstride;
   int ldw;
   int imul;                   // multiply the spectrum by i before the FFT
+  // FP64 unfold (irfft_unfold_wave64_kernel): the spectrum is assembled and
+  // inverted in FP64, orders m < m0 read from low + f low_stride ([m][p][i]
+  // folded (E, O) in FP64) instead of the fp32 waist
+  int f64;
+  const double2* low;
+  const double* rowscale64;
+  int m0;
+  int low_stride;
 };
 
 struct UnfoldArgs {
@@ -209,6 +233,29 @@ struct GemmLaunch {
   int ntiles, grid;
 };
 
+// ---- the vector operators' FP64 low orders (sht_lowm.hip) ----
+// Orders m < m0 of uv_from_vordiv and vordiv_from_uv run in FP64 end to end.
+// The low waist of a vector job is [level][f][m][p][i] double2 (complex),
+// f = 0 for u and 1 for v, p the parity, i the folded row (JHP slots).
+struct LowSynArgs {
+  const float2* vort;         // (nlev, T+1, T+1) [n][m]
+  const float2* div;
+  const double* dp;           // low_syn tables [m][r][i]: dP/dphi and P m / cos
+  const double* pm;
+  double2* low;               // (E, O) of U and V without the 1/a the unfold applies
+  double4* pc;                // workspace [level][m][n]: psi (re, im), chi (re, im) in FP64
+  double radius;
+  int T, JH, JHP, m0, nlev;
+};
+struct LowAnaArgs {
+  const double2* low;         // (E, O) of U and V from the FP64 fold, weights applied
+  const double* dp;           // low_ana tables [m][i][r]
+  const double* pm;
+  float2* vort;               // (nlev, T+1, T+1): entries n >= m, m < m0
+  float2* div;
+  int T, JH, JHP, m0, nlev;
+};
+
 // ---- spectral data movement (sht_pack.hip) ----
 enum PackMode : int { kPlain = 0, kIM = 1, kICoef = 2, kVector = 3 };
 
@@ -238,6 +285,8 @@ struct UnpackJob {
   int col0;
   int ldp;
   int vector;                 // columns per level [zeta.re, zeta.im, D.re, D.im]
+  int m0;                     // orders m < m0 inside the triangle are left alone
+                              // (written by the FP64 low-order kernel)
 };
 
 struct UnpackArgs {
@@ -269,6 +318,11 @@ hipError_t sht_launch_sample_pack(const sht::SamplePackArgs* a, hipStream_t s);
 hipError_t sht_launch_sample(const sht::SampleArgs* a, hipStream_t s);
 hipError_t sht_launch_vertical_solve(const sht::VerticalArgs* a, hipStream_t s);
 hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s);
+hipError_t sht_launch_low_synthesis(const sht::LowSynArgs* a, hipStream_t s);
+hipError_t sht_launch_low_analysis(const sht::LowAnaArgs* a, hipStream_t s);
+hipError_t sht_low_allow_lds(size_t bytes);
+size_t sht_fold64_lds_bytes(int N);
+size_t sht_unfold64_lds_bytes(int N);
 hipError_t sht_launch_unpack(const sht::UnpackArgs* a, hipStream_t s);
 hipError_t sht_launch_spectral_elementwise(const float2* in, float2* out, long long total, int T,
                                            int op, float radius, hipStream_t s);

@@ -1,0 +1,321 @@
+// The vector operators' FP64 low orders (uv_from_vordiv, vordiv_from_uv).
+//
+// A wind field's large scales sit in its lowest zonal orders.  Vorticity and
+// divergence at high degree are small differences of those large values
+// weighted by dP/dphi (which grows like n), so fp32 rounding anywhere on the
+// low orders (FFT, waist storage, table values, accumulation) is amplified
+// by ~T/m relative to the field's largest coefficient.  For the orders
+// m < m0 the engine therefore runs the whole Legendre stage in FP64: these
+// kernels contract the FP64 tables of those orders with the FP64 low waist
+// (written by the FP64 fold, read by the FP64 unfold, sht_fft.hip), while
+// the orders m >= m0 go through the grouped GEMM as before.
+//
+// Low waist of a vector job: [level][f][m][p][i] double2 (complex), f = 0
+// for u and 1 for v, p the parity, i the folded row.  Folded row i pairs the
+// north row nlat-1-i with the south row i; E (p = 0) and O (p = 1) are the
+// even and odd parts, north = E + O, south = E - O.  A table of parity
+// (n - m) mod 2 = q contributes to E when q = 0; dP/dphi has the opposite
+// parity of P and P m / cos.
+//
+// Synthesis (reference vector.py:36-52, transform.py:1748-1768), with
+// psi, chi = inverse_laplacian(vorticity, divergence) in FP64:
+//   U = -sum psi dP + i sum chi Pm,   V = i sum psi Pm + sum chi dP
+// (the 1/a is applied by the unfold).  Analysis (vector.py:113-134), with
+// the FP64 fold's weights 2 pi w / (a nlon) already in E and O:
+//   zeta = i sum Pm V_q + sum dP U_(1-q),   D = i sum Pm U_q - sum dP V_(1-q).
+#include <algorithm>
+
+#include "sht_kernels.h"
+
+namespace sht {
+
+// Both directions are small FP64 GEMMs per order m, run on FP64 MFMA
+// (v_mfma_f64_16x16x4_f64: A 16 x 4 with lane l holding A[l % 16][l / 16],
+// B 4 x 16 with lane l holding B[l / 16][l % 16], D 16 x 16 with lane l,
+// register r holding D[l / 16 + 4 r][l % 16] -- rows interleaved, unlike the
+// fp32 16x16x4 MFMA's 4 (l / 16) + r; checked on gfx950).  A block owns one
+// order m, one output parity, 8 levels (32 data columns: two 16-column MFMA
+// tiles) and up to 8 row tiles of 16, one per wavefront.  The B operand of
+// one segment (table kind) at a time is staged in LDS in FP64; each A value
+// a wavefront loads from its table feeds both column tiles.
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kLowCols = 32;    // data columns per block: 8 levels x 4 components
+constexpr int kLowLevels = 8;
+constexpr int kLowWaves = 8;    // wavefronts (row tiles) per block
+constexpr int kLowBatch = 8;    // k-steps of A loaded ahead (32 k)
+constexpr int kLowKPad = 4 * kLowBatch;  // k extents padded to whole batches
+
+__device__ __forceinline__ size_t low_index(int JHP, int m0, int lev, int f, int m, int p, int i) {
+  return ((((size_t)lev * 2 + f) * m0 + m) * 2 + p) * JHP + i;
+}
+
+// acc0 += A B[:, 0:16], acc1 += A B[:, 16:32] over kpad k (multiple of
+// kLowKPad); a(k) gives this lane's A value for k-step row k (k = j0 + lane/16
+// + 4 u), any finite value where the B row is zero.  A values come from
+// global memory 8 k-steps at a time, the next batch in flight while the
+// current one feeds the MFMAs.
+template <class LoadA>
+__device__ __forceinline__ void low_segment(d4& acc0, d4& acc1, const double* b, int kpad,
+                                            int lane, LoadA&& a) {
+  const int kq = lane >> 4, col = lane & 15;
+  double x[kLowBatch], y[kLowBatch];  // two batches in flight, alternating roles
+  auto load = [&](double (&v)[kLowBatch], int j0) {
+#pragma unroll
+    for (int u = 0; u < kLowBatch; ++u) v[u] = a(j0 + 4 * u + kq);
+  };
+  auto mma = [&](const double (&v)[kLowBatch], int j0) {
+#pragma unroll
+    for (int u = 0; u < kLowBatch; ++u) {
+      const double* row = b + (size_t)(j0 + 4 * u + kq) * kLowCols;
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(v[u], row[col], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(v[u], row[16 + col], acc1, 0, 0, 0);
+    }
+  };
+  load(x, 0);
+  for (int j0 = 0; j0 < kpad; j0 += 2 * kLowKPad) {
+    load(y, j0 + kLowKPad);      // past the end: clamped rows, never multiplied
+    mma(x, j0);
+    load(x, j0 + 2 * kLowKPad);
+    if (j0 + kLowKPad < kpad) mma(y, j0 + kLowKPad);
+  }
+}
+
+// Synthesis, output parity p, rows i (folded latitude), columns (level,
+// component c): c = 0, 1 the real and imaginary part of U, c = 2, 3 of V:
+//   out[i][col] = sum_{r of parity p} Pm[r][i] X[r][col] + sum_{r of parity 1-p} dP[r][i] Y[r][col]
+// with X = (i chi, i psi) and Y = (-psi, chi), r = n - m.
+// Grid: x = m0 x row-tile groups, y = level groups of 8, z = parity.
+__global__ void __launch_bounds__(64 * kLowWaves) low_synthesis_kernel(LowSynArgs a) {
+  extern __shared__ double lds_low[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int groups = (a.JH + 16 * kLowWaves - 1) / (16 * kLowWaves);
+  const int m = blockIdx.x / groups;
+  const int it = (blockIdx.x % groups) * kLowWaves + w;  // this wavefront's row tile
+  const bool active = it * 16 < a.JH;                     // wave-uniform
+  const int lev0 = blockIdx.y * kLowLevels;
+  const int p = blockIdx.z;
+  const int T1 = a.T + 1;
+  const int K = T1 - m;
+  const int i = it * 16 + (lane & 15);
+  const int ic = min(i, a.JHP - 1);
+  d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  // segment 0: P m / cos against X over r = 2 j + p; segment 1: dP/dphi
+  // against Y over r = 2 j + 1 - p
+  for (int seg = 0; seg < 2; ++seg) {
+    const int pr = seg == 0 ? p : 1 - p;
+    const int kn = (K - pr + 1) / 2;
+    const int kpad = (kn + kLowKPad - 1) / kLowKPad * kLowKPad;
+    if (seg) __syncthreads();  // the previous segment's B is consumed
+    // one thread per (level, k row): its psi and chi (from the low pack,
+    // contiguous in n) give the four columns of that level; all loads of a
+    // thread are issued before the LDS writes
+    constexpr int kPer = 4;
+    for (int e0 = threadIdx.x; e0 < kpad * kLowLevels; e0 += kPer * blockDim.x) {
+      double4 pc[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = e0 + u * blockDim.x;
+        const int l = e % kLowLevels, j = e / kLowLevels;
+        const int lev = min(lev0 + l, a.nlev - 1);
+        const int n = min(m + 2 * j + pr, a.T);
+        pc[u] = a.pc[((size_t)lev * a.m0 + m) * T1 + n];
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = e0 + u * blockDim.x;
+        const int l = e % kLowLevels, j = e / kLowLevels;
+        if (j >= kpad) continue;
+        const bool ok = j < kn && lev0 + l < a.nlev;
+        const double psr = ok ? pc[u].x : 0.0, psi = ok ? pc[u].y : 0.0;
+        const double chr = ok ? pc[u].z : 0.0, chi = ok ? pc[u].w : 0.0;
+        double* dst = lds_low + (size_t)j * kLowCols + 4 * l;
+        if (seg == 0) {  // X: U <- i chi, V <- i psi
+          dst[0] = -chi;
+          dst[1] = chr;
+          dst[2] = -psi;
+          dst[3] = psr;
+        } else {         // Y: U <- -psi, V <- chi
+          dst[0] = -psr;
+          dst[1] = -psi;
+          dst[2] = chr;
+          dst[3] = chi;
+        }
+      }
+    }
+    __syncthreads();
+    if (active && kn > 0) {
+      const double* tab = (seg == 0 ? a.pm : a.dp) + (size_t)m * T1 * a.JHP + ic;
+      // rows past the segment meet zero B rows, so any finite table value
+      // will do there: clamped addresses, no select (a select right after a
+      // load makes the compiler wait for it before issuing the next)
+      low_segment(acc0, acc1, lds_low, kpad, lane, [&](int j) {
+        return tab[(size_t)min(2 * j + pr, a.T) * a.JHP];
+      });
+    }
+  }
+  if (!active) return;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int col = 16 * h + (lane & 15);
+    const int lev = lev0 + col / 4, c = col % 4;
+    if (lev >= a.nlev) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int io = it * 16 + (lane >> 4) + 4 * q;
+      if (io < a.JH) {
+        double* dst = reinterpret_cast<double*>(
+            a.low + low_index(a.JHP, a.m0, lev, c >> 1, m, p, io));
+        dst[c & 1] = h ? acc1[q] : acc0[q];
+      }
+    }
+  }
+}
+
+// Analysis, output parity q, rows r = 2 j + q, columns (level, component):
+// c = 0, 1 the real and imaginary part of vorticity, c = 2, 3 of divergence:
+//   out[r][col] = sum_i Pm[r][i] B1[i][col] + sum_i dP[r][i] B2[i][col]
+// with B1 = (i V_q, i U_q) and B2 = (U_(1-q), -V_(1-q)) from the FP64 fold.
+// Grid: x = m0 x row-tile groups, y = level groups of 8, z = parity.
+__global__ void __launch_bounds__(64 * kLowWaves) low_analysis_kernel(LowAnaArgs a) {
+  extern __shared__ double lds_low[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int T1 = a.T + 1;
+  const int rows_max = (T1 + 1) / 2;                      // rows of parity 0 at m = 0
+  const int groups = (rows_max + 16 * kLowWaves - 1) / (16 * kLowWaves);
+  const int m = blockIdx.x / groups;
+  const int it = (blockIdx.x % groups) * kLowWaves + w;
+  const int lev0 = blockIdx.y * kLowLevels;
+  const int q = blockIdx.z;
+  const int K = T1 - m;
+  const int KQ = (K - q + 1) / 2;                         // rows of parity q
+  const bool active = it * 16 < KQ;                       // wave-uniform
+  const int IP = (a.JH + kLowKPad - 1) / kLowKPad * kLowKPad;
+  const int j = it * 16 + (lane & 15);
+  const int r = 2 * min(j, max(KQ - 1, 0)) + q;
+  d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  for (int seg = 0; seg < 2; ++seg) {
+    if (seg) __syncthreads();
+    // one thread per (level, folded row): the U and V values of the
+    // segment's parity give the four columns of that level; loads first
+    constexpr int kPer = 4;
+    const int pq = seg == 0 ? q : 1 - q;
+    for (int e0 = threadIdx.x; e0 < IP * kLowLevels; e0 += kPer * blockDim.x) {
+      double2 uu[kPer], vv[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = e0 + u * blockDim.x;
+        const int i = min(e % IP, a.JH - 1), l = e / IP;
+        const int lev = min(lev0 + l, a.nlev - 1);
+        uu[u] = a.low[low_index(a.JHP, a.m0, lev, 0, m, pq, i)];
+        vv[u] = a.low[low_index(a.JHP, a.m0, lev, 1, m, pq, i)];
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = e0 + u * blockDim.x;
+        const int i = e % IP, l = e / IP;
+        if (l >= kLowLevels) continue;
+        const bool ok = i < a.JH && lev0 + l < a.nlev;
+        const double2 U = ok ? uu[u] : make_double2(0.0, 0.0);
+        const double2 V = ok ? vv[u] : make_double2(0.0, 0.0);
+        double* dst = lds_low + (size_t)i * kLowCols + 4 * l;
+        if (seg == 0) {  // B1: (i V_q, i U_q)
+          dst[0] = -V.y;
+          dst[1] = V.x;
+          dst[2] = -U.y;
+          dst[3] = U.x;
+        } else {         // B2: (U_(1-q), -V_(1-q))
+          dst[0] = U.x;
+          dst[1] = U.y;
+          dst[2] = -V.x;
+          dst[3] = -V.y;
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+      const double* tab = (seg == 0 ? a.pm : a.dp) + (size_t)m * a.JHP * T1 + r;
+      low_segment(acc0, acc1, lds_low, IP, lane, [&](int i) {  // past JH: zero B rows
+        return tab[(size_t)min(i, a.JH - 1) * T1];
+      });
+    }
+  }
+  if (!active) return;
+  const size_t plane = (size_t)T1 * T1;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int col = 16 * h + (lane & 15);
+    const int lev = lev0 + col / 4, c = col % 4;
+    if (lev >= a.nlev) continue;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const int jo = it * 16 + (lane >> 4) + 4 * qq;
+      if (jo < KQ) {
+        const int n = m + 2 * jo + q;
+        float* dst = reinterpret_cast<float*>((c < 2 ? a.vort : a.div) + (size_t)lev * plane +
+                                              (size_t)n * T1 + m);
+        dst[c & 1] = (m == 0 && (c & 1)) ? 0.f : (float)(h ? acc1[qq] : acc0[qq]);
+      }
+    }
+  }
+}
+
+// psi, chi = inverse Laplacian (vorticity, divergence) of the orders m < m0
+// in FP64 (m = 0 realified, degree 0 zero), laid out [level][m][n] so the
+// synthesis kernel stages them with loads that run along n.
+__global__ void low_pack_kernel(LowSynArgs a) {
+  const int T1 = a.T + 1;
+  const long long total = (long long)a.nlev * a.m0 * T1;
+  const double a2 = a.radius * a.radius;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int n = (int)(e % T1);
+    const int m = (int)((e / T1) % a.m0);
+    const int lev = (int)(e / ((long long)T1 * a.m0));
+    double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (n >= 1 && m <= n) {
+      const double f = -a2 / ((double)n * ((double)n + 1.0));
+      const size_t o = ((size_t)lev * T1 + n) * T1 + m;
+      const float2 z = a.vort[o], d = a.div[o];
+      v = make_double4(f * (double)z.x, m == 0 ? 0.0 : f * (double)z.y, f * (double)d.x,
+                       m == 0 ? 0.0 : f * (double)d.y);
+    }
+    a.pc[e] = v;
+  }
+}
+
+}  // namespace sht
+
+extern "C" hipError_t sht_launch_low_synthesis(const sht::LowSynArgs* a, hipStream_t s) {
+  if (a->nlev <= 0 || a->m0 <= 0) return hipSuccess;
+  const int m0 = std::min(a->m0, a->T + 1);
+  const int groups = (a->JH + 16 * sht::kLowWaves - 1) / (16 * sht::kLowWaves);
+  const int KP = ((a->T + 2) / 2 + sht::kLowKPad - 1) / sht::kLowKPad * sht::kLowKPad;
+  const size_t lds = (size_t)KP * sht::kLowCols * sizeof(double);  // one segment's B
+  const dim3 g(m0 * groups, (a->nlev + sht::kLowLevels - 1) / sht::kLowLevels, 2);
+  const long long total = (long long)a->nlev * a->m0 * (a->T + 1);
+  hipLaunchKernelGGL(sht::low_pack_kernel, dim3((unsigned)std::min<long long>((total + 255) / 256, 4096)),
+                     dim3(256), 0, s, *a);
+  hipLaunchKernelGGL(sht::low_synthesis_kernel, g, dim3(64 * sht::kLowWaves), lds, s, *a);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t sht_launch_low_analysis(const sht::LowAnaArgs* a, hipStream_t s) {
+  if (a->nlev <= 0 || a->m0 <= 0) return hipSuccess;
+  const int m0 = std::min(a->m0, a->T + 1);
+  const int rows_max = (a->T + 2) / 2;
+  const int groups = (rows_max + 16 * sht::kLowWaves - 1) / (16 * sht::kLowWaves);
+  const int IP = (a->JH + sht::kLowKPad - 1) / sht::kLowKPad * sht::kLowKPad;
+  const size_t lds = (size_t)IP * sht::kLowCols * sizeof(double);
+  const dim3 g(m0 * groups, (a->nlev + sht::kLowLevels - 1) / sht::kLowLevels, 2);
+  hipLaunchKernelGGL(sht::low_analysis_kernel, g, dim3(64 * sht::kLowWaves), lds, s, *a);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t sht_low_allow_lds(size_t bytes) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(sht::low_synthesis_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(sht::low_analysis_kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}

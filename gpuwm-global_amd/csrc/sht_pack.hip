@@ -195,6 +195,7 @@ __global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs a) {
       const int mi = e % kTile, r = e / kTile;
       const int n = n0 + r, m = m0 + mi;
       if (n > T || m > T) continue;
+      if (m < a.job.m0 && m <= n) continue;  // the FP64 low-order kernel's entries
       const float2 v = m <= n ? make_float2(tile[2 * o][r][mi], tile[2 * o + 1][r][mi])
                               : make_float2(0.f, 0.f);
       out[(size_t)n * T1 + m] = v;

@@ -160,6 +160,17 @@ __global__ void legendre_tables_kernel(int T, int nlat, int JH, int JHP,
     store_value(out, 0, plain_idx, syn_idx, ana_idx, p);
     store_value(out, 1, plain_idx, syn_idx, ana_idx, d);
     store_value(out, 2, plain_idx, syn_idx, ana_idx, pm);
+    if (m < out.low_m0) {
+      const int r = n - m;
+      const long long ls = ((long long)m * (T + 1) + r) * JHP + i;
+      const long long la = ((long long)m * JHP + i) * (T + 1) + r;
+      const double vk[3] = {p, d, pm};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (out.f64_low_syn[k]) out.f64_low_syn[k][ls] = vk[k];
+        if (out.f64_low_ana[k]) out.f64_low_ana[k][la] = vk[k];
+      }
+    }
     p2 = p1;
     p1 = p;
   }

@@ -68,6 +68,17 @@ typedef struct arwen_sht_plan_desc {
   const double* sin_lat;  /* optional HOST nodes, ascending; NULL: the device
                              Newton iteration computes them                  */
   const double* weights;  /* HOST quadrature weights (with sin_lat)          */
+  int fp64_orders;        /* vector operators (uv_from_vordiv, vordiv_from_uv):
+                             zonal orders m < fp64_orders run in FP64 end to
+                             end (FP64 longitude FFTs for u and v, FP64
+                             tables and sums for those orders); the rest use
+                             the plan's precision.  -1: the default, 0 for
+                             ARWEN_SHT_FP32 and 4 ceil((T+1)/192) for the
+                             split-BF16 modes; 0: off; k > 0: rounded up to
+                             a multiple of 4.  Needs nlon = 64 L with
+                             L in {2,3,4,6,8,12,16,24}; other grids run with
+                             0.  No reference counterpart: the reference's
+                             float32 path runs these orders in fp32.        */
 } arwen_sht_plan_desc;
 
 /* Build a plan: Gauss nodes and weights (FP64 Newton on the device) and the
@@ -93,6 +104,10 @@ int arwen_sht_gauss_nodes(int nlat, double* sin_lat, double* weights);
 /* Resident device bytes of the plan's Legendre tables.  Replaces
  * legendre_table_nbytes (reference transform.py:935-951). */
 size_t arwen_sht_plan_table_bytes(const arwen_sht_plan* plan);
+
+/* The plan's FP64 vector orders (arwen_sht_plan_desc.fp64_orders as
+ * resolved: 0 or a multiple of 4). */
+int arwen_sht_plan_fp64_orders(const arwen_sht_plan* plan);
 
 /* FP64 Legendre table (kind 0 = P, 1 = dP/dphi, 2 = P*m/cos) for the folded
  * north rows, written to HOST memory `out` as out[m][n][i] (zero for n < m),

@@ -22,7 +22,7 @@ from _helpers import hip_transform, rel_err
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"float32": 2e-6, "bf16x6": 2e-6, "bf16x3": 6e-5}
+TOL = {"float32": 1e-6, "bf16x6": 1e-6, "bf16x3": 6e-5}
 
 
 @pytest.fixture(scope="module")
@@ -211,10 +211,9 @@ def test_t383_headline_schedule_matches_the_oracle():
     ez, ed = rel_err(ref["z2"], oz), rel_err(ref["d2"], od)
     print(f"T383 headline schedule: vorticity {ez:.2e}, divergence {ed:.2e}")
     # the vector analysis weights the winds by m / cos and the derivative
-    # table, which at T383 lifts the fp32-class error above the scalar bound
-    # (2.4e-6 / 3.1e-6 measured; the FP32 path's own vector round trip at
-    # T383 is 9.7e-6 / 4.5e-5), so its bound is five times the scalar one
-    assert max(ez, ed) < 5 * TOL["bf16x6"]
+    # table; its 8 lowest orders run in FP64 (fp64_orders), which keeps it
+    # within the scalar bound (2.4e-6 / 3.1e-6 without them)
+    assert max(ez, ed) < TOL["bf16x6"]
     for cols in ((32, 32), (64, 64), (96, 32)):
         batch.set_config(cols=cols)
         _run(batch)
@@ -230,3 +229,122 @@ def test_t383_headline_schedule_matches_the_oracle():
         torch.cuda.synchronize()
         for k, v in _outputs(bufs).items():
             assert torch.equal(v, ref[k]), ("graph", rep, k)
+
+
+def _member_batch(specs, precision, levels):
+    """Round-trip jobs (wind + inverse, then vordiv + forward) for every
+    instance of ``specs`` = [(T, instances)], each instance with ``levels``
+    levels of vorticity, divergence and temperature plus ln ps; instance k of
+    truncation T draws its red spectra from seeds 1000 T + 10 k + {1, 2, 3}."""
+    import torch
+    from arwen_spectral import SphericalHarmonicTransform
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    c64 = lambda a: torch.from_numpy(np.ascontiguousarray(a).astype(np.complex64)).to(dev)
+    parts, jobs = [], []
+    for T, count in specs:
+        tr = SphericalHarmonicTransform.create(T, dealias_factor=1.0, precision=precision)
+        g = lambda n: torch.empty((n, *tr.grid.shape), dtype=torch.float32, device=dev)
+        s = lambda n: torch.empty((n, *tr.spectral_shape), dtype=torch.complex64, device=dev)
+        for k in range(count):
+            seed = 1000 * T + 10 * k
+            zeta = red_spectrum(T, (levels,), seed=seed + 1, scale=1e-5)
+            div = red_spectrum(T, (levels,), seed=seed + 2, scale=2e-6)
+            zeta[..., 0, 0] = 0.0
+            div[..., 0, 0] = 0.0
+            scal = red_spectrum(T, (levels + 1,), seed=seed + 3, scale=1.0)
+            b = dict(T=T, tr=tr, zeta=c64(zeta), div=c64(div), scal=c64(scal),
+                     u=g(levels), v=g(levels), grid=g(levels + 1),
+                     z2=s(levels), d2=s(levels), s2=s(levels + 1))
+            jobs += [("wind", tr, (b["zeta"], b["div"]), (b["u"], b["v"])),
+                     ("inverse", tr, b["scal"], b["grid"]),
+                     ("vordiv", tr, (b["u"], b["v"]), (b["z2"], b["d2"])),
+                     ("forward", tr, b["grid"], b["s2"])]
+            parts.append(b)
+    return parts, jobs
+
+
+def _check_against_single_batches(parts, split_acc=0):
+    """Every instance of a combined batch is bit-identical to the same jobs
+    run as a batch of their own (with the same accumulator split; column
+    tiles and grids never change bits)."""
+    import torch
+    from arwen_spectral import TransformBatch
+
+    for b in parts:
+        out = {k: b[k].clone() for k in ("u", "v", "grid", "z2", "d2", "s2")}
+        tr = b["tr"]
+        alone = TransformBatch([("wind", tr, (b["zeta"], b["div"]), (b["u"], b["v"])),
+                                ("inverse", tr, b["scal"], b["grid"]),
+                                ("vordiv", tr, (b["u"], b["v"]), (b["z2"], b["d2"])),
+                                ("forward", tr, b["grid"], b["s2"])])
+        alone.set_config(split_acc=split_acc)
+        for k in out:
+            b[k].fill_(float("nan"))
+        _run(alone)
+        for k, v in out.items():
+            assert torch.equal(b[k], v), (b["T"], k)
+
+
+def _check_against_oracle(b, o):
+    zeta, div = b["zeta"].cpu().numpy(), b["div"].cpu().numpy()
+    scal = b["scal"].cpu().numpy()
+    u64, v64 = o.wind_from_vordiv(zeta, div)
+    z64, d64 = o.vordiv_from_wind(u64, v64)
+    g64 = o.inverse(scal)
+    errs = {"u": rel_err(b["u"], u64), "v": rel_err(b["v"], v64), "grid": rel_err(b["grid"], g64),
+            "vorticity": rel_err(b["z2"], z64), "divergence": rel_err(b["d2"], d64),
+            "scalars": rel_err(b["s2"], o.forward(g64))}
+    for k, e in errs.items():
+        assert e < TOL["bf16x6"], (b["T"], k, e)
+    return errs
+
+
+def test_ensemble_of_32_members_in_one_launch():
+    """C3 at reduced depth: 32 T383 members (4 levels each) in one batch, one
+    grouped GEMM launch per direction, autotuned and replayed from a graph.
+    Members 0 and 31 match the FP64 oracle within 1e-6; every member is
+    bit-identical to its own single-member batch."""
+    import torch
+    from arwen_spectral import TransformBatch
+
+    parts, jobs = _member_batch([(383, 32)], "bf16x6", levels=4)
+    batch = TransformBatch(jobs)
+    batch.set_config(split_acc=0)
+    batch.autotune(reps=1)
+    batch.capture()
+    batch.launch()
+    torch.cuda.synchronize()
+    o = OracleTransform.create(383, dealias_factor=1.0)
+    for k in (0, 31):
+        errs = _check_against_oracle(parts[k], o)
+        print(f"member {k}: " + ", ".join(f"{n} {e:.2e}" for n, e in errs.items()))
+    _check_against_single_batches(parts[:2] + parts[-2:])
+
+
+def test_mixed_truncations_in_one_launch():
+    """C4 at reduced depth: 4 T767 + 8 T383 + 16 T127 instances (2 levels
+    each) in one batch.  The first and last T127 and T383 instances match the
+    FP64 oracle within 1e-6; the T767 instances are bit-identical to a batch
+    of their own and their scalar and vector round trips return their inputs
+    within 1e-6; every instance checked is bit-identical to its own batch."""
+    import torch
+    from arwen_spectral import TransformBatch
+
+    parts, jobs = _member_batch([(767, 4), (383, 8), (127, 16)], "bf16x6", levels=2)
+    batch = TransformBatch(jobs)
+    batch.set_config(split_acc=0)
+    batch.autotune(reps=1)
+    _run(batch)
+    for T in (127, 383):
+        o = OracleTransform.create(T, dealias_factor=1.0)
+        mine = [b for b in parts if b["T"] == T]
+        for b in (mine[0], mine[-1]):
+            errs = _check_against_oracle(b, o)
+            print(f"T{T}: " + ", ".join(f"{n} {e:.2e}" for n, e in errs.items()))
+    for b in (b for b in parts if b["T"] == 767):
+        assert rel_err(b["z2"], b["zeta"].cpu().numpy()) < TOL["bf16x6"]
+        assert rel_err(b["d2"], b["div"].cpu().numpy()) < TOL["bf16x6"]
+        assert rel_err(b["s2"], b["scal"].cpu().numpy()) < TOL["bf16x6"]
+    checked = [b for b in parts if b["T"] == 767] + [parts[4], parts[11], parts[12], parts[-1]]
+    _check_against_single_batches(checked)

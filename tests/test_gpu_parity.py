@@ -3,7 +3,7 @@
 Inputs are seeded; the oracle runs the reference's numpy algorithm in
 float64.  The error measure is max|engine - oracle| / max|oracle| over the
 compared array.  Tolerances per precision mode:
-  float32 2e-6, bf16x6 2e-6, bf16x3 6e-5
+  float32 1e-6, bf16x6 1e-6 (the north star's bound), bf16x3 6e-5
 (float32 rounding of the data and tables alone gives a few 1e-7 at T63).
 """
 from __future__ import annotations
@@ -22,7 +22,7 @@ from _helpers import hip_transform, rel_err
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"float32": 2e-6, "bf16x6": 2e-6, "bf16x3": 6e-5}
+TOL = {"float32": 1e-6, "bf16x6": 1e-6, "bf16x3": 6e-5}
 PRECISIONS = ["float32", "bf16x6", "bf16x3"]
 RECORD = {}
 
@@ -204,7 +204,7 @@ def test_t383_roundtrip_and_parity(precision):
     back = tr.forward(grid)
     err = rel_err(back, coeff)
     _record(f"roundtrip383_{precision}", err)
-    assert err < 2 * tol
+    assert err < tol
     check = tr.transform_check(seed=1)
     _record(f"check383_{precision}", check)
     assert check["roundtrip_relative_linf"] < (1e-6 if precision != "bf16x3" else 1e-4)
@@ -216,8 +216,9 @@ def test_t767_roundtrips_on_the_longest_rows(precision):
     wave-resident FFT, and the u,v analysis there is the one launch whose
     eight-row blocks do not fit LDS (it runs four rows per block).  No FP64
     oracle at this size: round trips against the inputs instead, with the
-    bounds of the T383 round trips (scalars) and of the measured vector
-    amplification floor (about 3.5e-5 for D at T767)."""
+    bounds of the T383 round trips (scalars).  Vectors: bf16x6 runs its 16
+    lowest orders in FP64 and is held to 1e-6; the plain FP32 form keeps
+    its measured amplification floor (about 3.5e-5 for D at T767)."""
     from arwen_spectral import SphericalHarmonicTransform, VorticityDivergenceOperator
 
     tr = SphericalHarmonicTransform.create(767, dealias_factor=1.0, backend="hip",
@@ -227,7 +228,7 @@ def test_t767_roundtrips_on_the_longest_rows(precision):
     back = tr.forward(tr.inverse(coeff))
     err = rel_err(back, coeff)
     _record(f"roundtrip767_{precision}", err)
-    assert err < 2 * TOL[precision]
+    assert err < TOL[precision]
     op = VorticityDivergenceOperator(tr)
     zeta = red_spectrum(767, (2,), seed=8, scale=1e-5)
     div = red_spectrum(767, (2,), seed=9, scale=1e-5)
@@ -237,7 +238,7 @@ def test_t767_roundtrips_on_the_longest_rows(precision):
     z2, d2 = op.vordiv_from_wind(u, v)
     err = max(rel_err(z2, zeta), rel_err(d2, div))
     _record(f"vector_roundtrip767_{precision}", err)
-    assert err < 1e-4
+    assert err < (1e-6 if precision == "bf16x6" else 1e-4)
 
 
 def test_t767_on_the_native_1536_grid():
@@ -246,7 +247,9 @@ def test_t767_on_the_native_1536_grid():
     cos(5 lon) cos(3 lat) analyses to order 5 only.  nlon = 3072 takes the
     block Stockham FFT, and the u,v analysis's panels carry the longest step
     lists (two segments of 24 k-chunks).  Round trips against the inputs, as
-    in the T767 test above."""
+    in the T767 test above.  nlon = 3072 has no wave-resident FFT, so no FP64
+    vector orders either (fp64_orders resolves to 0): the vector round trip
+    keeps the plain form's amplification floor."""
     from arwen_spectral import GaussianGrid, SphericalHarmonicTransform, VorticityDivergenceOperator
 
     native = GaussianGrid.for_shape(1536, 3072)
@@ -259,7 +262,7 @@ def test_t767_on_the_native_1536_grid():
     coeff = red_spectrum(767, (2,), seed=17, scale=1.0)
     err = rel_err(tr.forward(tr.inverse(coeff)), coeff)
     _record("roundtrip767_native_bf16x6", err)
-    assert err < 2 * TOL["bf16x6"]
+    assert err < TOL["bf16x6"]
     op = VorticityDivergenceOperator(tr)
     zeta = red_spectrum(767, (2,), seed=18, scale=1e-5)
     div = red_spectrum(767, (2,), seed=19, scale=1e-5)
