@@ -30,6 +30,8 @@ OP_VORDIV_FROM_WAIST = 12
 WAIST_BASIS, WAIST_ZONAL, WAIST_DERIVATIVE = 0, 1, 2
 
 SPECTRAL_PROJECT, SPECTRAL_LAPLACIAN, SPECTRAL_INVERSE_LAPLACIAN = 0, 1, 2
+#: ARWEN_SHT_SAMPLE_STATIC_BYTES: the static sampler's per-GPU workspace array
+SAMPLE_STATIC_BYTES = 64 << 20
 
 #: Every symbol the header declares; the CPU suite checks the library exports them.
 EXPORTS = (
@@ -41,6 +43,7 @@ EXPORTS = (
     "arwen_sht_vordiv_from_uv", "arwen_sht_synthesis_to_waist", "arwen_sht_analysis_from_waist",
     "arwen_sht_vordiv_from_waist", "arwen_sht_waist_from_rows", "arwen_sht_rows_from_waist",
     "arwen_sht_spectral_op", "arwen_sht_sample_workspace_bytes", "arwen_sht_sample",
+    "arwen_sht_sample_static",
     "arwen_sht_vertical_solve", "arwen_sht_vertical_tendencies", "arwen_sht_vertical_half_apply",
     "arwen_sht_batch_create",
     "arwen_sht_batch_destroy", "arwen_sht_batch_run", "arwen_sht_batch_capture",
@@ -137,6 +140,7 @@ def load():
             "arwen_sht_spectral_op": (I, [P, I, P, P, I, P]),
             "arwen_sht_sample_workspace_bytes": (ctypes.c_size_t, [P, I, I, I]),
             "arwen_sht_sample": (I, [P, P, P, I, P, P, P, I, I, I, P, ctypes.c_size_t, P, P, P]),
+            "arwen_sht_sample_static": (I, [P, P, P, I, P, P, P, I, I, I, P, P, P]),
             "arwen_sht_vertical_solve": (I, [P, I, P, P, P, P, P, P, P, P, F, F, F, F, F,
                                              P, P, P, P]),
             "arwen_sht_vertical_tendencies": (I, [P, I, P, P, P, P, P, P, F, P, P, P, P]),

@@ -32,7 +32,7 @@ COMMON = [
 ]
 EXTRA = {"sht_tables.hip": ["-ffp-contract=off"], "sht_sample.hip": ["-ffp-contract=off"]}
 SOURCES = ["sht_tables.hip", "sht_fft.hip", "sht_gemm.hip", "sht_pack.hip", "sht_sample.hip",
-           "sht_vertical.hip", "sht_lowm.hip", "sht_capi.cpp"]
+           "sht_vertical.hip", "sht_lowm.hip", "sht_static.cpp", "sht_capi.cpp"]
 
 
 def hipcc() -> str:

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "sht_kernels.h"
+#include "sht_static.h"
 
 namespace {
 
@@ -335,6 +336,8 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   }
   // the point sampler's recurrence coefficients, so that sampling allocates nothing
   build_sample_tables(P);
+  // this device's copy of arwen_sht_sample_static's array, resolved once
+  hip_check(sht_static_setup(P->device), "static sampler workspace");
   hip_check(hipDeviceSynchronize(), "plan build");
 }
 
@@ -1270,33 +1273,83 @@ size_t arwen_sht_sample_workspace_bytes(const arwen_sht_plan* P, int count, int 
   return bytes;
 }
 
+namespace {
+// The sampler's argument checks; returns false when there is nothing to do.
+bool sample_checks(const arwen_sht_plan* P, const void* in0, const void* in1, int count,
+                   const double* sin_lat, const double* cos_lat, const double* lon_rad,
+                   int npoints, int kind, int precision, const double* out0, const double* out1) {
+  if (!P) fail(ARWEN_SHT_EINVAL, "plan is NULL");
+  if (kind < 0 || kind > 2)
+    fail(ARWEN_SHT_EINVAL, "sample kind must be 0 (values), 1 (gradient) or 2 (wind), got " +
+                               std::to_string(kind));
+  if (precision != 0 && precision != 1)
+    fail(ARWEN_SHT_EINVAL, "sample precision must be 0 (float64) or 1 (float32), got " +
+                               std::to_string(precision));
+  if (count < 0 || npoints < 0) fail(ARWEN_SHT_EINVAL, "count and npoints must be >= 0");
+  if ((long long)count * npoints == 0) return false;
+  if (!in0 || (kind == 2 && !in1)) fail(ARWEN_SHT_EINVAL, "an input field is NULL");
+  if (!sin_lat || !lon_rad || (kind != 0 && !cos_lat))
+    fail(ARWEN_SHT_EINVAL, "a point coordinate array is NULL");
+  if (!out0 || (kind != 0 && !out1)) fail(ARWEN_SHT_EINVAL, "an output is NULL");
+  int dev;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev != P->device) fail(ARWEN_SHT_EINVAL, "plan used on a device other than its own");
+  if (!P->sample_rec.p || !P->sample_diag.p)
+    fail(ARWEN_SHT_EHIP, "the plan's sampler tables are missing");
+  return true;
+}
+
+// The pack and sampler launches with the workspace at `workspace` (at least
+// arwen_sht_sample_workspace_bytes of the call); allocates nothing.
+void sample_launch(arwen_sht_plan* P, const void* in0, const void* in1, int count,
+                   const double* sin_lat, const double* cos_lat, const double* lon_rad,
+                   int npoints, int kind, int precision, void* workspace, double* out0,
+                   double* out1, hipStream_t st);
+}  // namespace
+
+int arwen_sht_sample_static(arwen_sht_plan* P, const void* in0, const void* in1, int count,
+                            const double* sin_lat, const double* cos_lat, const double* lon_rad,
+                            int npoints, int kind, int precision, double* out0, double* out1,
+                            void* s) {
+  return guarded([&] {
+    if (!sample_checks(P, in0, in1, count, sin_lat, cos_lat, lon_rad, npoints, kind, precision,
+                       out0, out1))
+      return;
+    const size_t need = arwen_sht_sample_workspace_bytes(P, count, kind, npoints);
+    if (need > ARWEN_SHT_SAMPLE_STATIC_BYTES)
+      fail(ARWEN_SHT_EINVAL, "the sample needs a " + std::to_string(need) +
+                                 "-byte workspace, more than the static array's " +
+                                 std::to_string((size_t)ARWEN_SHT_SAMPLE_STATIC_BYTES) +
+                                 " bytes: use arwen_sht_sample or fewer points");
+    void* scratch = sht_static_scratch(P->device);
+    if (!scratch) fail(ARWEN_SHT_EHIP, "the static sampler workspace of this device is unresolved");
+    sample_launch(P, in0, in1, count, sin_lat, cos_lat, lon_rad, npoints, kind, precision,
+                  scratch, out0, out1, static_cast<hipStream_t>(s));
+  });
+}
+
 int arwen_sht_sample(arwen_sht_plan* P, const void* in0, const void* in1, int count,
                      const double* sin_lat, const double* cos_lat, const double* lon_rad,
                      int npoints, int kind, int precision, void* workspace,
                      size_t workspace_bytes, double* out0, double* out1, void* s) {
   return guarded([&] {
-    if (!P) fail(ARWEN_SHT_EINVAL, "plan is NULL");
-    if (kind < 0 || kind > 2)
-      fail(ARWEN_SHT_EINVAL, "sample kind must be 0 (values), 1 (gradient) or 2 (wind), got " +
-                                 std::to_string(kind));
-    if (precision != 0 && precision != 1)
-      fail(ARWEN_SHT_EINVAL, "sample precision must be 0 (float64) or 1 (float32), got " +
-                                 std::to_string(precision));
-    if (count < 0 || npoints < 0) fail(ARWEN_SHT_EINVAL, "count and npoints must be >= 0");
-    if ((long long)count * npoints == 0) return;
-    if (!in0 || (kind == 2 && !in1)) fail(ARWEN_SHT_EINVAL, "an input field is NULL");
-    if (!sin_lat || !lon_rad || (kind != 0 && !cos_lat))
-      fail(ARWEN_SHT_EINVAL, "a point coordinate array is NULL");
-    if (!out0 || (kind != 0 && !out1)) fail(ARWEN_SHT_EINVAL, "an output is NULL");
+    if (!sample_checks(P, in0, in1, count, sin_lat, cos_lat, lon_rad, npoints, kind, precision,
+                       out0, out1))
+      return;
     const size_t need = arwen_sht_sample_workspace_bytes(P, count, kind, npoints);
     if (!workspace || workspace_bytes < need)
       fail(ARWEN_SHT_EINVAL, "sample workspace must hold " + std::to_string(need) + " bytes");
-    int dev;
-    hip_check(hipGetDevice(&dev), "hipGetDevice");
-    if (dev != P->device) fail(ARWEN_SHT_EINVAL, "plan used on a device other than its own");
-    if (!P->sample_rec.p || !P->sample_diag.p)
-      fail(ARWEN_SHT_EHIP, "the plan's sampler tables are missing");
-    hipStream_t st = static_cast<hipStream_t>(s);
+    sample_launch(P, in0, in1, count, sin_lat, cos_lat, lon_rad, npoints, kind, precision,
+                  workspace, out0, out1, static_cast<hipStream_t>(s));
+  });
+}
+
+namespace {
+void sample_launch(arwen_sht_plan* P, const void* in0, const void* in1, int count,
+                   const double* sin_lat, const double* cos_lat, const double* lon_rad,
+                   int npoints, int kind, int precision, void* workspace, double* out0,
+                   double* out1, hipStream_t st) {
+  {
     const int ldc = sample_columns(count, kind);
     sht::SamplePackArgs pa{};
     pa.in0 = static_cast<const float2*>(in0);
@@ -1332,8 +1385,9 @@ int arwen_sht_sample(arwen_sht_plan* P, const void* in0, const void* in1, int co
     a.kind = kind;
     a.f32 = precision;
     hip_check(sht_launch_sample(&a, st), "sample launch");
-  });
+  }
 }
+}  // namespace
 
 int arwen_sht_vertical_solve(arwen_sht_plan* P, int nlev, const void* div, const void* theta,
                              const void* lnps, const float* b_matrix, const float* c_matrix,

@@ -194,6 +194,21 @@ int arwen_sht_sample(arwen_sht_plan* plan, const void* in0, const void* in1, int
                      int npoints, int kind, int precision, void* workspace,
                      size_t workspace_bytes, double* out0, double* out1, void* stream);
 
+/* arwen_sht_sample without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * workspace (arwen_sht_sample_workspace_bytes of the call) lives in a
+ * library-owned array of ARWEN_SHT_SAMPLE_STATIC_BYTES per GPU, held until
+ * the process ends; a call that needs more returns ARWEN_SHT_EINVAL without
+ * launching anything.  The call allocates nothing.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race.  Results are bit-identical to arwen_sht_sample's. */
+#define ARWEN_SHT_SAMPLE_STATIC_BYTES (64u << 20)
+int arwen_sht_sample_static(arwen_sht_plan* plan, const void* in0, const void* in1, int count,
+                            const double* sin_lat, const double* cos_lat, const double* lon_rad,
+                            int npoints, int kind, int precision, double* out0, double* out1,
+                            void* stream);
+
 /* The semi-implicit vertical-mode solves on spectral columns (reference
  * semi_implicit.py: VerticalModeSemiImplicit.solve_shifted :921-959 and
  * apply :887-919, each followed by _finish :815-843).  Inputs (device):
