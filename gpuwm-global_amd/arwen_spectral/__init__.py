@@ -15,8 +15,10 @@ from __future__ import annotations
 
 from .backend import Backend, get_backend
 from .batch import TransformBatch
-from .constants import EARTH_RADIUS_M
+from .constants import EARTH_RADIUS_M, REFERENCE_PRESSURE_PA
 from .grid import GaussianGrid
+from .offsets import (LNPS_REFERENCE, lnps_anomaly_coefficients, lnps_from_anomaly_coefficients,
+                      theta_analysis_offsets)
 from .transform import DEFAULT_BAND, LegendreTable, SphericalHarmonicTransform
 from .vector import VorticityDivergenceOperator
 from .waist import FourierWaist, latitude_band_edges
@@ -24,5 +26,7 @@ from .waist import FourierWaist, latitude_band_edges
 __all__ = [
     "Backend", "get_backend", "TransformBatch", "EARTH_RADIUS_M", "GaussianGrid",
     "DEFAULT_BAND", "SphericalHarmonicTransform", "VorticityDivergenceOperator",
-    "FourierWaist", "LegendreTable", "latitude_band_edges",
+    "FourierWaist", "LegendreTable", "latitude_band_edges", "REFERENCE_PRESSURE_PA",
+    "LNPS_REFERENCE", "lnps_anomaly_coefficients", "lnps_from_anomaly_coefficients",
+    "theta_analysis_offsets",
 ]

@@ -26,6 +26,8 @@ OP_ZONAL_TO_WAIST = 9
 OP_MERIDIONAL_TO_WAIST = 10
 OP_ANALYSIS_FROM_WAIST = 11
 OP_VORDIV_FROM_WAIST = 12
+OP_SYNTHESIS_GRADIENT = 13
+JOB_EXP = 1
 # kind argument of arwen_sht_synthesis_to_waist
 WAIST_BASIS, WAIST_ZONAL, WAIST_DERIVATIVE = 0, 1, 2
 
@@ -39,7 +41,8 @@ EXPORTS = (
     "arwen_sht_gauss_nodes", "arwen_sht_plan_table_bytes", "arwen_sht_plan_fp64_orders",
     "arwen_sht_plan_table_f64",
     "arwen_sht_analysis", "arwen_sht_synthesis", "arwen_sht_zonal_derivative",
-    "arwen_sht_meridional_derivative", "arwen_sht_gradient", "arwen_sht_uv_from_vordiv",
+    "arwen_sht_meridional_derivative", "arwen_sht_gradient", "arwen_sht_synthesis_gradient",
+    "arwen_sht_analysis_about", "arwen_sht_synthesis_about", "arwen_sht_uv_from_vordiv",
     "arwen_sht_vordiv_from_uv", "arwen_sht_synthesis_to_waist", "arwen_sht_analysis_from_waist",
     "arwen_sht_vordiv_from_waist", "arwen_sht_waist_from_rows", "arwen_sht_rows_from_waist",
     "arwen_sht_spectral_op", "arwen_sht_sample_workspace_bytes", "arwen_sht_sample",
@@ -88,6 +91,10 @@ class Job(ctypes.Structure):
         ("in1", ctypes.c_void_p),
         ("out0", ctypes.c_void_p),
         ("out1", ctypes.c_void_p),
+        ("out2", ctypes.c_void_p),
+        ("flags", ctypes.c_int),
+        ("grid_offset", ctypes.c_void_p),
+        ("coeff_offset", ctypes.c_void_p),
     ]
 
 
@@ -130,6 +137,9 @@ def load():
             "arwen_sht_zonal_derivative": (I, [P, P, P, I, P]),
             "arwen_sht_meridional_derivative": (I, [P, P, P, I, P]),
             "arwen_sht_gradient": (I, [P, P, P, P, I, P]),
+            "arwen_sht_synthesis_gradient": (I, [P, P, P, P, P, I, P]),
+            "arwen_sht_analysis_about": (I, [P, P, P, I, P, P, P]),
+            "arwen_sht_synthesis_about": (I, [P, P, P, I, P, I, P]),
             "arwen_sht_uv_from_vordiv": (I, [P, P, P, P, P, I, P]),
             "arwen_sht_vordiv_from_uv": (I, [P, P, P, P, P, I, P]),
             "arwen_sht_synthesis_to_waist": (I, [P, P, P, I, I, P]),

@@ -24,6 +24,7 @@ from .transform import SphericalHarmonicTransform
 
 OPS = {
     "inverse": (_native.OP_SYNTHESIS, 1, 1),
+    "inverse_gradient": (_native.OP_SYNTHESIS_GRADIENT, 1, 3),
     "zonal": (_native.OP_ZONAL_DERIVATIVE, 1, 1),
     "meridional": (_native.OP_MERIDIONAL_DERIVATIVE, 1, 1),
     "gradient": (_native.OP_GRADIENT, 1, 2),
@@ -31,11 +32,20 @@ OPS = {
     "forward": (_native.OP_ANALYSIS, 1, 1),
     "vordiv": (_native.OP_VORDIV_FROM_UV, 2, 2),
 }
-_SPECTRAL_IN = {"inverse", "zonal", "meridional", "gradient", "wind"}
+_SPECTRAL_IN = {"inverse", "inverse_gradient", "zonal", "meridional", "gradient", "wind"}
+#: per-job options: "exp" (inverse: write exp of the grid), "grid_offset"
+#: (forward: float32 (count,) subtracted from each field's grid first),
+#: "coeff_offset" (forward, inverse, inverse_gradient: complex64 (count,)
+#: added to each field's (0, 0) coefficient, after the analysis or before the
+#: synthesis)
+_OPTION_OPS = {"exp": {"inverse"}, "grid_offset": {"forward"},
+               "coeff_offset": {"forward", "inverse", "inverse_gradient"}}
 
 
 class TransformBatch:
-    """``jobs``: list of ``(op, transform, inputs, outputs)`` with torch tensors.
+    """``jobs``: list of ``(op, transform, inputs, outputs)`` or ``(op, transform,
+    inputs, outputs, options)`` with torch tensors; ``inverse_gradient``
+    writes (grid, east, north) from one spectral input.
 
     Inputs and outputs are caller-owned device tensors in the boundary
     layouts; they must stay alive and keep their storage for the batch's
@@ -47,7 +57,9 @@ class TransformBatch:
         self._keep = []
         arr = (_native.Job * len(jobs))()
         precision = None
-        for k, (op, tr, ins, outs) in enumerate(jobs):
+        for k, job in enumerate(jobs):
+            op, tr, ins, outs = job[:4]
+            opts = dict(job[4]) if len(job) > 4 and job[4] else {}
             if op not in OPS:
                 raise ValueError(f"unknown op {op!r}; expected one of {sorted(OPS)}")
             if not isinstance(tr, SphericalHarmonicTransform):
@@ -75,10 +87,23 @@ class TransformBatch:
             elif precision != tr.backend.precision:
                 raise ValueError("all transforms of a batch must share one precision")
             count = SphericalHarmonicTransform._count(ins[0])
+            for name in opts:
+                if name not in _OPTION_OPS or op not in _OPTION_OPS[name]:
+                    raise ValueError(f"option {name!r} does not apply to {op}")
+            flags = _native.JOB_EXP if opts.get("exp") else 0
+            offsets = []
+            for name, dtype in (("grid_offset", "float32"), ("coeff_offset", "complex64")):
+                t = opts.get(name)
+                if t is not None:
+                    if str(t.dtype) != f"torch.{dtype}" or t.numel() != count \
+                            or not t.is_contiguous() or t.device.type != "cuda":
+                        raise ValueError(f"{name} must be a contiguous {dtype} device tensor of "
+                                         f"{count} entries")
+                offsets.append(t.data_ptr() if t is not None else None)
             ptr = lambda seq, i: seq[i].data_ptr() if i < len(seq) else None
             arr[k] = _native.Job(code, tr.plan_handle.value, count, ptr(ins, 0), ptr(ins, 1),
-                                 ptr(outs, 0), ptr(outs, 1))
-            self._keep.append((tr, ins, outs))
+                                 ptr(outs, 0), ptr(outs, 1), ptr(outs, 2), flags, *offsets)
+            self._keep.append((tr, ins, outs, opts))
         handle = ctypes.c_void_p()
         _native.check(self._lib.arwen_sht_batch_create(arr, len(jobs), ctypes.byref(handle)))
         self._handle = handle

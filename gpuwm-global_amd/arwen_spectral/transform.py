@@ -27,7 +27,7 @@ DEFAULT_BAND = 32  # reference legendre.py:36; accepted for API compatibility
 
 
 def _vp(t) -> ctypes.c_void_p:
-    return ctypes.c_void_p(t.data_ptr())
+    return ctypes.c_void_p(t.data_ptr() if t is not None else None)
 
 
 @dataclass(frozen=True)
@@ -255,6 +255,64 @@ class SphericalHarmonicTransform:
         _native.check(self._lib.arwen_sht_gradient(self._plan, _vp(c), _vp(east), _vp(north),
                                                    self._count(c), self.backend.stream()))
         return east, north
+
+    def _offset_tensor(self, value, count, dtype, what):
+        """A per-field offset as a contiguous device array of ``count``
+        entries: a scalar (every field), a sequence or an array."""
+        if value is None:
+            return None
+        torch = self.backend.xp
+        t = torch.as_tensor(value, dtype=dtype, device=self.backend.device())
+        if t.dim() == 0:
+            t = t.expand(count)
+        t = t.reshape(-1).contiguous()
+        if t.numel() != count:
+            raise ValueError(f"{what} has {t.numel()} entries for {count} fields")
+        return t
+
+    def forward_about(self, field, grid_offset=None, coefficient_offset=None):
+        """``forward(field - grid_offset) + coefficient_offset`` at (0, 0), one
+        offset per field (leading dims flattened) or one for all: the theta
+        analysis about whole-kelvin offsets of the reference
+        (semilag/step.py:128-163, 610-621) and ``lnps_from_anomaly_coefficients``
+        after an analysis (lnps_reference.py:47-49).  The subtraction happens
+        in the float32 fold ahead of the FFT and the addition in the unpack,
+        the same float32 operations the reference makes on the arrays."""
+        f = self._grid_tensor(field)
+        n = self._count(f)
+        g = self._offset_tensor(grid_offset, n, self.backend.float_dtype, "grid_offset")
+        c = self._offset_tensor(coefficient_offset, n, self.backend.complex_dtype,
+                                "coefficient_offset")
+        out = self._empty_spectral(f.shape[:-2])
+        _native.check(self._lib.arwen_sht_analysis_about(
+            self._plan, _vp(f), _vp(out), n, _vp(g), _vp(c), self.backend.stream()))
+        return out
+
+    def inverse_about(self, coeff, coefficient_offset=None, *, exp: bool = False):
+        """``inverse(coeff + coefficient_offset at (0, 0))``, and its exp with
+        ``exp=True``: ``lnps_from_anomaly_coefficients`` ahead of the inverse
+        and the surface pressure ``exp(ln ps)`` the dycore takes from it
+        (reference lnps_reference.py:47-49, dynamics.py:964-980)."""
+        c = self._spectral_tensor(coeff)
+        n = self._count(c)
+        off = self._offset_tensor(coefficient_offset, n, self.backend.complex_dtype,
+                                  "coefficient_offset")
+        out = self._empty_grid(c.shape[:-2])
+        _native.check(self._lib.arwen_sht_synthesis_about(
+            self._plan, _vp(c), _vp(out), n, _vp(off), _native.JOB_EXP if exp else 0,
+            self.backend.stream()))
+        return out
+
+    def inverse_with_gradient(self, coeff):
+        """``(inverse(coeff), *gradient(coeff))`` from one pack and two plane
+        contractions (the P and dP/dphi tables); the dycore takes both of
+        temperature and ln ps every step (reference dynamics.py:907-1047)."""
+        c = self._spectral_tensor(coeff)
+        outs = [self._empty_grid(c.shape[:-2]) for _ in range(3)]
+        _native.check(self._lib.arwen_sht_synthesis_gradient(
+            self._plan, _vp(c), *(_vp(o) for o in outs), self._count(c),
+            self.backend.stream()))
+        return tuple(outs)
 
     def _spectral_op(self, op, coeff):
         c = self._spectral_tensor(coeff)

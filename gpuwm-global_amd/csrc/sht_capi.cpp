@@ -14,6 +14,7 @@
 #include <cstring>
 #include <list>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -122,7 +123,7 @@ struct arwen_sht_plan {
   std::vector<double> mu, w;
   DevBuf d_mu, d_w, d_koff, d_tw;
   DevBuf tab_syn[3], tab_ana[3];
-  DevBuf scale_fold, scale_fold_vec, scale_one, scale_inv_a;
+  DevBuf scale_fold, scale_fold_vec, scale_one, scale_inv_a, scale_inv_acos;
   DevBuf scale_waist, scale_waist_vec;  // 2 pi w (x 1/a): folds of a reference waist (already / nlon)
   DevBuf sample_rec, sample_diag;       // point sampler recurrence tables, built on first use
   // vector operators' FP64 low orders m < m0: FP64 twiddles, per-row FP64
@@ -133,10 +134,9 @@ struct arwen_sht_plan {
   sht::FftPlan fft{};
   int rp_fold = 4, rp_fold_vec = 4, rp_unfold = 4;  // folded rows (wavefronts) per FFT block
   int fft_gpb = 2;                                   // row groups walked by one FFT block
-  // single-operation batch cache (most recent first)
-  std::list<std::pair<std::tuple<int, int, const void*, const void*, void*, void*>,
-                      arwen_sht_batch*>>
-      cache;
+  // single-operation batches by (op, count, stream), most recent first
+  std::list<std::pair<std::tuple<int, int, void*>, arwen_sht_batch*>> cache;
+  std::mutex cache_mu;
   ~arwen_sht_plan();
 
   int kp(int m, int p) const { return koff[2 * m + p + 1] - koff[2 * m + p]; }
@@ -305,6 +305,7 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
 
   // Per folded-row scales.
   std::vector<float> sf(P->JHP, 0.f), sfv(P->JHP, 0.f), one(P->JHP, 1.f), inva(P->JHP, 0.f);
+  std::vector<float> invacos(P->JHP, 0.f);
   std::vector<float> sw(P->JHP, 0.f), swv(P->JHP, 0.f);
   for (int i = 0; i < P->JH; ++i) {
     const int jn = P->nlat - 1 - i;
@@ -315,6 +316,8 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
     sw[i] = (float)(2.0 * M_PI * wq);
     swv[i] = (float)(2.0 * M_PI * wq / P->radius);
     inva[i] = (float)(1.0 / P->radius);
+    const double mu = P->mu[jn];
+    invacos[i] = (float)(1.0 / (P->radius * std::sqrt(std::max(0.0, 1.0 - mu * mu))));
   }
   upload(P->scale_fold, sf);
   upload(P->scale_fold_vec, sfv);
@@ -322,6 +325,7 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   upload(P->scale_waist_vec, swv);
   upload(P->scale_one, one);
   upload(P->scale_inv_a, inva);
+  upload(P->scale_inv_acos, invacos);
   if (P->m0 > 0) {
     std::vector<double> sfv64(P->JHP, 0.0), inva64(P->JHP, 0.0);
     for (int i = 0; i < P->JH; ++i) {
@@ -413,7 +417,8 @@ namespace {
 
 bool is_synthesis(int op) {
   return (op >= ARWEN_SHT_OP_SYNTHESIS && op <= ARWEN_SHT_OP_UV_FROM_VORDIV) ||
-         (op >= ARWEN_SHT_OP_SYNTHESIS_TO_WAIST && op <= ARWEN_SHT_OP_MERIDIONAL_TO_WAIST);
+         (op >= ARWEN_SHT_OP_SYNTHESIS_TO_WAIST && op <= ARWEN_SHT_OP_MERIDIONAL_TO_WAIST) ||
+         op == ARWEN_SHT_OP_SYNTHESIS_GRADIENT;
 }
 bool to_waist(int op) {
   return op >= ARWEN_SHT_OP_SYNTHESIS_TO_WAIST && op <= ARWEN_SHT_OP_MERIDIONAL_TO_WAIST;
@@ -634,12 +639,288 @@ void add_ana_group(std::vector<HostGroup>& b, const arwen_sht_plan* P, int mb, i
   b.push_back(g);
 }
 
+// One job's share of a batch.  prepare: allocate its workspaces, add its
+// grouped-GEMM work groups and its algorithmic work (once, at batch
+// creation); always: append its launch arguments, which carry the caller's
+// buffers (so a single-operation batch is rebound to new buffers without
+// allocating).  The work groups reference only workspaces and tables.
+void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
+  auto& syn = B->groups[0];
+  auto& ana = B->groups[1];
+  const arwen_sht_job& J = js->job;
+  const arwen_sht_plan* P = J.plan;
+  const int n = J.count;
+  const int T = P->T;
+  const bool vec = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV || J.op == ARWEN_SHT_OP_VORDIV_FROM_UV ||
+                   J.op == ARWEN_SHT_OP_VORDIV_FROM_WAIST;
+  const bool sgrad = J.op == ARWEN_SHT_OP_SYNTHESIS_GRADIENT;
+  const double pcf = 4.0 * (P->nlat / 2.0) * (T + 1.0) * (T + 2.0) / 2.0;  // FLOP per PC
+  const double fftb = 4.0 * P->nlat * P->nlon + 8.0 * P->nlat * (T + 1.0);
+  if (n == 0) return;
+  if (is_synthesis(J.op)) {
+    // ---- pack ----
+    const int ncol = vec ? 4 : 2;
+    js->ncols = ncol * n;
+    if (prepare) js->packed_a.alloc((size_t)js->ncols * P->ldp * 4);
+    if (prepare) js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
+    const bool grad = J.op == ARWEN_SHT_OP_GRADIENT;
+    if ((grad || sgrad) && prepare) js->waist_b.alloc((size_t)js->ncols * P->ldw * 4);
+    sht::PackArgs pa{};
+    pa.koff = P->d_koff.as<int>();
+    pa.T = T;
+    pa.mtiles = (T + 1 + 31) / 32;
+    pa.job.in0 = static_cast<const float2*>(J.in0);
+    pa.job.in1 = static_cast<const float2*>(J.in1);
+    pa.job.out0 = js->packed_a.as<float>();
+    pa.job.nfields = n;
+    pa.job.col0 = 0;
+    pa.job.ldp = P->ldp;
+    pa.job.radius = (float)P->radius;
+    pa.job.mode = (J.op == ARWEN_SHT_OP_ZONAL_DERIVATIVE || J.op == ARWEN_SHT_OP_ZONAL_TO_WAIST)
+                      ? sht::kIM
+                  : vec ? sht::kVector
+                        : sht::kPlain;
+    pa.job.off00 = static_cast<const float2*>(J.coeff_offset);
+    B->packs.push_back(pa);
+    // ---- FP64 low orders of u, v (orders m < m0) ----
+    const bool low = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV && P->m0 > 0;
+    if (low) {
+      if (prepare) js->low.alloc((size_t)n * 2 * P->m0 * 2 * P->JHP * sizeof(double2));
+      if (prepare) js->low_pc.alloc((size_t)n * P->m0 * (T + 1) * sizeof(double4));
+      sht::LowSynArgs ls{};
+      ls.pc = js->low_pc.as<double4>();
+      ls.vort = static_cast<const float2*>(J.in0);
+      ls.div = static_cast<const float2*>(J.in1);
+      ls.dp = P->low_syn[kD].as<double>();
+      ls.pm = P->low_syn[kPm].as<double>();
+      ls.low = js->low.as<double2>();
+      ls.radius = P->radius;
+      ls.T = T;
+      ls.JH = P->JH;
+      ls.JHP = P->JHP;
+      ls.m0 = P->m0;
+      ls.nlev = n;
+      B->low_syns.push_back(ls);
+    }
+    // ---- grouped GEMM: one group per (m-tile, output parity) ----
+    const DataRef pk{js->packed_a.as<char>()};
+    for (int mb = low ? P->m0 / sht::kMT : 0; prepare && mb < P->MB; ++mb)
+      for (int q = 0; q < 2; ++q) {
+        if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
+          // U,V = -dP/dphi (psi) + i P m/cos (chi), i P m/cos (psi) + dP/dphi (chi):
+          // dP/dphi flips the parity, so its segment reads the other parity.
+          SynSeg sg[2] = {{kD, 1, pk, 0}, {kPm, 0, pk, kMapSynVec}};
+          add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 2);
+        } else if (J.op == ARWEN_SHT_OP_MERIDIONAL_DERIVATIVE ||
+                   J.op == ARWEN_SHT_OP_MERIDIONAL_TO_WAIST) {
+          SynSeg sg[1] = {{kD, 1, pk, 0}};
+          add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
+        } else if (grad) {
+          SynSeg se[1] = {{kPm, 0, pk, 0}};
+          add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), se, 1);
+          SynSeg sn[1] = {{kD, 1, pk, 0}};
+          add_syn_group(syn, P, mb, q, js->ncols, js->waist_b.as<float>(), sn, 1);
+        } else if (sgrad) {  // the basis waist feeds the grid and (times i m) the east
+          SynSeg sp[1] = {{kP, 0, pk, 0}};
+          add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sp, 1);
+          SynSeg sn[1] = {{kD, 1, pk, 0}};
+          add_syn_group(syn, P, mb, q, js->ncols, js->waist_b.as<float>(), sn, 1);
+        } else {  // SYNTHESIS, ZONAL_DERIVATIVE and their waist forms
+          SynSeg sg[1] = {{kP, 0, pk, 0}};
+          add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
+        }
+      }
+    // algorithmic FLOPs: one plane contraction per table application
+    {
+      double pcs = (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) ? 4.0 * n
+                   : (grad || sgrad)                     ? 2.0 * n
+                                                         : 1.0 * n;
+      if (prepare) B->legendre_flops += pcs * pcf;
+    }
+    // ---- inverse FFTs ----
+    auto unfold = [&](const DevBuf& waist, void* grid, int col0, int fstride, const DevBuf& sc,
+                      int imul, int post_exp = 0) {
+      sht::UnfoldArgs ua{};
+      ua.fft = P->fft;
+      ua.T = T;
+      ua.nlat = P->nlat;
+      ua.JH = P->JH;
+      ua.JHP = P->JHP;
+      ua.rp = P->rp_unfold;
+      ua.ngroups = (P->JH + ua.rp - 1) / ua.rp;
+      ua.gpb = P->fft_gpb;
+      ua.nchunks = (ua.ngroups + ua.gpb - 1) / ua.gpb;
+      ua.job.waist = waist.as<float>();
+      ua.job.grid = static_cast<float*>(grid);
+      ua.job.rowscale = sc.as<float>();
+      ua.job.nfields = n;
+      ua.job.col0 = col0;
+      ua.job.fstride = fstride;
+      ua.job.ldw = P->ldw;
+      ua.job.imul = imul;
+      ua.job.post_exp = post_exp;
+      if (low) {  // FP64 unfold; orders m < m0 from the low waist of u (col0 0) or v
+        const int f = col0 == 0 ? 0 : 1;
+        ua.job.f64 = 1;
+        ua.job.low = js->low.as<double2>() + (size_t)f * P->m0 * 2 * P->JHP;
+        ua.job.low_stride = 2 * P->m0 * 2 * P->JHP;
+        ua.job.rowscale64 = P->scale_inv_a64.as<double>();
+        ua.job.m0 = P->m0;
+      }
+      B->unfolds.push_back(ua);
+      if (prepare) B->fft_bytes += n * fftb;
+    };
+    if (to_waist(J.op)) {
+      // the reference's waist (*lead, nlat, T+1): north = E + O, south = E - O
+      sht::WaistUnfoldArgs wu{};
+      wu.waist = js->waist_a.as<float>();
+      wu.out = static_cast<float2*>(J.out0);
+      wu.T = T;
+      wu.nlat = P->nlat;
+      wu.JH = P->JH;
+      wu.JHP = P->JHP;
+      wu.ldw = P->ldw;
+      wu.nfields = n;
+      wu.col0 = 0;
+      wu.fstride = 2;
+      B->waist_unfolds.push_back(wu);
+    } else if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
+      unfold(js->waist_a, J.out0, 0, 4, P->scale_inv_a, 0);
+      unfold(js->waist_a, J.out1, 2, 4, P->scale_inv_a, 0);
+    } else if (grad) {
+      unfold(js->waist_a, J.out0, 0, 2, P->scale_inv_a, 1);
+      unfold(js->waist_b, J.out1, 0, 2, P->scale_inv_a, 0);
+    } else if (sgrad) {
+      unfold(js->waist_a, J.out0, 0, 2, P->scale_one, 0);
+      unfold(js->waist_a, J.out1, 0, 2, P->scale_inv_acos, 2);  // i m / (a cos)
+      unfold(js->waist_b, J.out2, 0, 2, P->scale_inv_a, 0);
+    } else {
+      unfold(js->waist_a, J.out0, 0, 2, P->scale_one, 0, (J.flags & ARWEN_SHT_JOB_EXP) ? 1 : 0);
+    }
+  } else {
+    // ---- FFT + fold ----
+    const int ncol = vec ? 4 : 2;
+    js->ncols = ncol * n;
+    if (prepare) js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
+    if (prepare) js->packed_out.alloc((size_t)js->ncols * P->ldp * 4);
+    if (from_waist(J.op)) {
+      // a reference waist (*lead, nlat, T+1), already divided by nlon
+      sht::WaistFoldArgs wf{};
+      wf.w0 = static_cast<const float2*>(J.in0);
+      wf.w1 = static_cast<const float2*>(J.in1);
+      wf.waist = js->waist_a.as<float>();
+      wf.rowscale = vec ? P->scale_waist_vec.as<float>() : P->scale_waist.as<float>();
+      wf.T = T;
+      wf.nlat = P->nlat;
+      wf.JH = P->JH;
+      wf.JHP = P->JHP;
+      wf.ldw = P->ldw;
+      wf.nfields = n;
+      wf.col0 = 0;
+      wf.vector = vec ? 1 : 0;
+      B->waist_folds.push_back(wf);
+    }
+    sht::FoldArgs fa{};
+    fa.fft = P->fft;
+    fa.T = T;
+    fa.nlat = P->nlat;
+    fa.JH = P->JH;
+    fa.JHP = P->JHP;
+    fa.rp = vec ? P->rp_fold_vec : P->rp_fold;
+    fa.ngroups = P->JHP / fa.rp;
+    fa.gpb = P->fft_gpb;
+    fa.nchunks = (fa.ngroups + fa.gpb - 1) / fa.gpb;
+    fa.job.grid0 = static_cast<const float*>(J.in0);
+    fa.job.grid1 = static_cast<const float*>(J.in1);
+    fa.job.waist0 = js->waist_a.as<float>();
+    fa.job.rowscale = vec ? P->scale_fold_vec.as<float>() : P->scale_fold.as<float>();
+    fa.job.nfields = n;
+    fa.job.col0 = 0;
+    fa.job.ldw = P->ldw;
+    fa.job.vector = vec ? 1 : 0;
+    fa.job.goff = J.grid_offset;
+    const bool low = J.op == ARWEN_SHT_OP_VORDIV_FROM_UV && P->m0 > 0;
+    if (low) {
+      // FP64 folds of u and v, one field each: i U to level columns 2, 3 and
+      // i V to 0, 1 (the [iV, iU] arrangement), orders m < m0 also in FP64
+      if (prepare) js->low.alloc((size_t)n * 2 * P->m0 * 2 * P->JHP * sizeof(double2));
+      for (int f = 0; f < 2; ++f) {
+        sht::FoldArgs fh = fa;
+        fh.job.grid0 = static_cast<const float*>(f == 0 ? J.in0 : J.in1);
+        fh.job.grid1 = nullptr;
+        fh.job.vector = 0;
+        fh.job.col0 = f == 0 ? 2 : 0;
+        fh.job.fstride = 4;
+        fh.job.low = js->low.as<double2>() + (size_t)f * P->m0 * 2 * P->JHP;
+        fh.job.low_stride = 2 * P->m0 * 2 * P->JHP;
+        fh.job.rowscale64 = P->scale_fold_vec64.as<double>();
+        fh.job.m0 = P->m0;
+        B->folds.push_back(fh);
+      }
+      if (prepare) B->fft_bytes += 2.0 * n * fftb;
+      sht::LowAnaArgs la{};
+      la.low = js->low.as<double2>();
+      la.dp = P->low_ana[kD].as<double>();
+      la.pm = P->low_ana[kPm].as<double>();
+      la.vort = static_cast<float2*>(J.out0);
+      la.div = static_cast<float2*>(J.out1);
+      la.T = T;
+      la.JH = P->JH;
+      la.JHP = P->JHP;
+      la.m0 = P->m0;
+      la.nlev = n;
+      B->low_anas.push_back(la);
+    } else if (!from_waist(J.op)) {
+      B->folds.push_back(fa);
+      if (prepare) B->fft_bytes += (vec ? 2.0 : 1.0) * n * fftb;
+    }
+    const DataRef wst{js->waist_a.as<char>()};
+    for (int mb = low ? P->m0 / sht::kMT : 0; prepare && mb < P->MB; ++mb)
+      for (int q = 0; q < 2; ++q) {
+        if (vec) {
+          // zeta, D = Pm^T [iV, iU]_q + dP^T [U, -V]_{1-q}
+          AnaSeg sg[2] = {{kPm, 0, wst, 0}, {kD, 1, wst, kMapAnaVec}};
+          add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 2);
+        } else {
+          AnaSeg sg[1] = {{kP, 0, wst, 0}};
+          add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 1);
+        }
+      }
+    if (prepare) B->legendre_flops += (vec ? 4.0 : 1.0) * n * pcf;
+    sht::UnpackArgs ua{};
+    ua.koff = P->d_koff.as<int>();
+    ua.T = T;
+    ua.tiles = (T + 1 + 31) / 32;
+    ua.job.in = js->packed_out.as<float>();
+    ua.job.out0 = static_cast<float2*>(J.out0);
+    ua.job.out1 = static_cast<float2*>(J.out1);
+    ua.job.nfields = n;
+    ua.job.col0 = 0;
+    ua.job.ldp = P->ldp;
+    ua.job.vector = vec ? 1 : 0;
+    ua.job.m0 = low ? P->m0 : 0;
+    ua.job.off00 = vec ? nullptr : static_cast<const float2*>(J.coeff_offset);
+    B->unpacks.push_back(ua);
+  }
+}
+
+// The launch arguments of every job, rebuilt from the jobs' current buffers.
+void bind_jobs(arwen_sht_batch* B) {
+  B->packs.clear();
+  B->unfolds.clear();
+  B->folds.clear();
+  B->unpacks.clear();
+  B->waist_unfolds.clear();
+  B->waist_folds.clear();
+  B->low_syns.clear();
+  B->low_anas.clear();
+  for (auto& js : B->jobs) setup_job(B, js.get(), false);
+}
+
 void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
   if (njobs <= 0 || !jobs) fail(ARWEN_SHT_EINVAL, "a batch needs at least one job");
   hip_check(hipGetDevice(&B->device), "hipGetDevice");
   B->precision = jobs[0].plan ? jobs[0].plan->precision : 0;
-  auto& syn = B->groups[0];
-  auto& ana = B->groups[1];
   for (int j = 0; j < njobs; ++j) {
     const arwen_sht_job& J = jobs[j];
     const arwen_sht_plan* P = J.plan;
@@ -651,250 +932,26 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
     if (J.count < 0) fail(ARWEN_SHT_EINVAL, "job count must be >= 0");
     const bool vec = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV || J.op == ARWEN_SHT_OP_VORDIV_FROM_UV ||
                      J.op == ARWEN_SHT_OP_VORDIV_FROM_WAIST;
-    const bool two_out = vec || J.op == ARWEN_SHT_OP_GRADIENT;
-    if (J.op < ARWEN_SHT_OP_SYNTHESIS || J.op > ARWEN_SHT_OP_VORDIV_FROM_WAIST)
+    const bool sgrad = J.op == ARWEN_SHT_OP_SYNTHESIS_GRADIENT;
+    const bool two_out = vec || J.op == ARWEN_SHT_OP_GRADIENT || sgrad;
+    if (J.op < ARWEN_SHT_OP_SYNTHESIS || J.op > ARWEN_SHT_OP_SYNTHESIS_GRADIENT)
       fail(ARWEN_SHT_EINVAL, "unknown op " + std::to_string(J.op));
-    if (J.count > 0 && (!J.in0 || !J.out0 || (vec && !J.in1) || (two_out && !J.out1)))
+    if (J.count > 0 && (!J.in0 || !J.out0 || (vec && !J.in1) || (two_out && !J.out1) ||
+                        (sgrad && !J.out2)))
       fail(ARWEN_SHT_EINVAL, "job " + std::to_string(j) + " is missing a buffer");
+    if (J.flags & ~ARWEN_SHT_JOB_EXP)
+      fail(ARWEN_SHT_EINVAL, "job " + std::to_string(j) + " has unknown flags");
+    if ((J.flags & ARWEN_SHT_JOB_EXP) && J.op != ARWEN_SHT_OP_SYNTHESIS)
+      fail(ARWEN_SHT_EINVAL, "ARWEN_SHT_JOB_EXP applies to ARWEN_SHT_OP_SYNTHESIS only");
+    if (J.grid_offset && J.op != ARWEN_SHT_OP_ANALYSIS)
+      fail(ARWEN_SHT_EINVAL, "grid_offset applies to ARWEN_SHT_OP_ANALYSIS only");
+    if (J.coeff_offset && J.op != ARWEN_SHT_OP_ANALYSIS && J.op != ARWEN_SHT_OP_SYNTHESIS &&
+        !sgrad)
+      fail(ARWEN_SHT_EINVAL,
+           "coeff_offset applies to ARWEN_SHT_OP_ANALYSIS, _SYNTHESIS and _SYNTHESIS_GRADIENT only");
     auto js = std::make_unique<JobState>();
     js->job = J;
-    const int n = J.count;
-    const int T = P->T;
-    const double pcf = 4.0 * (P->nlat / 2.0) * (T + 1.0) * (T + 2.0) / 2.0;  // FLOP per PC
-    const double fftb = 4.0 * P->nlat * P->nlon + 8.0 * P->nlat * (T + 1.0);
-    if (n == 0) { B->jobs.push_back(std::move(js)); continue; }
-    if (is_synthesis(J.op)) {
-      // ---- pack ----
-      const int ncol = vec ? 4 : 2;
-      js->ncols = ncol * n;
-      js->packed_a.alloc((size_t)js->ncols * P->ldp * 4);
-      js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
-      const bool grad = J.op == ARWEN_SHT_OP_GRADIENT;
-      if (grad) js->waist_b.alloc((size_t)js->ncols * P->ldw * 4);
-      sht::PackArgs pa{};
-      pa.koff = P->d_koff.as<int>();
-      pa.T = T;
-      pa.mtiles = (T + 1 + 31) / 32;
-      pa.job.in0 = static_cast<const float2*>(J.in0);
-      pa.job.in1 = static_cast<const float2*>(J.in1);
-      pa.job.out0 = js->packed_a.as<float>();
-      pa.job.nfields = n;
-      pa.job.col0 = 0;
-      pa.job.ldp = P->ldp;
-      pa.job.radius = (float)P->radius;
-      pa.job.mode = (J.op == ARWEN_SHT_OP_ZONAL_DERIVATIVE || J.op == ARWEN_SHT_OP_ZONAL_TO_WAIST)
-                        ? sht::kIM
-                    : vec ? sht::kVector
-                          : sht::kPlain;
-      B->packs.push_back(pa);
-      // ---- FP64 low orders of u, v (orders m < m0) ----
-      const bool low = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV && P->m0 > 0;
-      if (low) {
-        js->low.alloc((size_t)n * 2 * P->m0 * 2 * P->JHP * sizeof(double2));
-        js->low_pc.alloc((size_t)n * P->m0 * (T + 1) * sizeof(double4));
-        sht::LowSynArgs ls{};
-        ls.pc = js->low_pc.as<double4>();
-        ls.vort = static_cast<const float2*>(J.in0);
-        ls.div = static_cast<const float2*>(J.in1);
-        ls.dp = P->low_syn[kD].as<double>();
-        ls.pm = P->low_syn[kPm].as<double>();
-        ls.low = js->low.as<double2>();
-        ls.radius = P->radius;
-        ls.T = T;
-        ls.JH = P->JH;
-        ls.JHP = P->JHP;
-        ls.m0 = P->m0;
-        ls.nlev = n;
-        B->low_syns.push_back(ls);
-      }
-      // ---- grouped GEMM: one group per (m-tile, output parity) ----
-      const DataRef pk{js->packed_a.as<char>()};
-      for (int mb = low ? P->m0 / sht::kMT : 0; mb < P->MB; ++mb)
-        for (int q = 0; q < 2; ++q) {
-          if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
-            // U,V = -dP/dphi (psi) + i P m/cos (chi), i P m/cos (psi) + dP/dphi (chi):
-            // dP/dphi flips the parity, so its segment reads the other parity.
-            SynSeg sg[2] = {{kD, 1, pk, 0}, {kPm, 0, pk, kMapSynVec}};
-            add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 2);
-          } else if (J.op == ARWEN_SHT_OP_MERIDIONAL_DERIVATIVE ||
-                     J.op == ARWEN_SHT_OP_MERIDIONAL_TO_WAIST) {
-            SynSeg sg[1] = {{kD, 1, pk, 0}};
-            add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
-          } else if (grad) {
-            SynSeg se[1] = {{kPm, 0, pk, 0}};
-            add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), se, 1);
-            SynSeg sn[1] = {{kD, 1, pk, 0}};
-            add_syn_group(syn, P, mb, q, js->ncols, js->waist_b.as<float>(), sn, 1);
-          } else {  // SYNTHESIS, ZONAL_DERIVATIVE and their waist forms
-            SynSeg sg[1] = {{kP, 0, pk, 0}};
-            add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
-          }
-        }
-      // algorithmic FLOPs: one plane contraction per table application
-      {
-        double pcs = (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) ? 4.0 * n
-                     : grad                                ? 2.0 * n
-                                                           : 1.0 * n;
-        B->legendre_flops += pcs * pcf;
-      }
-      // ---- inverse FFTs ----
-      auto unfold = [&](const DevBuf& waist, void* grid, int col0, int fstride, const DevBuf& sc,
-                        int imul) {
-        sht::UnfoldArgs ua{};
-        ua.fft = P->fft;
-        ua.T = T;
-        ua.nlat = P->nlat;
-        ua.JH = P->JH;
-        ua.JHP = P->JHP;
-        ua.rp = P->rp_unfold;
-        ua.ngroups = (P->JH + ua.rp - 1) / ua.rp;
-        ua.gpb = P->fft_gpb;
-        ua.nchunks = (ua.ngroups + ua.gpb - 1) / ua.gpb;
-        ua.job.waist = waist.as<float>();
-        ua.job.grid = static_cast<float*>(grid);
-        ua.job.rowscale = sc.as<float>();
-        ua.job.nfields = n;
-        ua.job.col0 = col0;
-        ua.job.fstride = fstride;
-        ua.job.ldw = P->ldw;
-        ua.job.imul = imul;
-        if (low) {  // FP64 unfold; orders m < m0 from the low waist of u (col0 0) or v
-          const int f = col0 == 0 ? 0 : 1;
-          ua.job.f64 = 1;
-          ua.job.low = js->low.as<double2>() + (size_t)f * P->m0 * 2 * P->JHP;
-          ua.job.low_stride = 2 * P->m0 * 2 * P->JHP;
-          ua.job.rowscale64 = P->scale_inv_a64.as<double>();
-          ua.job.m0 = P->m0;
-        }
-        B->unfolds.push_back(ua);
-        B->fft_bytes += n * fftb;
-      };
-      if (to_waist(J.op)) {
-        // the reference's waist (*lead, nlat, T+1): north = E + O, south = E - O
-        sht::WaistUnfoldArgs wu{};
-        wu.waist = js->waist_a.as<float>();
-        wu.out = static_cast<float2*>(J.out0);
-        wu.T = T;
-        wu.nlat = P->nlat;
-        wu.JH = P->JH;
-        wu.JHP = P->JHP;
-        wu.ldw = P->ldw;
-        wu.nfields = n;
-        wu.col0 = 0;
-        wu.fstride = 2;
-        B->waist_unfolds.push_back(wu);
-      } else if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
-        unfold(js->waist_a, J.out0, 0, 4, P->scale_inv_a, 0);
-        unfold(js->waist_a, J.out1, 2, 4, P->scale_inv_a, 0);
-      } else if (grad) {
-        unfold(js->waist_a, J.out0, 0, 2, P->scale_inv_a, 1);
-        unfold(js->waist_b, J.out1, 0, 2, P->scale_inv_a, 0);
-      } else {
-        unfold(js->waist_a, J.out0, 0, 2, P->scale_one, 0);
-      }
-    } else {
-      // ---- FFT + fold ----
-      const int ncol = vec ? 4 : 2;
-      js->ncols = ncol * n;
-      js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
-      js->packed_out.alloc((size_t)js->ncols * P->ldp * 4);
-      if (from_waist(J.op)) {
-        // a reference waist (*lead, nlat, T+1), already divided by nlon
-        sht::WaistFoldArgs wf{};
-        wf.w0 = static_cast<const float2*>(J.in0);
-        wf.w1 = static_cast<const float2*>(J.in1);
-        wf.waist = js->waist_a.as<float>();
-        wf.rowscale = vec ? P->scale_waist_vec.as<float>() : P->scale_waist.as<float>();
-        wf.T = T;
-        wf.nlat = P->nlat;
-        wf.JH = P->JH;
-        wf.JHP = P->JHP;
-        wf.ldw = P->ldw;
-        wf.nfields = n;
-        wf.col0 = 0;
-        wf.vector = vec ? 1 : 0;
-        B->waist_folds.push_back(wf);
-      }
-      sht::FoldArgs fa{};
-      fa.fft = P->fft;
-      fa.T = T;
-      fa.nlat = P->nlat;
-      fa.JH = P->JH;
-      fa.JHP = P->JHP;
-      fa.rp = vec ? P->rp_fold_vec : P->rp_fold;
-      fa.ngroups = P->JHP / fa.rp;
-      fa.gpb = P->fft_gpb;
-      fa.nchunks = (fa.ngroups + fa.gpb - 1) / fa.gpb;
-      fa.job.grid0 = static_cast<const float*>(J.in0);
-      fa.job.grid1 = static_cast<const float*>(J.in1);
-      fa.job.waist0 = js->waist_a.as<float>();
-      fa.job.rowscale = vec ? P->scale_fold_vec.as<float>() : P->scale_fold.as<float>();
-      fa.job.nfields = n;
-      fa.job.col0 = 0;
-      fa.job.ldw = P->ldw;
-      fa.job.vector = vec ? 1 : 0;
-      const bool low = J.op == ARWEN_SHT_OP_VORDIV_FROM_UV && P->m0 > 0;
-      if (low) {
-        // FP64 folds of u and v, one field each: i U to level columns 2, 3 and
-        // i V to 0, 1 (the [iV, iU] arrangement), orders m < m0 also in FP64
-        js->low.alloc((size_t)n * 2 * P->m0 * 2 * P->JHP * sizeof(double2));
-        for (int f = 0; f < 2; ++f) {
-          sht::FoldArgs fh = fa;
-          fh.job.grid0 = static_cast<const float*>(f == 0 ? J.in0 : J.in1);
-          fh.job.grid1 = nullptr;
-          fh.job.vector = 0;
-          fh.job.col0 = f == 0 ? 2 : 0;
-          fh.job.fstride = 4;
-          fh.job.low = js->low.as<double2>() + (size_t)f * P->m0 * 2 * P->JHP;
-          fh.job.low_stride = 2 * P->m0 * 2 * P->JHP;
-          fh.job.rowscale64 = P->scale_fold_vec64.as<double>();
-          fh.job.m0 = P->m0;
-          B->folds.push_back(fh);
-        }
-        B->fft_bytes += 2.0 * n * fftb;
-        sht::LowAnaArgs la{};
-        la.low = js->low.as<double2>();
-        la.dp = P->low_ana[kD].as<double>();
-        la.pm = P->low_ana[kPm].as<double>();
-        la.vort = static_cast<float2*>(J.out0);
-        la.div = static_cast<float2*>(J.out1);
-        la.T = T;
-        la.JH = P->JH;
-        la.JHP = P->JHP;
-        la.m0 = P->m0;
-        la.nlev = n;
-        B->low_anas.push_back(la);
-      } else if (!from_waist(J.op)) {
-        B->folds.push_back(fa);
-        B->fft_bytes += (vec ? 2.0 : 1.0) * n * fftb;
-      }
-      const DataRef wst{js->waist_a.as<char>()};
-      for (int mb = low ? P->m0 / sht::kMT : 0; mb < P->MB; ++mb)
-        for (int q = 0; q < 2; ++q) {
-          if (vec) {
-            // zeta, D = Pm^T [iV, iU]_q + dP^T [U, -V]_{1-q}
-            AnaSeg sg[2] = {{kPm, 0, wst, 0}, {kD, 1, wst, kMapAnaVec}};
-            add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 2);
-          } else {
-            AnaSeg sg[1] = {{kP, 0, wst, 0}};
-            add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 1);
-          }
-        }
-      B->legendre_flops += (vec ? 4.0 : 1.0) * n * pcf;
-      sht::UnpackArgs ua{};
-      ua.koff = P->d_koff.as<int>();
-      ua.T = T;
-      ua.tiles = (T + 1 + 31) / 32;
-      ua.job.in = js->packed_out.as<float>();
-      ua.job.out0 = static_cast<float2*>(J.out0);
-      ua.job.out1 = static_cast<float2*>(J.out1);
-      ua.job.nfields = n;
-      ua.job.col0 = 0;
-      ua.job.ldp = P->ldp;
-      ua.job.vector = vec ? 1 : 0;
-      ua.job.m0 = low ? P->m0 : 0;
-      B->unpacks.push_back(ua);
-    }
+    setup_job(B, js.get(), true);
     B->jobs.push_back(std::move(js));
   }
   for (int d = 0; d < 2; ++d) build_lists(d, B->groups[d], B->config.cols[d], B->lists[d]);
@@ -986,13 +1043,23 @@ void apply_config(arwen_sht_batch* B, const arwen_sht_gemm_config& c) {
   hip_check(hipDeviceSynchronize(), "config upload");
 }
 
-int single_op(arwen_sht_plan* P, int op, const void* in0, const void* in1, void* out0,
-              void* out1, int count, void* stream) {
+// A single operation runs as a one-job batch kept by the plan per (op,
+// count, stream): its workspaces and GEMM work lists depend on those only, so
+// a later call with other buffers rebinds the launch arguments on the host
+// and launches, with no allocation and no synchronization.  The first call of
+// a key allocates and uploads (and waits for that); at most kSingleCache keys
+// are kept, and evicting one waits for its stream.  Calls of one plan on one
+// stream share that key's workspaces in stream order.
+constexpr size_t kSingleCache = 16;
+
+int single_job(arwen_sht_plan* P, const arwen_sht_job& job, void* stream) {
   return guarded([&] {
     if (!P) fail(ARWEN_SHT_EINVAL, "plan is NULL");
+    const int op = job.op, count = job.count;
     if (count < 0) fail(ARWEN_SHT_EINVAL, "count must be >= 0");
     if (count == 0) return;
-    auto key = std::make_tuple(op, count, in0, in1, out0, out1);
+    std::lock_guard<std::mutex> lock(P->cache_mu);
+    const auto key = std::make_tuple(op, count, stream);
     arwen_sht_batch* B = nullptr;
     for (auto it = P->cache.begin(); it != P->cache.end(); ++it) {
       if (it->first == key) {
@@ -1001,20 +1068,38 @@ int single_op(arwen_sht_plan* P, int op, const void* in0, const void* in1, void*
         break;
       }
     }
+    arwen_sht_job J = job;
+    J.plan = P;
     if (!B) {
-      arwen_sht_job J{op, P, count, in0, in1, out0, out1};
       auto nb = std::make_unique<arwen_sht_batch>();
       plan_batch(nb.get(), &J, 1);
       B = nb.release();
       P->cache.push_front({key, B});
-      while (P->cache.size() > 16) {
-        hip_check(hipDeviceSynchronize(), "cache eviction");
-        delete P->cache.back().second;
+      while (P->cache.size() > kSingleCache) {
+        auto& victim = P->cache.back();
+        hip_check(hipStreamSynchronize(static_cast<hipStream_t>(std::get<2>(victim.first))),
+                  "single-operation cache eviction");
+        delete victim.second;
         P->cache.pop_back();
       }
+    } else {
+      B->jobs[0]->job = J;
+      bind_jobs(B);
     }
     run_batch(B, static_cast<hipStream_t>(stream));
   });
+}
+
+int single_op(arwen_sht_plan* P, int op, const void* in0, const void* in1, void* out0,
+              void* out1, int count, void* stream) {
+  arwen_sht_job J{};
+  J.op = op;
+  J.count = count;
+  J.in0 = in0;
+  J.in1 = in1;
+  J.out0 = out0;
+  J.out1 = out1;
+  return single_job(P, J, stream);
 }
 
 }  // namespace
@@ -1181,6 +1266,42 @@ int arwen_sht_gradient(arwen_sht_plan* P, const void* coeff, float* east, float*
                        int count, void* s) {
   return single_op(P, ARWEN_SHT_OP_GRADIENT, coeff, nullptr, east, north, count, s);
 }
+int arwen_sht_synthesis_gradient(arwen_sht_plan* P, const void* coeff, float* grid, float* east,
+                                 float* north, int count, void* s) {
+  arwen_sht_job J{};
+  J.op = ARWEN_SHT_OP_SYNTHESIS_GRADIENT;
+  J.count = count;
+  J.in0 = coeff;
+  J.out0 = grid;
+  J.out1 = east;
+  J.out2 = north;
+  return single_job(P, J, s);
+}
+
+int arwen_sht_analysis_about(arwen_sht_plan* P, const float* grid, void* coeff, int count,
+                             const float* grid_offset, const void* coeff_offset, void* s) {
+  arwen_sht_job J{};
+  J.op = ARWEN_SHT_OP_ANALYSIS;
+  J.count = count;
+  J.in0 = grid;
+  J.out0 = coeff;
+  J.grid_offset = grid_offset;
+  J.coeff_offset = coeff_offset;
+  return single_job(P, J, s);
+}
+
+int arwen_sht_synthesis_about(arwen_sht_plan* P, const void* coeff, float* grid, int count,
+                              const void* coeff_offset, int flags, void* s) {
+  arwen_sht_job J{};
+  J.op = ARWEN_SHT_OP_SYNTHESIS;
+  J.count = count;
+  J.in0 = coeff;
+  J.out0 = grid;
+  J.coeff_offset = coeff_offset;
+  J.flags = flags;
+  return single_job(P, J, s);
+}
+
 int arwen_sht_uv_from_vordiv(arwen_sht_plan* P, const void* vort, const void* div, float* u,
                              float* v, int count, void* s) {
   return single_op(P, ARWEN_SHT_OP_UV_FROM_VORDIV, vort, div, u, v, count, s);

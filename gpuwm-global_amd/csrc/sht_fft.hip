@@ -206,6 +206,9 @@ __global__ void __launch_bounds__(kFftThreads) rfft_fold_kernel(FoldArgs a) {
       }
     }
   };
+  // a scalar field's grid offset is subtracted as its rows enter the FFT
+  // (x - 0 = x exactly, so no offset changes no bit)
+  const float goff = (!a.job.vector && a.job.goff) ? a.job.goff[item] : 0.f;
   auto put = [&](int g, int src, float2* b) {
     const int i = g * a.rp + w;
     if (i >= a.JH) return;
@@ -213,12 +216,12 @@ __global__ void __launch_bounds__(kFftThreads) rfft_fold_kernel(FoldArgs a) {
 #pragma unroll
       for (int j = 0; j < PF; ++j) {
         const int k = lane + kWave * j;
-        if (k < N) b[k] = make_float2(rn[src][j], rs[src][j]);
+        if (k < N) b[k] = make_float2(rn[src][j] - goff, rs[src][j] - goff);
       }
     } else {
       const float* n_ = gsrc[src] + (size_t)(a.nlat - 1 - i) * N;
       const float* s_ = gsrc[src] + (size_t)i * N;
-      for (int k = lane; k < N; k += kWave) b[k] = make_float2(n_[k], s_[k]);
+      for (int k = lane; k < N; k += kWave) b[k] = make_float2(n_[k] - goff, s_[k] - goff);
     }
   };
 
@@ -356,8 +359,9 @@ __global__ void __launch_bounds__(kFftThreads) irfft_unfold_kernel(UnfoldArgs a)
     float2 A = make_float2((E.x + O.x) * s, (E.y + O.y) * s);
     float2 B = make_float2((E.x - O.x) * s, (E.y - O.y) * s);
     if (a.job.imul) {
-      A = make_float2(-A.y, A.x);
-      B = make_float2(-B.y, B.x);
+      const float k = a.job.imul == 2 ? (float)m : 1.f;  // i m or i
+      A = make_float2(-k * A.y, k * A.x);
+      B = make_float2(-k * B.y, k * B.x);
     }
     if (m == 0) {
       A.y = 0.f;
@@ -405,8 +409,8 @@ __global__ void __launch_bounds__(kFftThreads) irfft_unfold_kernel(UnfoldArgs a)
       float* rs = a.job.grid + (size_t)f * plane + (size_t)i * N;
       for (int k = lane; k < N; k += kWave) {
         float2 v = x[k];
-        rn[k] = v.x;
-        rs[k] = v.y;
+        rn[k] = a.job.post_exp ? expf(v.x) : v.x;
+        rs[k] = a.job.post_exp ? expf(v.y) : v.y;
       }
     }
     __syncthreads();  // buffers are rewritten by the next group
@@ -699,9 +703,11 @@ __global__ void __launch_bounds__(64 * kWaveBlock, (fold_waves_per_simd<L, F, kW
       const float* gp = (f == 0 ? a.job.grid0 : a.job.grid1) + (size_t)fld * plane;
       const float* rn = gp + (size_t)(a.nlat - 1 - i) * N;
       const float* rs = gp + (size_t)i * N;
+      // a scalar field's grid offset (x - 0 = x exactly)
+      const float off = (F == 1 && a.job.goff) ? a.job.goff[fld] : 0.f;
       static_for<L>([&](auto n1_) {
         constexpr int n1 = decltype(n1_)::value;
-        v[f][n1] = make_float2(rn[64 * n1 + lane], rs[64 * n1 + lane]);
+        v[f][n1] = make_float2(rn[64 * n1 + lane] - off, rs[64 * n1 + lane] - off);
       });
     });
   };
@@ -875,8 +881,9 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
         float2 A = make_float2((er[r] + orr[r]) * s, (ei[r] + oi[r]) * s);
         float2 B = make_float2((er[r] - orr[r]) * s, (ei[r] - oi[r]) * s);
         if (a.job.imul) {
-          A = make_float2(-A.y, A.x);
-          B = make_float2(-B.y, B.x);
+          const float k = a.job.imul == 2 ? (float)m : 1.f;  // i m or i
+          A = make_float2(-k * A.y, k * A.x);
+          B = make_float2(-k * B.y, k * B.x);
         }
         if (m == 0) {
           A.y = 0.f;
@@ -892,11 +899,12 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
       wave_fft64<L, 1, +1>(v, buf, tw, lane);
       float* rn = a.job.grid + (size_t)fld * plane + (size_t)(a.nlat - 1 - i) * N;
       float* rs = a.job.grid + (size_t)fld * plane + (size_t)i * N;
+      const bool post_exp = a.job.post_exp != 0;
       static_for<L>([&](auto q_) {
         const int k = lane + 64 * decltype(q_)::value;
         const float2 x = buf[k];
-        rn[k] = x.x;
-        rs[k] = x.y;
+        rn[k] = post_exp ? expf(x.x) : x.x;
+        rs[k] = post_exp ? expf(x.y) : x.y;
       });
     }
     __syncthreads();  // every span is consumed before the next group's runs land
@@ -1096,8 +1104,9 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave64_kernel(Un
         double2 A = make_double2((E.x + O.x) * s, (E.y + O.y) * s);
         double2 B = make_double2((E.x - O.x) * s, (E.y - O.y) * s);
         if (a.job.imul) {
-          A = make_double2(-A.y, A.x);
-          B = make_double2(-B.y, B.x);
+          const double k = a.job.imul == 2 ? (double)m : 1.0;  // i m or i
+          A = make_double2(-k * A.y, k * A.x);
+          B = make_double2(-k * B.y, k * B.x);
         }
         if (m == 0) {
           A.y = 0.0;

@@ -59,6 +59,7 @@ struct FoldJob {
   int col0;                   // first data column of this job in the waist
   int ldw;                    // floats per data column
   int vector;                 // 0 scalar, 1 vector pair
+  const float* goff;          // scalar jobs: per field, subtracted from its rows (or NULL)
   // FP64 fold of one field of a vector pair (rfft_fold_wave64_kernel):
   // writes i X to columns col0 + fstride f + {0, 1} of the fp32 waist and,
   // for orders m < m0, the folded (E, O) of X in FP64 to low + f low_stride,
@@ -88,7 +89,8 @@ struct UnfoldJob {
   int col0;                   // columns of field f: col0 + fstride*f + {0 re, 1 im}
   int fstride;
   int ldw;
-  int imul;                   // multiply the spectrum by i before the FFT
+  int imul;                   // multiply the spectrum by 1: i, 2: i m before the FFT
+  int post_exp;               // write exp() of the grid values
   // FP64 unfold (irfft_unfold_wave64_kernel): the spectrum is assembled and
   // inverted in FP64, orders m < m0 read from low + f low_stride ([m][p][i]
   // folded (E, O) in FP64) instead of the fp32 waist
@@ -268,6 +270,7 @@ struct PackJob {
   int ldp;                    // floats per packed column
   int mode;
   float radius;
+  const float2* off00;        // kPlain: per field, added to its (0, 0) coefficient (or NULL)
 };
 
 struct PackArgs {
@@ -287,6 +290,7 @@ struct UnpackJob {
   int vector;                 // columns per level [zeta.re, zeta.im, D.re, D.im]
   int m0;                     // orders m < m0 inside the triangle are left alone
                               // (written by the FP64 low-order kernel)
+  const float2* off00;        // scalar jobs: per field, added to its (0, 0) output (or NULL)
 };
 
 struct UnpackArgs {

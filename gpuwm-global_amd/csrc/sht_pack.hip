@@ -10,7 +10,10 @@
 // into bf16 pieces as the GEMM loads them.
 //
 // Fused transforms (pack), per field or per vorticity/divergence level:
-//   PLAIN   c
+//   PLAIN   c, plus an optional per-field constant at (0, 0) (the reference's
+//           add_grid_constant, transform.py:1832-1835, as lnps_reference.py:32-49
+//           applies it); unpack adds one after an analysis likewise
+//           (semilag/step.py:159-163)
 //   IM      i*m*c                (reference transform.py:1752-1760)
 //   ICOEF   i*c                  (zonal derivative against the P*m/cos table)
 //   VECTOR  psi = zeta/lambda_n, chi = D/lambda_n, lambda_n = -n(n+1)/a^2,
@@ -110,8 +113,12 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
     const int nidx = (n - m - p) >> 1;
     const int r = n - n0;
     float o[4] = {0.f, 0.f, 0.f, 0.f};
-    const float2 cv = tile[0][r][mi];
+    float2 cv = tile[0][r][mi];
     if (a.job.mode == kPlain) {
+      if (a.job.off00 && n == 0) {  // m = 0 as well: (0, 0) is the only n = 0 entry
+        const float2 d = a.job.off00[f];
+        cv = make_float2(cv.x + d.x, cv.y + d.y);
+      }
       o[0] = cv.x; o[1] = cv.y;
     } else if (a.job.mode == kIM) {
       o[0] = -(float)m * cv.y; o[1] = (float)m * cv.x;
@@ -196,8 +203,12 @@ __global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs a) {
       const int n = n0 + r, m = m0 + mi;
       if (n > T || m > T) continue;
       if (m < a.job.m0 && m <= n) continue;  // the FP64 low-order kernel's entries
-      const float2 v = m <= n ? make_float2(tile[2 * o][r][mi], tile[2 * o + 1][r][mi])
-                              : make_float2(0.f, 0.f);
+      float2 v = m <= n ? make_float2(tile[2 * o][r][mi], tile[2 * o + 1][r][mi])
+                        : make_float2(0.f, 0.f);
+      if (NCOL == 2 && a.job.off00 && n == 0 && m == 0) {
+        const float2 d = a.job.off00[f];
+        v = make_float2(v.x + d.x, v.y + d.y);
+      }
       out[(size_t)n * T1 + m] = v;
     }
   }

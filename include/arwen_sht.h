@@ -55,6 +55,17 @@ extern "C" {
 #define ARWEN_SHT_OP_MERIDIONAL_TO_WAIST 10 /* in0 coeff -> out0 waist (dP/dphi)     */
 #define ARWEN_SHT_OP_ANALYSIS_FROM_WAIST 11 /* in0 waist -> out0 coeff               */
 #define ARWEN_SHT_OP_VORDIV_FROM_WAIST 12  /* in0 u waist, in1 v waist -> vort, div  */
+/* inverse, plus the east and north gradients of the same field: out0 grid,
+ * out1 east = d/dlambda / (a cos), out2 north = d/dphi / a (reference
+ * transform.py:1742-1768).  Two plane contractions per field: the east
+ * derivative is the grid's own waist times i m, as inverse(c i m) / (a cos)
+ * is in the reference (inverse_zonal_derivative, :1752-1760). */
+#define ARWEN_SHT_OP_SYNTHESIS_GRADIENT 13
+
+/* arwen_sht_job.flags */
+#define ARWEN_SHT_JOB_EXP 1  /* SYNTHESIS: out0 = exp(inverse(in0)), the surface
+                                pressure from ln ps (reference dynamics.py:964-965)
+                                or a moisture field kept >= 0 by a log transform */
 
 typedef struct arwen_sht_plan arwen_sht_plan;
 typedef struct arwen_sht_batch arwen_sht_batch;
@@ -116,8 +127,12 @@ int arwen_sht_plan_fp64_orders(const arwen_sht_plan* plan);
  * legendre.py:76-146) so it can be checked against them. */
 int arwen_sht_plan_table_f64(const arwen_sht_plan* plan, int kind, double* out);
 
-/* Single operations (one grouped-GEMM launch each; workspaces cached per
- * shape and pointers inside the plan). */
+/* Single operations (one grouped-GEMM launch each).  The plan keeps a
+ * one-job batch per (operation, count, stream): the first call of a key
+ * allocates its workspaces and uploads its work lists (and waits for that);
+ * every later call rebinds the caller's buffers on the host and launches,
+ * with no allocation and no synchronization.  At most 16 keys are kept per
+ * plan; evicting one waits for its stream. */
 /* forward (reference transform.py:1486-1490) */
 int arwen_sht_analysis(arwen_sht_plan* plan, const float* grid, void* coeff, int count,
                        void* stream);
@@ -133,6 +148,27 @@ int arwen_sht_meridional_derivative(arwen_sht_plan* plan, const void* coeff, flo
 /* gradient (transform.py:1762-1768): east = d/dlambda / (a cos), north = d/dphi / a */
 int arwen_sht_gradient(arwen_sht_plan* plan, const void* coeff, float* east, float* north,
                        int count, void* stream);
+/* inverse with both gradients of the same field (ARWEN_SHT_OP_SYNTHESIS_GRADIENT):
+ * grid, east = d/dlambda / (a cos), north = d/dphi / a, from two plane
+ * contractions (reference transform.py:1742-1768, as the dycore calls
+ * inverse and gradient on temperature and ln ps, dynamics.py:907-1047). */
+int arwen_sht_synthesis_gradient(arwen_sht_plan* plan, const void* coeff, float* grid,
+                                 float* east, float* north, int count, void* stream);
+/* forward about offsets: forward(grid - grid_offset[f]) with coeff_offset[f]
+ * added to the (0, 0) coefficient (either offset may be NULL; device arrays
+ * of count float32 / complex64): the theta analysis offsets of the reference
+ * (semilag/step.py:128-163, 610-621) and lnps_from_anomaly_coefficients
+ * after an analysis (lnps_reference.py:47-49), fused into the fold and the
+ * unpack. */
+int arwen_sht_analysis_about(arwen_sht_plan* plan, const float* grid, void* coeff, int count,
+                             const float* grid_offset, const void* coeff_offset, void* stream);
+/* inverse of coeff with coeff_offset[f] (complex64, or NULL) added to its
+ * (0, 0) coefficient in the pack (lnps_anomaly_coefficients before an
+ * inverse, reference lnps_reference.py:32-44, dynamics.py:978-980); flags
+ * ARWEN_SHT_JOB_EXP writes exp of the grid (ps = exp(ln ps),
+ * dynamics.py:964-965). */
+int arwen_sht_synthesis_about(arwen_sht_plan* plan, const void* coeff, float* grid, int count,
+                              const void* coeff_offset, int flags, void* stream);
 /* VorticityDivergenceOperator.wind_from_vordiv (reference vector.py:36-52) */
 int arwen_sht_uv_from_vordiv(arwen_sht_plan* plan, const void* vort, const void* div, float* u,
                              float* v, int count, void* stream);
@@ -263,6 +299,18 @@ typedef struct arwen_sht_job {
   const void* in1;
   void* out0;
   void* out1;
+  void* out2;             /* ARWEN_SHT_OP_SYNTHESIS_GRADIENT: north */
+  int flags;              /* ARWEN_SHT_JOB_* */
+  /* Offsets about which a field is transformed (NULL: none), device arrays
+   * of `count` entries.  ANALYSIS: grid_offset[f] (float32) is subtracted
+   * from field f's grid before its FFT, and coeff_offset[f] (complex64) is
+   * added to its (0, 0) coefficient afterwards: forward(g - c) + c sqrt(4 pi)
+   * with the theta analysis offsets of the reference (semilag/step.py:128-163,
+   * 610-621).  SYNTHESIS, SYNTHESIS_GRADIENT: coeff_offset[f] is added to the
+   * (0, 0) coefficient before the transform, as add_grid_constant does
+   * (transform.py:1832-1835; lnps_reference.py:32-49). */
+  const float* grid_offset;
+  const void* coeff_offset;
 } arwen_sht_job;
 
 int arwen_sht_batch_create(const arwen_sht_job* jobs, int njobs, arwen_sht_batch** out);
