@@ -12,8 +12,16 @@ run as one TransformBatch (one grouped Legendre GEMM launch per direction)
 replayed from a hipGraph, inputs resident in HBM.  ``value`` is the split-BF16
 (bf16x6) round trips/s; the same workload in FP32 MFMA is timed beside it.
 
-    python bench.py [--steps K] [--warmup W] [--workload t383|t383x32|mixed|sample|vertical]
-                    [--precision bf16x6|bf16x3|float32] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--precision bf16x6|bf16x3|float32]
+                    [--workload t383|t383x32|mixed|t63|step|stepx32|sample|vertical]
+                    [--no-cpu-baseline] [--no-extras]
+
+``step`` is the dycore's per-step set at T383 (inverse: u,v, temperature with
+both gradients, q = exp(ln q), ln ps with both gradients; forward: vorticity
+and divergence from u,v, temperature, q, geopotential, ln ps) in one batch;
+``stepx32`` is 32 members of it.  The default run also times single calls at
+the dycore's chunk shapes (6 x 40 fields, one ln ps plane) and the T63
+round trip beside the oracle's T63 figure.
 
 Prints one JSON line.  Only ``cpu_baseline()`` touches the FP64 oracle: it
 times the oracle (the reference's numpy algorithm) on a bounded sample of the
@@ -39,7 +47,8 @@ MFMA_PEAK = {"bf16x6": 2516.6, "bf16x3": 2516.6, "float32": 157.3}   # dense TFL
 PRODUCTS = {"bf16x6": 6, "bf16x3": 3, "float32": 1}
 HBM_PEAK_GBS = 8000.0
 LEVELS = 40
-CLASS_SCALE = {"vorticity": 1e-5, "divergence": 2e-6, "temperature": 10.0, "lnps": 1e-2}
+CLASS_SCALE = {"vorticity": 1e-5, "divergence": 2e-6, "temperature": 10.0, "lnps": 1e-2,
+               "lnq": 0.3, "geopotential": 3e3}
 
 
 def red_spectrum(truncation, lead, seed, scale):
@@ -59,11 +68,14 @@ def red_spectrum(truncation, lead, seed, scale):
     return c
 
 
-def member_fields(truncation, members, seed0=0):
-    """(vorticity, divergence, scalars) per member; member 0 is the control,
-    member k adds a 1% seeded red-spectrum perturbation."""
+def member_fields(truncation, members, seed0=0, step=False):
+    """Spectral fields per member: vorticity, divergence, temperature, ln ps
+    (and, for the per-step set, ln q and geopotential); member 0 is the
+    control, member k adds a 1% seeded red-spectrum perturbation."""
     out = {}
     seeds = {"vorticity": 11, "divergence": 12, "temperature": 13, "lnps": 14}
+    if step:
+        seeds.update(lnq=15, geopotential=16)
     for name, s in seeds.items():
         lev = 1 if name == "lnps" else LEVELS
         ctrl = red_spectrum(truncation, (lev,), seed0 + s, CLASS_SCALE[name])
@@ -75,13 +87,24 @@ def member_fields(truncation, members, seed0=0):
     return out
 
 
-class RoundTrip:
-    """The device buffers and the batch of one round-trip workload."""
+#: coefficient planes read and waist planes written by the synthesis GEMM
+#: (per level), and waist planes read and coefficient planes written by the
+#: analysis GEMM, per job kind
+GEMM_PLANES = {"wind": (2, 2, 0, 0), "inverse": (1, 1, 0, 0), "inverse_gradient": (1, 2, 0, 0),
+               "vordiv": (0, 0, 2, 2), "forward": (0, 0, 1, 1)}
 
-    def __init__(self, specs, precision, torch, tune=True, split_acc=None, fp64_orders=None):
+
+class RoundTrip:
+    """The device buffers and the batch of one workload: the 121-field round
+    trip, or with ``step`` the dycore's per-step set (243 grid outputs from
+    the inverse, 201 grid fields into the forward)."""
+
+    def __init__(self, specs, precision, torch, tune=True, split_acc=None, fp64_orders=None,
+                 step=False):
         from arwen_spectral import SphericalHarmonicTransform, TransformBatch
 
         self.parts = []
+        self.step = step
         jobs = []
         dev = torch.device("cuda", torch.cuda.current_device())
         for spec in specs:
@@ -91,16 +114,52 @@ class RoundTrip:
             c64 = lambda a: torch.from_numpy(a.astype(np.complex64)).to(dev)
             zeta = c64(f["vorticity"]).contiguous()
             div = c64(f["divergence"]).contiguous()
-            scal = c64(np.concatenate([f["temperature"], f["lnps"]], axis=1)).contiguous()
             M = zeta.shape[0]
             g = lambda n: torch.empty((M, n, *tr.grid.shape), dtype=torch.float32, device=dev)
             s = lambda n: torch.empty((M, n, *tr.spectral_shape), dtype=torch.complex64, device=dev)
-            u, v, grid = g(LEVELS), g(LEVELS), g(LEVELS + 1)
-            z2, d2, s2 = s(LEVELS), s(LEVELS), s(LEVELS + 1)
-            jobs += [("wind", tr, (zeta, div), (u, v)), ("inverse", tr, scal, grid),
-                     ("vordiv", tr, (u, v), (z2, d2)), ("forward", tr, grid, s2)]
-            self.parts.append(dict(tr=tr, zeta=zeta, div=div, scal=scal, u=u, v=v, grid=grid,
-                                   z2=z2, d2=d2, s2=s2, spec=spec))
+            L = LEVELS
+            if not step:
+                scal = c64(np.concatenate([f["temperature"], f["lnps"]], axis=1)).contiguous()
+                u, v, grid = g(L), g(L), g(L + 1)
+                z2, d2, s2 = s(L), s(L), s(L + 1)
+                pj = [("wind", tr, (zeta, div), (u, v)), ("inverse", tr, scal, grid),
+                      ("vordiv", tr, (u, v), (z2, d2)), ("forward", tr, grid, s2)]
+                part = dict(tr=tr, zeta=zeta, div=div, scal=scal, u=u, v=v, grid=grid,
+                            z2=z2, d2=d2, s2=s2, spec=spec)
+                part["pairs"] = {"vorticity": (z2, "vorticity"), "divergence": (d2, "divergence"),
+                                 "temperature": (s2[:, :L], "temperature"),
+                                 "lnps": (s2[:, L:], "lnps")}
+            else:
+                temp, lnq = c64(f["temperature"]).contiguous(), c64(f["lnq"]).contiguous()
+                lnps = c64(f["lnps"]).contiguous()
+                # geopotential on the grid: a resident input of the forward
+                phig = tr.inverse(c64(f["geopotential"]))
+                u, v, tg, tx, ty, q = g(L), g(L), g(L), g(L), g(L), g(L)
+                lg, lx, ly = g(1), g(1), g(1)
+                z2, d2, t2, q2, p2, l2 = s(L), s(L), s(L), s(L), s(L), s(1)
+                pj = [("wind", tr, (zeta, div), (u, v)),
+                      ("inverse_gradient", tr, temp, (tg, tx, ty)),
+                      ("inverse", tr, lnq, q, {"exp": True}),      # q = exp(ln q) >= 0
+                      ("inverse_gradient", tr, lnps, (lg, lx, ly)),
+                      ("vordiv", tr, (u, v), (z2, d2)),
+                      ("forward", tr, tg, t2), ("forward", tr, q, q2),
+                      ("forward", tr, phig, p2), ("forward", tr, lg, l2)]
+                part = dict(tr=tr, zeta=zeta, div=div, u=u, v=v, q=q, spec=spec)
+                part["pairs"] = {"vorticity": (z2, "vorticity"), "divergence": (d2, "divergence"),
+                                 "temperature": (t2, "temperature"), "lnps": (l2, "lnps"),
+                                 "geopotential": (p2, "geopotential")}
+            part["planes"] = [0, 0, 0, 0]  # GEMM planes: syn coef, syn waist, ana waist, ana coef
+            for job in pj:
+                nlev = job[2][0].shape[1] if isinstance(job[2], tuple) else job[2].shape[1]
+                for k, w in enumerate(GEMM_PLANES[job[0]]):
+                    part["planes"][k] += M * nlev * w
+            seq = lambda x: x if isinstance(x, tuple) else (x,)  # noqa: E731
+            part["grid_out"] = sum(o.shape[0] * o.shape[1] for j in pj if GEMM_PLANES[j[0]][0]
+                                   for o in seq(j[3]))
+            part["grid_in"] = sum(i.shape[0] * i.shape[1] for j in pj if GEMM_PLANES[j[0]][2]
+                                  for i in seq(j[2]))
+            jobs += pj
+            self.parts.append(part)
         self.batch = TransformBatch(jobs)
         if split_acc is not None:
             self.batch.set_config(split_acc=split_acc)
@@ -128,62 +187,101 @@ def class_errors(rt):
     for part in rt.parts:
         M = part["zeta"].shape[0]
         sel = sorted({0, M - 1})
-        pairs = {"vorticity": (part["z2"], part["zeta"]), "divergence": (part["d2"], part["div"]),
-                 "temperature": (part["s2"][:, :LEVELS], part["scal"][:, :LEVELS]),
-                 "lnps": (part["s2"][:, LEVELS:], part["scal"][:, LEVELS:])}
-        for name, (have, want) in pairs.items():
+        for name, (have, key) in part["pairs"].items():
             h = have[sel].cpu().numpy().astype(np.complex128)
-            w = part["spec"]["fields"][name][sel]
+            w = part["spec"]["fields"][key][sel]
             e = float(np.max(np.abs(h - w)) / np.max(np.abs(w)))
             key = f"T{part['spec']['T']}_{name}"
             out[key] = max(out.get(key, 0.0), e)
     return out
 
 
-def cpu_baseline(part, sample_levels=LEVELS):
-    """FP64 oracle (reference numpy algorithm) on the same workload: by default
-    all 121 fields of one member (about 5 s per round trip on 16 cores), or the
-    first `sample_levels` levels of each class plus ln ps, scaled linearly.
-    Returns the timing entry and the oracle's outputs."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
+def _oracle_round_trip(o, zeta, div, scal):
+    u, v = o.wind_from_vordiv(zeta, div)
+    g = o.inverse(scal)
+    z2, d2 = o.vordiv_from_wind(u, v)
+    s2 = o.forward(g)
+    return dict(u=u, v=v, grid=g, zeta=z2, div=d2, scal=s2)
+
+
+def _time_oracle(o, fields, threads, warmups=2, iterations=5):
+    """The reference CLI benchmark's protocol (spectral/cli.py:153-202): the
+    first call timed on its own, then warmups, then the steady mean."""
+    from threadpoolctl import threadpool_limits
+
+    with threadpool_limits(threads):
+        t0 = time.perf_counter()
+        out = _oracle_round_trip(o, *fields)
+        first = time.perf_counter() - t0
+        for _ in range(warmups):
+            _oracle_round_trip(o, *fields)
+        times = []
+        for _ in range(iterations):
+            t0 = time.perf_counter()
+            _oracle_round_trip(o, *fields)
+            times.append(time.perf_counter() - t0)
+    return first, float(np.mean(times)), out
+
+
+def cpu_baseline(part):
+    """FP64 oracle (the reference's per-order BLAS algorithm in numpy) on the
+    host cores: the 121-field T383 round trip of member 0 on all cores (up to
+    16), 2 warmups and 5 timed iterations; the same on one thread over a
+    4-field sample (one level of vorticity, divergence, temperature + ln ps);
+    and the T63 121-field round trip on all cores and on one thread.
+    Returns the timing entry and the T383 oracle outputs."""
     sys.path.insert(0, str(ROOT))
     from oracle.sht import OracleTransform
 
-    try:
-        from threadpoolctl import threadpool_limits
-    except ImportError:  # pragma: no cover
-        threadpool_limits = None
     cores = min(len(os.sched_getaffinity(0)), 16)
     T = part["spec"]["T"]
     f = part["spec"]["fields"]
-    L = sample_levels
-    zeta, div = f["vorticity"][0, :L], f["divergence"][0, :L]
-    scal = np.concatenate([f["temperature"][0, :L], f["lnps"][0]], axis=0)
-    ctx = threadpool_limits(cores) if threadpool_limits else None
-    try:
-        o = OracleTransform.create(T, dealias_factor=1.0)
-        times = []
-        for rep in range(3):  # first call, then two steady calls
-            t0 = time.perf_counter()
-            u, v = o.wind_from_vordiv(zeta, div)
-            g = o.inverse(scal)
-            z2, d2 = o.vordiv_from_wind(u, v)
-            s2 = o.forward(g)
-            times.append(time.perf_counter() - t0)
-    finally:
-        if ctx is not None:
-            ctx.__exit__(None, None, None)
-    fields_sample = 3 * L + 1
-    steady = float(np.mean(times[1:]))
+    full = (f["vorticity"][0], f["divergence"][0],
+            np.concatenate([f["temperature"][0], f["lnps"][0]], axis=0))
+    o = OracleTransform.create(T, dealias_factor=1.0)
+    first, steady, out = _time_oracle(o, full, cores)
+    sample = (full[0][:1], full[1][:1], np.concatenate([full[2][:1], full[2][LEVELS:]], axis=0))
+    first1, steady1, _ = _time_oracle(o, sample, 1)
+    t63 = member_fields(63, 1)
+    o63 = OracleTransform.create(63, dealias_factor=1.0)
+    f63 = (t63["vorticity"][0], t63["divergence"][0],
+           np.concatenate([t63["temperature"][0], t63["lnps"][0]], axis=0))
+    _, s63, _ = _time_oracle(o63, f63, cores)
+    _, s63_1, _ = _time_oracle(o63, f63, 1)
     fields_total = 3 * LEVELS + 1
-    value = (fields_sample / fields_total) / steady  # 121-field round trips/s
-    what = (f"all {fields_total} fields of one member" if fields_sample == fields_total else
-             f"{fields_sample} of {fields_total} fields ({L} levels of vorticity, divergence, "
-             f"temperature + ln ps), scaled linearly to {fields_total} fields")
-    entry = {"value": value, "unit": "round trips/s", "cores": cores, "kind": "port",
-             "sample": f"FP64 numpy oracle (reference per-order BLAS algorithm), T{T}, {what}; "
-                       f"steady mean of 2 round trips ({steady:.2f} s each) after a first call "
-                       f"({times[0]:.2f} s)"}
-    return entry, dict(u=u, v=v, grid=g, zeta=z2, div=d2, scal=s2, L=L)
+    entry = {
+        "value": 1.0 / steady, "unit": "round trips/s", "cores": cores, "kind": "port",
+        "cpu_model": cpu_model(),
+        "sample": f"FP64 numpy oracle (the reference's per-order BLAS algorithm), T{T} "
+                  f"{fields_total}-field round trip of member 0 (u, v from vorticity/divergence "
+                  f"and back, temperature, ln ps) on {cores} threads: first call {first:.2f} s, "
+                  f"2 warmups, steady mean of 5 iterations {steady:.2f} s",
+        "first_call_s": first, "steady_s": steady, "warmups": 2, "iterations": 5,
+        "single_thread": {"value": (4.0 / fields_total) / steady1, "unit": "round trips/s",
+                          "cores": 1, "first_call_s": first1, "steady_s": steady1,
+                          "sample": "4 of 121 fields (one level of vorticity, divergence, "
+                                    "temperature + ln ps), 2 warmups + 5 iterations, scaled "
+                                    "linearly to 121 fields (the oracle's per-call table "
+                                    "expansion makes the scaled rate a lower bound)"},
+        "c0_t63": {"value": 1.0 / s63, "unit": "round trips/s", "cores": cores,
+                   "single_thread_value": 1.0 / s63_1,
+                   "sample": "T63 64x128, 121 fields, 2 warmups + 5 iterations each"},
+    }
+    out["L"] = LEVELS
+    return entry, out
 
 
 def oracle_errors(part, ref):
@@ -228,28 +326,29 @@ TABLE_BYTES = {"float32": 4, "bf16x3": 4, "bf16x6": 6}  # bytes per table elemen
 
 
 def gemm_algorithmic_bytes(rt, precision):
-    """Unique HBM bytes of the round trip's two GEMM launches: each Legendre
+    """Unique HBM bytes of the workload's two GEMM launches: each Legendre
     table the launch uses read once (P, dP/dphi, P m/cos: K_tri x nlat/2
-    folded elements, no padding), and every field's coefficients (K_tri
-    complex64) and waist (nlat x (T+1) complex64) read or written once per
-    direction."""
+    folded elements, no padding), and every coefficient plane (K_tri
+    complex64) and waist plane (nlat x (T+1) complex64) the launch reads or
+    writes, once."""
     total = 0.0
     for part in rt.parts:
         T, nlat = part["spec"]["T"], part["tr"].grid.nlat
-        members = part["zeta"].shape[0]
         k_tri = (T + 1) * (T + 2) // 2
         tables = 3 * k_tri * (nlat / 2) * TABLE_BYTES[precision]
-        fields = members * (3 * LEVELS + 1)
-        data = fields * (8.0 * k_tri + 8.0 * nlat * (T + 1))
-        total += 2 * (tables + data)
+        coef = 8.0 * k_tri
+        waist = 8.0 * nlat * (T + 1)
+        sc, sw, aw, ac = part["planes"]
+        total += 2 * tables + sc * coef + sw * waist + aw * waist + ac * coef
     return total
 
 
 def roofline(rt, phases, precision, workload="t383"):
-    """Roofline of the dominant kernel.  For the Legendre GEMM the bound is
-    chosen by arithmetic intensity against the ridge point: MFMA products per
-    algorithmic byte above peak_mfma / peak_hbm is MFMA-bound, below it
-    HBM-bound; the other roof is reported beside it."""
+    """Roofline of the dominant kernel.  The Legendre GEMM is priced on its
+    MFMA roof (MFMA products per second against the dense peak of the
+    instruction it issues); its HBM roof (algorithmic bytes per second) and
+    the arithmetic intensity against the ridge point are reported beside it
+    as ``other_roof``."""
     gemm_ms = phases["synthesis_gemm"] + phases["analysis_gemm"]
     fft_ms = phases["inverse_fft"] + phases["forward_fft"]
     s = PRODUCTS[precision]
@@ -264,7 +363,7 @@ def roofline(rt, phases, precision, workload="t383"):
                "unit": "GB/s"}
         for r in (mfma, hbm):
             r["frac"] = r["achieved"] / r["peak"]
-        main, other = (mfma, hbm) if intensity >= ridge else (hbm, mfma)
+        main, other = mfma, hbm
         return {**main, "kernel": "legendre_gemm (synthesis + analysis launches)",
                 "traffic": gemm_traffic(rt.config, precision, workload),
                 "traffic_unit": "HBM bytes per round trip (both launches)",
@@ -458,13 +557,85 @@ def vertical_bench(args, torch):
 
 
 def workload_specs(name):
-    if name == "t383":
-        return [dict(T=383, members=1)]
-    if name == "t383x32":
-        return [dict(T=383, members=32)]
-    if name == "mixed":
-        return [dict(T=767, members=4), dict(T=383, members=8), dict(T=127, members=16)]
-    raise SystemExit(f"unknown workload {name}")
+    """(specs, step): the truncations and members of a workload, and whether
+    it is the per-step set rather than the 121-field round trip."""
+    table = {
+        "t383": ([dict(T=383, members=1)], False),
+        "t383x32": ([dict(T=383, members=32)], False),
+        "mixed": ([dict(T=767, members=4), dict(T=383, members=8), dict(T=127, members=16)],
+                  False),
+        "t63": ([dict(T=63, members=1)], False),
+        "step": ([dict(T=383, members=1)], True),
+        "stepx32": ([dict(T=383, members=32)], True),
+    }
+    if name not in table:
+        raise SystemExit(f"unknown workload {name}; expected one of {sorted(table)} or sample, "
+                         "vertical")
+    specs, step = table[name]
+    return [dict(sp) for sp in specs], step
+
+
+WORKLOAD_TEXT = {
+    "t383": "T383 768x384 linear Gaussian grid, 121 fields per member (40 vorticity, 40 "
+            "divergence, 40 temperature, ln ps): synthesis incl. u,v from vorticity/divergence, "
+            "then analysis incl. vorticity/divergence from u,v",
+    "step": "T383 768x384, the dycore's per-step transform set per member: inverse u,v from "
+            "40 vorticity/divergence, 40 temperature with both gradients, 40 q = exp(ln q), "
+            "ln ps with both gradients (243 grid outputs, 282 plane contractions); forward "
+            "vorticity/divergence from 40 u,v, 40 temperature, 40 q, 40 geopotential, ln ps "
+            "(201 grid fields, 281 plane contractions)",
+}
+
+
+def _event_ms(torch, fn, reps, stream=None):
+    """Mean milliseconds of ``fn`` over ``reps`` calls, HIP events on the
+    current stream after one untimed call."""
+    fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def single_op_leg(torch, precision, reps=10):
+    """Single calls at the dycore's chunk shapes (T383): a chunk of 6 fields x
+    40 levels through inverse and forward, and a single ln ps plane through
+    inverse, forward and gradient, each one grouped-GEMM launch per direction
+    from the plan's cached workspaces (no allocation after the first call of a
+    shape)."""
+    from arwen_spectral import SphericalHarmonicTransform
+
+    tr = SphericalHarmonicTransform.create(383, dealias_factor=1.0, precision=precision)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    chunk = torch.from_numpy(red_spectrum(383, (6 * LEVELS,), 51, 10.0).astype(np.complex64)).to(dev)
+    plane = chunk[:1].contiguous()
+    gchunk = tr.inverse(chunk)
+    gplane = gchunk[:1].contiguous()
+    out = {}
+    for name, fn, nf in (("inverse_6x40", lambda: tr.inverse(chunk), 240),
+                         ("forward_6x40", lambda: tr.forward(gchunk), 240),
+                         ("inverse_plane", lambda: tr.inverse(plane), 1),
+                         ("forward_plane", lambda: tr.forward(gplane), 1),
+                         ("gradient_plane", lambda: tr.gradient(plane), 1)):
+        ms = _event_ms(torch, fn, reps)
+        out[name] = {"ms_per_call": ms, "fields_per_s": nf / (ms * 1e-3)}
+    return out
+
+
+def gpu_t63(torch, precision, steps, warmup):
+    """The T63 64x128 121-field round trip on the GPU (beside the oracle's
+    T63 figure)."""
+    specs, _ = workload_specs("t63")
+    for sp in specs:
+        sp["fields"] = member_fields(sp["T"], sp["members"])
+    rt = RoundTrip(specs, precision, torch, True)
+    sec = time_steps(rt, steps, warmup, torch)
+    return {"value": 1.0 / sec, "unit": "round trips/s", "ms_per_step": sec * 1e3,
+            "roundtrip_error": class_errors(rt)}
 
 
 def main():
@@ -476,6 +647,8 @@ def main():
     ap.add_argument("--precision", default="bf16x6")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the single-call and T63 legs")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--split-acc", type=int, default=0,
                     help="split modes: 1 = small products in their own accumulators; 0 (the "
@@ -483,7 +656,7 @@ def main():
                          "keeps the FP32 path's error class")
     ap.add_argument("--fp64-orders", type=int, default=None,
                     help="vector operators: zonal orders below this run in FP64 end to end "
-                         "(default: 0 for float32, 4 ceil((T+1)/96) for the split modes)")
+                         "(default: 0 for float32, 4 ceil((T+1)/192) for the split modes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB on the device three times (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -510,24 +683,25 @@ def main():
     if args.workload == "vertical":
         vertical_bench(args, torch)
         return
-    specs = workload_specs(args.workload)
+    specs, step = workload_specs(args.workload)
     for sp in specs:
-        sp["fields"] = member_fields(sp["T"], sp["members"])
-    nfields = sum(sp["members"] * (3 * LEVELS + 1) for sp in specs)
-    print(f"[bench] workload {args.workload}: {nfields} fields, precision {args.precision}",
-          file=sys.stderr, flush=True)
+        sp["fields"] = member_fields(sp["T"], sp["members"], step=step)
+    print(f"[bench] workload {args.workload}, precision {args.precision}", file=sys.stderr,
+          flush=True)
 
     rt = RoundTrip(specs, args.precision, torch, not args.no_autotune, args.split_acc,
-                   args.fp64_orders)
+                   args.fp64_orders, step=step)
     sec = time_steps(rt, args.steps, args.warmup, torch)
     phases = rt.batch.profile(reps=3)
     errors = class_errors(rt)
     print(f"[bench] {args.precision}: {sec * 1e3:.3f} ms/step, phases {phases}, "
           f"gemm config {rt.config}", file=sys.stderr, flush=True)
+    grid_out = sum(p["grid_out"] for p in rt.parts)
+    grid_in = sum(p["grid_in"] for p in rt.parts)
 
     fp32 = None
     if not args.no_fp32 and args.precision != "float32":
-        rt32 = RoundTrip(specs, "float32", torch, not args.no_autotune)
+        rt32 = RoundTrip(specs, "float32", torch, not args.no_autotune, step=step)
         sec32 = time_steps(rt32, args.steps, args.warmup, torch)
         ph32 = rt32.batch.profile(reps=3)
         fp32 = {"value": 1.0 / sec32, "ms_per_step": sec32 * 1e3, "phases_ms": ph32,
@@ -537,21 +711,32 @@ def main():
         print(f"[bench] float32: {sec32 * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
         del rt32
 
+    extras = None
+    if not args.no_extras and args.workload == "t383":
+        extras = {"single_calls": {args.precision: single_op_leg(torch, args.precision)},
+                  "t63_gpu": {args.precision: gpu_t63(torch, args.precision, args.steps,
+                                                      args.warmup)}}
+        if not args.no_fp32 and args.precision != "float32":
+            extras["single_calls"]["float32"] = single_op_leg(torch, "float32")
+            extras["t63_gpu"]["float32"] = gpu_t63(torch, "float32", args.steps, args.warmup)
+        print(f"[bench] extras {json.dumps(extras)}", file=sys.stderr, flush=True)
+
     cpu = None
     oracle_err = None
-    if not args.no_cpu_baseline:
-        part = next(p for p in rt.parts if p["spec"]["T"] == 383) if any(
-            p["spec"]["T"] == 383 for p in rt.parts) else rt.parts[0]
+    if not args.no_cpu_baseline and not step:
+        part = next((p for p in rt.parts if p["spec"]["T"] == 383), rt.parts[0])
         cpu, ref = cpu_baseline(part)
         oracle_err = oracle_errors(part, ref)
         print(f"[bench] cpu baseline {cpu['value']:.4f} round trips/s on {cpu['cores']} cores",
               file=sys.stderr, flush=True)
 
     members = sum(sp["members"] for sp in specs)
+    unit = "steps/s" if step else "round trips/s"
     result = {
-        "metric": METRIC,
+        "metric": METRIC if args.workload == "t383" else
+                  f"{args.workload}: {unit} ({args.precision})",
         "value": 1.0 / sec,
-        "unit": "round trips/s",
+        "unit": unit,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -564,12 +749,9 @@ def main():
                   "float32": "fp32"}[args.precision],
         "data": "synthetic: seeded red-spectrum coefficients (n^-1.5), temperature and ln ps as "
                 "anomalies about fixed references",
-        "config": {"workload": "T383 768x384 linear Gaussian grid, 121 fields per member (40 "
-                               "vorticity, 40 divergence, 40 temperature, ln ps): synthesis "
-                               "incl. u,v from vorticity/divergence, then analysis incl. "
-                               "vorticity/divergence from u,v" if args.workload == "t383"
-                   else args.workload,
-                   "members": members, "fields": nfields, "precision": args.precision,
+        "config": {"workload": WORKLOAD_TEXT.get(args.workload, args.workload),
+                   "members": members, "grid_outputs_of_synthesis": grid_out,
+                   "grid_inputs_of_analysis": grid_in, "precision": args.precision,
                    "specs": [{k: v for k, v in sp.items() if k != "fields"} for sp in specs]},
         "phases_ms": phases,
         "gemm_config": rt.config,
@@ -580,6 +762,7 @@ def main():
         "legendre_gflop_per_step": rt.flops / 1e9,
         "fft_mb_per_step": rt.fft_bytes / 1e6,
         "roofline": roofline(rt, phases, args.precision, args.workload),
+        "extras": extras,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result))

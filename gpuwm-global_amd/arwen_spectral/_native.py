@@ -27,6 +27,7 @@ OP_MERIDIONAL_TO_WAIST = 10
 OP_ANALYSIS_FROM_WAIST = 11
 OP_VORDIV_FROM_WAIST = 12
 OP_SYNTHESIS_GRADIENT = 13
+OP_DIVERGENCE_FROM_WAIST = 14
 JOB_EXP = 1
 # kind argument of arwen_sht_synthesis_to_waist
 WAIST_BASIS, WAIST_ZONAL, WAIST_DERIVATIVE = 0, 1, 2
@@ -44,7 +45,7 @@ EXPORTS = (
     "arwen_sht_meridional_derivative", "arwen_sht_gradient", "arwen_sht_synthesis_gradient",
     "arwen_sht_analysis_about", "arwen_sht_synthesis_about", "arwen_sht_uv_from_vordiv",
     "arwen_sht_vordiv_from_uv", "arwen_sht_synthesis_to_waist", "arwen_sht_analysis_from_waist",
-    "arwen_sht_vordiv_from_waist", "arwen_sht_waist_from_rows", "arwen_sht_rows_from_waist",
+    "arwen_sht_vordiv_from_waist", "arwen_sht_divergence_from_waist", "arwen_sht_waist_from_rows", "arwen_sht_rows_from_waist",
     "arwen_sht_spectral_op", "arwen_sht_sample_workspace_bytes", "arwen_sht_sample",
     "arwen_sht_sample_static",
     "arwen_sht_vertical_solve", "arwen_sht_vertical_tendencies", "arwen_sht_vertical_half_apply",
@@ -145,6 +146,7 @@ def load():
             "arwen_sht_synthesis_to_waist": (I, [P, P, P, I, I, P]),
             "arwen_sht_analysis_from_waist": (I, [P, P, P, I, P]),
             "arwen_sht_vordiv_from_waist": (I, [P, P, P, P, P, I, P]),
+            "arwen_sht_divergence_from_waist": (I, [P, P, P, P, I, P]),
             "arwen_sht_waist_from_rows": (I, [P, P, P, I, I, I, P]),
             "arwen_sht_rows_from_waist": (I, [P, P, P, I, I, I, P]),
             "arwen_sht_spectral_op": (I, [P, I, P, P, I, P]),

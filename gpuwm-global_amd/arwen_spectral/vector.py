@@ -90,8 +90,17 @@ class VorticityDivergenceOperator:
         return zeta, div
 
     def divergence_from_fourier(self, fourier):
-        """The divergence half of :meth:`_vordiv_from_fourier` (vector.py:136-165)."""
-        return self._vordiv_from_fourier(fourier)[1]
+        """The divergence alone from the (u, v) waist (vector.py:136-166): the
+        zonal component against P m / cos and the meridional one against
+        dP/dphi, two plane contractions per level instead of the pair's four;
+        projected.  The same weights and products as the pair's divergence,
+        in one grouped launch at half the columns."""
+        t = self.transform
+        uw, vw = self._pair_waists(fourier)
+        div = t._empty_spectral(uw.shape[:-2])
+        _native.check(t._lib.arwen_sht_divergence_from_waist(t._plan, _vp(uw), _vp(vw), _vp(div),
+                                                             t._count(uw), t.backend.stream()))
+        return div
 
     def flux_curl_divergence(self, eastward_flux, northward_flux):
         return self.vordiv_from_wind(eastward_flux, northward_flux)

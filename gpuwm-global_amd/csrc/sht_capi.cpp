@@ -424,7 +424,8 @@ bool to_waist(int op) {
   return op >= ARWEN_SHT_OP_SYNTHESIS_TO_WAIST && op <= ARWEN_SHT_OP_MERIDIONAL_TO_WAIST;
 }
 bool from_waist(int op) {
-  return op == ARWEN_SHT_OP_ANALYSIS_FROM_WAIST || op == ARWEN_SHT_OP_VORDIV_FROM_WAIST;
+  return op == ARWEN_SHT_OP_ANALYSIS_FROM_WAIST || op == ARWEN_SHT_OP_VORDIV_FROM_WAIST ||
+         op == ARWEN_SHT_OP_DIVERGENCE_FROM_WAIST;
 }
 
 constexpr int kXcds = 8;  // MI355X: 8 XCDs x 32 CUs
@@ -799,9 +800,11 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     }
   } else {
     // ---- FFT + fold ----
+    const bool divw = J.op == ARWEN_SHT_OP_DIVERGENCE_FROM_WAIST;
     const int ncol = vec ? 4 : 2;
     js->ncols = ncol * n;
     if (prepare) js->waist_a.alloc((size_t)js->ncols * P->ldw * 4);
+    if (prepare && divw) js->waist_b.alloc((size_t)js->ncols * P->ldw * 4);
     if (prepare) js->packed_out.alloc((size_t)js->ncols * P->ldp * 4);
     if (from_waist(J.op)) {
       // a reference waist (*lead, nlat, T+1), already divided by nlon
@@ -809,7 +812,8 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
       wf.w0 = static_cast<const float2*>(J.in0);
       wf.w1 = static_cast<const float2*>(J.in1);
       wf.waist = js->waist_a.as<float>();
-      wf.rowscale = vec ? P->scale_waist_vec.as<float>() : P->scale_waist.as<float>();
+      wf.waist1 = divw ? js->waist_b.as<float>() : nullptr;
+      wf.rowscale = (vec || divw) ? P->scale_waist_vec.as<float>() : P->scale_waist.as<float>();
       wf.T = T;
       wf.nlat = P->nlat;
       wf.JH = P->JH;
@@ -817,7 +821,7 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
       wf.ldw = P->ldw;
       wf.nfields = n;
       wf.col0 = 0;
-      wf.vector = vec ? 1 : 0;
+      wf.vector = divw ? 2 : vec ? 1 : 0;
       B->waist_folds.push_back(wf);
     }
     sht::FoldArgs fa{};
@@ -881,12 +885,16 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
           // zeta, D = Pm^T [iV, iU]_q + dP^T [U, -V]_{1-q}
           AnaSeg sg[2] = {{kPm, 0, wst, 0}, {kD, 1, wst, kMapAnaVec}};
           add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 2);
+        } else if (divw) {
+          // D = Pm^T (iU)_q + dP^T (-V)_{1-q}
+          AnaSeg sg[2] = {{kPm, 0, wst, 0}, {kD, 1, DataRef{js->waist_b.as<char>()}, 0}};
+          add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 2);
         } else {
           AnaSeg sg[1] = {{kP, 0, wst, 0}};
           add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 1);
         }
       }
-    if (prepare) B->legendre_flops += (vec ? 4.0 : 1.0) * n * pcf;
+    if (prepare) B->legendre_flops += (vec ? 4.0 : divw ? 2.0 : 1.0) * n * pcf;
     sht::UnpackArgs ua{};
     ua.koff = P->d_koff.as<int>();
     ua.T = T;
@@ -934,9 +942,10 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
                      J.op == ARWEN_SHT_OP_VORDIV_FROM_WAIST;
     const bool sgrad = J.op == ARWEN_SHT_OP_SYNTHESIS_GRADIENT;
     const bool two_out = vec || J.op == ARWEN_SHT_OP_GRADIENT || sgrad;
-    if (J.op < ARWEN_SHT_OP_SYNTHESIS || J.op > ARWEN_SHT_OP_SYNTHESIS_GRADIENT)
+    const bool two_in = vec || J.op == ARWEN_SHT_OP_DIVERGENCE_FROM_WAIST;
+    if (J.op < ARWEN_SHT_OP_SYNTHESIS || J.op > ARWEN_SHT_OP_DIVERGENCE_FROM_WAIST)
       fail(ARWEN_SHT_EINVAL, "unknown op " + std::to_string(J.op));
-    if (J.count > 0 && (!J.in0 || !J.out0 || (vec && !J.in1) || (two_out && !J.out1) ||
+    if (J.count > 0 && (!J.in0 || !J.out0 || (two_in && !J.in1) || (two_out && !J.out1) ||
                         (sgrad && !J.out2)))
       fail(ARWEN_SHT_EINVAL, "job " + std::to_string(j) + " is missing a buffer");
     if (J.flags & ~ARWEN_SHT_JOB_EXP)
@@ -1194,6 +1203,11 @@ int arwen_sht_synthesis_to_waist(arwen_sht_plan* P, const void* coeff, void* wai
 int arwen_sht_analysis_from_waist(arwen_sht_plan* P, const void* waist, void* coeff, int count,
                                   void* s) {
   return single_op(P, ARWEN_SHT_OP_ANALYSIS_FROM_WAIST, waist, nullptr, coeff, nullptr, count, s);
+}
+
+int arwen_sht_divergence_from_waist(arwen_sht_plan* P, const void* uw, const void* vw, void* div,
+                                    int count, void* s) {
+  return single_op(P, ARWEN_SHT_OP_DIVERGENCE_FROM_WAIST, uw, vw, div, nullptr, count, s);
 }
 
 int arwen_sht_vordiv_from_waist(arwen_sht_plan* P, const void* uw, const void* vw, void* vort,

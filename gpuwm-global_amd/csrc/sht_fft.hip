@@ -1350,6 +1350,18 @@ __global__ void waist_fold_kernel(WaistFoldArgs a) {
       a.waist[waist_index(c, a.ldw, a.JHP, m, 1, i)] = o0[0].x;
       a.waist[waist_index(c + 1, a.ldw, a.JHP, m, 0, i)] = e0[0].y;
       a.waist[waist_index(c + 1, a.ldw, a.JHP, m, 1, i)] = o0[0].y;
+    } else if (a.vector == 2) {
+      // divergence alone: iU = [-U.im, U.re] against P m / cos, -V against dP/dphi
+      const int c = a.col0 + 2 * f;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const float2 U = p == 0 ? e0[0] : o0[0];
+        const float2 V = p == 0 ? e0[1] : o0[1];
+        a.waist[waist_index(c, a.ldw, a.JHP, m, p, i)] = -U.y;
+        a.waist[waist_index(c + 1, a.ldw, a.JHP, m, p, i)] = U.x;
+        a.waist1[waist_index(c, a.ldw, a.JHP, m, p, i)] = -V.x;
+        a.waist1[waist_index(c + 1, a.ldw, a.JHP, m, p, i)] = -V.y;
+      }
     } else {
       // level columns [iV, iU] = [-V.im, V.re, -U.im, U.re] (w0 = u, w1 = v)
       const int c = a.col0 + 4 * f;

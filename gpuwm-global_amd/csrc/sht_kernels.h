@@ -130,8 +130,10 @@ struct WaistFoldArgs {
   const float2* w0;           // (nfields, nlat, T+1): scalar fields, or u
   const float2* w1;           // v (vector jobs)
   float* waist;
+  float* waist1;              // vector == 2: the second waist (-V)
   const float* rowscale;      // per folded row: 2 pi w (x 1/a for vector jobs)
-  int T, nlat, JH, JHP, ldw, nfields, col0, vector;
+  int T, nlat, JH, JHP, ldw, nfields, col0;
+  int vector;                 // 0 scalar; 1 [iV, iU] per level; 2 iU to waist, -V to waist1
 };
 // The synthesis GEMM's folded waist -> reference waist (north = E + O,
 // south = E - O).

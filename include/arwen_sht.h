@@ -61,6 +61,11 @@ extern "C" {
  * derivative is the grid's own waist times i m, as inverse(c i m) / (a cos)
  * is in the reference (inverse_zonal_derivative, :1752-1760). */
 #define ARWEN_SHT_OP_SYNTHESIS_GRADIENT 13
+/* in0 u waist, in1 v waist -> out0 divergence alone: the zonal component
+ * against P m / cos and the meridional one against dP/dphi, two plane
+ * contractions per level where the vorticity-and-divergence pair takes four
+ * (reference vector.py:136-166, divergence_from_fourier). */
+#define ARWEN_SHT_OP_DIVERGENCE_FROM_WAIST 14
 
 /* arwen_sht_job.flags */
 #define ARWEN_SHT_JOB_EXP 1  /* SYNTHESIS: out0 = exp(inverse(in0)), the surface
@@ -169,6 +174,10 @@ int arwen_sht_analysis_about(arwen_sht_plan* plan, const float* grid, void* coef
  * dynamics.py:964-965). */
 int arwen_sht_synthesis_about(arwen_sht_plan* plan, const void* coeff, float* grid, int count,
                               const void* coeff_offset, int flags, void* stream);
+/* VorticityDivergenceOperator.divergence_from_fourier (reference
+ * vector.py:136-166): divergence from (u, v) reference waists. */
+int arwen_sht_divergence_from_waist(arwen_sht_plan* plan, const void* u_waist,
+                                    const void* v_waist, void* div, int count, void* stream);
 /* VorticityDivergenceOperator.wind_from_vordiv (reference vector.py:36-52) */
 int arwen_sht_uv_from_vordiv(arwen_sht_plan* plan, const void* vort, const void* div, float* u,
                              float* v, int count, void* stream);
