@@ -58,6 +58,13 @@ int guarded(F&& f) {
   }
 }
 
+// Set-up work of a single operation's first call goes to the caller's stream
+// (zero fills and uploads ordered before its kernels, no device-wide wait);
+// null elsewhere.  The host copies of what is uploaded there live in
+// g_setup_keep until the owner takes them.
+thread_local hipStream_t g_setup_stream = nullptr;
+thread_local std::vector<std::vector<char>>* g_setup_keep = nullptr;
+
 // Owning device buffer.
 struct DevBuf {
   void* p = nullptr;
@@ -69,7 +76,12 @@ struct DevBuf {
     bytes = n;
     if (n == 0) return;
     hip_check(hipMalloc(&p, n), "hipMalloc");
-    if (zero) hip_check(hipMemset(p, 0, n), "hipMemset");
+    if (zero) {
+      if (g_setup_stream)
+        hip_check(hipMemsetAsync(p, 0, n, g_setup_stream), "hipMemsetAsync");
+      else
+        hip_check(hipMemset(p, 0, n), "hipMemset");
+    }
   }
   void release() {
     if (p) (void)hipFree(p);
@@ -88,9 +100,18 @@ struct DevBuf {
 template <class T>
 void upload(DevBuf& buf, const std::vector<T>& host) {
   buf.alloc(host.size() * sizeof(T), false);
-  if (!host.empty())
-    hip_check(hipMemcpy(buf.p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice),
-              "hipMemcpy H2D");
+  if (host.empty()) return;
+  const size_t n = host.size() * sizeof(T);
+  if (g_setup_stream && g_setup_keep) {
+    // the copy may still be pending when this returns: copy from a kept buffer
+    g_setup_keep->emplace_back(reinterpret_cast<const char*>(host.data()),
+                               reinterpret_cast<const char*>(host.data()) + n);
+    hip_check(hipMemcpyAsync(buf.p, g_setup_keep->back().data(), n, hipMemcpyHostToDevice,
+                             g_setup_stream),
+              "hipMemcpyAsync H2D");
+  } else {
+    hip_check(hipMemcpy(buf.p, host.data(), n, hipMemcpyHostToDevice), "hipMemcpy H2D");
+  }
 }
 
 std::vector<int> factor_radices(int n) {
@@ -388,6 +409,7 @@ struct JobState {
 struct arwen_sht_batch {
   int device = 0;
   int precision = 0;
+  std::vector<std::vector<char>> setup_keep;  // host sources of stream-ordered uploads
   std::vector<std::unique_ptr<JobState>> jobs;
   std::vector<sht::PackArgs> packs;
   std::vector<sht::UnfoldArgs> unfolds;
@@ -964,7 +986,23 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
     B->jobs.push_back(std::move(js));
   }
   for (int d = 0; d < 2; ++d) build_lists(d, B->groups[d], B->config.cols[d], B->lists[d]);
-  hip_check(hipDeviceSynchronize(), "batch setup");
+  if (!g_setup_stream) hip_check(hipDeviceSynchronize(), "batch setup");
+}
+
+// plan_batch with its zero fills and uploads on `s`, before whatever the
+// caller queues there next; nothing waits for them on the host.
+void plan_batch_on(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs, hipStream_t s) {
+  struct Scope {
+    Scope(hipStream_t st, std::vector<std::vector<char>>* keep) {
+      g_setup_stream = st;
+      g_setup_keep = keep;
+    }
+    ~Scope() {
+      g_setup_stream = nullptr;
+      g_setup_keep = nullptr;
+    }
+  } scope(s, &B->setup_keep);
+  plan_batch(B, jobs, njobs);
 }
 
 // Persistent grid of direction d: a multiple of the XCD count (see
@@ -1056,9 +1094,10 @@ void apply_config(arwen_sht_batch* B, const arwen_sht_gemm_config& c) {
 // count, stream): its workspaces and GEMM work lists depend on those only, so
 // a later call with other buffers rebinds the launch arguments on the host
 // and launches, with no allocation and no synchronization.  The first call of
-// a key allocates and uploads (and waits for that); at most kSingleCache keys
-// are kept, and evicting one waits for its stream.  Calls of one plan on one
-// stream share that key's workspaces in stream order.
+// a key allocates, then zero-fills and uploads on the caller's stream ahead
+// of its kernels (no host wait); at most kSingleCache keys are kept, and
+// evicting one waits for its stream.  Calls of one plan on one stream share
+// that key's workspaces in stream order.
 constexpr size_t kSingleCache = 16;
 
 int single_job(arwen_sht_plan* P, const arwen_sht_job& job, void* stream) {
@@ -1081,7 +1120,7 @@ int single_job(arwen_sht_plan* P, const arwen_sht_job& job, void* stream) {
     J.plan = P;
     if (!B) {
       auto nb = std::make_unique<arwen_sht_batch>();
-      plan_batch(nb.get(), &J, 1);
+      plan_batch_on(nb.get(), &J, 1, static_cast<hipStream_t>(stream));
       B = nb.release();
       P->cache.push_front({key, B});
       while (P->cache.size() > kSingleCache) {
