@@ -13,7 +13,7 @@ replayed from a hipGraph, inputs resident in HBM.  ``value`` is the split-BF16
 (bf16x6) round trips/s; the same workload in FP32 MFMA is timed beside it.
 
     python bench.py [--steps K] [--warmup W] [--precision bf16x6|bf16x3|float32]
-                    [--workload t383|t383x32|mixed|t63|step|stepx32|sample|vertical]
+                    [--workload t383|t383x32|mixed|t63|step|stepx32|sample|vertical|ensemble]
                     [--no-cpu-baseline] [--no-extras]
 
 ``step`` is the dycore's per-step set at T383 (inverse: u,v, temperature with
@@ -556,6 +556,55 @@ def vertical_bench(args, torch):
         "cpu_baseline": cpu}))
 
 
+def ensemble_bench(args, torch):
+    """Every member's grid fields (u, v, theta, qv, ln ps, psi, chi) of a
+    32-member T383 L40 ensemble from one batch (EnsembleSynthesis), against
+    the reference's member loop run on the engine (the same synthesis one
+    member at a time); inputs resident, HIP events on the launch stream."""
+    from arwen_spectral import EnsembleSynthesis, SphericalHarmonicTransform, TransformBatch
+
+    T, R, L = 383, 32, LEVELS
+    tr = SphericalHarmonicTransform.create(T, dealias_factor=1.0, precision=args.precision)
+    ens = EnsembleSynthesis(tr, R, L)
+    for name, buf in ens.inputs.items():
+        lead = (R,) if name == "log_surface_pressure" else (R, L)
+        scale = {"vorticity": 1e-5, "divergence": 2e-6, "theta": 10.0, "qv": 1e-3,
+                 "log_surface_pressure": 1e-2}[name]
+        buf.copy_(torch.from_numpy(red_spectrum(T, lead, 60 + len(name), scale)
+                                   .astype(np.complex64)))
+    ms_batch = _event_ms(torch, lambda: ens.run(graph=True), args.steps)
+    # the member loop: the same syntheses as one batch per member, bound to
+    # that member's slices of the same buffers, each replayed from its graph
+    x, y = ens.inputs, ens.outputs
+    singles = []
+    for k in range(R):
+        b = TransformBatch([
+            ("wind", tr, (x["vorticity"][k], x["divergence"][k]), (y["u"][k], y["v"][k])),
+            ("inverse", tr, x["theta"][k], y["theta"][k]),
+            ("inverse", tr, x["qv"][k], y["qv"][k]),
+            ("inverse", tr, x["log_surface_pressure"][k], y["lnps"][k]),
+            ("inverse", tr, x["vorticity"][k], y["psi"][k], {"inverse_laplacian": True}),
+            ("inverse", tr, x["divergence"][k], y["chi"][k], {"inverse_laplacian": True})])
+        b.capture()
+        singles.append(b)
+
+    def loop():
+        for b in singles:
+            b.launch()
+    ms_loop = _event_ms(torch, loop, max(1, args.steps // 4))
+    fields = R * (6 * L + 1)
+    print(json.dumps({
+        "metric": f"T{T} ensemble grid fields/s ({R} members x (6 x {L} + 1) fields, one batch)",
+        "value": fields / (ms_batch * 1e-3), "unit": "grid fields/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_batch,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic: seeded red-spectrum member fields",
+        "config": {"workload": f"EnsembleSynthesis T{T}, {R} members, {L} levels: u, v, theta, "
+                               "qv, ln ps, psi, chi", "members": R, "levels": L},
+        "member_loop_ms": ms_loop, "speedup_vs_member_loop": ms_loop / ms_batch,
+        "roofline": None, "cpu_baseline": None}))
+
+
 def workload_specs(name):
     """(specs, step): the truncations and members of a workload, and whether
     it is the per-step set rather than the 121-field round trip."""
@@ -570,7 +619,7 @@ def workload_specs(name):
     }
     if name not in table:
         raise SystemExit(f"unknown workload {name}; expected one of {sorted(table)} or sample, "
-                         "vertical")
+                         "vertical, ensemble")
     specs, step = table[name]
     return [dict(sp) for sp in specs], step
 
@@ -682,6 +731,9 @@ def main():
         return
     if args.workload == "vertical":
         vertical_bench(args, torch)
+        return
+    if args.workload == "ensemble":
+        ensemble_bench(args, torch)
         return
     specs, step = workload_specs(args.workload)
     for sp in specs:

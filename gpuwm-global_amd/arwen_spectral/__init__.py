@@ -16,6 +16,7 @@ from __future__ import annotations
 from .backend import Backend, get_backend
 from .batch import TransformBatch
 from .constants import EARTH_RADIUS_M, REFERENCE_PRESSURE_PA
+from .ensemble import EnsembleSynthesis, ensemble_grid_fields
 from .grid import GaussianGrid
 from .offsets import (LNPS_REFERENCE, lnps_anomaly_coefficients, lnps_from_anomaly_coefficients,
                       theta_analysis_offsets)
@@ -28,5 +29,5 @@ __all__ = [
     "DEFAULT_BAND", "SphericalHarmonicTransform", "VorticityDivergenceOperator",
     "FourierWaist", "LegendreTable", "latitude_band_edges", "REFERENCE_PRESSURE_PA",
     "LNPS_REFERENCE", "lnps_anomaly_coefficients", "lnps_from_anomaly_coefficients",
-    "theta_analysis_offsets",
+    "theta_analysis_offsets", "EnsembleSynthesis", "ensemble_grid_fields",
 ]

@@ -29,6 +29,7 @@ OP_VORDIV_FROM_WAIST = 12
 OP_SYNTHESIS_GRADIENT = 13
 OP_DIVERGENCE_FROM_WAIST = 14
 JOB_EXP = 1
+JOB_INVLAP = 2
 # kind argument of arwen_sht_synthesis_to_waist
 WAIST_BASIS, WAIST_ZONAL, WAIST_DERIVATIVE = 0, 1, 2
 

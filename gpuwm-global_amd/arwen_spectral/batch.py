@@ -33,12 +33,14 @@ OPS = {
     "vordiv": (_native.OP_VORDIV_FROM_UV, 2, 2),
 }
 _SPECTRAL_IN = {"inverse", "inverse_gradient", "zonal", "meridional", "gradient", "wind"}
-#: per-job options: "exp" (inverse: write exp of the grid), "grid_offset"
-#: (forward: float32 (count,) subtracted from each field's grid first),
-#: "coeff_offset" (forward, inverse, inverse_gradient: complex64 (count,)
-#: added to each field's (0, 0) coefficient, after the analysis or before the
-#: synthesis)
-_OPTION_OPS = {"exp": {"inverse"}, "grid_offset": {"forward"},
+#: per-job options: "exp" (inverse: write exp of the grid),
+#: "inverse_laplacian" (inverse: synthesize the inverse Laplacian of the
+#: input, the streamfunction or velocity potential of a vorticity or
+#: divergence), "grid_offset" (forward: float32 (count,) subtracted from each
+#: field's grid first), "coeff_offset" (forward, inverse, inverse_gradient:
+#: complex64 (count,) added to each field's (0, 0) coefficient, after the
+#: analysis or before the synthesis)
+_OPTION_OPS = {"exp": {"inverse"}, "inverse_laplacian": {"inverse"}, "grid_offset": {"forward"},
                "coeff_offset": {"forward", "inverse", "inverse_gradient"}}
 
 
@@ -90,7 +92,10 @@ class TransformBatch:
             for name in opts:
                 if name not in _OPTION_OPS or op not in _OPTION_OPS[name]:
                     raise ValueError(f"option {name!r} does not apply to {op}")
-            flags = _native.JOB_EXP if opts.get("exp") else 0
+            flags = (_native.JOB_EXP if opts.get("exp") else 0) | \
+                (_native.JOB_INVLAP if opts.get("inverse_laplacian") else 0)
+            if opts.get("inverse_laplacian") and opts.get("coeff_offset") is not None:
+                raise ValueError("inverse_laplacian takes no coeff_offset")
             offsets = []
             for name, dtype in (("grid_offset", "float32"), ("coeff_offset", "complex64")):
                 t = opts.get(name)

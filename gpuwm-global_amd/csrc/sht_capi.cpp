@@ -701,8 +701,9 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     pa.job.radius = (float)P->radius;
     pa.job.mode = (J.op == ARWEN_SHT_OP_ZONAL_DERIVATIVE || J.op == ARWEN_SHT_OP_ZONAL_TO_WAIST)
                       ? sht::kIM
-                  : vec ? sht::kVector
-                        : sht::kPlain;
+                  : vec                              ? sht::kVector
+                  : (J.flags & ARWEN_SHT_JOB_INVLAP) ? sht::kInvLap
+                                                     : sht::kPlain;
     pa.job.off00 = static_cast<const float2*>(J.coeff_offset);
     B->packs.push_back(pa);
     // ---- FP64 low orders of u, v (orders m < m0) ----
@@ -970,10 +971,13 @@ void plan_batch(arwen_sht_batch* B, const arwen_sht_job* jobs, int njobs) {
     if (J.count > 0 && (!J.in0 || !J.out0 || (two_in && !J.in1) || (two_out && !J.out1) ||
                         (sgrad && !J.out2)))
       fail(ARWEN_SHT_EINVAL, "job " + std::to_string(j) + " is missing a buffer");
-    if (J.flags & ~ARWEN_SHT_JOB_EXP)
+    if (J.flags & ~(ARWEN_SHT_JOB_EXP | ARWEN_SHT_JOB_INVLAP))
       fail(ARWEN_SHT_EINVAL, "job " + std::to_string(j) + " has unknown flags");
     if ((J.flags & ARWEN_SHT_JOB_EXP) && J.op != ARWEN_SHT_OP_SYNTHESIS)
       fail(ARWEN_SHT_EINVAL, "ARWEN_SHT_JOB_EXP applies to ARWEN_SHT_OP_SYNTHESIS only");
+    if ((J.flags & ARWEN_SHT_JOB_INVLAP) && (J.op != ARWEN_SHT_OP_SYNTHESIS || J.coeff_offset))
+      fail(ARWEN_SHT_EINVAL,
+           "ARWEN_SHT_JOB_INVLAP applies to ARWEN_SHT_OP_SYNTHESIS without a coeff_offset");
     if (J.grid_offset && J.op != ARWEN_SHT_OP_ANALYSIS)
       fail(ARWEN_SHT_EINVAL, "grid_offset applies to ARWEN_SHT_OP_ANALYSIS only");
     if (J.coeff_offset && J.op != ARWEN_SHT_OP_ANALYSIS && J.op != ARWEN_SHT_OP_SYNTHESIS &&

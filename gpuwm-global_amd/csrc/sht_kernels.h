@@ -261,7 +261,7 @@ struct LowAnaArgs {
 };
 
 // ---- spectral data movement (sht_pack.hip) ----
-enum PackMode : int { kPlain = 0, kIM = 1, kICoef = 2, kVector = 3 };
+enum PackMode : int { kPlain = 0, kIM = 1, kICoef = 2, kVector = 3, kInvLap = 4 };
 
 struct PackJob {
   const float2* in0;          // [f][n][m]; vorticity for VECTOR

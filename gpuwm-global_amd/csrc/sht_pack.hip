@@ -124,6 +124,13 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
       o[0] = -(float)m * cv.y; o[1] = (float)m * cv.x;
     } else if (a.job.mode == kICoef) {
       o[0] = -cv.y; o[1] = cv.x;
+    } else if (a.job.mode == kInvLap) {
+      // the spectral inverse Laplacian's own arithmetic (n = 0 -> 0, m = 0 real)
+      if (n >= 1) {
+        const float eig = (float)(-((double)n * ((double)n + 1.0)) / a2);
+        o[0] = cv.x / eig;
+        o[1] = m == 0 ? 0.f : cv.y / eig;
+      }
     } else if constexpr (NIN == 2) {
       const float2 d = tile[NIN - 1][r][mi];
       float2 psi = make_float2(0.f, 0.f), chi = make_float2(0.f, 0.f);

@@ -71,6 +71,10 @@ extern "C" {
 #define ARWEN_SHT_JOB_EXP 1  /* SYNTHESIS: out0 = exp(inverse(in0)), the surface
                                 pressure from ln ps (reference dynamics.py:964-965)
                                 or a moisture field kept >= 0 by a log transform */
+#define ARWEN_SHT_JOB_INVLAP 2 /* SYNTHESIS: out0 = inverse(inverse_laplacian(in0)),
+                                  the streamfunction or velocity potential of a
+                                  vorticity or divergence (reference
+                                  da/ensemble.py:569-570), fused into the pack */
 
 typedef struct arwen_sht_plan arwen_sht_plan;
 typedef struct arwen_sht_batch arwen_sht_batch;
