@@ -65,12 +65,21 @@ __device__ __forceinline__ void low_segment(d4& acc0, d4& acc1, const double* b,
     for (int u = 0; u < kLowBatch; ++u) v[u] = a(j0 + 4 * u + kq);
   };
   auto mma = [&](const double (&v)[kLowBatch], int j0) {
+    // the batch's B values first (one LDS wait), then its MFMAs
+    double b0[kLowBatch], b1[kLowBatch];
 #pragma unroll
     for (int u = 0; u < kLowBatch; ++u) {
       const double* row = b + (size_t)(j0 + 4 * u + kq) * kLowCols;
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(v[u], row[col], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(v[u], row[16 + col], acc1, 0, 0, 0);
+      b0[u] = row[col];
+      b1[u] = row[16 + col];
     }
+#pragma unroll
+    for (int u = 0; u < kLowBatch; ++u) {
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(v[u], b0[u], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(v[u], b1[u], acc1, 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x100, 2 * kLowBatch, 0);  // the LDS reads first
+    __builtin_amdgcn_sched_group_barrier(0x008, 2 * kLowBatch, 0);  // then the MFMAs
   };
   load(x, 0);
   for (int j0 = 0; j0 < kpad; j0 += 2 * kLowKPad) {
