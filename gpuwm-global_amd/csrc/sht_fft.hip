@@ -576,8 +576,26 @@ constexpr int wave_span() { return L * kRowSpan; }  // float2 per sequence (>= 6
 // consecutive lanes) fall in eight different bank groups.
 constexpr int kSpanPad = 4;
 
+// Twiddle entries of the wave-resident kernels' LDS table: N + 64.
+template <int L>
+constexpr int wave_tw() { return 64 * L + 64; }
+
+// The wave-resident FFT's twiddles in LDS, in the order its lanes read them:
+// [k1][lane] = w^{lane k1} for the first pass (k1 < L), then [c][b] =
+// w^{b c L} for the second (c, b < 8), w = e^{-2 pi i / N} from the table.
+// Consecutive lanes read consecutive entries (and in the second pass the
+// eight b-groups of a wavefront read the same eight), so neither pass has
+// bank conflicts; read at w^{lane k1} and w^{b c L} straight from the table,
+// 16-byte entries collided up to 8 ways.
+template <int L, class C>
+__device__ __forceinline__ void fill_wave_twiddles(C* tw, const C* table) {
+  constexpr int N = 64 * L;
+  for (int k = threadIdx.x; k < wave_tw<L>(); k += blockDim.x)
+    tw[k] = k < N ? table[(k >> 6) * (k & 63)] : table[((k - N) & 7) * ((k - N) >> 3) * L];
+}
+
 // F sequences of N = 64 L points: v[f][n1] = x_f[64 n1 + lane] on entry; on
-// return buf[f * span + k] holds X_f[k], k < N.
+// return buf[f * span + k] holds X_f[k], k < N.  tw: fill_wave_twiddles.
 template <int L, int F, int S, class C>
 __device__ __forceinline__ void wave_fft64(C (&v)[F][L], C* buf, const C* tw, int lane) {
   constexpr int SP = wave_span<L>();
@@ -587,7 +605,7 @@ __device__ __forceinline__ void wave_fft64(C (&v)[F][L], C* buf, const C* tw, in
     static_for<L>([&](auto k1_) {
       constexpr int k1 = decltype(k1_)::value;
       C x = v[f][k1];
-      if constexpr (k1 > 0) x = cmul(x, twiddle(tw, lane * k1, S));
+      if constexpr (k1 > 0) x = cmul(x, twiddle(tw, 64 * k1 + lane, S));
       buf[f * SP + k1 * kRowSpan + lane] = x;
     });
   });
@@ -605,7 +623,7 @@ __device__ __forceinline__ void wave_fft64(C (&v)[F][L], C* buf, const C* tw, in
       dft<8, S>(w[r]);
       static_for<8>([&](auto c_) {
         constexpr int c = decltype(c_)::value;
-        if constexpr (c > 0) w[r][c] = cmul(w[r][c], twiddle(tw, b * c * L, S));
+        if constexpr (c > 0) w[r][c] = cmul(w[r][c], twiddle(tw, 64 * L + 8 * c + b, S));
       });
     }
   });
@@ -656,7 +674,7 @@ __device__ __forceinline__ void wave_fft64(C (&v)[F][L], C* buf, const C* tw, in
 // spill at that budget and the vector fold's LDS allows one block.
 template <int L, int F, int RB>
 constexpr int wave_blocks_per_cu() {
-  return (64 * L + RB * (F * L * kRowSpan + kSpanPad)) * 8 * 2 <= 160 * 1024 ? 2 : 1;
+  return (wave_tw<L>() + RB * (F * L * kRowSpan + kSpanPad)) * 8 * 2 <= 160 * 1024 ? 2 : 1;
 }
 template <int L, int F, int RB>
 constexpr int fold_waves_per_simd() {
@@ -685,7 +703,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock, (fold_waves_per_simd<L, F, kW
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int WS = F * SP + kSpanPad;  // span stride per wavefront
   float2* tw = lds;
-  float2* buf = lds + N + (size_t)w * WS;
+  float2* buf = lds + wave_tw<L>() + (size_t)w * WS;
   const int ngroups = a.job.nfields * (a.JHP / kWaveBlock);
   const int T1 = a.T + 1;
   const int MT = (T1 + kMT - 1) / kMT;  // m-tiles
@@ -712,7 +730,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock, (fold_waves_per_simd<L, F, kW
     });
   };
   load_rows(g);
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
+  fill_wave_twiddles<L>(tw, a.fft.tw);
   __syncthreads();
   for (; g < ngroups; g += gridDim.x) {
     const int item0 = g * kWaveBlock;
@@ -786,7 +804,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock, (fold_waves_per_simd<L, F, kW
       const int e = (int)(((float)rn + 0.5f) * inv_mt), mt = rn - e * MT;
       const int c = a.job.col0 + ncolf * fld + (e >> 1), p = e & 1;
       *reinterpret_cast<f4*>(a.job.waist0 + waist_index(c, a.job.ldw, a.JHP, mt * kMT, p, i0 + ww)) =
-          *reinterpret_cast<const f4*>(reinterpret_cast<const float*>(lds + N + (size_t)ww * WS) +
+          *reinterpret_cast<const f4*>(reinterpret_cast<const float*>(lds + wave_tw<L>() + (size_t)ww * WS) +
                                        ((size_t)e * MT + mt) * 4);
     }
     __syncthreads();  // the spans are free for the next group
@@ -808,7 +826,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int WS = SP + kSpanPad;  // span stride per wavefront
   float2* tw = lds;
-  float2* buf = lds + N + (size_t)w * WS;
+  float2* buf = lds + wave_tw<L>() + (size_t)w * WS;
   const int JH4 = (a.JH + kWaveBlock - 1) / kWaveBlock;  // row groups
   const int ngroups = a.job.nfields * JH4;
   const int T1 = a.T + 1;
@@ -836,7 +854,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
     const unsigned e = ok ? (unsigned)(rn / MT) : 0u, mt = ok ? (unsigned)(rn % MT) : 0u;
     rvalid |= (ok ? 1u : 0u) << r;
     roff[r] = (e >> 1) * (unsigned)a.job.ldw + 4u * ((2u * mt + (e & 1u)) * (unsigned)a.JHP + ww);
-    loff[r] = (unsigned)(2 * N + 2 * ww * WS) + 4u * (e * (unsigned)MT + mt);
+    loff[r] = (unsigned)(2 * wave_tw<L>() + 2 * ww * WS) + 4u * (e * (unsigned)MT + mt);
   }
   auto load_runs = [&](int grp) {
     const int fld = grp / JH4, i0 = (grp % JH4) * kWaveBlock;
@@ -846,7 +864,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
       if (rvalid & (1u << r)) run[r] = *reinterpret_cast<const f4*>(base + roff[r]);
   };
   load_runs(g);
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
+  fill_wave_twiddles<L>(tw, a.fft.tw);
   for (; g < ngroups; g += gridDim.x) {
     const int fld = g / JH4, i0 = (g % JH4) * kWaveBlock, i = i0 + w;
     const bool live = i < a.JH;
@@ -931,7 +949,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave64_kernel(FoldA
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int WS = SP + kSpanPad;
   double2* tw = lds_d;
-  double2* buf = lds_d + N + (size_t)w * WS;
+  double2* buf = lds_d + wave_tw<L>() + (size_t)w * WS;
   const int ngroups = a.job.nfields * (a.JHP / kWaveBlock);
   const int T1 = a.T + 1;
   const int MT = (T1 + kMT - 1) / kMT;
@@ -953,7 +971,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave64_kernel(FoldA
     });
   };
   load_rows(g);
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw64[k];
+  fill_wave_twiddles<L>(tw, a.fft.tw64);
   __syncthreads();
   for (; g < ngroups; g += gridDim.x) {
     const int item0 = g * kWaveBlock;
@@ -1015,7 +1033,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave64_kernel(FoldA
       const int e = (int)(((float)rn + 0.5f) * inv_mt), mt = rn - e * MT;
       const int c = a.job.col0 + a.job.fstride * fld + (e >> 1), p = e & 1;
       *reinterpret_cast<f4*>(a.job.waist0 + waist_index(c, a.job.ldw, a.JHP, mt * kMT, p, i0 + ww)) =
-          *reinterpret_cast<const f4*>(reinterpret_cast<const float*>(lds_d + N + (size_t)ww * WS) +
+          *reinterpret_cast<const f4*>(reinterpret_cast<const float*>(lds_d + wave_tw<L>() + (size_t)ww * WS) +
                                        ((size_t)e * MT + mt) * 4);
     }
     __syncthreads();  // the spans are free for the next group
@@ -1031,7 +1049,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave64_kernel(Un
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int WS = SP + kSpanPad;  // span stride per wavefront, double2
   double2* tw = lds_d;
-  double2* buf = lds_d + N + (size_t)w * WS;
+  double2* buf = lds_d + wave_tw<L>() + (size_t)w * WS;
   const int JH4 = (a.JH + kWaveBlock - 1) / kWaveBlock;
   const int ngroups = a.job.nfields * JH4;
   const int T1 = a.T + 1;
@@ -1054,7 +1072,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave64_kernel(Un
     const unsigned e = ok ? (unsigned)(rn / MT) : 0u, mt = ok ? (unsigned)(rn % MT) : 0u;
     rvalid |= (ok ? 1u : 0u) << r;
     roff[r] = (e >> 1) * (unsigned)a.job.ldw + 4u * ((2u * mt + (e & 1u)) * (unsigned)a.JHP + ww);
-    loff[r] = (unsigned)(4 * N + 4 * ww * WS) + 4u * (e * (unsigned)MT + mt);
+    loff[r] = (unsigned)(4 * wave_tw<L>() + 4 * ww * WS) + 4u * (e * (unsigned)MT + mt);
   }
   auto load_runs = [&](int grp) {
     const int fld = grp / JH4, i0 = (grp % JH4) * kWaveBlock;
@@ -1064,7 +1082,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave64_kernel(Un
       if (rvalid & (1u << r)) run[r] = *reinterpret_cast<const f4*>(base + roff[r]);
   };
   load_runs(g);
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw64[k];
+  fill_wave_twiddles<L>(tw, a.fft.tw64);
   for (; g < ngroups; g += gridDim.x) {
     const int fld = g / JH4, i0 = (g % JH4) * kWaveBlock, i = i0 + w;
     const bool live = i < a.JH;
@@ -1187,15 +1205,15 @@ hipError_t launch_persistent(void (*kernel)(Args), int groups, int rb, size_t ld
 
 size_t wave64_lds_bytes(int N, int rb) {
   const int L = N / 64;
-  return ((size_t)N + (size_t)rb * (L * kRowSpan + kSpanPad)) * sizeof(double2);
+  return ((size_t)N + 64 + (size_t)rb * (L * kRowSpan + kSpanPad)) * sizeof(double2);
 }
 size_t fold_wave_lds_bytes(int N, int vector, int rb) {
   const int L = N / 64;
-  return ((size_t)N + (size_t)rb * ((vector ? 2 : 1) * L * kRowSpan + kSpanPad)) * sizeof(float2);
+  return ((size_t)N + 64 + (size_t)rb * ((vector ? 2 : 1) * L * kRowSpan + kSpanPad)) * sizeof(float2);
 }
 size_t unfold_wave_lds_bytes(int N, int rb) {
   const int L = N / 64;
-  return ((size_t)N + (size_t)rb * (L * kRowSpan + kSpanPad)) * sizeof(float2);
+  return ((size_t)N + 64 + (size_t)rb * (L * kRowSpan + kSpanPad)) * sizeof(float2);
 }
 constexpr size_t kWaveLdsMax = 160 * 1024;  // LDS of one gfx950 workgroup
 // rows per block: 8 when their LDS fits, else 4
@@ -1229,8 +1247,8 @@ __global__ void __launch_bounds__(64 * kRowWaves) waist_from_rows_kernel(WaistRo
   constexpr int N = 64 * L, SP = wave_span<L>();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float2* tw = lds;
-  float2* buf = lds + N + (size_t)w * SP;
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
+  float2* buf = lds + wave_tw<L>() + (size_t)w * SP;
+  fill_wave_twiddles<L>(tw, a.fft.tw);
   __syncthreads();
   const int r = blockIdx.x * kRowWaves + w;
   if (r >= a.nlead * a.nrows) return;  // no block barrier below
@@ -1257,8 +1275,8 @@ __global__ void __launch_bounds__(64 * kRowWaves) rows_from_waist_kernel(WaistRo
   constexpr int N = 64 * L, SP = wave_span<L>();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float2* tw = lds;
-  float2* buf = lds + N + (size_t)w * SP;
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = a.fft.tw[k];
+  float2* buf = lds + wave_tw<L>() + (size_t)w * SP;
+  fill_wave_twiddles<L>(tw, a.fft.tw);
   __syncthreads();
   const int r = blockIdx.x * kRowWaves + w;
   if (r >= a.nlead * a.nrows) return;  // no block barrier below
@@ -1401,7 +1419,7 @@ __global__ void waist_unfold_kernel(WaistUnfoldArgs a) {
 
 size_t rows_wave_lds_bytes(int N) {
   const int L = N / 64;
-  return ((size_t)N + (size_t)kRowWaves * L * kRowSpan) * sizeof(float2);
+  return ((size_t)N + 64 + (size_t)kRowWaves * L * kRowSpan) * sizeof(float2);
 }
 
 // Register prefetch depth per lane: PF * 64 >= N, or 0 (no prefetch) above 1536.
