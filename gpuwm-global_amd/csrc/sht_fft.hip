@@ -668,10 +668,10 @@ __device__ __forceinline__ void wave_fft64(C (&v)[F][L], C* buf, const C* tw, in
 }
 
 // Blocks per CU that a wave-resident launch's LDS allows (2 when a block
-// takes at most half of the 160 KB).  The scalar fold is compiled for that
-// many (its second launch bound, waves per SIMD, is blocks x RB / 4), so that
-// registers do not cap its residency below what LDS allows; the unfold would
-// spill at that budget and the vector fold's LDS allows one block.
+// takes at most half of the 160 KB).  The scalar fold and the unfold are
+// compiled for that many (their second launch bound, waves per SIMD, is
+// blocks x RB / 4), so that registers do not cap their residency below what
+// LDS allows; the vector fold's LDS allows one block.
 template <int L, int F, int RB>
 constexpr int wave_blocks_per_cu() {
   return (wave_tw<L>() + RB * (F * L * kRowSpan + kSpanPad)) * 8 * 2 <= 160 * 1024 ? 2 : 1;
@@ -818,7 +818,9 @@ __global__ void __launch_bounds__(64 * kWaveBlock, (fold_waves_per_simd<L, F, kW
 // the next group's waist runs are loaded into registers while the current
 // group is transformed.
 template <int L, int kWaveBlock>
-__global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(UnfoldArgs a) {
+__global__ void __launch_bounds__(64 * kWaveBlock,
+                                  (wave_blocks_per_cu<L, 1, kWaveBlock>() * kWaveBlock / 4))
+    irfft_unfold_wave_kernel(UnfoldArgs a) {
   extern __shared__ float2 lds[];
   constexpr int N = 64 * L, SP = wave_span<L>();
   static_assert(kMT == 4, "a row's four orders are one 16-byte vector");
@@ -866,6 +868,12 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave_kernel(Unfo
   load_runs(g);
   fill_wave_twiddles<L>(tw, a.fft.tw);
   for (; g < ngroups; g += gridDim.x) {
+    // the lane index is made opaque in each group, so that the FFT's
+    // lane-derived LDS addresses are recomputed there rather than held in
+    // registers across the loop: the kernel then fits the 128 VGPRs of two
+    // blocks per CU without spilling
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
     const int fld = g / JH4, i0 = (g % JH4) * kWaveBlock, i = i0 + w;
     const bool live = i < a.JH;
 #pragma unroll
