@@ -35,16 +35,27 @@ namespace sht {
 // register r holding D[l / 16 + 4 r][l % 16] -- rows interleaved, unlike the
 // fp32 16x16x4 MFMA's 4 (l / 16) + r; checked on gfx950).  A block owns one
 // order m, one output parity, 8 levels (32 data columns: two 16-column MFMA
-// tiles) and up to 8 row tiles of 16, one per wavefront.  The B operand of
-// one segment (table kind) at a time is staged in LDS in FP64; each A value
-// a wavefront loads from its table feeds both column tiles.
+// tiles) and up to 4 row tiles of 16, two wavefronts per row tile: each takes
+// one half of every segment's k range, and the halves' sums meet in LDS at
+// the end (the grids are a few hundred blocks, so the k loop's length, not
+// the MFMA rate, sets the time).  The B operand of one segment (table kind)
+// at a time is staged in LDS in FP64; each A value a wavefront loads from its
+// table feeds both column tiles.
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kLowCols = 32;    // data columns per block: 8 levels x 4 components
 constexpr int kLowLevels = 8;
-constexpr int kLowWaves = 8;    // wavefronts (row tiles) per block
+constexpr int kLowWaves = 8;    // wavefronts per block
+constexpr int kLowTiles = kLowWaves / 2;  // row tiles per block (two wavefronts each)
 constexpr int kLowBatch = 8;    // k-steps of A loaded ahead (32 k)
 constexpr int kLowKPad = 4 * kLowBatch;  // k extents padded to whole batches
+// LDS for the two halves' sums: per row tile, 64 lanes x two d4 accumulators
+constexpr size_t kLowReduceBytes = (size_t)kLowTiles * 2 * 64 * 4 * sizeof(double);
+
+// Where the second half of a k range of kpad (a multiple of kLowKPad) starts.
+__device__ __forceinline__ int low_split(int kpad) {
+  return (kpad / kLowKPad + 1) / 2 * kLowKPad;
+}
 
 __device__ __forceinline__ size_t low_index(int JHP, int m0, int lev, int f, int m, int p, int i) {
   return ((((size_t)lev * 2 + f) * m0 + m) * 2 + p) * JHP + i;
@@ -90,6 +101,25 @@ __device__ __forceinline__ void low_segment(d4& acc0, d4& acc1, const double* b,
   }
 }
 
+// The second wavefront of each row tile (kh = 1) hands its sums to the first
+// through LDS (after every wavefront is done with the staged B); returns
+// whether this wavefront holds the tile's totals and stores them.
+__device__ __forceinline__ bool low_reduce(d4& acc0, d4& acc1, double* lds, int w, int lane,
+                                           bool active) {
+  const int kh = w & 1;
+  d4* red = reinterpret_cast<d4*>(lds) + (size_t)(w >> 1) * 2 * 64;
+  __syncthreads();
+  if (kh) {
+    red[lane] = acc0;
+    red[64 + lane] = acc1;
+  }
+  __syncthreads();
+  if (kh || !active) return false;
+  acc0 += red[lane];
+  acc1 += red[64 + lane];
+  return true;
+}
+
 // Synthesis, output parity p, rows i (folded latitude), columns (level,
 // component c): c = 0, 1 the real and imaginary part of U, c = 2, 3 of V:
 //   out[i][col] = sum_{r of parity p} Pm[r][i] X[r][col] + sum_{r of parity 1-p} dP[r][i] Y[r][col]
@@ -98,10 +128,11 @@ __device__ __forceinline__ void low_segment(d4& acc0, d4& acc1, const double* b,
 __global__ void __launch_bounds__(64 * kLowWaves) low_synthesis_kernel(LowSynArgs a) {
   extern __shared__ double lds_low[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int groups = (a.JH + 16 * kLowWaves - 1) / (16 * kLowWaves);
+  const int groups = (a.JH + 16 * kLowTiles - 1) / (16 * kLowTiles);
   const int m = blockIdx.x / groups;
-  const int it = (blockIdx.x % groups) * kLowWaves + w;  // this wavefront's row tile
-  const bool active = it * 16 < a.JH;                     // wave-uniform
+  const int it = (blockIdx.x % groups) * kLowTiles + (w >> 1);  // this wavefront's row tile
+  const int kh = w & 1;                                          // and its half of k
+  const bool active = it * 16 < a.JH;                            // wave-uniform
   const int lev0 = blockIdx.y * kLowLevels;
   const int p = blockIdx.z;
   const int T1 = a.T + 1;
@@ -154,16 +185,17 @@ __global__ void __launch_bounds__(64 * kLowWaves) low_synthesis_kernel(LowSynArg
     }
     __syncthreads();
     if (active && kn > 0) {
+      const int k0 = kh ? low_split(kpad) : 0, k1 = kh ? kpad : low_split(kpad);
       const double* tab = (seg == 0 ? a.pm : a.dp) + (size_t)m * T1 * a.JHP + ic;
       // rows past the segment meet zero B rows, so any finite table value
       // will do there: clamped addresses, no select (a select right after a
       // load makes the compiler wait for it before issuing the next)
-      low_segment(acc0, acc1, lds_low, kpad, lane, [&](int j) {
-        return tab[(size_t)min(2 * j + pr, a.T) * a.JHP];
+      low_segment(acc0, acc1, lds_low + (size_t)k0 * kLowCols, k1 - k0, lane, [&](int j) {
+        return tab[(size_t)min(2 * (j + k0) + pr, a.T) * a.JHP];
       });
     }
   }
-  if (!active) return;
+  if (!low_reduce(acc0, acc1, lds_low, w, lane, active)) return;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int col = 16 * h + (lane & 15);
@@ -191,9 +223,10 @@ __global__ void __launch_bounds__(64 * kLowWaves) low_analysis_kernel(LowAnaArgs
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int T1 = a.T + 1;
   const int rows_max = (T1 + 1) / 2;                      // rows of parity 0 at m = 0
-  const int groups = (rows_max + 16 * kLowWaves - 1) / (16 * kLowWaves);
+  const int groups = (rows_max + 16 * kLowTiles - 1) / (16 * kLowTiles);
   const int m = blockIdx.x / groups;
-  const int it = (blockIdx.x % groups) * kLowWaves + w;
+  const int it = (blockIdx.x % groups) * kLowTiles + (w >> 1);
+  const int kh = w & 1;
   const int lev0 = blockIdx.y * kLowLevels;
   const int q = blockIdx.z;
   const int K = T1 - m;
@@ -243,13 +276,15 @@ __global__ void __launch_bounds__(64 * kLowWaves) low_analysis_kernel(LowAnaArgs
     }
     __syncthreads();
     if (active) {
+      const int k0 = kh ? low_split(IP) : 0, k1 = kh ? IP : low_split(IP);
       const double* tab = (seg == 0 ? a.pm : a.dp) + (size_t)m * a.JHP * T1 + r;
-      low_segment(acc0, acc1, lds_low, IP, lane, [&](int i) {  // past JH: zero B rows
-        return tab[(size_t)min(i, a.JH - 1) * T1];
-      });
+      low_segment(acc0, acc1, lds_low + (size_t)k0 * kLowCols, k1 - k0, lane,
+                  [&](int i) {  // past JH: zero B rows
+                    return tab[(size_t)min(i + k0, a.JH - 1) * T1];
+                  });
     }
   }
-  if (!active) return;
+  if (!low_reduce(acc0, acc1, lds_low, w, lane, active)) return;
   const size_t plane = (size_t)T1 * T1;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -298,9 +333,10 @@ __global__ void low_pack_kernel(LowSynArgs a) {
 extern "C" hipError_t sht_launch_low_synthesis(const sht::LowSynArgs* a, hipStream_t s) {
   if (a->nlev <= 0 || a->m0 <= 0) return hipSuccess;
   const int m0 = std::min(a->m0, a->T + 1);
-  const int groups = (a->JH + 16 * sht::kLowWaves - 1) / (16 * sht::kLowWaves);
+  const int groups = (a->JH + 16 * sht::kLowTiles - 1) / (16 * sht::kLowTiles);
   const int KP = ((a->T + 2) / 2 + sht::kLowKPad - 1) / sht::kLowKPad * sht::kLowKPad;
-  const size_t lds = (size_t)KP * sht::kLowCols * sizeof(double);  // one segment's B
+  const size_t lds = std::max((size_t)KP * sht::kLowCols * sizeof(double),  // one segment's B
+                              sht::kLowReduceBytes);
   const dim3 g(m0 * groups, (a->nlev + sht::kLowLevels - 1) / sht::kLowLevels, 2);
   const long long total = (long long)a->nlev * a->m0 * (a->T + 1);
   hipLaunchKernelGGL(sht::low_pack_kernel, dim3((unsigned)std::min<long long>((total + 255) / 256, 4096)),
@@ -313,9 +349,9 @@ extern "C" hipError_t sht_launch_low_analysis(const sht::LowAnaArgs* a, hipStrea
   if (a->nlev <= 0 || a->m0 <= 0) return hipSuccess;
   const int m0 = std::min(a->m0, a->T + 1);
   const int rows_max = (a->T + 2) / 2;
-  const int groups = (rows_max + 16 * sht::kLowWaves - 1) / (16 * sht::kLowWaves);
+  const int groups = (rows_max + 16 * sht::kLowTiles - 1) / (16 * sht::kLowTiles);
   const int IP = (a->JH + sht::kLowKPad - 1) / sht::kLowKPad * sht::kLowKPad;
-  const size_t lds = (size_t)IP * sht::kLowCols * sizeof(double);
+  const size_t lds = std::max((size_t)IP * sht::kLowCols * sizeof(double), sht::kLowReduceBytes);
   const dim3 g(m0 * groups, (a->nlev + sht::kLowLevels - 1) / sht::kLowLevels, 2);
   hipLaunchKernelGGL(sht::low_analysis_kernel, g, dim3(64 * sht::kLowWaves), lds, s, *a);
   return hipGetLastError();
