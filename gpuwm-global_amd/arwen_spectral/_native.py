@@ -167,7 +167,7 @@ def load():
             "arwen_sht_batch_get_config": (I, [P, ctypes.POINTER(GemmConfig)]),
             "arwen_sht_batch_set_config": (I, [P, ctypes.POINTER(GemmConfig)]),
             "arwen_sht_batch_autotune": (I, [P, P, I, ctypes.POINTER(GemmConfig)]),
-            "arwen_sht_batch_work": (I, [P, D, D]),
+            "arwen_sht_batch_work": (I, [P, D, D, D]),
             "arwen_sht_batch_profile": (I, [P, P, I, D]),
             "arwen_sht_last_error": (ctypes.c_char_p, []),
         }

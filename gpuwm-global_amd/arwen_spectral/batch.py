@@ -159,7 +159,8 @@ class TransformBatch:
                                                          int(reps), ctypes.byref(c)))
         return c.as_dict()
 
-    PHASES = ("pack", "synthesis_gemm", "inverse_fft", "forward_fft", "analysis_gemm", "unpack")
+    PHASES = ("pack", "synthesis_gemm", "synthesis_low", "inverse_fft", "forward_fft",
+              "analysis_gemm", "analysis_low", "unpack")
 
     def profile(self, reps: int = 5) -> dict:
         """Mean milliseconds per phase over ``reps`` runs, HIP events on the
@@ -171,9 +172,12 @@ class TransformBatch:
                                                         int(reps), _native.dptr(ms)))
         return dict(zip(self.PHASES, ms.tolist()))
 
-    def work(self) -> tuple[float, float]:
-        """(Legendre FLOPs, FFT bytes) per run, algorithmic."""
+    def work(self) -> tuple[float, float, float]:
+        """(Legendre FLOPs, FFT bytes, the FP64 low-order kernels' share of
+        the Legendre FLOPs) per run, algorithmic."""
         f = ctypes.c_double()
         b = ctypes.c_double()
-        _native.check(self._lib.arwen_sht_batch_work(self._handle, ctypes.byref(f), ctypes.byref(b)))
-        return f.value, b.value
+        low = ctypes.c_double()
+        _native.check(self._lib.arwen_sht_batch_work(self._handle, ctypes.byref(f), ctypes.byref(b),
+                                                     ctypes.byref(low)))
+        return f.value, b.value, low.value

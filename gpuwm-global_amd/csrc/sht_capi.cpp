@@ -423,6 +423,7 @@ struct arwen_sht_batch {
   GemmLists lists[2];
   arwen_sht_gemm_config config{{32, 32}, {0, 0}, 1};
   double legendre_flops = 0, fft_bytes = 0;
+  double low_flops = 0;  // the part of legendre_flops the FP64 low-order kernels do
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~arwen_sht_batch() {
@@ -678,6 +679,9 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
                    J.op == ARWEN_SHT_OP_VORDIV_FROM_WAIST;
   const bool sgrad = J.op == ARWEN_SHT_OP_SYNTHESIS_GRADIENT;
   const double pcf = 4.0 * (P->nlat / 2.0) * (T + 1.0) * (T + 2.0) / 2.0;  // FLOP per PC
+  // the orders m < m0 of that (vector jobs in the split modes: FP64 kernels)
+  const double pcf_low =
+      4.0 * (P->nlat / 2.0) * (P->m0 * (T + 1.0) - P->m0 * (P->m0 - 1.0) / 2.0);
   const double fftb = 4.0 * P->nlat * P->nlon + 8.0 * P->nlat * (T + 1.0);
   if (n == 0) return;
   if (is_synthesis(J.op)) {
@@ -760,6 +764,7 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
                    : (grad || sgrad)                     ? 2.0 * n
                                                          : 1.0 * n;
       if (prepare) B->legendre_flops += pcs * pcf;
+      if (prepare && low) B->low_flops += pcs * pcf_low;
     }
     // ---- inverse FFTs ----
     auto unfold = [&](const DevBuf& waist, void* grid, int col0, int fstride, const DevBuf& sc,
@@ -918,6 +923,7 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
         }
       }
     if (prepare) B->legendre_flops += (vec ? 4.0 : divw ? 2.0 : 1.0) * n * pcf;
+    if (prepare && low) B->low_flops += 4.0 * n * pcf_low;
     sht::UnpackArgs ua{};
     ua.koff = P->d_koff.as<int>();
     ua.T = T;
@@ -1035,7 +1041,9 @@ hipError_t launch_gemm(const arwen_sht_batch* B, int d, hipStream_t s) {
 
 // Phases of a batch run, in order; `ev` (optional) gets one event before
 // each phase and one after the last.
-constexpr int kPhases = 6;  // pack, synthesis GEMM, inverse FFT, FFT, analysis GEMM, unpack
+// pack, synthesis GEMM, its FP64 low orders, inverse FFT, forward FFT, analysis
+// GEMM, its FP64 low orders, unpack
+constexpr int kPhases = 8;
 
 void run_batch(arwen_sht_batch* B, hipStream_t s, hipEvent_t* ev = nullptr) {
   int dev;
@@ -1048,19 +1056,21 @@ void run_batch(arwen_sht_batch* B, hipStream_t s, hipEvent_t* ev = nullptr) {
   for (auto& a : B->packs) hip_check(sht_launch_pack(&a, s), "pack launch");
   mark(1);
   hip_check(launch_gemm(B, 0, s), "synthesis GEMM launch");
-  for (auto& a : B->low_syns) hip_check(sht_launch_low_synthesis(&a, s), "low-order synthesis launch");
   mark(2);
+  for (auto& a : B->low_syns) hip_check(sht_launch_low_synthesis(&a, s), "low-order synthesis launch");
+  mark(3);
   for (auto& a : B->unfolds) hip_check(sht_launch_irfft_unfold(&a, s), "irfft launch");
   for (auto& a : B->waist_unfolds) hip_check(sht_launch_waist_unfold(&a, s), "waist unfold launch");
-  mark(3);
+  mark(4);
   for (auto& a : B->folds) hip_check(sht_launch_rfft_fold(&a, s), "rfft launch");
   for (auto& a : B->waist_folds) hip_check(sht_launch_waist_fold(&a, s), "waist fold launch");
-  mark(4);
-  hip_check(launch_gemm(B, 1, s), "analysis GEMM launch");
-  for (auto& a : B->low_anas) hip_check(sht_launch_low_analysis(&a, s), "low-order analysis launch");
   mark(5);
-  for (auto& a : B->unpacks) hip_check(sht_launch_unpack(&a, s), "unpack launch");
+  hip_check(launch_gemm(B, 1, s), "analysis GEMM launch");
   mark(6);
+  for (auto& a : B->low_anas) hip_check(sht_launch_low_analysis(&a, s), "low-order analysis launch");
+  mark(7);
+  for (auto& a : B->unpacks) hip_check(sht_launch_unpack(&a, s), "unpack launch");
+  mark(8);
 }
 
 void drop_graph(arwen_sht_batch* B) {
@@ -1828,11 +1838,13 @@ int arwen_sht_batch_autotune(arwen_sht_batch* B, void* stream, int reps,
   });
 }
 
-int arwen_sht_batch_work(const arwen_sht_batch* B, double* flops, double* bytes) {
+int arwen_sht_batch_work(const arwen_sht_batch* B, double* flops, double* bytes,
+                         double* low_flops) {
   return guarded([&] {
     if (!B) fail(ARWEN_SHT_EINVAL, "batch is NULL");
     if (flops) *flops = B->legendre_flops;
     if (bytes) *bytes = B->fft_bytes;
+    if (low_flops) *low_flops = B->low_flops;
   });
 }
 

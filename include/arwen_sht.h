@@ -335,8 +335,10 @@ int arwen_sht_batch_capture(arwen_sht_batch* batch, void* stream);
 /* Launch the captured graph (captures first if needed). */
 int arwen_sht_batch_launch(arwen_sht_batch* batch, void* stream);
 /* Run the batch `reps` times with HIP events recorded on `stream` around each
- * phase and return the mean milliseconds per phase in ms[6]: pack, synthesis
- * GEMM, inverse FFTs, forward FFTs, analysis GEMM, unpack.  Synchronizes on
+ * phase and return the mean milliseconds per phase in ms[8]: pack, synthesis
+ * GEMM, the vector jobs' FP64 low-order synthesis (its pack included),
+ * inverse FFTs, forward FFTs, analysis GEMM, FP64 low-order analysis,
+ * unpack.  Synchronizes on
  * the last event.  Measurement hook; no reference counterpart (the
  * reference's StepProfiler times whole sections, profile.py:87-330). */
 int arwen_sht_batch_profile(arwen_sht_batch* batch, void* stream, int reps, double* ms);
@@ -359,9 +361,13 @@ int arwen_sht_batch_set_config(arwen_sht_batch* batch, const arwen_sht_gemm_conf
  * (optional).  Synchronizes; overwrites the batch's intermediate buffers. */
 int arwen_sht_batch_autotune(arwen_sht_batch* batch, void* stream, int reps,
                              arwen_sht_gemm_config* out);
-/* Algorithmic work of the batch: Legendre FLOPs (parity folded, triangular)
- * and FFT bytes (grid + waist once each) per run. */
-int arwen_sht_batch_work(const arwen_sht_batch* batch, double* legendre_flops, double* fft_bytes);
+/* Algorithmic work of the batch: Legendre FLOPs (parity folded, triangular,
+ * all orders), FFT bytes (grid + waist once each) and the part of the
+ * Legendre FLOPs that the FP64 low-order kernels do (the vector jobs' orders
+ * m < fp64_orders; the grouped GEMMs do the rest) per run.  Any output
+ * pointer may be NULL. */
+int arwen_sht_batch_work(const arwen_sht_batch* batch, double* legendre_flops, double* fft_bytes,
+                         double* fp64_low_flops);
 
 const char* arwen_sht_last_error(void);
 
