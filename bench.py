@@ -564,7 +564,7 @@ def vertical_bench(args, torch):
         "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: seeded red-spectrum fields, synthetic well-posed vertical operator",
         "config": {"workload": f"solve_shifted T{T}, {L} levels", "T": T, "levels": L},
-        "roofline": {"bound": "hbm", "kernel": "vertical_solve_kernel", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "vertical_rows_kernel", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes": alg},
         "cpu_baseline": cpu}))

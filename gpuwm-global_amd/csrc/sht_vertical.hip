@@ -15,11 +15,11 @@
 // is s_bx = s_c = tau w, s_md = s_cd = 0, s_cn = 1; apply is s_bx = s_c = dt,
 // s_md = -alpha (1 - alpha) dt^2, s_cd = 1 - alpha, s_cn = alpha.
 //
-// One block per (degree n, 32 orders): the matrices and the block's columns
-// sit in LDS, each thread owns one column and a strided eighth of the
-// levels, and every stage is a short fp32 dot product per output (the work
-// is ~3 L^2 multiply-adds per column against 16 L bytes of HBM traffic, so
-// the kernel is bound by HBM, not by arithmetic).
+// Per column the work is ~3 L^2 multiply-adds against 16 L bytes of HBM
+// traffic.  Two kernels: vertical_rows_kernel (solve, apply, tendencies)
+// keeps each lane's stage column in registers, and vertical_kernel (the half
+// map, whose four per-degree matrices need the LDS that the former gives its
+// columns) reads both operands of every multiply-add from LDS.
 #include "sht_kernels.h"
 
 namespace sht {
@@ -207,6 +207,171 @@ __global__ void __launch_bounds__(256) vertical_kernel(VerticalArgs a) {
   }
 }
 
+// Solve, apply and tendencies (modes 0 and 1): one block of four wavefronts
+// per (degree n, 64 orders m), lane = order.  The block's matrices sit in LDS
+// (row stride LP, zero padded); wavefront w computes the rows i = w, w + 4,
+// ... of each stage from the lane's column of the stage's right-hand side,
+// held in registers (v[j], zero from its length on), and reads each group of
+// four matrix values with one 16-byte load at a wave-uniform address.  LDS
+// then carries 4 bytes per lane and multiply-add pair, not 12, and passes
+// each stage's result column (64 orders wide) on to the next stage.
+template <int LP>
+__device__ __forceinline__ float2 lds_row_dot(const float* row, const float2 (&v)[LP]) {
+  typedef __attribute__((ext_vector_type(4))) float f4;
+  float ax = 0.f, ay = 0.f;
+#pragma unroll
+  for (int q = 0; q < LP / 4; ++q) {
+    const f4 c = *reinterpret_cast<const f4*>(row + 4 * q);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      ax = fmaf(c[k], v[4 * q + k].x, ax);
+      ay = fmaf(c[k], v[4 * q + k].y, ay);
+    }
+  }
+  return make_float2(ax, ay);
+}
+
+template <int LP>
+__global__ void __launch_bounds__(256) vertical_rows_kernel(VerticalArgs a) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  extern __shared__ float smem[];
+  const int L = a.nlev, L1 = a.nlev + 1, T1 = a.T + 1;
+  const int n = blockIdx.x;
+  const int m0 = blockIdx.y * 64;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int m = m0 + lane;
+  const bool live = m < T1;
+  const bool in_tri = live && m <= n;
+  const unsigned plane8 = (unsigned)T1 * (unsigned)T1 * 8u;  // bytes per level plane
+  // Columns through raw buffers: a level's value at (n, m) is plane row i
+  // (scalar offset i x plane8) plus the lane's byte offset; lanes off the
+  // triangle (loads) or past T (stores) take an offset beyond every buffer,
+  // so their loads return zeros and their stores are dropped, with no
+  // per-lane address registers beyond one offset.
+  constexpr unsigned kOff = 0x80000000u;
+  const unsigned at8 = ((unsigned)n * (unsigned)T1 + (unsigned)m) * 8u;
+  const unsigned ld_off = in_tri ? at8 : kOff, st_off = live ? at8 : kOff;
+  auto rsrc = [](const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+  };
+  const auto rD = rsrc(a.div, (unsigned)L * plane8), rT = rsrc(a.theta, (unsigned)L * plane8);
+  const auto rP = rsrc(a.lnps, plane8);
+  const auto rDo = rsrc(a.d_out, (unsigned)L * plane8), rTo = rsrc(a.theta_out, (unsigned)L * plane8);
+  const auto rPo = rsrc(a.lnps_out, plane8);
+  auto load = [&](__amdgpu_buffer_rsrc_t r, int i) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(
+                                          r, (int)ld_off, (int)((unsigned)i * plane8), 0));
+  };
+  auto store = [&](__amdgpu_buffer_rsrc_t r, int i, float2 v) {  // projected (m = 0 real)
+    if (m == 0) v.y = 0.f;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)st_off,
+                                          (int)((unsigned)i * plane8), 0);
+  };
+  auto D = [&](int i) { return load(rD, i); };
+  auto X = [&](int i) { return i < L ? load(rT, i) : load(rP, 0); };
+  auto store_x = [&](int i, float2 v) {
+    if (i < L) store(rTo, i, v);
+    else store(rPo, 0, v);
+  };
+  const float2 zero = make_float2(0.f, 0.f);
+  if (m0 > n) {  // every order of the block is above the triangle: zeros
+    for (int i = w; i < L1; i += 4) {
+      if (i < L) store(rDo, i, zero);
+      store_x(i, zero);
+    }
+    return;
+  }
+  // LDS: B (L rows), C (L1 rows), [mode 0: M (L rows, with M), H_n (L rows)],
+  // row stride LP; then one result column [L][64]
+  float* sB = smem;
+  float* sC = sB + (size_t)L * LP;
+  float* sM = sC + (size_t)L1 * LP;
+  float* sH = a.m_matrix ? sM + (size_t)L * LP : sM;
+  float2* scol = reinterpret_cast<float2*>(a.mode == 0 ? sH + (size_t)L * LP : sM);
+  auto stage = [&](float* dst, const float* src, int rows, int len) {
+    for (int r = w; r < rows; r += 4)
+      for (int j = lane; j < LP; j += 64) dst[(size_t)r * LP + j] = j < len ? src[(size_t)r * len + j] : 0.f;
+  };
+  stage(sB, a.b_matrix, L, L1);
+  stage(sC, a.c_matrix, L1, L);
+  if (a.mode == 0) {
+    if (a.m_matrix) stage(sM, a.m_matrix, L, L);
+    stage(sH, a.inverse + (size_t)n * L * L, L, L);
+  }
+  auto S = [&](int i) -> float2& { return scol[(size_t)i * 64 + lane]; };
+  float2 v[LP];
+  auto load_v = [&](auto&& src, int len) {
+#pragma unroll
+    for (int j = 0; j < LP; ++j) v[j] = j < len ? src(j) : zero;
+  };
+  const float k2 = a.k2[n];
+  const bool pin = n == 0 && m == 0 && a.mode == 0;
+  __syncthreads();  // the matrices are in LDS
+
+  if (a.mode == 1) {
+    // d = s_bx k2 B x; x' = s_c C D
+    load_v(X, L1);
+    for (int i = w; i < L; i += 4) {
+      const float2 bx = lds_row_dot<LP>(sB + (size_t)i * LP, v);
+      store(rDo, i, make_float2(a.s_bx * k2 * bx.x, a.s_bx * k2 * bx.y));
+    }
+    load_v(D, L);
+    for (int i = w; i < L1; i += 4) {
+      const float2 cd = lds_row_dot<LP>(sC + (size_t)i * LP, v);
+      store_x(i, make_float2(a.s_c * cd.x, a.s_c * cd.y));
+    }
+    return;
+  }
+  // A row's own D or x value is loaded one row ahead (the next row of the
+  // wavefront, i + 4), so that its latency hides behind the current row.
+  // forcing = D + k2 (s_bx B x + s_md M D)
+  load_v(X, L1);
+  float2 dn = w < L ? D(w) : zero;
+  for (int i = w; i < L; i += 4) {
+    const float2 d = dn;
+    if (i + 4 < L) dn = D(i + 4);
+    const float2 bx = lds_row_dot<LP>(sB + (size_t)i * LP, v);
+    if (a.m_matrix) S(i) = bx;
+    else S(i) = make_float2(d.x + k2 * a.s_bx * bx.x, d.y + k2 * a.s_bx * bx.y);
+  }
+  if (a.m_matrix) {  // the rows this wavefront wrote above
+    load_v(D, L);
+    dn = w < L ? D(w) : zero;
+    for (int i = w; i < L; i += 4) {
+      const float2 d = dn;
+      if (i + 4 < L) dn = D(i + 4);
+      const float2 md = lds_row_dot<LP>(sM + (size_t)i * LP, v);
+      const float2 bx = S(i);
+      S(i) = make_float2(d.x + k2 * (a.s_bx * bx.x + a.s_md * md.x),
+                         d.y + k2 * (a.s_bx * bx.y + a.s_md * md.y));
+    }
+  }
+  __syncthreads();  // the whole forcing column is in LDS
+  // d = H_n forcing; the column then becomes u = s_cd D + s_cn d
+  load_v(S, L);
+  __syncthreads();  // every wavefront holds the forcing before it is overwritten
+  dn = w < L ? D(w) : zero;
+  for (int i = w; i < L; i += 4) {
+    const float2 di = dn;
+    if (i + 4 < L) dn = D(i + 4);
+    const float2 dd = lds_row_dot<LP>(sH + (size_t)i * LP, v);
+    store(rDo, i, pin ? di : dd);
+    S(i) = make_float2(a.s_cd * di.x + a.s_cn * dd.x, a.s_cd * di.y + a.s_cn * dd.y);
+  }
+  __syncthreads();
+  // x' = x + s_c C u
+  load_v(S, L);
+  float2 xn = w < L1 ? X(w) : zero;
+  for (int i = w; i < L1; i += 4) {
+    const float2 x = xn;
+    if (i + 4 < L1) xn = X(i + 4);
+    const float2 cu = lds_row_dot<LP>(sC + (size_t)i * LP, v);
+    store_x(i, pin ? x : make_float2(x.x + a.s_c * cu.x, x.y + a.s_c * cu.y));
+  }
+}
+
+
+
 }  // namespace sht
 
 // Orders per block: 32, or 16 for the half map, whose four per-degree
@@ -220,9 +385,40 @@ static size_t vertical_lds_bytes(int nlev, int mode, bool with_m) {
   return (2 * L * L1 + mat) * sizeof(float) + colsets * cols * sizeof(float2);
 }
 
+// LDS of vertical_rows_kernel: the staged matrices at row stride lp and one
+// result column of 64 orders.
+static size_t vertical_rows_lds_bytes(int nlev, int mode, bool with_m, int lp) {
+  const size_t L = nlev, L1 = nlev + 1;
+  const size_t rows = L + L1 + (mode == 0 ? (with_m ? 2 : 1) * L : 0);
+  return rows * lp * sizeof(float) + L * 64 * sizeof(float2);
+}
+
 extern "C" hipError_t sht_launch_vertical_solve(const sht::VerticalArgs* a, hipStream_t s) {
   if (a->nlev < 1 || a->nlev > sht::kMaxLevels || a->mode < 0 || a->mode > 2)
     return hipErrorInvalidValue;
+  // vertical_rows_kernel for solve, apply and tendencies while its buffer
+  // offsets (32-bit, beyond 2^31 for the lanes it masks) cover the planes
+  const size_t planes = (size_t)a->nlev * (size_t)(a->T + 1) * (size_t)(a->T + 1) * 8;
+  if (a->mode != 2 && planes < ((size_t)1 << 31)) {
+    // the stage columns in registers: nlev + 1 values, rounded up to groups of 8
+    const int lp = (a->nlev + 1 + 7) / 8 * 8;
+    const size_t lds = vertical_rows_lds_bytes(a->nlev, a->mode, a->m_matrix != nullptr, lp);
+    const void* k = nullptr;
+    switch (lp) {
+#define SHT_VK(LP) \
+  case LP: k = reinterpret_cast<const void*>(sht::vertical_rows_kernel<LP>); break;
+      SHT_VK(8) SHT_VK(16) SHT_VK(24) SHT_VK(32) SHT_VK(40) SHT_VK(48) SHT_VK(56) SHT_VK(64)
+      SHT_VK(72)
+#undef SHT_VK
+      default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    const dim3 g((unsigned)(a->T + 1), (unsigned)((a->T + 1 + 63) / 64)), b(256);
+    sht::VerticalArgs copy = *a;
+    void* args[] = {&copy};
+    return hipLaunchKernel(k, g, b, args, lds, s);
+  }
   const int cols = vertical_cols(a->mode);
   const size_t lds = vertical_lds_bytes(a->nlev, a->mode, a->m_matrix != nullptr);
   const void* k = cols == 16 ? reinterpret_cast<const void*>(sht::vertical_kernel<16>)
