@@ -363,7 +363,7 @@ def roofline(rt, phases, precision, workload="t383"):
     low = None
     if rt.low_flops > 0 and low_ms > 0:
         a = rt.low_flops / (low_ms * 1e-3) / 1e12
-        low = {"kernel": "low_synthesis + low_analysis (+ low_pack)", "ms": low_ms,
+        low = {"kernel": "low_synthesis + low_analysis", "ms": low_ms,
                "flops": rt.low_flops, "achieved": a, "peak": FP64_PEAK_TFLOPS,
                "unit": "TFLOP/s", "frac": a / FP64_PEAK_TFLOPS}
     if gemm_ms >= fft_ms:

@@ -402,7 +402,6 @@ struct JobState {
   DevBuf waist_a, waist_b;    // synthesis output (b: gradient north) / analysis input
   DevBuf packed_out;          // analysis output
   DevBuf low;                 // vector jobs: FP64 low waist [level][u|v][m < m0][p][i]
-  DevBuf low_pc;              // uv_from_vordiv: FP64 psi, chi of the orders m < m0
   int ncols = 0;
 };
 
@@ -714,9 +713,7 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     const bool low = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV && P->m0 > 0;
     if (low) {
       if (prepare) js->low.alloc((size_t)n * 2 * P->m0 * 2 * P->JHP * sizeof(double2));
-      if (prepare) js->low_pc.alloc((size_t)n * P->m0 * (T + 1) * sizeof(double4));
       sht::LowSynArgs ls{};
-      ls.pc = js->low_pc.as<double4>();
       ls.vort = static_cast<const float2*>(J.in0);
       ls.div = static_cast<const float2*>(J.in1);
       ls.dp = P->low_syn[kD].as<double>();

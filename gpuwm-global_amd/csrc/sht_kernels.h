@@ -247,7 +247,6 @@ struct LowSynArgs {
   const double* dp;           // low_syn tables [m][r][i]: dP/dphi and P m / cos
   const double* pm;
   double2* low;               // (E, O) of U and V without the 1/a the unfold applies
-  double4* pc;                // workspace [level][m][n]: psi (re, im), chi (re, im) in FP64
   double radius;
   int T, JH, JHP, m0, nlev;
 };

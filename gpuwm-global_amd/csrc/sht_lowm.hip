@@ -147,28 +147,34 @@ __global__ void __launch_bounds__(64 * kLowWaves) low_synthesis_kernel(LowSynArg
     const int kn = (K - pr + 1) / 2;
     const int kpad = (kn + kLowKPad - 1) / kLowKPad * kLowKPad;
     if (seg) __syncthreads();  // the previous segment's B is consumed
-    // one thread per (level, k row): its psi and chi (from the low pack,
-    // contiguous in n) give the four columns of that level; all loads of a
-    // thread are issued before the LDS writes
+    // one thread per (level, k row): psi, chi = inverse Laplacian (vorticity,
+    // divergence) of that entry in FP64 (degree 0 -> 0, m = 0 realified) give
+    // the four columns of that level; all loads of a thread are issued before
+    // the LDS writes
     constexpr int kPer = 4;
+    const double a2 = a.radius * a.radius;
     for (int e0 = threadIdx.x; e0 < kpad * kLowLevels; e0 += kPer * blockDim.x) {
-      double4 pc[kPer];
+      float2 zv[kPer], dv[kPer];
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
         const int e = e0 + u * blockDim.x;
         const int l = e % kLowLevels, j = e / kLowLevels;
         const int lev = min(lev0 + l, a.nlev - 1);
         const int n = min(m + 2 * j + pr, a.T);
-        pc[u] = a.pc[((size_t)lev * a.m0 + m) * T1 + n];
+        const size_t o = ((size_t)lev * T1 + n) * T1 + m;
+        zv[u] = a.vort[o];
+        dv[u] = a.div[o];
       }
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
         const int e = e0 + u * blockDim.x;
         const int l = e % kLowLevels, j = e / kLowLevels;
         if (j >= kpad) continue;
-        const bool ok = j < kn && lev0 + l < a.nlev;
-        const double psr = ok ? pc[u].x : 0.0, psi = ok ? pc[u].y : 0.0;
-        const double chr = ok ? pc[u].z : 0.0, chi = ok ? pc[u].w : 0.0;
+        const int n = m + 2 * j + pr;
+        const bool ok = j < kn && lev0 + l < a.nlev && n >= 1;
+        const double f = ok ? -a2 / ((double)n * ((double)n + 1.0)) : 0.0;
+        const double psr = f * (double)zv[u].x, psi = m == 0 ? 0.0 : f * (double)zv[u].y;
+        const double chr = f * (double)dv[u].x, chi = m == 0 ? 0.0 : f * (double)dv[u].y;
         double* dst = lds_low + (size_t)j * kLowCols + 4 * l;
         if (seg == 0) {  // X: U <- i chi, V <- i psi
           dst[0] = -chi;
@@ -304,30 +310,6 @@ __global__ void __launch_bounds__(64 * kLowWaves) low_analysis_kernel(LowAnaArgs
   }
 }
 
-// psi, chi = inverse Laplacian (vorticity, divergence) of the orders m < m0
-// in FP64 (m = 0 realified, degree 0 zero), laid out [level][m][n] so the
-// synthesis kernel stages them with loads that run along n.
-__global__ void low_pack_kernel(LowSynArgs a) {
-  const int T1 = a.T + 1;
-  const long long total = (long long)a.nlev * a.m0 * T1;
-  const double a2 = a.radius * a.radius;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-       e += (long long)gridDim.x * blockDim.x) {
-    const int n = (int)(e % T1);
-    const int m = (int)((e / T1) % a.m0);
-    const int lev = (int)(e / ((long long)T1 * a.m0));
-    double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (n >= 1 && m <= n) {
-      const double f = -a2 / ((double)n * ((double)n + 1.0));
-      const size_t o = ((size_t)lev * T1 + n) * T1 + m;
-      const float2 z = a.vort[o], d = a.div[o];
-      v = make_double4(f * (double)z.x, m == 0 ? 0.0 : f * (double)z.y, f * (double)d.x,
-                       m == 0 ? 0.0 : f * (double)d.y);
-    }
-    a.pc[e] = v;
-  }
-}
-
 }  // namespace sht
 
 extern "C" hipError_t sht_launch_low_synthesis(const sht::LowSynArgs* a, hipStream_t s) {
@@ -338,9 +320,6 @@ extern "C" hipError_t sht_launch_low_synthesis(const sht::LowSynArgs* a, hipStre
   const size_t lds = std::max((size_t)KP * sht::kLowCols * sizeof(double),  // one segment's B
                               sht::kLowReduceBytes);
   const dim3 g(m0 * groups, (a->nlev + sht::kLowLevels - 1) / sht::kLowLevels, 2);
-  const long long total = (long long)a->nlev * a->m0 * (a->T + 1);
-  hipLaunchKernelGGL(sht::low_pack_kernel, dim3((unsigned)std::min<long long>((total + 255) / 256, 4096)),
-                     dim3(256), 0, s, *a);
   hipLaunchKernelGGL(sht::low_synthesis_kernel, g, dim3(64 * sht::kLowWaves), lds, s, *a);
   return hipGetLastError();
 }
