@@ -503,17 +503,22 @@ def sampler_bench(args, torch):
 
 
 def vertical_bench(args, torch):
-    """Semi-implicit stage solve on spectral columns (solve_shifted): T383,
-    40 levels of divergence and theta plus ln ps, synthetic well-posed
+    """Semi-implicit stage solve on spectral columns (solve_shifted, `value`):
+    T383, 40 levels of divergence and theta plus ln ps, synthetic well-posed
     vertical operator, inputs resident on the device; HIP events on the
     launch stream.  Algorithmic bytes per call: the triangles of the 2L + 1
     input planes read once (complex64), the 2L + 1 output planes written
     once (whole (T+1)^2 squares: the outputs are fresh arrays, zero above
-    the triangle), the per-degree inverses read once."""
+    the triangle), the per-degree inverses read once.  apply,
+    linear_tendencies and half_apply are timed beside it on the same inputs
+    (`other_operations`, each with its own algorithmic bytes: half_apply
+    reads four per-degree matrices, tendencies none)."""
     import ctypes  # noqa: F401
 
     from arwen_spectral import SphericalHarmonicTransform
-    from arwen_spectral.semi_implicit import vertical_solve_shifted, wave_operator
+    from arwen_spectral.semi_implicit import (vertical_apply, vertical_half_apply,
+                                              vertical_linear_tendencies, vertical_solve_shifted,
+                                              wave_operator)
 
     sys.path.insert(0, str(ROOT))
     from oracle import semi_implicit as osi
@@ -543,8 +548,39 @@ def vertical_bench(args, torch):
     ms = e0.elapsed_time(e1) / args.steps
     k_tri = (T + 1) * (T + 2) // 2
     planes = 2 * L + 1
-    alg = planes * (8.0 * k_tri + 8.0 * (T + 1) ** 2) + 4.0 * (T + 1) * L * L
+    cols = planes * (8.0 * k_tri + 8.0 * (T + 1) ** 2)  # triangles in, whole squares out
+    alg = cols + 4.0 * (T + 1) * L * L
     achieved = alg / (ms * 1e-3) / 1e9
+    # the other column operators on the same inputs: apply (with M and its
+    # own inverse), linear_tendencies, half_apply (per-degree half maps)
+    dt, alpha = 600.0, 0.6
+    inv_a = f32(osi.helmholtz_inverse(modal, k2, dt, alpha))
+    mm = f32(m)
+    maps = [f32(x) for x in osi.symmetric_half_maps(modal, osi.x_modes(b, c), k2, dt, alpha)]
+    ops = {
+        "apply": (lambda: vertical_apply(t, d, th, ps, bm, cm, mm, inv_a, km, dt, alpha),
+                  cols + 4.0 * (T + 1) * L * L),
+        "linear_tendencies": (lambda: vertical_linear_tendencies(t, d, th, ps, bm, cm, km, 0.7),
+                              cols),
+        "half_apply": (lambda: vertical_half_apply(t, d, th, ps, bm, cm, *maps, km),
+                       cols + 4.0 * (T + 1) * 2 * (L * L + (L + 1) ** 2)),
+    }
+    others = {}
+    for name, (fn, nbytes) in ops.items():
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        oms = e0.elapsed_time(e1) / args.steps
+        gbs = nbytes / (oms * 1e-3) / 1e9
+        others[name] = {"ms_per_call": oms, "algorithmic_bytes": nbytes, "achieved_gbs": gbs,
+                        "frac": gbs / HBM_PEAK_GBS}
+        print(f"[bench] vertical {name}: {oms:.3f} ms per call, {gbs / HBM_PEAK_GBS:.3f} of HBM",
+              file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         from oracle.sht import OracleTransform
@@ -567,6 +603,7 @@ def vertical_bench(args, torch):
         "roofline": {"bound": "hbm", "kernel": "vertical_rows_kernel", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes": alg},
+        "other_operations": others,
         "cpu_baseline": cpu}))
 
 
