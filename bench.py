@@ -93,6 +93,12 @@ def member_fields(truncation, members, seed0=0, step=False):
 #: analysis GEMM, per job kind
 GEMM_PLANES = {"wind": (2, 2, 0, 0), "inverse": (1, 1, 0, 0), "inverse_gradient": (1, 2, 0, 0),
                "vordiv": (0, 0, 2, 2), "forward": (0, 0, 1, 1)}
+# Legendre tables each job's GEMM segments read (direction, kinds): P the
+# basis, D = dP/dphi, M = P m / cos.  The wind and vordiv jobs of a plan with
+# FP64 low orders run their other orders in the recurrence form (the basis
+# alone, one degree past the truncation).
+GEMM_TABLES = {"wind": ("syn", "DM"), "inverse": ("syn", "P"), "inverse_gradient": ("syn", "PD"),
+               "vordiv": ("ana", "MD"), "forward": ("ana", "P")}
 
 
 class RoundTrip:
@@ -150,7 +156,12 @@ class RoundTrip:
                                  "temperature": (t2, "temperature"), "lnps": (l2, "lnps"),
                                  "geopotential": (p2, "geopotential")}
             part["planes"] = [0, 0, 0, 0]  # GEMM planes: syn coef, syn waist, ana waist, ana coef
+            part["tables"] = {"syn": set(), "ana": set()}
             for job in pj:
+                d, kinds = GEMM_TABLES[job[0]]
+                if job[0] in ("wind", "vordiv") and tr.vector_fp64_orders > 0:
+                    kinds = "R"  # the basis to degree T+1
+                part["tables"][d].update(kinds)
                 nlev = job[2][0].shape[1] if isinstance(job[2], tuple) else job[2].shape[1]
                 for k, w in enumerate(GEMM_PLANES[job[0]]):
                     part["planes"][k] += M * nlev * w
@@ -328,19 +339,26 @@ TABLE_BYTES = {"float32": 4, "bf16x3": 4, "bf16x6": 6}  # bytes per table elemen
 
 def gemm_algorithmic_bytes(rt, precision):
     """Unique HBM bytes of the workload's two GEMM launches: each Legendre
-    table the launch uses read once (P, dP/dphi, P m/cos: K_tri x nlat/2
-    folded elements, no padding), and every coefficient plane (K_tri
-    complex64) and waist plane (nlat x (T+1) complex64) the launch reads or
-    writes, once."""
+    table a launch uses read once (P, dP/dphi, P m/cos: K_tri x nlat/2 folded
+    elements, no padding; the basis to degree T+1 for the recurrence form:
+    (T+2)(T+3)/2 x nlat/2; P and that one table are one array), and every
+    coefficient plane (K_tri complex64) and waist plane (nlat x (T+1)
+    complex64) the launch reads or writes, once."""
     total = 0.0
     for part in rt.parts:
         T, nlat = part["spec"]["T"], part["tr"].grid.nlat
         k_tri = (T + 1) * (T + 2) // 2
-        tables = 3 * k_tri * (nlat / 2) * TABLE_BYTES[precision]
+        k_ext = (T + 2) * (T + 3) // 2
+        for kinds in part["tables"].values():
+            ext = "R" in kinds
+            n = len(set(kinds) - {"R", "P"}) + (1 if ext or "P" in kinds else 0)
+            elems = (k_ext if ext else k_tri) * (nlat / 2)
+            # P (or the extended basis) first, then the other kinds
+            total += elems * TABLE_BYTES[precision] + (n - 1) * k_tri * (nlat / 2) * TABLE_BYTES[precision]
         coef = 8.0 * k_tri
         waist = 8.0 * nlat * (T + 1)
         sc, sw, aw, ac = part["planes"]
-        total += 2 * tables + sc * coef + sw * waist + aw * waist + ac * coef
+        total += sc * coef + sw * waist + aw * waist + ac * coef
     return total
 
 
@@ -352,7 +370,9 @@ def roofline(rt, phases, precision, workload="t383"):
     (algorithmic bytes per second) and the arithmetic intensity against the
     ridge point are reported beside it as ``other_roof``.  The vector jobs'
     orders m < m0 run in the FP64 low-order kernels, priced apart against the
-    FP64 peak (``fp64_low_orders``)."""
+    FP64 peak (``fp64_low_orders``); the FFT phases (all fold and unfold
+    launches, which together can take longer than the two GEMM launches, each
+    of them shorter) against HBM (``fft_phases``)."""
     gemm_ms = phases["synthesis_gemm"] + phases["analysis_gemm"]
     fft_ms = phases["inverse_fft"] + phases["forward_fft"]
     s = PRODUCTS[precision]
@@ -366,7 +386,12 @@ def roofline(rt, phases, precision, workload="t383"):
         low = {"kernel": "low_synthesis + low_analysis", "ms": low_ms,
                "flops": rt.low_flops, "achieved": a, "peak": FP64_PEAK_TFLOPS,
                "unit": "TFLOP/s", "frac": a / FP64_PEAK_TFLOPS}
-    if gemm_ms >= fft_ms:
+    fft = None
+    if fft_ms > 0:
+        a = rt.fft_bytes / (fft_ms * 1e-3) / 1e9
+        fft = {"kernel": "rfft_fold + irfft_unfold", "ms": fft_ms, "bytes": rt.fft_bytes,
+               "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS}
+    if gemm_ms > 0:
         t = gemm_ms * 1e-3
         alg_bytes = gemm_algorithmic_bytes(rt, precision)
         intensity = s * flops / alg_bytes
@@ -384,7 +409,7 @@ def roofline(rt, phases, precision, workload="t383"):
                 "algorithmic_bytes": alg_bytes, "intensity_flop_per_byte": intensity,
                 "ridge_flop_per_byte": ridge, "other_roof": other,
                 "fp32_equivalent_tflops": flops / t / 1e12, "mfma_products_per_fma": s,
-                "gemm_flops": flops, "fp64_low_orders": low}
+                "gemm_flops": flops, "fp64_low_orders": low, "fft_phases": fft}
     achieved = rt.fft_bytes / (fft_ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": "rfft_fold + irfft_unfold", "achieved": achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}

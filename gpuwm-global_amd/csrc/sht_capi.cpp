@@ -151,6 +151,8 @@ struct arwen_sht_plan {
   // scales and the FP64 tables of those orders (kinds dP/dphi and P m / cos)
   int m0 = 0;
   DevBuf d_tw64, scale_fold_vec64, scale_inv_a64;
+  // the same with 1/cos(phi): the recurrence form's orders m >= m0
+  DevBuf scale_fold_vec64_cos, scale_inv_acos64;
   DevBuf low_syn[3], low_ana[3];
   sht::FftPlan fft{};
   int rp_fold = 4, rp_fold_vec = 4, rp_unfold = 4;  // folded rows (wavefronts) per FFT block
@@ -202,10 +204,13 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   P->JH = (d->nlat + 1) / 2;
   P->JHP = SHT_ROUND_UP(P->JH, sht::kPad);
   P->MB = (T + 1 + sht::kMT - 1) / sht::kMT;
+  // blocks hold degrees m..T+1: the vector operators' recurrence form
+  // contracts the basis one degree past the truncation (the other operations
+  // leave that row zero in their data and ignore it in their results)
   P->koff.assign(2 * (T + 1) + 1, 0);
   for (int m = 0; m <= T; ++m)
     for (int p = 0; p < 2; ++p)
-      P->koff[2 * m + p + 1] = P->koff[2 * m + p] + sht::k_padded(T, m, p);
+      P->koff[2 * m + p + 1] = P->koff[2 * m + p] + sht::k_padded(T + 1, m, p);
   P->ldp = P->koff.back();
   P->tab_elems = (long long)P->ldp * P->JHP;
   P->ldw = P->MB * 2 * P->JHP * sht::kMT;
@@ -349,15 +354,22 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   upload(P->scale_inv_acos, invacos);
   if (P->m0 > 0) {
     std::vector<double> sfv64(P->JHP, 0.0), inva64(P->JHP, 0.0);
+    std::vector<double> sfvc64(P->JHP, 0.0), invac64(P->JHP, 0.0);
     for (int i = 0; i < P->JH; ++i) {
       const int jn = P->nlat - 1 - i;
       double wq = P->w[jn];
       if (jn == i) wq *= 0.5;
+      const double mu = P->mu[jn];
+      const double cosl = std::sqrt(std::max(0.0, 1.0 - mu * mu));
       sfv64[i] = 2.0 * M_PI * wq / (P->radius * P->nlon);
       inva64[i] = 1.0 / P->radius;
+      sfvc64[i] = sfv64[i] / cosl;
+      invac64[i] = 1.0 / (P->radius * cosl);
     }
     upload(P->scale_fold_vec64, sfv64);
     upload(P->scale_inv_a64, inva64);
+    upload(P->scale_fold_vec64_cos, sfvc64);
+    upload(P->scale_inv_acos64, invac64);
   }
   // the point sampler's recurrence coefficients, so that sampling allocates nothing
   build_sample_tables(P);
@@ -702,15 +714,18 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     pa.job.col0 = 0;
     pa.job.ldp = P->ldp;
     pa.job.radius = (float)P->radius;
+    // u, v with FP64 low orders: the orders m >= m0 in the recurrence form
+    const bool low = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV && P->m0 > 0;
     pa.job.mode = (J.op == ARWEN_SHT_OP_ZONAL_DERIVATIVE || J.op == ARWEN_SHT_OP_ZONAL_TO_WAIST)
                       ? sht::kIM
+                  : low                              ? sht::kVectorRec
                   : vec                              ? sht::kVector
                   : (J.flags & ARWEN_SHT_JOB_INVLAP) ? sht::kInvLap
                                                      : sht::kPlain;
     pa.job.off00 = static_cast<const float2*>(J.coeff_offset);
+    pa.job.m0 = low ? P->m0 : 0;
     B->packs.push_back(pa);
     // ---- FP64 low orders of u, v (orders m < m0) ----
-    const bool low = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV && P->m0 > 0;
     if (low) {
       if (prepare) js->low.alloc((size_t)n * 2 * P->m0 * 2 * P->JHP * sizeof(double2));
       sht::LowSynArgs ls{};
@@ -731,7 +746,11 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     const DataRef pk{js->packed_a.as<char>()};
     for (int mb = low ? P->m0 / sht::kMT : 0; prepare && mb < P->MB; ++mb)
       for (int q = 0; q < 2; ++q) {
-        if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
+        if (low) {
+          // recurrence form: a cos(phi) [u, v] = P [U, V] (pack_vector_rec_kernel)
+          SynSeg sg[1] = {{kP, 0, pk, 0}};
+          add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
+        } else if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
           // U,V = -dP/dphi (psi) + i P m/cos (chi), i P m/cos (psi) + dP/dphi (chi):
           // dP/dphi flips the parity, so its segment reads the other parity.
           SynSeg sg[2] = {{kD, 1, pk, 0}, {kPm, 0, pk, kMapSynVec}};
@@ -760,8 +779,12 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
       double pcs = (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) ? 4.0 * n
                    : (grad || sgrad)                     ? 2.0 * n
                                                          : 1.0 * n;
-      if (prepare) B->legendre_flops += pcs * pcf;
-      if (prepare && low) B->low_flops += pcs * pcf_low;
+      if (prepare && low) {  // FP64 kernels: four per level below m0; GEMM: two above
+        B->legendre_flops += 2.0 * n * (pcf - pcf_low) + 4.0 * n * pcf_low;
+        B->low_flops += 4.0 * n * pcf_low;
+      } else if (prepare) {
+        B->legendre_flops += pcs * pcf;
+      }
     }
     // ---- inverse FFTs ----
     auto unfold = [&](const DevBuf& waist, void* grid, int col0, int fstride, const DevBuf& sc,
@@ -791,6 +814,7 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
         ua.job.low = js->low.as<double2>() + (size_t)f * P->m0 * 2 * P->JHP;
         ua.job.low_stride = 2 * P->m0 * 2 * P->JHP;
         ua.job.rowscale64 = P->scale_inv_a64.as<double>();
+        ua.job.rowscale64_hi = P->scale_inv_acos64.as<double>();  // recurrence form
         ua.job.m0 = P->m0;
       }
       B->unfolds.push_back(ua);
@@ -883,6 +907,7 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
         fh.job.low = js->low.as<double2>() + (size_t)f * P->m0 * 2 * P->JHP;
         fh.job.low_stride = 2 * P->m0 * 2 * P->JHP;
         fh.job.rowscale64 = P->scale_fold_vec64.as<double>();
+        fh.job.rowscale64_hi = P->scale_fold_vec64_cos.as<double>();  // recurrence form
         fh.job.m0 = P->m0;
         B->folds.push_back(fh);
       }
@@ -906,7 +931,11 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     const DataRef wst{js->waist_a.as<char>()};
     for (int mb = low ? P->m0 / sht::kMT : 0; prepare && mb < P->MB; ++mb)
       for (int q = 0; q < 2; ++q) {
-        if (vec) {
+        if (low) {
+          // recurrence form: [iB, iA] = P^T [iV, iU] / cos(phi); unpack_vector_rec_kernel
+          AnaSeg sg[1] = {{kP, 0, wst, 0}};
+          add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 1);
+        } else if (vec) {
           // zeta, D = Pm^T [iV, iU]_q + dP^T [U, -V]_{1-q}
           AnaSeg sg[2] = {{kPm, 0, wst, 0}, {kD, 1, wst, kMapAnaVec}};
           add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 2);
@@ -919,8 +948,12 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
           add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 1);
         }
       }
-    if (prepare) B->legendre_flops += (vec ? 4.0 : divw ? 2.0 : 1.0) * n * pcf;
-    if (prepare && low) B->low_flops += 4.0 * n * pcf_low;
+    if (prepare && low) {  // FP64 kernels: four per level below m0; GEMM: two above
+      B->legendre_flops += 2.0 * n * (pcf - pcf_low) + 4.0 * n * pcf_low;
+      B->low_flops += 4.0 * n * pcf_low;
+    } else if (prepare) {
+      B->legendre_flops += (vec ? 4.0 : divw ? 2.0 : 1.0) * n * pcf;
+    }
     sht::UnpackArgs ua{};
     ua.koff = P->d_koff.as<int>();
     ua.T = T;
@@ -933,6 +966,7 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     ua.job.ldp = P->ldp;
     ua.job.vector = vec ? 1 : 0;
     ua.job.m0 = low ? P->m0 : 0;
+    ua.job.rec = low ? 1 : 0;
     ua.job.off00 = vec ? nullptr : static_cast<const float2*>(J.coeff_offset);
     B->unpacks.push_back(ua);
   }

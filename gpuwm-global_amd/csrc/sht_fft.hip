@@ -945,8 +945,11 @@ __global__ void __launch_bounds__(64 * kWaveBlock,
 // field's mirror row pair in FP64 (grid values fp32 as stored), and the
 // folded values go to the fp32 waist (orders the split-BF16 GEMM contracts)
 // and, for m < m0, in FP64 to the low waist (orders the FP64 low-order kernels
-// contract).  Same row grouping, persistence and block writes as the fp32
-// kernels above; LDS spans hold double2.
+// contract).  The two parts take their own row scales (rowscale64 for the
+// low orders, rowscale64_hi for the rest: the recurrence form's extra
+// 1/cos(phi) applies to the GEMM's orders only).  Same row grouping,
+// persistence and block writes as the fp32 kernels above; LDS spans hold
+// double2.
 // ---------------------------------------------------------------------------
 template <int L, int kWaveBlock>
 __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave64_kernel(FoldArgs a) {
@@ -995,6 +998,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave64_kernel(FoldA
     }
     if (g + (int)gridDim.x < ngroups) load_rows(g + gridDim.x);
     const double s = live ? a.job.rowscale64[i] : 0.0;
+    const double sh = live && a.job.rowscale64_hi ? a.job.rowscale64_hi[i] : s;
     double2* low = a.job.low + (size_t)fld * a.job.low_stride;
     constexpr int RO = (N / 2 + 4 + 63) / 64;  // rounds: MO <= N/2 + 4
     float val[RO][NV];
@@ -1008,8 +1012,9 @@ __global__ void __launch_bounds__(64 * kWaveBlock) rfft_fold_wave64_kernel(FoldA
         const double2 xn = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
         const double2 d = csub(zm, zc);
         const double2 xs = make_double2(0.5 * d.y, -0.5 * d.x);
-        e = make_double2((xn.x + xs.x) * s, (xn.y + xs.y) * s);
-        o = make_double2((xn.x - xs.x) * s, (xn.y - xs.y) * s);
+        const double sc = m < a.job.m0 ? s : sh;
+        e = make_double2((xn.x + xs.x) * sc, (xn.y + xs.y) * sc);
+        o = make_double2((xn.x - xs.x) * sc, (xn.y - xs.y) * sc);
         if (m < a.job.m0) {
           low[((size_t)m * 2 + 0) * a.JHP + i] = e;
           low[((size_t)m * 2 + 1) * a.JHP + i] = o;
@@ -1114,6 +1119,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave64_kernel(Un
       }
       wave_sync();  // the staged values are in registers; the span becomes the spectrum
       const double s = a.job.rowscale64[i];
+      const double sh = a.job.rowscale64_hi ? a.job.rowscale64_hi[i] : s;
       const double2* low = a.job.low + (size_t)fld * a.job.low_stride;
       for (int k = lane; k < N; k += 64) buf[k] = make_double2(0.0, 0.0);
       wave_sync();
@@ -1123,12 +1129,14 @@ __global__ void __launch_bounds__(64 * kWaveBlock) irfft_unfold_wave64_kernel(Un
         if (m >= T1) continue;
         double2 E = make_double2((double)er[r], (double)ei[r]);
         double2 O = make_double2((double)orr[r], (double)oi[r]);
+        double sc = sh;
         if (m < a.job.m0) {
           E = low[((size_t)m * 2 + 0) * a.JHP + i];
           O = low[((size_t)m * 2 + 1) * a.JHP + i];
+          sc = s;
         }
-        double2 A = make_double2((E.x + O.x) * s, (E.y + O.y) * s);
-        double2 B = make_double2((E.x - O.x) * s, (E.y - O.y) * s);
+        double2 A = make_double2((E.x + O.x) * sc, (E.y + O.y) * sc);
+        double2 B = make_double2((E.x - O.x) * sc, (E.y - O.y) * sc);
         if (a.job.imul) {
           const double k = a.job.imul == 2 ? (double)m : 1.0;  // i m or i
           A = make_double2(-k * A.y, k * A.x);

@@ -66,6 +66,7 @@ struct FoldJob {
   // laid out [m][p][i]
   double2* low;
   const double* rowscale64;   // per folded row, FP64
+  const double* rowscale64_hi;  // orders m >= m0 of the fp32 waist (NULL: rowscale64)
   int fstride;
   int m0;
   int low_stride;             // double2 per field
@@ -97,6 +98,7 @@ struct UnfoldJob {
   int f64;
   const double2* low;
   const double* rowscale64;
+  const double* rowscale64_hi;  // orders m >= m0 (NULL: rowscale64)
   int m0;
   int low_stride;
 };
@@ -260,7 +262,7 @@ struct LowAnaArgs {
 };
 
 // ---- spectral data movement (sht_pack.hip) ----
-enum PackMode : int { kPlain = 0, kIM = 1, kICoef = 2, kVector = 3, kInvLap = 4 };
+enum PackMode : int { kPlain = 0, kIM = 1, kICoef = 2, kVector = 3, kInvLap = 4, kVectorRec = 5 };
 
 struct PackJob {
   const float2* in0;          // [f][n][m]; vorticity for VECTOR
@@ -272,6 +274,7 @@ struct PackJob {
   int mode;
   float radius;
   const float2* off00;        // kPlain: per field, added to its (0, 0) coefficient (or NULL)
+  int m0;                     // kVectorRec: orders m < m0 are not written
 };
 
 struct PackArgs {
@@ -291,6 +294,8 @@ struct UnpackJob {
   int vector;                 // columns per level [zeta.re, zeta.im, D.re, D.im]
   int m0;                     // orders m < m0 inside the triangle are left alone
                               // (written by the FP64 low-order kernel)
+  int rec;                    // vector jobs in the recurrence form: columns [iB, iA]
+                              // over degrees m..T+1 (see unpack_vector_rec_kernel)
   const float2* off00;        // scalar jobs: per field, added to its (0, 0) output (or NULL)
 };
 

@@ -22,6 +22,8 @@
 //           segment needs [i chi, i psi] = [-a3, a2, a1, -a0] of those
 //           columns; the GEMM reads that arrangement from the same block
 //           through a column map instead of a second copy.
+//   VECTOR_REC  the winds' orders m >= m0 in the recurrence form (vector jobs
+//           whose orders m < m0 run in FP64): see pack_vector_rec_kernel.
 #include "sht_kernels.h"
 
 namespace sht {
@@ -149,6 +151,124 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
   }
 }
 
+// The recurrence form of the vector operators.  With the basis normalized as
+// here, cos(phi) dP_n/dphi = (n+1) e_n P_{n-1} - n e_{n+1} P_{n+1}, where
+// e_n = sqrt((n^2 - m^2) / (4 n^2 - 1)) (e_m = 0).  Substituted into
+//   u = (-dpsi/dphi + (1/cos) dchi/dlambda) / a,
+//   v = ((1/cos) dpsi/dlambda + dchi/dphi) / a,
+// the winds of order m are contractions of the basis alone over the degrees
+// k = m..T+1:
+//   a cos(phi) u = sum_k P_k U_k,  U_k = (k-1) e_k psi_{k-1} - (k+2) e_{k+1} psi_{k+1} + i m chi_k
+//   a cos(phi) v = sum_k P_k V_k,  V_k = i m psi_k - (k-1) e_k chi_{k-1} + (k+2) e_{k+1} chi_{k+1}
+// (psi_k = 0 outside m <= k <= T), one plane contraction per wind component
+// against the P*m/cos and dP/dphi tables' two.  The pack forms U and V in FP64
+// from psi, chi = zeta / lambda_n, D / lambda_n (fp32, as the other vector
+// pack forms them) and writes them in fp32 as the four
+// columns [U, V] per level; the unfold multiplies by 1/(a cos(phi)).  Near a
+// pole the terms of order 0 cancel to O(cos^2), so the form is used only where
+// the orders below m0 >= 4 run in FP64; above them every term carries cos^m.
+// A block owns 32 orders x 64 degrees plus the degrees either side (halo rows
+// n0 - 1 and n0 + 64), so neighbours in degree come from LDS.
+// e_n^m = sqrt((n^2 - m^2) / (4 n^2 - 1)) in FP64 without a divide or a
+// square root: x rsqrt(x y) from an fp32 seed refined by two Newton steps
+// r <- r (3 - z r^2) / 2 (relative error ~1e-16; x y is exact); 0 for n <= m.
+__device__ __forceinline__ double rec_eps(int n, int m) {
+  const double x = (double)n * n - (double)m * m, y = 4.0 * (double)n * n - 1.0;
+  if (!(x > 0.0)) return 0.0;
+  const double z = x * y;
+  double r = (double)__builtin_amdgcn_rsqf((float)z);
+  r = r * (1.5 - 0.5 * z * r * r);
+  r = r * (1.5 - 0.5 * z * r * r);
+  return x * r;
+}
+
+// 1 / lambda_n = -a^2 / (n (n+1)) in FP64 without a divide: an fp32
+// reciprocal seed refined by two Newton steps r <- r (2 - d r); 0 at n = 0.
+__device__ __forceinline__ double inv_lambda(double a2, int n) {
+  if (n < 1) return 0.0;
+  const double d = (double)n * ((double)n + 1.0);
+  double r = (double)__builtin_amdgcn_rcpf((float)d);
+  r = r * (2.0 - d * r);
+  r = r * (2.0 - d * r);
+  return -a2 * r;
+}
+
+__global__ void __launch_bounds__(256) pack_vector_rec_kernel(PackArgs a) {
+  constexpr int kRows = kChunk + 2;
+  __shared__ float2 tile[2][kRows][kTile + 1];  // [input][n - n0 + 1][m - m0]
+  __shared__ int skoff[2 * kTile];
+  const int T = a.T, T1 = T + 1;
+  const int nchunks = (T + 2 + kChunk - 1) / kChunk;  // degrees up to T+1
+  const int b = xcd_block();
+  const int c = b % nchunks;
+  const int mt = (b / nchunks) % a.mtiles;
+  const int f = b / nchunks / a.mtiles;
+  if (f >= a.job.nfields) return;
+  const int m0 = mt * kTile;
+  const int n0 = m0 + kChunk * c;
+  if (n0 > T + 1 || m0 + kTile <= a.job.m0) return;  // block-uniform
+  constexpr int E = kRows * kTile;
+  constexpr int IT = (E + 255) / 256;
+  const size_t plane = (size_t)T1 * T1;
+  const int tid = threadIdx.x;
+  stage_koff(skoff, a.koff, m0, T);
+  {
+    float2 v[2][IT];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const float2* in = (s == 0 ? a.job.in0 : a.job.in1) + (size_t)f * plane;
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int e = tid + 256 * it;
+        const int n = n0 - 1 + e / kTile, m = m0 + e % kTile;
+        const bool ok = e < E && n <= T && m <= n;  // m <= n <= T
+        const float2 x = in[ok ? (size_t)n * T1 + m : 0];
+        v[s][it] = ok ? x : make_float2(0.f, 0.f);
+      }
+    }
+    const double a2 = (double)a.job.radius * (double)a.job.radius;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + 256 * it;
+      // the tile holds psi, chi in fp32 (FP64 here, between the loads and the
+      // LDS stores, costs a quarter of the kernel's time)
+      const float g = (float)inv_lambda(a2, n0 - 1 + e / kTile);
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+        if (e < E) tile[s][e / kTile][e % kTile] = make_float2(g * v[s][it].x, g * v[s][it].y);
+    }
+  }
+  __syncthreads();
+  const int j = tid % 32, p = (tid / 32) % 2;
+#pragma unroll
+  for (int it = 0; it < kTile * 2 * 32 / 256; ++it) {
+    const int mi = tid / 64 + 4 * it;
+    const int m = m0 + mi;
+    const int n = chunk_degree(n0, m, p, j);
+    if (m > T || m < a.job.m0 || n > T + 1 || n < m + p) continue;
+    const int nidx = (n - m - p) >> 1;
+    const int r = n - n0 + 1;
+    // psi, chi of degree n + d from tile row r + d (zero outside m <= deg <= T)
+    auto pot = [&](int s, int d) {
+      const float2 z = tile[s][r + d][mi];
+      return make_double2((double)z.x, (double)z.y);
+    };
+    const double2 pm = pot(0, -1), p0 = pot(0, 0), pp = pot(0, 1);
+    const double2 cm = pot(1, -1), c0 = pot(1, 0), cp = pot(1, 1);
+    const double dn = (double)n, dm = (double)m;
+    const double ca = (dn - 1.0) * rec_eps(n, m), cb = (dn + 2.0) * rec_eps(n + 1, m);
+    float o[4];
+    o[0] = (float)(ca * pm.x - cb * pp.x - dm * c0.y);  // U
+    o[1] = (float)(ca * pm.y - cb * pp.y + dm * c0.x);
+    o[2] = (float)(-dm * p0.y - ca * cm.x + cb * cp.x);  // V
+    o[3] = (float)(dm * p0.x - ca * cm.y + cb * cp.y);
+    float* dst = a.job.out0 + ((size_t)a.job.col0 + (size_t)f * 4) * a.job.ldp +
+                 skoff[2 * mi + p] + nidx;
+#pragma unroll
+    for (int cl = 0; cl < 4; ++cl) dst[(size_t)cl * a.job.ldp] = o[cl];
+  }
+}
+
 // One block per (field or level, 32 orders, 64 rows).  Entries above the
 // triangle (m > n) are written as zeros; chunks that lie wholly above it need
 // no reads.  As in pack, every thread issues all of its loads first.
@@ -221,6 +341,111 @@ __global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs a) {
   }
 }
 
+// The analysis side of the recurrence form (orders m >= m0 of vector jobs
+// whose orders below m0 run in FP64).  The fold weights the winds by
+// 2 pi w / (a nlon cos(phi)) and the GEMM contracts the basis over degrees
+// m..T+1, giving [iB, iA] per level: A_k = sum_j (2 pi w_j / (a cos_j)) u_j P_k,
+// B_k likewise of v.  With the derivative recurrence above,
+//   D_n    = i m A_n + n e_{n+1} B_{n+1} - (n+1) e_n B_{n-1}
+//   zeta_n = i m B_n - n e_{n+1} A_{n+1} + (n+1) e_n A_{n-1},
+// formed here in FP64 from the packed columns (degrees n - 1 .. n + 1 of the
+// chunk's rows and its halo).  The orders m < m0 are left to the FP64
+// low-order kernel, entries above the triangle are written as zeros.
+__global__ void __launch_bounds__(256) unpack_vector_rec_kernel(UnpackArgs a) {
+  constexpr int kRows = kChunk + 2;
+  __shared__ float tile[4][kRows][kTile + 1];  // [col][n - n0 + 1][m - m0]
+  __shared__ int skoff[2 * kTile];
+  const int T = a.T, T1 = T + 1;
+  const int nchunks = (T1 + kChunk - 1) / kChunk;
+  const int b = xcd_block();
+  const int c = b % nchunks;
+  const int mt = (b / nchunks) % a.tiles;
+  const int f = b / nchunks / a.tiles;
+  if (f >= a.job.nfields) return;
+  const int m0 = mt * kTile;
+  const int n0 = kChunk * c;
+  constexpr int IT = kTile * 2 * 32 / 256;  // loads per thread and column
+  const size_t plane = (size_t)T1 * T1;
+  const int tid = threadIdx.x;
+  const float* in = a.job.in + ((size_t)a.job.col0 + (size_t)f * 4) * a.job.ldp;
+  if (n0 + kChunk >= m0) {  // some m <= n + 1 in this chunk (block-uniform)
+    stage_koff(skoff, a.koff, m0, T);
+    __syncthreads();
+    const int j = tid % 32, p = (tid / 32) % 2;
+    float v[4][IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int mi = tid / 64 + 4 * it;
+      const int m = m0 + mi;
+      const int n = chunk_degree(n0, m, p, j);
+      const bool ok = m <= T && n <= T + 1 && n >= m + p;
+      const size_t idx = ok ? (size_t)skoff[2 * mi + p] + ((n - m - p) >> 1) : 0;
+#pragma unroll
+      for (int cl = 0; cl < 4; ++cl) {
+        const float x = in[(size_t)cl * a.job.ldp + idx];
+        v[cl][it] = ok ? x : 0.f;
+      }
+    }
+    // halo: degrees n0 - 1 and n0 + 64 of the tile's orders
+    float h[4] = {0.f, 0.f, 0.f, 0.f};
+    const int hm = m0 + (tid & 31), hn = (tid >> 5) ? n0 + kChunk : n0 - 1;
+    if (tid < 64) {
+      const bool ok = hm <= T && hn >= hm && hn <= T + 1;
+      const int hp = (hn - hm) & 1;
+      const size_t idx = ok ? (size_t)skoff[2 * (tid & 31) + hp] + ((hn - hm - hp) >> 1) : 0;
+#pragma unroll
+      for (int cl = 0; cl < 4; ++cl) {
+        const float x = in[(size_t)cl * a.job.ldp + idx];
+        h[cl] = ok ? x : 0.f;
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int mi = tid / 64 + 4 * it;
+      const int n = chunk_degree(n0, m0 + mi, p, j);
+      if (n - n0 < kChunk) {
+#pragma unroll
+        for (int cl = 0; cl < 4; ++cl) tile[cl][n - n0 + 1][mi] = v[cl][it];
+      }
+    }
+    if (tid < 64) {
+#pragma unroll
+      for (int cl = 0; cl < 4; ++cl) tile[cl][(tid >> 5) ? kRows - 1 : 0][tid & 31] = h[cl];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int it = 0; it < kChunk * kTile / 256; ++it) {
+    const int e = tid + 256 * it;
+    const int mi = e % kTile, r = e / kTile;
+    const int n = n0 + r, m = m0 + mi;
+    if (n > T || m > T) continue;
+    if (m < a.job.m0 && m <= n) continue;  // the FP64 low-order kernel's entries
+    float2 z = make_float2(0.f, 0.f), d = make_float2(0.f, 0.f);
+    if (m <= n) {
+      const int rr = r + 1;
+      const double dn = (double)n, dm = (double)m;
+      const double ca = (dn + 1.0) * rec_eps(n, m), cb = dn * rec_eps(n + 1, m);  // X_{n-1}, X_{n+1}
+      // columns 0, 1: i B; 2, 3: i A
+      const double bx = tile[0][rr][mi], by = tile[1][rr][mi];
+      const double ax = tile[2][rr][mi], ay = tile[3][rr][mi];
+      const double sax = cb * tile[2][rr + 1][mi] - ca * tile[2][rr - 1][mi];
+      const double say = cb * tile[3][rr + 1][mi] - ca * tile[3][rr - 1][mi];
+      const double sbx = cb * tile[0][rr + 1][mi] - ca * tile[0][rr - 1][mi];
+      const double sby = cb * tile[1][rr + 1][mi] - ca * tile[1][rr - 1][mi];
+      // zeta = m (iB) + i (cb iA_{n+1} - ca iA_{n-1}); D = m (iA) - i (cb iB_{n+1} - ca iB_{n-1})
+      z = make_float2((float)(dm * bx - say), (float)(dm * by + sax));
+      d = make_float2((float)(dm * ax + sby), (float)(dm * ay - sbx));
+      if (m == 0) {
+        z.y = 0.f;
+        d.y = 0.f;
+      }
+    }
+    a.job.out0[(size_t)f * plane + (size_t)n * T1 + m] = z;
+    a.job.out1[(size_t)f * plane + (size_t)n * T1 + m] = d;
+  }
+}
+
 // Elementwise spectral operators on the boundary layout.
 //   op 0: project   (zero above the triangle, m = 0 realified; transform.py:1124-1144)
 //   op 1: laplacian (c * lambda_n; transform.py:1798-1802)
@@ -253,10 +478,13 @@ __global__ void spectral_elementwise_kernel(const float2* __restrict__ in,
 }  // namespace sht
 
 extern "C" hipError_t sht_launch_pack(const sht::PackArgs* a, hipStream_t s) {
-  int blocks = a->job.nfields * a->mtiles * ((a->T + 1 + 63) / 64);
+  const int degrees = a->job.mode == sht::kVectorRec ? a->T + 2 : a->T + 1;
+  int blocks = a->job.nfields * a->mtiles * ((degrees + 63) / 64);
   if (blocks == 0) return hipSuccess;
   blocks = (blocks + 7) / 8 * 8;  // XCD-aware order (xcd_block)
-  if (a->job.mode == sht::kVector)
+  if (a->job.mode == sht::kVectorRec)
+    hipLaunchKernelGGL(sht::pack_vector_rec_kernel, dim3(blocks), dim3(256), 0, s, *a);
+  else if (a->job.mode == sht::kVector)
     hipLaunchKernelGGL(sht::pack_kernel<2>, dim3(blocks), dim3(256), 0, s, *a);
   else
     hipLaunchKernelGGL(sht::pack_kernel<1>, dim3(blocks), dim3(256), 0, s, *a);
@@ -267,7 +495,9 @@ extern "C" hipError_t sht_launch_unpack(const sht::UnpackArgs* a, hipStream_t s)
   int blocks = a->job.nfields * a->tiles * ((a->T + 1 + 63) / 64);
   if (blocks == 0) return hipSuccess;
   blocks = (blocks + 7) / 8 * 8;  // XCD-aware order (xcd_block)
-  if (a->job.vector)
+  if (a->job.rec)
+    hipLaunchKernelGGL(sht::unpack_vector_rec_kernel, dim3(blocks), dim3(256), 0, s, *a);
+  else if (a->job.vector)
     hipLaunchKernelGGL(sht::unpack_kernel<4>, dim3(blocks), dim3(256), 0, s, *a);
   else
     hipLaunchKernelGGL(sht::unpack_kernel<2>, dim3(blocks), dim3(256), 0, s, *a);

@@ -129,7 +129,9 @@ __global__ void legendre_tables_kernel(int T, int nlat, int JH, int JHP,
   for (int mm = 1; mm <= m; ++mm) diag = -sqrt((2 * mm + 1) / (2.0 * mm)) * root * diag;
 
   double p2 = 0.0, p1 = 0.0;  // P_{n-2}, P_{n-1}
-  for (int n = m; n <= T; ++n) {
+  // degrees m..T, and T+1 of the basis alone: the vector operators'
+  // recurrence form contracts the basis one degree past the truncation
+  for (int n = m; n <= T + 1; ++n) {
     double p;
     if (n == m) {
       p = diag;
@@ -157,6 +159,10 @@ __global__ void legendre_tables_kernel(int T, int nlat, int JH, int JHP,
     const long long plain_idx = base + (long long)i * kp + nidx;
     const long long syn_idx = base + syn_tiled(JHP, i, nidx);
     const long long ana_idx = base + ana_tiled(kp, i, nidx);
+    if (n > T) {  // the extra degree: P only (the other kinds keep their zeros)
+      if (nidx < kp) store_value(out, 0, plain_idx, syn_idx, ana_idx, p);
+      break;
+    }
     store_value(out, 0, plain_idx, syn_idx, ana_idx, p);
     store_value(out, 1, plain_idx, syn_idx, ana_idx, d);
     store_value(out, 2, plain_idx, syn_idx, ana_idx, pm);

@@ -207,6 +207,80 @@ def test_vector_known_answers():
     assert z[0, 0] == 0.0 and d[0, 0] == 0.0
 
 
+def _eps(m, n):
+    n = np.asarray(n, dtype=np.float64)
+    return np.sqrt(np.maximum(n * n - m * m, 0.0) / (4.0 * n * n - 1.0))
+
+
+def test_derivative_recurrence_of_the_vector_kernels():
+    """cos(phi) dP_n/dphi = (n+1) e_n P_{n-1} - n e_{n+1} P_{n+1}, the identity
+    behind the engine's recurrence form of the vector operators (pack/unpack
+    kernels: the winds of order m as contractions of the basis alone over
+    degrees m..T+1), checked against the oracle's dP/dphi columns."""
+    T = 40
+    g = gaussian_grid(T, dealias_factor=1.0)
+    x, cos = g.sin_lat, g.cos_lat
+    cols = dict(legendre_columns(T + 1, x))
+    for m in range(T + 1):
+        col = cols[m]                      # degrees m..T+1
+        d = derivative_column(m, x, col[:-1])
+        n = np.arange(m, T + 1)
+        below = np.vstack([np.zeros((1, x.size)), col[:-2]])   # P_{n-1} (0 at n = m)
+        rec = ((n + 1) * _eps(m, n))[:, None] * below - (n * _eps(m, n + 1))[:, None] * col[1:]
+        assert np.max(np.abs(cos * d - rec)) < 1e-12 * max(1.0, np.max(np.abs(rec)))
+
+
+def test_vector_operators_in_the_recurrence_form_match_the_oracle():
+    """The engine's recurrence form, restated in FP64: the pack's [U, V]
+    contracted with P over degrees m..T+1 and divided by a cos(phi) gives the
+    oracle's winds; the fold weight 2 pi w / (a cos(phi)), P^T and the unpack's
+    neighbour sums give its vorticity and divergence."""
+    T = 21
+    o = OracleTransform.create(T, dealias_factor=1.0)
+    g = o.grid
+    x, cos, w, a = g.sin_lat, g.cos_lat, g.quadrature_weights, g.radius_m
+    cols = dict(legendre_columns(T + 1, x))
+    zeta = red_spectrum(T, (2,), seed=5, scale=1e-5)
+    div = red_spectrum(T, (2,), seed=6, scale=2e-6)
+    zeta[..., 0, 0] = 0
+    div[..., 0, 0] = 0
+    u_ref, v_ref = o.wind_from_vordiv(zeta, div)
+    psi, chi = o.inverse_laplacian(zeta), o.inverse_laplacian(div)
+    wu = np.zeros((2, g.nlat, T + 1), complex)
+    wv = np.zeros_like(wu)
+    for m in range(T + 1):
+        k = np.arange(m, T + 2)
+        pad = lambda c: np.concatenate([np.zeros((2, 1)), c[:, m:, m], np.zeros((2, 2))], 1)
+        ps, ch = pad(psi), pad(chi)            # index j <-> degree m - 1 + j
+        ca, cb = (k - 1) * _eps(m, k), (k + 2) * _eps(m, k + 1)
+        U = ca * ps[:, :-2] - cb * ps[:, 2:] + 1j * m * ch[:, 1:-1]
+        V = 1j * m * ps[:, 1:-1] - ca * ch[:, :-2] + cb * ch[:, 2:]
+        wu[:, :, m] = U @ cols[m] / (a * cos)
+        wv[:, :, m] = V @ cols[m] / (a * cos)
+    u, v = o.waist_to_grid(wu), o.waist_to_grid(wv)
+    scale = max(np.max(np.abs(u_ref)), np.max(np.abs(v_ref)))
+    assert np.max(np.abs(u - u_ref)) < 1e-12 * scale
+    assert np.max(np.abs(v - v_ref)) < 1e-12 * scale
+
+    z_ref, d_ref = o.vordiv_from_wind(u_ref, v_ref)
+    fu, fv = o.fourier_waist(u_ref), o.fourier_waist(v_ref)
+    z = np.zeros_like(zeta)
+    d = np.zeros_like(div)
+    wt = 2 * math.pi * w / (a * cos)
+    for m in range(T + 1):
+        A = (fu[:, :, m] * wt) @ cols[m].T          # degrees m..T+1
+        B = (fv[:, :, m] * wt) @ cols[m].T
+        n = np.arange(m, T + 1)
+        ca, cb = (n + 1) * _eps(m, n), n * _eps(m, n + 1)
+        Am = np.concatenate([np.zeros((2, 1)), A[:, :-2]], 1)   # X_{n-1}, 0 at n = m
+        Bm = np.concatenate([np.zeros((2, 1)), B[:, :-2]], 1)
+        d[:, m:, m] = 1j * m * A[:, :-1] + cb * B[:, 1:] - ca * Bm
+        z[:, m:, m] = 1j * m * B[:, :-1] - cb * A[:, 1:] + ca * Am
+    z, d = o.project(z), o.project(d)
+    assert np.max(np.abs(z - z_ref)) < 1e-11 * np.max(np.abs(z_ref))
+    assert np.max(np.abs(d - d_ref)) < 1e-11 * np.max(np.abs(d_ref))
+
+
 def test_gram_solved_rows_are_the_quadrature_rows_to_roundoff():
     g = gaussian_grid(63, dealias_factor=1.0)
     for m, col in legendre_columns(63, g.sin_lat):
