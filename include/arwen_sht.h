@@ -92,7 +92,10 @@ typedef struct arwen_sht_plan_desc {
                              zonal orders m < fp64_orders run in FP64 end to
                              end (FP64 longitude FFTs for u and v, FP64
                              tables and sums for those orders); the rest use
-                             the plan's precision.  -1: the default, 0 for
+                             the plan's precision, as contractions of the
+                             basis alone through the derivative recurrence
+                             (two per level instead of four; off when 0).
+                             -1: the default, 0 for
                              ARWEN_SHT_FP32 and 4 ceil((T+1)/192) for the
                              split-BF16 modes; 0: off; k > 0: rounded up to
                              a multiple of 4.  Needs nlon = 64 L with
