@@ -157,12 +157,14 @@ class RoundTrip:
                                  "geopotential": (p2, "geopotential")}
             part["planes"] = [0, 0, 0, 0]  # GEMM planes: syn coef, syn waist, ana waist, ana coef
             part["tables"] = {"syn": set(), "ana": set()}
+            part["rec_levels"] = 0  # vector levels in the recurrence form (both directions)
             for job in pj:
+                nlev = job[2][0].shape[1] if isinstance(job[2], tuple) else job[2].shape[1]
                 d, kinds = GEMM_TABLES[job[0]]
                 if job[0] in ("wind", "vordiv") and tr.vector_fp64_orders > 0:
                     kinds = "R"  # the basis to degree T+1
+                    part["rec_levels"] += M * nlev
                 part["tables"][d].update(kinds)
-                nlev = job[2][0].shape[1] if isinstance(job[2], tuple) else job[2].shape[1]
                 for k, w in enumerate(GEMM_PLANES[job[0]]):
                     part["planes"][k] += M * nlev * w
             seq = lambda x: x if isinstance(x, tuple) else (x,)  # noqa: E731
@@ -362,6 +364,29 @@ def gemm_algorithmic_bytes(rt, precision):
     return total
 
 
+def two_segment_equivalent(rt, precision, t, s, flops):
+    """The GEMM launches' rate counted in the two-segment form's work: four
+    plane contractions per vector level and direction above m0 (the
+    reference's P m/cos and dP/dphi formulation), where the recurrence form
+    runs two.  Not an MFMA utilisation (the MFMAs those FLOPs would take are
+    not issued); the work-equivalent rate of the launches."""
+    extra = 0.0
+    for part in rt.parts:
+        m0 = part["tr"].vector_fp64_orders
+        if m0 <= 0 or not part["rec_levels"]:
+            continue
+        T, nlat = part["spec"]["T"], part["tr"].grid.nlat
+        pcf = 4.0 * (nlat / 2) * (T + 1) * (T + 2) / 2
+        pcf_low = 4.0 * (nlat / 2) * (m0 * (T + 1) - m0 * (m0 - 1) / 2)
+        extra += 2.0 * part["rec_levels"] * (pcf - pcf_low)
+    if extra == 0.0:
+        return None
+    f = flops + extra
+    a = s * f / t / 1e12
+    return {"flops": f, "achieved": a, "peak": MFMA_PEAK[precision], "unit": "TFLOP/s",
+            "frac": a / MFMA_PEAK[precision]}
+
+
 def roofline(rt, phases, precision, workload="t383"):
     """Roofline of the dominant kernel.  The Legendre GEMM (the grouped
     synthesis and analysis launches, timed by their own phase events) is
@@ -372,7 +397,10 @@ def roofline(rt, phases, precision, workload="t383"):
     orders m < m0 run in the FP64 low-order kernels, priced apart against the
     FP64 peak (``fp64_low_orders``); the FFT phases (all fold and unfold
     launches, which together can take longer than the two GEMM launches, each
-    of them shorter) against HBM (``fft_phases``)."""
+    of them shorter) against HBM (``fft_phases``).  The MFMA figures count
+    the plane contractions the launches run (two per vector level and
+    direction in the recurrence form); ``two_segment_equivalent`` gives the
+    same launches' rate in the two-segment form's work (four)."""
     gemm_ms = phases["synthesis_gemm"] + phases["analysis_gemm"]
     fft_ms = phases["inverse_fft"] + phases["forward_fft"]
     s = PRODUCTS[precision]
@@ -409,7 +437,8 @@ def roofline(rt, phases, precision, workload="t383"):
                 "algorithmic_bytes": alg_bytes, "intensity_flop_per_byte": intensity,
                 "ridge_flop_per_byte": ridge, "other_roof": other,
                 "fp32_equivalent_tflops": flops / t / 1e12, "mfma_products_per_fma": s,
-                "gemm_flops": flops, "fp64_low_orders": low, "fft_phases": fft}
+                "gemm_flops": flops, "fp64_low_orders": low, "fft_phases": fft,
+                "two_segment_equivalent": two_segment_equivalent(rt, precision, t, s, flops)}
     achieved = rt.fft_bytes / (fft_ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": "rfft_fold + irfft_unfold", "achieved": achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
