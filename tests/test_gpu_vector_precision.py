@@ -69,6 +69,34 @@ def test_vector_round_trip_within_the_bound(T, oracle383):
     assert max(rel_err(za, zr), rel_err(da, dr)) < BOUND
 
 
+@pytest.mark.parametrize("T,nlat,nlon,fp64", [(94, 96, 192, None), (62, 72, 128, 12),
+                                               (100, 102, 256, 36)])
+def test_recurrence_form_on_uneven_shapes(T, nlat, nlon, fp64):
+    """The orders m >= m0 run in the recurrence form (the basis to degree
+    T+1, pack and unpack with halo degrees): truncations whose T+1 and T+2
+    fall on and off the kernels' 64-degree chunk edges, latitudes padded past
+    the truncation, and m0 inside the first 32-order tile."""
+    from arwen_spectral import SphericalHarmonicTransform, VorticityDivergenceOperator
+
+    tr = SphericalHarmonicTransform.create(T, nlat=nlat, nlon=nlon, dealias_factor=1.0,
+                                           backend="hip", precision="bf16x6",
+                                           fp64_orders=fp64)
+    assert tr.vector_fp64_orders == (4 if fp64 is None else fp64)
+    o = OracleTransform.create(T, nlat=nlat, nlon=nlon, dealias_factor=1.0)
+    op = VorticityDivergenceOperator(tr)
+    zeta, div = _vector_fields(T, 3, seed=200 + T)
+    u64, v64 = o.wind_from_vordiv(zeta, div)
+    u, v = op.wind_from_vordiv(zeta, div)
+    assert max(rel_err(u, u64), rel_err(v, v64)) < BOUND
+    uh = u.cpu().numpy().astype(np.float64)
+    vh = v.cpu().numpy().astype(np.float64)
+    z64, d64 = o.vordiv_from_wind(uh, vh)
+    z, d = op.vordiv_from_wind(u, v)
+    print(f"T{T} {nlat}x{nlon} m0={tr.vector_fp64_orders}: analysis vorticity "
+          f"{rel_err(z, z64):.2e}, divergence {rel_err(d, d64):.2e}")
+    assert max(rel_err(z, z64), rel_err(d, d64)) < BOUND
+
+
 def test_the_fp64_orders_carry_the_bound(oracle383):
     """Without the FP64 orders the same round trip misses the bound by an
     order of magnitude (the reason they exist); with them, split-BF16 and
