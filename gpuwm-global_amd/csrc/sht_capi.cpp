@@ -150,8 +150,9 @@ struct arwen_sht_plan {
   // vector operators' FP64 low orders m < m0: FP64 twiddles, per-row FP64
   // scales and the FP64 tables of those orders (kinds dP/dphi and P m / cos)
   int m0 = 0;
-  DevBuf d_tw64, scale_fold_vec64, scale_inv_a64;
-  // the same with 1/cos(phi): the recurrence form's orders m >= m0
+  DevBuf d_tw64, scale_fold_vec64;
+  // with 1/cos(phi): the fold's orders m >= m0 (the recurrence form) and the
+  // unfold's orders (every order of u and v is a cos(phi) times the wind)
   DevBuf scale_fold_vec64_cos, scale_inv_acos64;
   DevBuf low_syn[3], low_ana[3];
   sht::FftPlan fft{};
@@ -269,12 +270,13 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   }
   if (P->m0 > 0) {
     const size_t low_bytes = (size_t)P->m0 * (T + 1) * P->JHP * sizeof(double);
-    for (int k = 1; k < 3; ++k) {  // dP/dphi and P m / cos
-      P->low_syn[k].alloc(low_bytes, true);
+    for (int k = 1; k < 3; ++k) {  // analysis: dP/dphi and P m / cos
       P->low_ana[k].alloc(low_bytes, true);
-      out.f64_low_syn[k] = P->low_syn[k].as<double>();
       out.f64_low_ana[k] = P->low_ana[k].as<double>();
     }
+    // synthesis: the basis over degrees m..T+1 (the recurrence form in FP64)
+    P->low_syn[kP].alloc((size_t)P->m0 * (T + 2) * P->JHP * sizeof(double), true);
+    out.f64_low_syn[kP] = P->low_syn[kP].as<double>();
     out.low_m0 = P->m0;
   }
   hip_check(sht_launch_legendre_tables(T, P->nlat, P->JH, P->JHP, P->d_mu.as<double>(),
@@ -353,7 +355,7 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   upload(P->scale_inv_a, inva);
   upload(P->scale_inv_acos, invacos);
   if (P->m0 > 0) {
-    std::vector<double> sfv64(P->JHP, 0.0), inva64(P->JHP, 0.0);
+    std::vector<double> sfv64(P->JHP, 0.0);
     std::vector<double> sfvc64(P->JHP, 0.0), invac64(P->JHP, 0.0);
     for (int i = 0; i < P->JH; ++i) {
       const int jn = P->nlat - 1 - i;
@@ -362,12 +364,10 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
       const double mu = P->mu[jn];
       const double cosl = std::sqrt(std::max(0.0, 1.0 - mu * mu));
       sfv64[i] = 2.0 * M_PI * wq / (P->radius * P->nlon);
-      inva64[i] = 1.0 / P->radius;
       sfvc64[i] = sfv64[i] / cosl;
       invac64[i] = 1.0 / (P->radius * cosl);
     }
     upload(P->scale_fold_vec64, sfv64);
-    upload(P->scale_inv_a64, inva64);
     upload(P->scale_fold_vec64_cos, sfvc64);
     upload(P->scale_inv_acos64, invac64);
   }
@@ -731,8 +731,7 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
       sht::LowSynArgs ls{};
       ls.vort = static_cast<const float2*>(J.in0);
       ls.div = static_cast<const float2*>(J.in1);
-      ls.dp = P->low_syn[kD].as<double>();
-      ls.pm = P->low_syn[kPm].as<double>();
+      ls.p = P->low_syn[kP].as<double>();
       ls.low = js->low.as<double2>();
       ls.radius = P->radius;
       ls.T = T;
@@ -779,9 +778,9 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
       double pcs = (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) ? 4.0 * n
                    : (grad || sgrad)                     ? 2.0 * n
                                                          : 1.0 * n;
-      if (prepare && low) {  // FP64 kernels: four per level below m0; GEMM: two above
-        B->legendre_flops += 2.0 * n * (pcf - pcf_low) + 4.0 * n * pcf_low;
-        B->low_flops += 4.0 * n * pcf_low;
+      if (prepare && low) {  // two per level: the GEMM above m0, the FP64 kernel below
+        B->legendre_flops += 2.0 * n * pcf;
+        B->low_flops += 2.0 * n * pcf_low;
       } else if (prepare) {
         B->legendre_flops += pcs * pcf;
       }
@@ -813,8 +812,9 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
         ua.job.f64 = 1;
         ua.job.low = js->low.as<double2>() + (size_t)f * P->m0 * 2 * P->JHP;
         ua.job.low_stride = 2 * P->m0 * 2 * P->JHP;
-        ua.job.rowscale64 = P->scale_inv_a64.as<double>();
-        ua.job.rowscale64_hi = P->scale_inv_acos64.as<double>();  // recurrence form
+        // both parts in the recurrence form: a cos(phi) (u, v)
+        ua.job.rowscale64 = P->scale_inv_acos64.as<double>();
+        ua.job.rowscale64_hi = P->scale_inv_acos64.as<double>();
         ua.job.m0 = P->m0;
       }
       B->unfolds.push_back(ua);

@@ -35,6 +35,7 @@ struct TableOut {
   // FP64 tables of the orders m < low_m0 for the vector operators' FP64
   // correction, over r = n - m (T+1 slots) and folded rows i (JHP):
   //   low_syn [m][r][i]   low_ana [m][i][r]
+  // except low_syn[0], the basis over degrees m..T+1: [m][r < T+2][i]
   double* f64_low_syn[3];
   double* f64_low_ana[3];
   int low_m0;
@@ -246,9 +247,8 @@ struct GemmLaunch {
 struct LowSynArgs {
   const float2* vort;         // (nlev, T+1, T+1) [n][m]
   const float2* div;
-  const double* dp;           // low_syn tables [m][r][i]: dP/dphi and P m / cos
-  const double* pm;
-  double2* low;               // (E, O) of U and V without the 1/a the unfold applies
+  const double* p;            // low_syn[0]: the basis [m][r < T+2][i]
+  double2* low;               // (E, O) of a cos(phi) (u, v): the unfold applies 1/(a cos)
   double radius;
   int T, JH, JHP, m0, nlev;
 };

@@ -17,11 +17,11 @@
 // (n - m) mod 2 = q contributes to E when q = 0; dP/dphi has the opposite
 // parity of P and P m / cos.
 //
-// Synthesis (reference vector.py:36-52, transform.py:1748-1768), with
-// psi, chi = inverse_laplacian(vorticity, divergence) in FP64:
-//   U = -sum psi dP + i sum chi Pm,   V = i sum psi Pm + sum chi dP
-// (the 1/a is applied by the unfold).  Analysis (vector.py:113-134), with
-// the FP64 fold's weights 2 pi w / (a nlon) already in E and O:
+// Synthesis (reference vector.py:36-52, transform.py:1748-1768): the winds
+// times a cos(phi) as one contraction of the basis over degrees m..T+1 (the
+// recurrence form, below; the unfold applies 1/(a cos)).  Analysis
+// (vector.py:113-134), with the FP64 fold's weights 2 pi w / (a nlon)
+// already in E and O:
 //   zeta = i sum Pm V_q + sum dP U_(1-q),   D = i sum Pm U_q - sum dP V_(1-q).
 #include <algorithm>
 
@@ -120,11 +120,19 @@ __device__ __forceinline__ bool low_reduce(d4& acc0, d4& acc1, double* lds, int 
   return true;
 }
 
-// Synthesis, output parity p, rows i (folded latitude), columns (level,
-// component c): c = 0, 1 the real and imaginary part of U, c = 2, 3 of V:
-//   out[i][col] = sum_{r of parity p} Pm[r][i] X[r][col] + sum_{r of parity 1-p} dP[r][i] Y[r][col]
-// with X = (i chi, i psi) and Y = (-psi, chi), r = n - m.
-// Grid: x = m0 x row-tile groups, y = level groups of 8, z = parity.
+// Synthesis in the recurrence form (as the GEMM's orders above m0; sht_pack.hip
+// states the identity): output parity p, rows i (folded latitude), columns
+// (level, component c), c = 0, 1 the real and imaginary part of a cos(phi) u,
+// c = 2, 3 of a cos(phi) v:
+//   out[i][col] = sum_{r of parity p} P[r][i] W[r][col],  W = (U, V) over
+//   degrees k = m + r <= T + 1,
+//   U_k = (k-1) e_k psi_{k-1} - (k+2) e_{k+1} psi_{k+1} + i m chi_k,
+//   V_k = i m psi_k - (k-1) e_k chi_{k-1} + (k+2) e_{k+1} chi_{k+1},
+// formed in FP64 from psi, chi = inverse Laplacian (vorticity, divergence) as
+// the block stages them.  In FP64 the order-0 cancellation near the poles
+// (terms O(1), sum O(cos^2)) costs ~1/cos x 1e-16, so every order m < m0
+// takes this form: one table segment instead of the P m / cos and dP/dphi
+// pair.  Grid: x = m0 x row-tile groups, y = level groups of 8, z = parity.
 __global__ void __launch_bounds__(64 * kLowWaves) low_synthesis_kernel(LowSynArgs a) {
   extern __shared__ double lds_low[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -136,70 +144,73 @@ __global__ void __launch_bounds__(64 * kLowWaves) low_synthesis_kernel(LowSynArg
   const int lev0 = blockIdx.y * kLowLevels;
   const int p = blockIdx.z;
   const int T1 = a.T + 1;
-  const int K = T1 - m;
+  const int kn = (a.T + 3 - m - p) / 2;  // degrees k = m + p + 2 j <= T + 1
+  const int kpad = (kn + kLowKPad - 1) / kLowKPad * kLowKPad;
   const int i = it * 16 + (lane & 15);
   const int ic = min(i, a.JHP - 1);
   d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-  // segment 0: P m / cos against X over r = 2 j + p; segment 1: dP/dphi
-  // against Y over r = 2 j + 1 - p
-  for (int seg = 0; seg < 2; ++seg) {
-    const int pr = seg == 0 ? p : 1 - p;
-    const int kn = (K - pr + 1) / 2;
-    const int kpad = (kn + kLowKPad - 1) / kLowKPad * kLowKPad;
-    if (seg) __syncthreads();  // the previous segment's B is consumed
-    // one thread per (level, k row): psi, chi = inverse Laplacian (vorticity,
-    // divergence) of that entry in FP64 (degree 0 -> 0, m = 0 realified) give
-    // the four columns of that level; all loads of a thread are issued before
-    // the LDS writes
-    constexpr int kPer = 4;
-    const double a2 = a.radius * a.radius;
-    for (int e0 = threadIdx.x; e0 < kpad * kLowLevels; e0 += kPer * blockDim.x) {
-      float2 zv[kPer], dv[kPer];
+  // one thread per (level, k row): psi, chi of degrees k - 1, k, k + 1 give
+  // U_k, V_k of that level; all loads of a thread are issued before the LDS
+  // writes
+  constexpr int kPer = 2;
+  const double a2 = a.radius * a.radius;
+  const double dm = (double)m;
+  for (int e0 = threadIdx.x; e0 < kpad * kLowLevels; e0 += kPer * blockDim.x) {
+    float2 zv[kPer][3], dv[kPer][3];
 #pragma unroll
-      for (int u = 0; u < kPer; ++u) {
-        const int e = e0 + u * blockDim.x;
-        const int l = e % kLowLevels, j = e / kLowLevels;
-        const int lev = min(lev0 + l, a.nlev - 1);
-        const int n = min(m + 2 * j + pr, a.T);
+    for (int u = 0; u < kPer; ++u) {
+      const int e = e0 + u * blockDim.x;
+      const int l = e % kLowLevels, j = e / kLowLevels;
+      const int lev = min(lev0 + l, a.nlev - 1);
+      const int k = m + p + 2 * j;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const int n = min(max(k - 1 + s, m), a.T);  // clamped: never used outside m..T
         const size_t o = ((size_t)lev * T1 + n) * T1 + m;
-        zv[u] = a.vort[o];
-        dv[u] = a.div[o];
+        zv[u][s] = a.vort[o];
+        dv[u][s] = a.div[o];
       }
+    }
 #pragma unroll
-      for (int u = 0; u < kPer; ++u) {
-        const int e = e0 + u * blockDim.x;
-        const int l = e % kLowLevels, j = e / kLowLevels;
-        if (j >= kpad) continue;
-        const int n = m + 2 * j + pr;
-        const bool ok = j < kn && lev0 + l < a.nlev && n >= 1;
-        const double f = ok ? -a2 / ((double)n * ((double)n + 1.0)) : 0.0;
-        const double psr = f * (double)zv[u].x, psi = m == 0 ? 0.0 : f * (double)zv[u].y;
-        const double chr = f * (double)dv[u].x, chi = m == 0 ? 0.0 : f * (double)dv[u].y;
-        double* dst = lds_low + (size_t)j * kLowCols + 4 * l;
-        if (seg == 0) {  // X: U <- i chi, V <- i psi
-          dst[0] = -chi;
-          dst[1] = chr;
-          dst[2] = -psi;
-          dst[3] = psr;
-        } else {         // Y: U <- -psi, V <- chi
-          dst[0] = -psr;
-          dst[1] = -psi;
-          dst[2] = chr;
-          dst[3] = chi;
-        }
+    for (int u = 0; u < kPer; ++u) {
+      const int e = e0 + u * blockDim.x;
+      const int l = e % kLowLevels, j = e / kLowLevels;
+      if (j >= kpad) continue;
+      const int k = m + p + 2 * j;
+      const bool ok = j < kn && lev0 + l < a.nlev;
+      double ps[3][2], ch[3][2];  // psi, chi of degrees k - 1, k, k + 1 (re, im)
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const int n = k - 1 + s;
+        const bool in = ok && n >= m && n <= a.T && n >= 1;
+        const double f = in ? -a2 / ((double)n * ((double)n + 1.0)) : 0.0;
+        ps[s][0] = f * (double)zv[u][s].x;
+        ps[s][1] = m == 0 ? 0.0 : f * (double)zv[u][s].y;
+        ch[s][0] = f * (double)dv[u][s].x;
+        ch[s][1] = m == 0 ? 0.0 : f * (double)dv[u][s].y;
       }
+      const double dk = (double)k;
+      const double e0k = k > m ? sqrt((dk * dk - dm * dm) / (4.0 * dk * dk - 1.0)) : 0.0;
+      const double e1k = sqrt(((dk + 1.0) * (dk + 1.0) - dm * dm) /
+                              (4.0 * (dk + 1.0) * (dk + 1.0) - 1.0));
+      const double ca = (dk - 1.0) * e0k, cb = (dk + 2.0) * e1k;
+      double* dst = lds_low + (size_t)j * kLowCols + 4 * l;
+      dst[0] = ca * ps[0][0] - cb * ps[2][0] - dm * ch[1][1];  // U
+      dst[1] = ca * ps[0][1] - cb * ps[2][1] + dm * ch[1][0];
+      dst[2] = -dm * ps[1][1] - ca * ch[0][0] + cb * ch[2][0];  // V
+      dst[3] = dm * ps[1][0] - ca * ch[0][1] + cb * ch[2][1];
     }
-    __syncthreads();
-    if (active && kn > 0) {
-      const int k0 = kh ? low_split(kpad) : 0, k1 = kh ? kpad : low_split(kpad);
-      const double* tab = (seg == 0 ? a.pm : a.dp) + (size_t)m * T1 * a.JHP + ic;
-      // rows past the segment meet zero B rows, so any finite table value
-      // will do there: clamped addresses, no select (a select right after a
-      // load makes the compiler wait for it before issuing the next)
-      low_segment(acc0, acc1, lds_low + (size_t)k0 * kLowCols, k1 - k0, lane, [&](int j) {
-        return tab[(size_t)min(2 * (j + k0) + pr, a.T) * a.JHP];
-      });
-    }
+  }
+  __syncthreads();
+  if (active && kn > 0) {
+    const int k0 = kh ? low_split(kpad) : 0, k1 = kh ? kpad : low_split(kpad);
+    const double* tab = a.p + (size_t)m * (T1 + 1) * a.JHP + ic;
+    // rows past the order's degrees meet zero B rows, so any finite table
+    // value will do there: clamped addresses, no select (a select right
+    // after a load makes the compiler wait for it before issuing the next)
+    low_segment(acc0, acc1, lds_low + (size_t)k0 * kLowCols, k1 - k0, lane, [&](int j) {
+      return tab[(size_t)min(2 * (j + k0) + p, T1) * a.JHP];
+    });
   }
   if (!low_reduce(acc0, acc1, lds_low, w, lane, active)) return;
 #pragma unroll
@@ -316,8 +327,8 @@ extern "C" hipError_t sht_launch_low_synthesis(const sht::LowSynArgs* a, hipStre
   if (a->nlev <= 0 || a->m0 <= 0) return hipSuccess;
   const int m0 = std::min(a->m0, a->T + 1);
   const int groups = (a->JH + 16 * sht::kLowTiles - 1) / (16 * sht::kLowTiles);
-  const int KP = ((a->T + 2) / 2 + sht::kLowKPad - 1) / sht::kLowKPad * sht::kLowKPad;
-  const size_t lds = std::max((size_t)KP * sht::kLowCols * sizeof(double),  // one segment's B
+  const int KP = ((a->T + 3) / 2 + sht::kLowKPad - 1) / sht::kLowKPad * sht::kLowKPad;
+  const size_t lds = std::max((size_t)KP * sht::kLowCols * sizeof(double),  // the B operand
                               sht::kLowReduceBytes);
   const dim3 g(m0 * groups, (a->nlev + sht::kLowLevels - 1) / sht::kLowLevels, 2);
   hipLaunchKernelGGL(sht::low_synthesis_kernel, g, dim3(64 * sht::kLowWaves), lds, s, *a);

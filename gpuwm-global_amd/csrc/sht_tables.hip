@@ -159,6 +159,8 @@ __global__ void legendre_tables_kernel(int T, int nlat, int JH, int JHP,
     const long long plain_idx = base + (long long)i * kp + nidx;
     const long long syn_idx = base + syn_tiled(JHP, i, nidx);
     const long long ana_idx = base + ana_tiled(kp, i, nidx);
+    if (m < out.low_m0 && out.f64_low_syn[0])  // FP64 basis of the low orders, degrees m..T+1
+      out.f64_low_syn[0][((long long)m * (T + 2) + (n - m)) * JHP + i] = p;
     if (n > T) {  // the extra degree: P only (the other kinds keep their zeros)
       if (nidx < kp) store_value(out, 0, plain_idx, syn_idx, ana_idx, p);
       break;
@@ -172,7 +174,7 @@ __global__ void legendre_tables_kernel(int T, int nlat, int JH, int JHP,
       const long long la = ((long long)m * JHP + i) * (T + 1) + r;
       const double vk[3] = {p, d, pm};
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
+      for (int k = 1; k < 3; ++k) {  // (kind 0: the basis above, T+2 rows per order)
         if (out.f64_low_syn[k]) out.f64_low_syn[k][ls] = vk[k];
         if (out.f64_low_ana[k]) out.f64_low_ana[k][la] = vk[k];
       }
