@@ -150,9 +150,9 @@ struct arwen_sht_plan {
   // vector operators' FP64 low orders m < m0: FP64 twiddles, per-row FP64
   // scales and the FP64 tables of those orders (kinds dP/dphi and P m / cos)
   int m0 = 0;
-  DevBuf d_tw64, scale_fold_vec64;
-  // with 1/cos(phi): the fold's orders m >= m0 (the recurrence form) and the
-  // unfold's orders (every order of u and v is a cos(phi) times the wind)
+  DevBuf d_tw64;
+  // FP64 row scales of the recurrence form (every order of u and v):
+  // fold 2 pi w / (a nlon cos(phi)), unfold 1 / (a cos(phi))
   DevBuf scale_fold_vec64_cos, scale_inv_acos64;
   DevBuf low_syn[3], low_ana[3];
   sht::FftPlan fft{};
@@ -269,14 +269,12 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
     }
   }
   if (P->m0 > 0) {
-    const size_t low_bytes = (size_t)P->m0 * (T + 1) * P->JHP * sizeof(double);
-    for (int k = 1; k < 3; ++k) {  // analysis: dP/dphi and P m / cos
-      P->low_ana[k].alloc(low_bytes, true);
-      out.f64_low_ana[k] = P->low_ana[k].as<double>();
-    }
-    // synthesis: the basis over degrees m..T+1 (the recurrence form in FP64)
-    P->low_syn[kP].alloc((size_t)P->m0 * (T + 2) * P->JHP * sizeof(double), true);
+    // the basis over degrees m..T+1 in both layouts (the recurrence form in FP64)
+    const size_t low_bytes = (size_t)P->m0 * (T + 2) * P->JHP * sizeof(double);
+    P->low_syn[kP].alloc(low_bytes, true);
+    P->low_ana[kP].alloc(low_bytes, true);
     out.f64_low_syn[kP] = P->low_syn[kP].as<double>();
+    out.f64_low_ana[kP] = P->low_ana[kP].as<double>();
     out.low_m0 = P->m0;
   }
   hip_check(sht_launch_legendre_tables(T, P->nlat, P->JH, P->JHP, P->d_mu.as<double>(),
@@ -355,7 +353,7 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
   upload(P->scale_inv_a, inva);
   upload(P->scale_inv_acos, invacos);
   if (P->m0 > 0) {
-    std::vector<double> sfv64(P->JHP, 0.0);
+
     std::vector<double> sfvc64(P->JHP, 0.0), invac64(P->JHP, 0.0);
     for (int i = 0; i < P->JH; ++i) {
       const int jn = P->nlat - 1 - i;
@@ -363,11 +361,9 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
       if (jn == i) wq *= 0.5;
       const double mu = P->mu[jn];
       const double cosl = std::sqrt(std::max(0.0, 1.0 - mu * mu));
-      sfv64[i] = 2.0 * M_PI * wq / (P->radius * P->nlon);
-      sfvc64[i] = sfv64[i] / cosl;
+      sfvc64[i] = 2.0 * M_PI * wq / (P->radius * P->nlon) / cosl;
       invac64[i] = 1.0 / (P->radius * cosl);
     }
-    upload(P->scale_fold_vec64, sfv64);
     upload(P->scale_fold_vec64_cos, sfvc64);
     upload(P->scale_inv_acos64, invac64);
   }
@@ -414,6 +410,7 @@ struct JobState {
   DevBuf waist_a, waist_b;    // synthesis output (b: gradient north) / analysis input
   DevBuf packed_out;          // analysis output
   DevBuf low;                 // vector jobs: FP64 low waist [level][u|v][m < m0][p][i]
+  DevBuf low_ab;              // vordiv_from_uv: FP64 A, B [level][A|B][n < T+2][m < m0]
   int ncols = 0;
 };
 
@@ -906,16 +903,18 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
         fh.job.fstride = 4;
         fh.job.low = js->low.as<double2>() + (size_t)f * P->m0 * 2 * P->JHP;
         fh.job.low_stride = 2 * P->m0 * 2 * P->JHP;
-        fh.job.rowscale64 = P->scale_fold_vec64.as<double>();
-        fh.job.rowscale64_hi = P->scale_fold_vec64_cos.as<double>();  // recurrence form
+        // both parts in the recurrence form: weights 2 pi w / (a nlon cos(phi))
+        fh.job.rowscale64 = P->scale_fold_vec64_cos.as<double>();
+        fh.job.rowscale64_hi = P->scale_fold_vec64_cos.as<double>();
         fh.job.m0 = P->m0;
         B->folds.push_back(fh);
       }
       if (prepare) B->fft_bytes += 2.0 * n * fftb;
+      if (prepare) js->low_ab.alloc((size_t)n * P->m0 * 2 * (T + 2) * sizeof(double2));
       sht::LowAnaArgs la{};
       la.low = js->low.as<double2>();
-      la.dp = P->low_ana[kD].as<double>();
-      la.pm = P->low_ana[kPm].as<double>();
+      la.p = P->low_ana[kP].as<double>();
+      la.ab = js->low_ab.as<double2>();
       la.vort = static_cast<float2*>(J.out0);
       la.div = static_cast<float2*>(J.out1);
       la.T = T;
@@ -948,9 +947,9 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
           add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 1);
         }
       }
-    if (prepare && low) {  // FP64 kernels: four per level below m0; GEMM: two above
-      B->legendre_flops += 2.0 * n * (pcf - pcf_low) + 4.0 * n * pcf_low;
-      B->low_flops += 4.0 * n * pcf_low;
+    if (prepare && low) {  // two per level: the GEMM above m0, the FP64 kernel below
+      B->legendre_flops += 2.0 * n * pcf;
+      B->low_flops += 2.0 * n * pcf_low;
     } else if (prepare) {
       B->legendre_flops += (vec ? 4.0 : divw ? 2.0 : 1.0) * n * pcf;
     }

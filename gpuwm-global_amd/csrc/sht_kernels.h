@@ -35,7 +35,8 @@ struct TableOut {
   // FP64 tables of the orders m < low_m0 for the vector operators' FP64
   // correction, over r = n - m (T+1 slots) and folded rows i (JHP):
   //   low_syn [m][r][i]   low_ana [m][i][r]
-  // except low_syn[0], the basis over degrees m..T+1: [m][r < T+2][i]
+  // except the basis (kind 0) over degrees m..T+1: low_syn[0] [m][r < T+2][i],
+  // low_ana[0] [m][i][r < T+2]
   double* f64_low_syn[3];
   double* f64_low_ana[3];
   int low_m0;
@@ -253,9 +254,10 @@ struct LowSynArgs {
   int T, JH, JHP, m0, nlev;
 };
 struct LowAnaArgs {
-  const double2* low;         // (E, O) of U and V from the FP64 fold, weights applied
-  const double* dp;           // low_ana tables [m][i][r]
-  const double* pm;
+  const double2* low;         // (E, O) of u and v from the FP64 fold, weights
+                              // 2 pi w / (a nlon cos(phi)) applied
+  const double* p;            // low_ana[0]: the basis [m][i][r < T+2]
+  double2* ab;                // A, B of the orders m < m0: [level][A|B][n < T+2][m < m0]
   float2* vort;               // (nlev, T+1, T+1): entries n >= m, m < m0
   float2* div;
   int T, JH, JHP, m0, nlev;

@@ -230,35 +230,37 @@ __global__ void __launch_bounds__(64 * kLowWaves) low_synthesis_kernel(LowSynArg
   }
 }
 
-// Analysis, output parity q, rows r = 2 j + q, columns (level, component):
-// c = 0, 1 the real and imaginary part of vorticity, c = 2, 3 of divergence:
-//   out[r][col] = sum_i Pm[r][i] B1[i][col] + sum_i dP[r][i] B2[i][col]
-// with B1 = (i V_q, i U_q) and B2 = (U_(1-q), -V_(1-q)) from the FP64 fold.
-// Grid: x = m0 x row-tile groups, y = level groups of 8, z = parity.
+// Analysis in the recurrence form: output parity q, rows r = 2 j + q (degrees
+// k = m + r <= T + 1), columns (level, component c), c = 0, 1 the real and
+// imaginary part of A, c = 2, 3 of B:
+//   A_k = sum_i P[r][i] U_q[i],   B_k = sum_i P[r][i] V_q[i]
+// with U, V the FP64 fold's (E, O) of u and v, weighted by
+// 2 pi w / (a nlon cos(phi)).  A and B go to the job's FP64 buffer, and the
+// combination below forms vorticity and divergence from neighbouring degrees
+// in FP64 (as unpack_vector_rec_kernel does for the other orders): one table
+// segment instead of the P m / cos and dP/dphi pair.  Grid: x = m0 x row-tile
+// groups, y = level groups of 8, z = parity.
 __global__ void __launch_bounds__(64 * kLowWaves) low_analysis_kernel(LowAnaArgs a) {
   extern __shared__ double lds_low[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int T1 = a.T + 1;
-  const int rows_max = (T1 + 1) / 2;                      // rows of parity 0 at m = 0
+  const int T2 = a.T + 2;
+  const int rows_max = (a.T + 3) / 2;                     // rows of parity 0 at m = 0
   const int groups = (rows_max + 16 * kLowTiles - 1) / (16 * kLowTiles);
   const int m = blockIdx.x / groups;
   const int it = (blockIdx.x % groups) * kLowTiles + (w >> 1);
   const int kh = w & 1;
   const int lev0 = blockIdx.y * kLowLevels;
   const int q = blockIdx.z;
-  const int K = T1 - m;
-  const int KQ = (K - q + 1) / 2;                         // rows of parity q
+  const int KQ = (a.T + 3 - m - q) / 2;                   // rows of parity q
   const bool active = it * 16 < KQ;                       // wave-uniform
   const int IP = (a.JH + kLowKPad - 1) / kLowKPad * kLowKPad;
   const int j = it * 16 + (lane & 15);
   const int r = 2 * min(j, max(KQ - 1, 0)) + q;
   d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-  for (int seg = 0; seg < 2; ++seg) {
-    if (seg) __syncthreads();
-    // one thread per (level, folded row): the U and V values of the
-    // segment's parity give the four columns of that level; loads first
+  {
+    // one thread per (level, folded row): U and V of parity q give the four
+    // columns of that level; loads first
     constexpr int kPer = 4;
-    const int pq = seg == 0 ? q : 1 - q;
     for (int e0 = threadIdx.x; e0 < IP * kLowLevels; e0 += kPer * blockDim.x) {
       double2 uu[kPer], vv[kPer];
 #pragma unroll
@@ -266,8 +268,8 @@ __global__ void __launch_bounds__(64 * kLowWaves) low_analysis_kernel(LowAnaArgs
         const int e = e0 + u * blockDim.x;
         const int i = min(e % IP, a.JH - 1), l = e / IP;
         const int lev = min(lev0 + l, a.nlev - 1);
-        uu[u] = a.low[low_index(a.JHP, a.m0, lev, 0, m, pq, i)];
-        vv[u] = a.low[low_index(a.JHP, a.m0, lev, 1, m, pq, i)];
+        uu[u] = a.low[low_index(a.JHP, a.m0, lev, 0, m, q, i)];
+        vv[u] = a.low[low_index(a.JHP, a.m0, lev, 1, m, q, i)];
       }
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
@@ -278,31 +280,23 @@ __global__ void __launch_bounds__(64 * kLowWaves) low_analysis_kernel(LowAnaArgs
         const double2 U = ok ? uu[u] : make_double2(0.0, 0.0);
         const double2 V = ok ? vv[u] : make_double2(0.0, 0.0);
         double* dst = lds_low + (size_t)i * kLowCols + 4 * l;
-        if (seg == 0) {  // B1: (i V_q, i U_q)
-          dst[0] = -V.y;
-          dst[1] = V.x;
-          dst[2] = -U.y;
-          dst[3] = U.x;
-        } else {         // B2: (U_(1-q), -V_(1-q))
-          dst[0] = U.x;
-          dst[1] = U.y;
-          dst[2] = -V.x;
-          dst[3] = -V.y;
-        }
+        dst[0] = U.x;
+        dst[1] = U.y;
+        dst[2] = V.x;
+        dst[3] = V.y;
       }
     }
-    __syncthreads();
-    if (active) {
-      const int k0 = kh ? low_split(IP) : 0, k1 = kh ? IP : low_split(IP);
-      const double* tab = (seg == 0 ? a.pm : a.dp) + (size_t)m * a.JHP * T1 + r;
-      low_segment(acc0, acc1, lds_low + (size_t)k0 * kLowCols, k1 - k0, lane,
-                  [&](int i) {  // past JH: zero B rows
-                    return tab[(size_t)min(i + k0, a.JH - 1) * T1];
-                  });
-    }
+  }
+  __syncthreads();
+  if (active) {
+    const int k0 = kh ? low_split(IP) : 0, k1 = kh ? IP : low_split(IP);
+    const double* tab = a.p + (size_t)m * a.JHP * T2 + r;
+    low_segment(acc0, acc1, lds_low + (size_t)k0 * kLowCols, k1 - k0, lane,
+                [&](int i) {  // past JH: zero B rows
+                  return tab[(size_t)min(i + k0, a.JH - 1) * T2];
+                });
   }
   if (!low_reduce(acc0, acc1, lds_low, w, lane, active)) return;
-  const size_t plane = (size_t)T1 * T1;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int col = 16 * h + (lane & 15);
@@ -312,13 +306,51 @@ __global__ void __launch_bounds__(64 * kLowWaves) low_analysis_kernel(LowAnaArgs
     for (int qq = 0; qq < 4; ++qq) {
       const int jo = it * 16 + (lane >> 4) + 4 * qq;
       if (jo < KQ) {
-        const int n = m + 2 * jo + q;
-        float* dst = reinterpret_cast<float*>((c < 2 ? a.vort : a.div) + (size_t)lev * plane +
-                                              (size_t)n * T1 + m);
-        dst[c & 1] = (m == 0 && (c & 1)) ? 0.f : (float)(h ? acc1[qq] : acc0[qq]);
+        double* dst = reinterpret_cast<double*>(
+            a.ab + (((size_t)lev * 2 + (c >> 1)) * T2 + m + 2 * jo + q) * a.m0 + m);
+        dst[c & 1] = h ? acc1[qq] : acc0[qq];
       }
     }
   }
+}
+
+// Vorticity and divergence of the orders m < m0 from the FP64 A, B of the
+// analysis above (degrees n - 1, n, n + 1; the unpack's recurrence sums for
+// the other orders, here in FP64 throughout):
+//   zeta_n = i m B_n - n e_{n+1} A_{n+1} + (n+1) e_n A_{n-1}
+//   D_n    = i m A_n + n e_{n+1} B_{n+1} - (n+1) e_n B_{n-1}
+// One thread per (level, n, m < m0), m fastest (consecutive A, B entries).
+__global__ void __launch_bounds__(256) low_combine_kernel(LowAnaArgs a) {
+  const int T1 = a.T + 1, T2 = a.T + 2, M0 = min(a.m0, T1);
+  const long long total = (long long)a.nlev * T1 * M0;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int m = (int)(e % M0);
+  const int n = (int)((e / M0) % T1);
+  const int lev = (int)(e / ((long long)M0 * T1));
+  if (n < m) return;  // above the triangle: the unpack writes those zeros
+  const double2* A = a.ab + (size_t)lev * 2 * T2 * a.m0 + m;
+  const double2* B = A + (size_t)T2 * a.m0;
+  const double2 an = A[(size_t)n * a.m0], bn = B[(size_t)n * a.m0];
+  const double2 ap = A[(size_t)(n + 1) * a.m0], bp = B[(size_t)(n + 1) * a.m0];
+  const double2 am = n > m ? A[(size_t)(n - 1) * a.m0] : make_double2(0.0, 0.0);
+  const double2 bm = n > m ? B[(size_t)(n - 1) * a.m0] : make_double2(0.0, 0.0);
+  const double dn = (double)n, dm = (double)m;
+  auto eps = [](double nn, double mm) {
+    return nn > mm ? sqrt((nn * nn - mm * mm) / (4.0 * nn * nn - 1.0)) : 0.0;
+  };
+  const double ca = (dn + 1.0) * eps(dn, dm), cb = dn * eps(dn + 1.0, dm);
+  float2 z = make_float2((float)(-dm * bn.y - cb * ap.x + ca * am.x),
+                         (float)(dm * bn.x - cb * ap.y + ca * am.y));
+  float2 d = make_float2((float)(-dm * an.y + cb * bp.x - ca * bm.x),
+                         (float)(dm * an.x + cb * bp.y - ca * bm.y));
+  if (m == 0) {
+    z.y = 0.f;
+    d.y = 0.f;
+  }
+  const size_t o = ((size_t)lev * T1 + n) * T1 + m;
+  a.vort[o] = z;
+  a.div[o] = d;
 }
 
 }  // namespace sht
@@ -338,12 +370,17 @@ extern "C" hipError_t sht_launch_low_synthesis(const sht::LowSynArgs* a, hipStre
 extern "C" hipError_t sht_launch_low_analysis(const sht::LowAnaArgs* a, hipStream_t s) {
   if (a->nlev <= 0 || a->m0 <= 0) return hipSuccess;
   const int m0 = std::min(a->m0, a->T + 1);
-  const int rows_max = (a->T + 2) / 2;
+  const int rows_max = (a->T + 3) / 2;  // degrees up to T+1
   const int groups = (rows_max + 16 * sht::kLowTiles - 1) / (16 * sht::kLowTiles);
   const int IP = (a->JH + sht::kLowKPad - 1) / sht::kLowKPad * sht::kLowKPad;
   const size_t lds = std::max((size_t)IP * sht::kLowCols * sizeof(double), sht::kLowReduceBytes);
   const dim3 g(m0 * groups, (a->nlev + sht::kLowLevels - 1) / sht::kLowLevels, 2);
   hipLaunchKernelGGL(sht::low_analysis_kernel, g, dim3(64 * sht::kLowWaves), lds, s, *a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long total = (long long)a->nlev * (a->T + 1) * m0;
+  hipLaunchKernelGGL(sht::low_combine_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                     0, s, *a);
   return hipGetLastError();
 }
 

@@ -350,7 +350,8 @@ __global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs a) {
 //   zeta_n = i m B_n - n e_{n+1} A_{n+1} + (n+1) e_n A_{n-1},
 // formed here in FP64 from the packed columns (degrees n - 1 .. n + 1 of the
 // chunk's rows and its halo).  The orders m < m0 are left to the FP64
-// low-order kernel, entries above the triangle are written as zeros.
+// low-order kernels (which form them the same way in FP64); entries above
+// the triangle are written as zeros.
 __global__ void __launch_bounds__(256) unpack_vector_rec_kernel(UnpackArgs a) {
   constexpr int kRows = kChunk + 2;
   __shared__ float tile[4][kRows][kTile + 1];  // [col][n - n0 + 1][m - m0]
@@ -368,6 +369,8 @@ __global__ void __launch_bounds__(256) unpack_vector_rec_kernel(UnpackArgs a) {
   const size_t plane = (size_t)T1 * T1;
   const int tid = threadIdx.x;
   const float* in = a.job.in + ((size_t)a.job.col0 + (size_t)f * 4) * a.job.ldp;
+  float2* out0 = a.job.out0 + (size_t)f * plane;
+  float2* out1 = a.job.out1 + (size_t)f * plane;
   if (n0 + kChunk >= m0) {  // some m <= n + 1 in this chunk (block-uniform)
     stage_koff(skoff, a.koff, m0, T);
     __syncthreads();
@@ -420,7 +423,7 @@ __global__ void __launch_bounds__(256) unpack_vector_rec_kernel(UnpackArgs a) {
     const int mi = e % kTile, r = e / kTile;
     const int n = n0 + r, m = m0 + mi;
     if (n > T || m > T) continue;
-    if (m < a.job.m0 && m <= n) continue;  // the FP64 low-order kernel's entries
+    if (m < a.job.m0 && m <= n) continue;  // the FP64 low-order kernels' entries
     float2 z = make_float2(0.f, 0.f), d = make_float2(0.f, 0.f);
     if (m <= n) {
       const int rr = r + 1;
@@ -441,8 +444,8 @@ __global__ void __launch_bounds__(256) unpack_vector_rec_kernel(UnpackArgs a) {
         d.y = 0.f;
       }
     }
-    a.job.out0[(size_t)f * plane + (size_t)n * T1 + m] = z;
-    a.job.out1[(size_t)f * plane + (size_t)n * T1 + m] = d;
+    out0[(size_t)n * T1 + m] = z;
+    out1[(size_t)n * T1 + m] = d;
   }
 }
 

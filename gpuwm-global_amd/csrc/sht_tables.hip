@@ -159,8 +159,10 @@ __global__ void legendre_tables_kernel(int T, int nlat, int JH, int JHP,
     const long long plain_idx = base + (long long)i * kp + nidx;
     const long long syn_idx = base + syn_tiled(JHP, i, nidx);
     const long long ana_idx = base + ana_tiled(kp, i, nidx);
-    if (m < out.low_m0 && out.f64_low_syn[0])  // FP64 basis of the low orders, degrees m..T+1
-      out.f64_low_syn[0][((long long)m * (T + 2) + (n - m)) * JHP + i] = p;
+    if (m < out.low_m0) {  // FP64 basis of the low orders, degrees m..T+1
+      if (out.f64_low_syn[0]) out.f64_low_syn[0][((long long)m * (T + 2) + (n - m)) * JHP + i] = p;
+      if (out.f64_low_ana[0]) out.f64_low_ana[0][((long long)m * JHP + i) * (T + 2) + (n - m)] = p;
+    }
     if (n > T) {  // the extra degree: P only (the other kinds keep their zeros)
       if (nidx < kp) store_value(out, 0, plain_idx, syn_idx, ana_idx, p);
       break;
