@@ -909,6 +909,13 @@ def main():
         "config": {"workload": WORKLOAD_TEXT.get(args.workload, args.workload),
                    "members": members, "grid_outputs_of_synthesis": grid_out,
                    "grid_inputs_of_analysis": grid_in, "precision": args.precision,
+                   "vector_fp64_orders": {f"T{part['spec']['T']}": part["tr"].vector_fp64_orders
+                                          for part in rt.parts},
+                   "vector_form": "orders m < vector_fp64_orders in FP64, the others as one "
+                                  "contraction of the basis per wind component (derivative "
+                                  "recurrence)" if any(part["tr"].vector_fp64_orders
+                                                       for part in rt.parts)
+                                  else "two table segments (P m/cos, dP/dphi)",
                    "specs": [{k: v for k, v in sp.items() if k != "fields"} for sp in specs]},
         "phases_ms": phases,
         "gemm_config": rt.config,
