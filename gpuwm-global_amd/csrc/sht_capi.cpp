@@ -144,7 +144,7 @@ struct arwen_sht_plan {
   std::vector<double> mu, w;
   DevBuf d_mu, d_w, d_koff, d_tw;
   DevBuf tab_syn[3], tab_ana[3];
-  DevBuf scale_fold, scale_fold_vec, scale_one, scale_inv_a, scale_inv_acos;
+  DevBuf scale_fold, scale_fold_vec, scale_fold_vec_cos, scale_one, scale_inv_a, scale_inv_acos;
   DevBuf scale_waist, scale_waist_vec;  // 2 pi w (x 1/a): folds of a reference waist (already / nlon)
   DevBuf sample_rec, sample_diag;       // point sampler recurrence tables, built on first use
   // vector operators' FP64 low orders m < m0: FP64 twiddles, per-row FP64
@@ -331,7 +331,7 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
 
   // Per folded-row scales.
   std::vector<float> sf(P->JHP, 0.f), sfv(P->JHP, 0.f), one(P->JHP, 1.f), inva(P->JHP, 0.f);
-  std::vector<float> invacos(P->JHP, 0.f);
+  std::vector<float> invacos(P->JHP, 0.f), sfvc(P->JHP, 0.f);
   std::vector<float> sw(P->JHP, 0.f), swv(P->JHP, 0.f);
   for (int i = 0; i < P->JH; ++i) {
     const int jn = P->nlat - 1 - i;
@@ -344,9 +344,12 @@ void build_plan(arwen_sht_plan* P, const arwen_sht_plan_desc* d) {
     inva[i] = (float)(1.0 / P->radius);
     const double mu = P->mu[jn];
     invacos[i] = (float)(1.0 / (P->radius * std::sqrt(std::max(0.0, 1.0 - mu * mu))));
+    sfvc[i] = (float)(2.0 * M_PI * wq / (P->radius * P->nlon) /
+                      std::sqrt(std::max(0.0, 1.0 - mu * mu)));
   }
   upload(P->scale_fold, sf);
   upload(P->scale_fold_vec, sfv);
+  upload(P->scale_fold_vec_cos, sfvc);
   upload(P->scale_waist, sw);
   upload(P->scale_waist_vec, swv);
   upload(P->scale_one, one);
@@ -690,6 +693,9 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
   // the orders m < m0 of that (vector jobs in the split modes: FP64 kernels)
   const double pcf_low =
       4.0 * (P->nlat / 2.0) * (P->m0 * (T + 1.0) - P->m0 * (P->m0 - 1.0) / 2.0);
+  // the first order tile's share (orders m < 4), four contractions per level
+  // there in the plain FP32 recurrence form
+  const double pcf_t0 = 4.0 * (P->nlat / 2.0) * (sht::kMT * (T + 1.0) - sht::kMT * (sht::kMT - 1.0) / 2.0);
   const double fftb = 4.0 * P->nlat * P->nlon + 8.0 * P->nlat * (T + 1.0);
   if (n == 0) return;
   if (is_synthesis(J.op)) {
@@ -711,16 +717,20 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     pa.job.col0 = 0;
     pa.job.ldp = P->ldp;
     pa.job.radius = (float)P->radius;
-    // u, v with FP64 low orders: the orders m >= m0 in the recurrence form
+    // u, v with FP64 low orders: the orders m >= m0 in the recurrence form;
+    // without them (plain FP32) on a wave-resident grid: the recurrence form
+    // from m = 4 on, the first order tile in the two-segment form
     const bool low = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV && P->m0 > 0;
+    const bool recp = J.op == ARWEN_SHT_OP_UV_FROM_VORDIV && P->m0 == 0 && P->fft.wave_L > 0;
     pa.job.mode = (J.op == ARWEN_SHT_OP_ZONAL_DERIVATIVE || J.op == ARWEN_SHT_OP_ZONAL_TO_WAIST)
                       ? sht::kIM
-                  : low                              ? sht::kVectorRec
+                  : low || recp                      ? sht::kVectorRec
                   : vec                              ? sht::kVector
                   : (J.flags & ARWEN_SHT_JOB_INVLAP) ? sht::kInvLap
                                                      : sht::kPlain;
     pa.job.off00 = static_cast<const float2*>(J.coeff_offset);
     pa.job.m0 = low ? P->m0 : 0;
+    pa.job.mdp = recp ? sht::kMT : 0;
     B->packs.push_back(pa);
     // ---- FP64 low orders of u, v (orders m < m0) ----
     if (low) {
@@ -742,7 +752,7 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     const DataRef pk{js->packed_a.as<char>()};
     for (int mb = low ? P->m0 / sht::kMT : 0; prepare && mb < P->MB; ++mb)
       for (int q = 0; q < 2; ++q) {
-        if (low) {
+        if (low || (recp && mb >= 1)) {
           // recurrence form: a cos(phi) [u, v] = P [U, V] (pack_vector_rec_kernel)
           SynSeg sg[1] = {{kP, 0, pk, 0}};
           add_syn_group(syn, P, mb, q, js->ncols, js->waist_a.as<float>(), sg, 1);
@@ -778,6 +788,8 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
       if (prepare && low) {  // two per level: the GEMM above m0, the FP64 kernel below
         B->legendre_flops += 2.0 * n * pcf;
         B->low_flops += 2.0 * n * pcf_low;
+      } else if (prepare && recp) {  // four per level in the first order tile, two above
+        B->legendre_flops += 2.0 * n * pcf + 2.0 * n * pcf_t0;
       } else if (prepare) {
         B->legendre_flops += pcs * pcf;
       }
@@ -834,6 +846,11 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     } else if (J.op == ARWEN_SHT_OP_UV_FROM_VORDIV) {
       unfold(js->waist_a, J.out0, 0, 4, P->scale_inv_a, 0);
       unfold(js->waist_a, J.out1, 2, 4, P->scale_inv_a, 0);
+      if (recp)  // the recurrence form's orders: a cos(phi) (u, v)
+        for (size_t k = B->unfolds.size() - 2; k < B->unfolds.size(); ++k) {
+          B->unfolds[k].job.rowscale_hi = P->scale_inv_acos.as<float>();
+          B->unfolds[k].job.m_hi = sht::kMT;
+        }
     } else if (grad) {
       unfold(js->waist_a, J.out0, 0, 2, P->scale_inv_a, 1);
       unfold(js->waist_b, J.out1, 0, 2, P->scale_inv_a, 0);
@@ -890,6 +907,13 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     fa.job.vector = vec ? 1 : 0;
     fa.job.goff = J.grid_offset;
     const bool low = J.op == ARWEN_SHT_OP_VORDIV_FROM_UV && P->m0 > 0;
+    // without FP64 low orders (plain FP32) on a wave-resident grid: the
+    // recurrence form from m = 4 on, its weights 1/cos(phi) in the fold
+    const bool recp = J.op == ARWEN_SHT_OP_VORDIV_FROM_UV && P->m0 == 0 && P->fft.wave_L > 0;
+    if (recp) {
+      fa.job.rowscale_hi = P->scale_fold_vec_cos.as<float>();
+      fa.job.m_hi = sht::kMT;
+    }
     if (low) {
       // FP64 folds of u and v, one field each: i U to level columns 2, 3 and
       // i V to 0, 1 (the [iV, iU] arrangement), orders m < m0 also in FP64
@@ -930,7 +954,7 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     const DataRef wst{js->waist_a.as<char>()};
     for (int mb = low ? P->m0 / sht::kMT : 0; prepare && mb < P->MB; ++mb)
       for (int q = 0; q < 2; ++q) {
-        if (low) {
+        if (low || (recp && mb >= 1)) {
           // recurrence form: [iB, iA] = P^T [iV, iU] / cos(phi); unpack_vector_rec_kernel
           AnaSeg sg[1] = {{kP, 0, wst, 0}};
           add_ana_group(ana, P, mb, q, js->ncols, js->packed_out.as<float>(), sg, 1);
@@ -950,6 +974,8 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     if (prepare && low) {  // two per level: the GEMM above m0, the FP64 kernel below
       B->legendre_flops += 2.0 * n * pcf;
       B->low_flops += 2.0 * n * pcf_low;
+    } else if (prepare && recp) {  // four per level in the first order tile, two above
+      B->legendre_flops += 2.0 * n * pcf + 2.0 * n * pcf_t0;
     } else if (prepare) {
       B->legendre_flops += (vec ? 4.0 : divw ? 2.0 : 1.0) * n * pcf;
     }
@@ -965,7 +991,8 @@ void setup_job(arwen_sht_batch* B, JobState* js, bool prepare) {
     ua.job.ldp = P->ldp;
     ua.job.vector = vec ? 1 : 0;
     ua.job.m0 = low ? P->m0 : 0;
-    ua.job.rec = low ? 1 : 0;
+    ua.job.rec = low || recp ? 1 : 0;
+    ua.job.mdp = recp ? sht::kMT : 0;
     ua.job.off00 = vec ? nullptr : static_cast<const float2*>(J.coeff_offset);
     B->unpacks.push_back(ua);
   }

@@ -741,12 +741,14 @@ __global__ void __launch_bounds__(64 * kWaveBlock, (fold_waves_per_simd<L, F, kW
     // fold into registers, one order per lane and round (consecutive orders
     // across lanes, so the spectrum reads are contiguous): val[r][e] for
     // order m = lane + 64 r, e = 2 * column + parity
-    const float s = live ? a.job.rowscale[i] : 0.f;
+    const float s0 = live ? a.job.rowscale[i] : 0.f;
+    const float s1 = live && a.job.rowscale_hi ? a.job.rowscale_hi[i] : s0;
     constexpr int RO = (N / 2 + 4 + 63) / 64;  // rounds: MO <= N/2 + 4
     float val[RO][NV];
 #pragma unroll
     for (int r = 0; r < RO; ++r) {
       const int m = lane + 64 * r;
+      const float s = m < a.job.m_hi ? s0 : s1;
       float2 e[2], o[2];
 #pragma unroll
       for (int f = 0; f < 2; ++f) {
@@ -896,7 +898,8 @@ __global__ void __launch_bounds__(64 * kWaveBlock,
         oi[r] = in ? stage[(size_t)3 * MO + m] : 0.f;
       }
       wave_sync();  // the staged values are in registers; the span becomes the spectrum
-      const float s = a.job.rowscale[i];
+      const float s0 = a.job.rowscale[i];
+      const float s1 = a.job.rowscale_hi ? a.job.rowscale_hi[i] : s0;
       // Z[k] = A + iB at k = m, conj(A) + i conj(B) at k = N - m; zero above T
       for (int k = lane; k < N; k += 64) buf[k] = make_float2(0.f, 0.f);
       wave_sync();
@@ -904,6 +907,7 @@ __global__ void __launch_bounds__(64 * kWaveBlock,
       for (int r = 0; r < RO; ++r) {
         const int m = lane + 64 * r;
         if (m >= T1) continue;
+        const float s = m < a.job.m_hi ? s0 : s1;
         float2 A = make_float2((er[r] + orr[r]) * s, (ei[r] + oi[r]) * s);
         float2 B = make_float2((er[r] - orr[r]) * s, (ei[r] - oi[r]) * s);
         if (a.job.imul) {
@@ -1509,6 +1513,7 @@ extern "C" hipError_t sht_launch_rfft_fold(const sht::FoldArgs* a, hipStream_t s
   size_t lds = sht::fold_lds_bytes(a->fft.N, a->T, a->job.vector, a->rp);
   int blocks = a->nchunks * a->job.nfields;
   if (blocks == 0) return hipSuccess;
+  if (a->job.rowscale_hi) return hipErrorInvalidValue;  // order-dependent scales: wave path only
   SHT_PF_DISPATCH(rfft_fold_kernel, sht::fft_pf(a->fft.N), dim3(blocks), dim3(sht::kWave * a->rp),
                   lds, s, *a);
   return hipGetLastError();
@@ -1548,6 +1553,7 @@ extern "C" hipError_t sht_launch_irfft_unfold(const sht::UnfoldArgs* a, hipStrea
   size_t lds = sht::unfold_lds_bytes(a->fft.N, a->T, a->rp);
   int blocks = a->nchunks * a->job.nfields;
   if (blocks == 0) return hipSuccess;
+  if (a->job.rowscale_hi) return hipErrorInvalidValue;  // order-dependent scales: wave path only
   SHT_PF_DISPATCH(irfft_unfold_kernel, sht::fft_pf(a->fft.N), dim3(blocks),
                   dim3(sht::kWave * a->rp), lds, s, *a);
   return hipGetLastError();

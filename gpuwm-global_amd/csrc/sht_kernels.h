@@ -57,6 +57,8 @@ struct FoldJob {
   const float* grid1;         // v rows for vector jobs, else NULL
   float* waist0;              // scalar: waist; vector: [iV, iU] per level (fp32)
   const float* rowscale;      // per folded row: 1/nlon x quadrature weight (x 1/a)
+  const float* rowscale_hi;   // wave-resident fold: orders m >= m_hi take this scale
+  int m_hi;                   // (the recurrence form's 1/cos(phi)); NULL: rowscale
   int nfields;                // scalar fields, or u/v level pairs
   int col0;                   // first data column of this job in the waist
   int ldw;                    // floats per data column
@@ -88,6 +90,8 @@ struct UnfoldJob {
   const float* waist;         // folded E/O waist
   float* grid;                // field f rows at grid + f*nlat*nlon
   const float* rowscale;      // per folded row multiplier
+  const float* rowscale_hi;   // wave-resident unfold: orders m >= m_hi take this scale
+  int m_hi;                   // (the recurrence form's 1/cos(phi)); NULL: rowscale
   int nfields;
   int col0;                   // columns of field f: col0 + fstride*f + {0 re, 1 im}
   int fstride;
@@ -277,6 +281,7 @@ struct PackJob {
   float radius;
   const float2* off00;        // kPlain: per field, added to its (0, 0) coefficient (or NULL)
   int m0;                     // kVectorRec: orders m < m0 are not written
+  int mdp;                    // kVectorRec: orders m < mdp as [-psi, chi] (two-segment form)
 };
 
 struct PackArgs {
@@ -298,6 +303,7 @@ struct UnpackJob {
                               // (written by the FP64 low-order kernel)
   int rec;                    // vector jobs in the recurrence form: columns [iB, iA]
                               // over degrees m..T+1 (see unpack_vector_rec_kernel)
+  int mdp;                    // rec: orders m < mdp hold [zeta, D] (two-segment form)
   const float2* off00;        // scalar jobs: per field, added to its (0, 0) output (or NULL)
 };
 

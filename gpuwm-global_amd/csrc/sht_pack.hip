@@ -166,7 +166,9 @@ __global__ void __launch_bounds__(256) pack_kernel(PackArgs a) {
 // pack forms them) and writes them in fp32 as the four
 // columns [U, V] per level; the unfold multiplies by 1/(a cos(phi)).  Near a
 // pole the terms of order 0 cancel to O(cos^2), so the form is used only where
-// the orders below m0 >= 4 run in FP64; above them every term carries cos^m.
+// the orders below m0 >= 4 run in FP64, or from m = mdp = 4 on, the orders
+// below in the two-segment form's columns [-psi, chi] (the plain FP32 path);
+// from there every term carries cos^m.
 // A block owns 32 orders x 64 degrees plus the degrees either side (halo rows
 // n0 - 1 and n0 + 64), so neighbours in degree come from LDS.
 // e_n^m = sqrt((n^2 - m^2) / (4 n^2 - 1)) in FP64 without a divide or a
@@ -256,12 +258,21 @@ __global__ void __launch_bounds__(256) pack_vector_rec_kernel(PackArgs a) {
     const double2 pm = pot(0, -1), p0 = pot(0, 0), pp = pot(0, 1);
     const double2 cm = pot(1, -1), c0 = pot(1, 0), cp = pot(1, 1);
     const double dn = (double)n, dm = (double)m;
-    const double ca = (dn - 1.0) * rec_eps(n, m), cb = (dn + 2.0) * rec_eps(n + 1, m);
     float o[4];
-    o[0] = (float)(ca * pm.x - cb * pp.x - dm * c0.y);  // U
-    o[1] = (float)(ca * pm.y - cb * pp.y + dm * c0.x);
-    o[2] = (float)(-dm * p0.y - ca * cm.x + cb * cp.x);  // V
-    o[3] = (float)(dm * p0.x - ca * cm.y + cb * cp.y);
+    if (m < a.job.mdp) {  // the two-segment form's [-psi, chi] (no degree T+1)
+      if (n > T) continue;
+      const float2 ps = tile[0][r][mi], ch = tile[1][r][mi];
+      o[0] = -ps.x;
+      o[1] = m == 0 ? 0.f : -ps.y;
+      o[2] = ch.x;
+      o[3] = m == 0 ? 0.f : ch.y;
+    } else {
+      const double ca = (dn - 1.0) * rec_eps(n, m), cb = (dn + 2.0) * rec_eps(n + 1, m);
+      o[0] = (float)(ca * pm.x - cb * pp.x - dm * c0.y);  // U
+      o[1] = (float)(ca * pm.y - cb * pp.y + dm * c0.x);
+      o[2] = (float)(-dm * p0.y - ca * cm.x + cb * cp.x);  // V
+      o[3] = (float)(dm * p0.x - ca * cm.y + cb * cp.y);
+    }
     float* dst = a.job.out0 + ((size_t)a.job.col0 + (size_t)f * 4) * a.job.ldp +
                  skoff[2 * mi + p] + nidx;
 #pragma unroll
@@ -350,8 +361,9 @@ __global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs a) {
 //   zeta_n = i m B_n - n e_{n+1} A_{n+1} + (n+1) e_n A_{n-1},
 // formed here in FP64 from the packed columns (degrees n - 1 .. n + 1 of the
 // chunk's rows and its halo).  The orders m < m0 are left to the FP64
-// low-order kernels (which form them the same way in FP64); entries above
-// the triangle are written as zeros.
+// low-order kernels (which form them the same way in FP64), the orders
+// m < mdp hold [zeta, D] from the two-segment form; entries above the
+// triangle are written as zeros.
 __global__ void __launch_bounds__(256) unpack_vector_rec_kernel(UnpackArgs a) {
   constexpr int kRows = kChunk + 2;
   __shared__ float tile[4][kRows][kTile + 1];  // [col][n - n0 + 1][m - m0]
@@ -425,7 +437,11 @@ __global__ void __launch_bounds__(256) unpack_vector_rec_kernel(UnpackArgs a) {
     if (n > T || m > T) continue;
     if (m < a.job.m0 && m <= n) continue;  // the FP64 low-order kernels' entries
     float2 z = make_float2(0.f, 0.f), d = make_float2(0.f, 0.f);
-    if (m <= n) {
+    if (m < a.job.mdp && m <= n) {  // the two-segment form: [zeta, D] as they are
+      const int rr = r + 1;
+      z = make_float2(tile[0][rr][mi], m == 0 ? 0.f : tile[1][rr][mi]);
+      d = make_float2(tile[2][rr][mi], m == 0 ? 0.f : tile[3][rr][mi]);
+    } else if (m <= n) {
       const int rr = r + 1;
       const double dn = (double)n, dm = (double)m;
       const double ca = (dn + 1.0) * rec_eps(n, m), cb = dn * rec_eps(n + 1, m);  // X_{n-1}, X_{n+1}
