@@ -101,6 +101,16 @@ GEMM_TABLES = {"wind": ("syn", "DM"), "inverse": ("syn", "P"), "inverse_gradient
                "vordiv": ("ana", "MD"), "forward": ("ana", "P")}
 
 
+def recurrence_orders(tr):
+    """First order of the vector operators' recurrence form in the GEMM, or
+    None: m0 with FP64 low orders, else 4 on a wave-resident grid (nlon =
+    64 L; the first order tile keeps the two-segment form), else none."""
+    if tr.vector_fp64_orders > 0:
+        return tr.vector_fp64_orders
+    nlon = tr.grid.nlon
+    return 4 if nlon % 64 == 0 and nlon // 64 in (2, 3, 4, 6, 8, 12, 16, 24) else None
+
+
 class RoundTrip:
     """The device buffers and the batch of one workload: the 121-field round
     trip, or with ``step`` the dycore's per-step set (243 grid outputs from
@@ -161,8 +171,8 @@ class RoundTrip:
             for job in pj:
                 nlev = job[2][0].shape[1] if isinstance(job[2], tuple) else job[2].shape[1]
                 d, kinds = GEMM_TABLES[job[0]]
-                if job[0] in ("wind", "vordiv") and tr.vector_fp64_orders > 0:
-                    kinds = "R"  # the basis to degree T+1
+                if job[0] in ("wind", "vordiv") and recurrence_orders(tr) is not None:
+                    kinds = "R"  # the basis to degree T+1 (the first tile's two tables: ~1%)
                     part["rec_levels"] += M * nlev
                 part["tables"][d].update(kinds)
                 for k, w in enumerate(GEMM_PLANES[job[0]]):
@@ -372,8 +382,8 @@ def two_segment_equivalent(rt, precision, t, s, flops):
     not issued); the work-equivalent rate of the launches."""
     extra = 0.0
     for part in rt.parts:
-        m0 = part["tr"].vector_fp64_orders
-        if m0 <= 0 or not part["rec_levels"]:
+        m0 = recurrence_orders(part["tr"])
+        if m0 is None or not part["rec_levels"]:
             continue
         T, nlat = part["spec"]["T"], part["tr"].grid.nlat
         pcf = 4.0 * (nlat / 2) * (T + 1) * (T + 2) / 2
@@ -915,6 +925,9 @@ def main():
                                   "contraction of the basis per wind component (derivative "
                                   "recurrence)" if any(part["tr"].vector_fp64_orders
                                                        for part in rt.parts)
+                                  else "orders m < 4 against the P m/cos and dP/dphi tables, the "
+                                  "others as one contraction of the basis per wind component"
+                                  if all(recurrence_orders(part["tr"]) for part in rt.parts)
                                   else "two table segments (P m/cos, dP/dphi)",
                    "specs": [{k: v for k, v in sp.items() if k != "fields"} for sp in specs]},
         "phases_ms": phases,
