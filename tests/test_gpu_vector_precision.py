@@ -97,6 +97,34 @@ def test_recurrence_form_on_uneven_shapes(T, nlat, nlon, fp64):
     assert max(rel_err(z, z64), rel_err(d, d64)) < BOUND
 
 
+@pytest.mark.parametrize("T,nlat,nlon", [(94, 96, 192), (62, 72, 128), (100, 102, 256)])
+def test_plain_fp32_recurrence_form_on_uneven_shapes(T, nlat, nlon):
+    """The plain FP32 path (no FP64 orders) on wave-resident grids: orders
+    m >= 4 in the recurrence form, the first order tile against the P m/cos
+    and dP/dphi tables, the fold and unfold scaling the two apart.  Same
+    checks and bound as the T21/T63 vector parity tests."""
+    from arwen_spectral import SphericalHarmonicTransform, VorticityDivergenceOperator
+
+    tr = SphericalHarmonicTransform.create(T, nlat=nlat, nlon=nlon, dealias_factor=1.0,
+                                           backend="hip", precision="float32")
+    assert tr.vector_fp64_orders == 0
+    o = OracleTransform.create(T, nlat=nlat, nlon=nlon, dealias_factor=1.0)
+    op = VorticityDivergenceOperator(tr)
+    zeta = o.project(red_spectrum(T, (2,), seed=300 + T, scale=1e-5))
+    div = o.project(red_spectrum(T, (2,), seed=400 + T, scale=1e-5))
+    u, v = op.wind_from_vordiv(zeta, div)
+    u64, v64 = o.wind_from_vordiv(zeta, div)
+    assert max(rel_err(u, u64), rel_err(v, v64)) < BOUND
+    rng = np.random.default_rng(T)
+    gu = rng.standard_normal((2, nlat, nlon)) * 10
+    gv = rng.standard_normal((2, nlat, nlon)) * 10
+    z, d = op.vordiv_from_wind(gu, gv)
+    z64, d64 = o.vordiv_from_wind(gu, gv)
+    print(f"FP32 T{T} {nlat}x{nlon}: winds {max(rel_err(u, u64), rel_err(v, v64)):.2e}, "
+          f"vorticity {rel_err(z, z64):.2e}, divergence {rel_err(d, d64):.2e}")
+    assert max(rel_err(z, z64), rel_err(d, d64)) < BOUND
+
+
 def test_the_fp64_orders_carry_the_bound(oracle383):
     """Without the FP64 orders the same round trip misses the bound by an
     order of magnitude (the reason they exist); with them, split-BF16 and
