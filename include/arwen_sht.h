@@ -95,8 +95,8 @@ typedef struct arwen_sht_plan_desc {
                              the plan's precision, as contractions of the
                              basis alone through the derivative recurrence
                              (two per level instead of four; with 0, from
-                             order 4 on where nlon = 64 L).
-                             -1: the default, 0 for
+                             order 4 on where nlon = 64 L).  -1: the
+                             default, 0 for
                              ARWEN_SHT_FP32 and 4 ceil((T+1)/192) for the
                              split-BF16 modes; 0: off; k > 0: rounded up to
                              a multiple of 4.  Needs nlon = 64 L with
